@@ -1,0 +1,157 @@
+/* lbmh.h -- C ABI of the MI355X (gfx950) block-lattice backend.
+ *
+ * One lattice handle is one OpenLB block: a padded core of nx*ny*nz cells
+ * (padding 1), one periodic-shift (PS) population column per discrete
+ * velocity in HBM, shifted storage f_i - t_i in FP64, FP32 or FP16, and one
+ * dynamics-id byte per core cell.  A time step is one fused kernel launch:
+ * collision of every core cell as an in-place read-modify-write, the
+ * periodic overlap refresh folded into the same pass, and streaming as a
+ * shift of every column.
+ *
+ * Every function returns 0 on success or a negative LBMH_E* code; the text
+ * of the last error of the calling thread is lbmh_last_error().  No C++
+ * exception crosses this boundary.  Handles are not thread-safe: one host
+ * thread drives a lattice at a time.
+ *
+ * Each entry point names the OpenLB 1.8 interface it replaces (paths are
+ * relative to the OpenLB source tree).
+ */
+#ifndef LBMH_H
+#define LBMH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBMH_ABI_VERSION 1
+
+/* status codes */
+#define LBMH_OK 0
+#define LBMH_E_INVALID (-1)   /* bad argument                               */
+#define LBMH_E_DEVICE (-2)    /* HIP runtime error                          */
+#define LBMH_E_MEMORY (-3)    /* allocation failed                          */
+#define LBMH_E_STATE (-4)     /* call not valid in the handle's state       */
+
+/* population storage (shifted deviations f_i - t_i) */
+#define LBMH_F64 0
+#define LBMH_F32 1
+#define LBMH_F16 2            /* FP16 storage, FP32 collision arithmetic    */
+
+/* bulk collision kinds, applied to cells whose dynamics id is 1 */
+#define LBMH_BGK 0            /* BGKdynamics             src/dynamics/dynamics.h:89-98   */
+#define LBMH_FORCED_BGK 1     /* ForcedBGKdynamics (Guo)  src/dynamics/dynamics.h:213-220 */
+#define LBMH_TRT 2            /* TRTdynamics              src/dynamics/dynamics.h:445-451 */
+#define LBMH_FORCED_TRT 3     /* ForcedTRTdynamics        src/dynamics/dynamics.h:457-464 */
+#define LBMH_SMAGORINSKY 4    /* SmagorinskyBGKdynamics   src/dynamics/smagorinskyBGKdynamics.h:38-44 */
+
+/* per-cell dynamics ids (one byte per core cell):
+ *   0       NoDynamics               src/dynamics/dynamics.h:50-62   (not collided)
+ *   1       the lattice's bulk kind
+ *   2       BounceBack               src/dynamics/dynamics.h:472-483
+ *   3..255  BounceBackVelocity wall  src/dynamics/dynamics.h:550-555, each id
+ *           with its own fixed density and wall velocity (lbmh_set_wall) */
+#define LBMH_ID_NONE 0
+#define LBMH_ID_BULK 1
+#define LBMH_ID_BOUNCE_BACK 2
+#define LBMH_ID_FIRST_WALL 3
+
+typedef struct lbmh_lattice lbmh_lattice;
+
+typedef struct lbmh_lattice_desc {
+  int q;            /* 19 (D3Q19) or 27 (D3Q27)                               */
+  int nx, ny, nz;   /* core extent; z is contiguous in memory                 */
+  int periodic[3];  /* periodic overlap refresh per axis (0/1)                */
+  int storage;      /* LBMH_F64 / LBMH_F32 / LBMH_F16                         */
+  int bulk_kind;    /* LBMH_BGK ... LBMH_SMAGORINSKY                          */
+  int device;       /* HIP device ordinal                                     */
+} lbmh_lattice_desc;
+
+typedef struct lbmh_bulk_params {
+  double omega;     /* descriptors::OMEGA = 1/tau                             */
+  double magic;     /* collision::TRT::MAGIC (TRT kinds)                      */
+  double smago;     /* collision::LES::SMAGORINSKY (Smagorinsky kind)         */
+  double force[3];  /* descriptors::FORCE, uniform over the bulk cells         */
+} lbmh_bulk_params;
+
+/* Create a block lattice.  Replaces the GPU_CUDA construction path of
+ * ConcreteBlockLattice (src/core/blockLattice.hh:393-416), which allocates one
+ * CyclicColumn per population (src/core/platform/gpu/cuda/column.hh:217-267).
+ * Populations start at zero (f_i = t_i), every cell has id 0. */
+int lbmh_lattice_create(const lbmh_lattice_desc* desc, lbmh_lattice** out);
+int lbmh_lattice_destroy(lbmh_lattice* lattice);
+
+/* Per-cell dynamics ids, host array [nx][ny][nz].  Replaces the dynamics masks
+ * and DYNAMICS pointer field of ConcreteBlockCollisionO<GPU_CUDA>
+ * (src/core/platform/gpu/cuda/operator.h:46-124, mask.h:31-85). */
+int lbmh_set_dynamics_ids(lbmh_lattice* lattice, const uint8_t* ids);
+
+/* Bulk parameters.  Replaces SuperLattice::setParameter<OMEGA/MAGIC/SMAGORINSKY>
+ * (src/core/superLattice.h:318-336) and the per-cell FORCE field of
+ * ForcedBGKdynamics for a uniform force (src/dynamics/forcing.h:37-74). */
+int lbmh_set_bulk_params(lbmh_lattice* lattice, const lbmh_bulk_params* params);
+
+/* Moving wall id (3..255): fixed density rho and wall velocity u.  Replaces the
+ * RHO / VELOCITY fields of BounceBackVelocity cells
+ * (src/dynamics/momenta/elements.h:237-262, collision.h:323-348). */
+int lbmh_set_wall(lbmh_lattice* lattice, int id, double rho, const double u[3]);
+
+/* Populations of the core cells, shifted values f_i - t_i, host layout
+ * [q][nx][ny][nz] in host_dtype (LBMH_F64/F32/F16).  Replaces the
+ * processing-context copies of CyclicColumn (src/core/platform/gpu/cuda/
+ * column.hh:285-296) and ConcreteBlockLattice::setProcessingContext. */
+int lbmh_populations_upload(lbmh_lattice* lattice, const void* host, int host_dtype);
+int lbmh_populations_download(lbmh_lattice* lattice, void* host, int host_dtype);
+
+/* Equilibrium initialisation on the device from per-cell moments, host layout
+ * [4][nx][ny][nz] = rho, ux, uy, uz.  Replaces SuperLattice::iniEquilibrium
+ * (src/core/blockLattice.hh:202-220; dynamics/interface.h:111-121): forced
+ * bulk cells inverse-shift u by F/2; cells with id 0 are left untouched. */
+int lbmh_ini_equilibrium(lbmh_lattice* lattice, const double* rho_u);
+
+/* Advance n_steps time steps on the given HIP stream (NULL = null stream),
+ * asynchronously.  Replaces SuperLattice::collideAndStream
+ * (src/core/superLattice.hh:601-663) for one block: collide, PostCollide
+ * overlap-1 refresh (src/communication/superCommunicator.hh:189-218), stream
+ * (src/core/blockLattice.hh:317-339). */
+int lbmh_step(lbmh_lattice* lattice, int64_t n_steps, void* stream);
+
+/* Number of completed time steps and the PS shift of every column, in the
+ * convention of CyclicColumn::_shift, normalised to [0, n_padded_cells)
+ * (src/core/platform/cpu/sisd/column.h:193-204). */
+int lbmh_get_step_count(const lbmh_lattice* lattice, int64_t* steps);
+int lbmh_get_shifts(const lbmh_lattice* lattice, int64_t* shifts);
+
+/* Per-cell moments in FP64, host layout [4][nx][ny][nz]: rho and u as the
+ * cell's momenta define them (bulk: lbm<D>::computeRhoU, src/dynamics/lbm.h:
+ * 300-307, forced: u += F/2).  Replaces SuperLatticePhysVelocity3D-style host
+ * evaluation after setProcessingContext(Evaluation). */
+int lbmh_moments(lbmh_lattice* lattice, double* rho_u);
+
+/* FP64 sums over the cells with dynamics id 1: out = {sum rho, sum j_x,
+ * sum j_y, sum j_z} (j = rho u without forcing shift).  Replaces
+ * SuperLatticeFieldReductionO / reductionFunctionDevice
+ * (src/core/platform/gpu/cuda/fieldReduction.hh:152-170). */
+int lbmh_reduce_moments(lbmh_lattice* lattice, double out[4]);
+
+/* Wait for all work queued on the lattice's device.  Replaces
+ * gpu::cuda::device::synchronize (src/core/platform/gpu/cuda/device.hh:64-69). */
+int lbmh_synchronize(lbmh_lattice* lattice);
+
+/* Bytes of device memory the lattice holds. */
+int lbmh_lattice_footprint(const lbmh_lattice* lattice, size_t* bytes);
+
+/* Library and device queries.  Replaces checkPlatform<GPU_CUDA>
+ * (src/core/platform/gpu/cuda/communicator.hh:45-81). */
+int lbmh_abi_version(void);
+int lbmh_device_count(int* count);
+int lbmh_device_arch(int device, char* name, size_t len);
+const char* lbmh_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

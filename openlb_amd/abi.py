@@ -1,0 +1,110 @@
+"""ctypes binding of the C ABI in include/lbmh.h (library openlb_amd/_lib/liblbmh.so).
+
+Loading fails with an exception if the HIP library has not been built. The
+Python layer never computes a time step itself.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "_lib", "liblbmh.so")
+
+F64, F32, F16 = 0, 1, 2
+STORAGE = {"f64": F64, "f32": F32, "f16": F16}
+KINDS = {"bgk": 0, "forced_bgk": 1, "trt": 2, "forced_trt": 3, "smagorinsky": 4}
+
+OK, E_INVALID, E_DEVICE, E_MEMORY, E_STATE = 0, -1, -2, -3, -4
+
+# every function include/lbmh.h declares
+EXPORTS = (
+    "lbmh_lattice_create", "lbmh_lattice_destroy", "lbmh_set_dynamics_ids",
+    "lbmh_set_bulk_params", "lbmh_set_wall", "lbmh_populations_upload",
+    "lbmh_populations_download", "lbmh_ini_equilibrium", "lbmh_step",
+    "lbmh_get_step_count", "lbmh_get_shifts", "lbmh_moments", "lbmh_reduce_moments",
+    "lbmh_synchronize", "lbmh_lattice_footprint", "lbmh_abi_version", "lbmh_device_count",
+    "lbmh_device_arch", "lbmh_last_error",
+)
+
+
+class LatticeDesc(ctypes.Structure):
+    _fields_ = [
+        ("q", ctypes.c_int),
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("periodic", ctypes.c_int * 3),
+        ("storage", ctypes.c_int),
+        ("bulk_kind", ctypes.c_int),
+        ("device", ctypes.c_int),
+    ]
+
+
+class BulkParams(ctypes.Structure):
+    _fields_ = [
+        ("omega", ctypes.c_double),
+        ("magic", ctypes.c_double),
+        ("smago", ctypes.c_double),
+        ("force", ctypes.c_double * 3),
+    ]
+
+
+class LbmhError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"lbmh error {code}: {message}")
+        self.code = code
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load liblbmh.so. It raises if the library is missing, and there is no fallback."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"HIP backend library not built: {LIB_PATH} is missing "
+            "(run `python -m openlb_amd.build` or __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
+    L.lbmh_lattice_create.argtypes = [ctypes.POINTER(LatticeDesc), ctypes.POINTER(vp)]
+    L.lbmh_lattice_destroy.argtypes = [vp]
+    L.lbmh_set_dynamics_ids.argtypes = [vp, vp]
+    L.lbmh_set_bulk_params.argtypes = [vp, ctypes.POINTER(BulkParams)]
+    L.lbmh_set_wall.argtypes = [vp, i32, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
+    L.lbmh_populations_upload.argtypes = [vp, vp, i32]
+    L.lbmh_populations_download.argtypes = [vp, vp, i32]
+    L.lbmh_ini_equilibrium.argtypes = [vp, vp]
+    L.lbmh_step.argtypes = [vp, i64, vp]
+    L.lbmh_get_step_count.argtypes = [vp, ctypes.POINTER(i64)]
+    L.lbmh_get_shifts.argtypes = [vp, vp]
+    L.lbmh_moments.argtypes = [vp, vp]
+    L.lbmh_reduce_moments.argtypes = [vp, vp]
+    L.lbmh_synchronize.argtypes = [vp]
+    L.lbmh_lattice_footprint.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
+    L.lbmh_device_count.argtypes = [ctypes.POINTER(i32)]
+    L.lbmh_device_arch.argtypes = [i32, ctypes.c_char_p, ctypes.c_size_t]
+    L.lbmh_last_error.restype = ctypes.c_char_p
+    for name in EXPORTS:
+        if name != "lbmh_last_error":
+            getattr(L, name).restype = i32
+    _lib = L
+    return L
+
+
+def check(code: int):
+    if code != OK:
+        raise LbmhError(code, load().lbmh_last_error().decode())
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    check(load().lbmh_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def device_arch(device: int = 0) -> str:
+    buf = ctypes.create_string_buffer(64)
+    check(load().lbmh_device_arch(device, buf, 64))
+    return buf.value.decode()

@@ -1,0 +1,342 @@
+// HIP kernels of the block-lattice time step for gfx950 (wave64).
+//
+// Memory layout in HBM: one column per discrete velocity i, each holding all
+// n_padded cells of the padded block (padding 1, x-major, z contiguous,
+// src/core/blockStructure.h:65-78), columns `stride` elements apart.  A column
+// is addressed through its periodic shift: logical cell n of column i lives at
+// element (n + shift_i) mod n_padded (src/core/platform/cpu/sisd/column.h:
+// 160-204).  Streaming is shift_i -= c_i . (sx, sy, 1) on the host, so a time
+// step is a single pass that reads and writes every population once.
+//
+// One lane owns one core cell.  Lanes of a wave take consecutive z, so each of
+// the Q loads and stores of a wave is one contiguous 64-element run of a
+// column (256 B at FP32).  The periodic overlap refresh that OpenLB performs
+// as a separate copy after collision (superLattice.hh:632-637) is folded into
+// the same pass: a boundary cell also writes each population that will be
+// pulled across a periodic face into the halo slot it is pulled from.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fastdiv.hpp"
+#include "lbm_collision.hpp"
+
+namespace lbmh {
+
+template <typename S> struct StorageTraits;
+template <> struct StorageTraits<double> { using V = double; };
+template <> struct StorageTraits<float> { using V = float; };
+template <> struct StorageTraits<_Float16> { using V = float; };
+
+constexpr int kBlock = 256;
+constexpr int kMaxWalls = 256;
+constexpr int kIdBulk = 1;
+
+template <int Q, typename V>
+struct StepArgs {
+  void* pop;
+  uint64_t stride;          // elements between population columns
+  uint32_t n_padded;        // cells per column (modulus of the shifts)
+  uint32_t shift[Q];        // PS shift of each column, in [0, n_padded)
+  const uint8_t* ids;       // per core cell dynamics id (flat core index)
+  uint32_t ncore;
+  uint32_t nx, ny, nz;
+  uint32_t sx, sy;          // padded strides of x and y
+  FastDiv div_z, div_y;
+  uint32_t periodic;        // bit d set: axis d periodic
+  uint32_t xcd_remap;       // map consecutive chunks onto one XCD
+  const double* walls;      // [kMaxWalls][4] rho, ux, uy, uz
+  BulkParams<V> bp;
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8).  Remap so that every XCD walks one contiguous range of cell chunks;
+// the partially shared cache lines at chunk edges then stay in one L2.
+__device__ __forceinline__ uint32_t remap_xcd(uint32_t b, uint32_t nblocks)
+{
+  constexpr uint32_t kXcd = 8;
+  const uint32_t per = nblocks / kXcd, rem = nblocks % kXcd;
+  const uint32_t xcd = b % kXcd, k = b / kXcd;
+  // XCDs below rem own per+1 chunks
+  const uint32_t start = xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per;
+  return start + k;
+}
+
+template <typename S, typename V>
+__device__ __forceinline__ V load_pop(const S* p) { return V(*p); }
+template <typename S, typename V>
+__device__ __forceinline__ void store_pop(S* p, V v) { *p = S(v); }
+
+// Element k of a column through a 32-bit byte offset from the column base, so
+// the base stays in scalar registers and each address costs one VGPR.
+template <typename S>
+__device__ __forceinline__ S* column_at(S* column, uint32_t k)
+{
+  return reinterpret_cast<S*>(reinterpret_cast<char*>(column) + uint32_t(k * uint32_t(sizeof(S))));
+}
+
+__device__ __forceinline__ uint32_t wrap_index(uint32_t n, uint32_t shift, uint32_t N)
+{
+  const uint32_t k = n + shift;
+  return k >= N ? k - N : k;
+}
+
+// Write populations that cross a periodic face into the halo cell they are
+// pulled from next step.  For velocity i of core cell r the target core cell
+// is m = wrap(r + c_i); if m != r + c_i the value must sit at m - c_i.
+template <int Q, typename S, typename V>
+__device__ __forceinline__ void push_periodic_halo(const StepArgs<Q, V>& a, S* pop, const V (&f)[Q],
+                                                int x, int y, int z)
+{
+  using L = Lattice<Q>;
+  const int n[3] = {int(a.nx), int(a.ny), int(a.nz)};
+  const int r[3] = {x, y, z};
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    bool crossed = false, valid = true;
+    int h[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int o = r[d] + L::c[i][d];
+      h[d] = r[d];
+      if (o < 0 || o >= n[d]) {
+        if ((a.periodic >> d) & 1u) {
+          const int w = o < 0 ? n[d] - 1 : 0;
+          h[d] = w - L::c[i][d];
+          crossed = true;
+        } else {
+          valid = false;
+        }
+      }
+    }
+    if (valid && crossed) {
+      const uint32_t hn = uint32_t(h[0] + 1) * a.sx + uint32_t(h[1] + 1) * a.sy + uint32_t(h[2] + 1);
+      const uint32_t k = wrap_index(hn, a.shift[i], a.n_padded);
+      store_pop<S, V>(column_at(pop + uint64_t(i) * a.stride, k), f[i]);
+    }
+  }
+}
+
+// One time step of a block: collide every core cell in place, refresh the
+// periodic overlap.  The caller advances the shifts afterwards.
+template <int Q, typename S, int KIND, bool IDS>
+__global__ __launch_bounds__(kBlock) void collide_stream_kernel(StepArgs<Q, typename StorageTraits<S>::V> a)
+{
+  using V = typename StorageTraits<S>::V;
+  uint32_t b = blockIdx.x;
+  if (a.xcd_remap) b = remap_xcd(b, gridDim.x);
+  const uint32_t c = b * kBlock + threadIdx.x;
+  if (c >= a.ncore) return;
+  int id = kIdBulk;
+  if constexpr (IDS) {
+    id = a.ids[c];
+    if (id == 0) return;
+  }
+  const uint32_t t = a.div_z.div(c);
+  const uint32_t z = c - t * a.nz;
+  const uint32_t x = a.div_y.div(t);
+  const uint32_t y = t - x * a.ny;
+  const uint32_t n = (x + 1) * a.sx + (y + 1) * a.sy + (z + 1);
+  S* pop = static_cast<S*>(a.pop);
+
+  V f[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    f[i] = load_pop<S, V>(column_at(pop + uint64_t(i) * a.stride, wrap_index(n, a.shift[i], a.n_padded)));
+  }
+  if (!IDS || id == kIdBulk) {
+    collide_bulk<Q, V, KIND>(f, a.bp);
+  } else if (id == 2) {
+    revert<Q, V>(f);
+  } else {
+    const double* w = a.walls + 4 * id;
+    const V uw[3] = {V(w[1]), V(w[2]), V(w[3])};
+    moving_wall<Q, V>(f, V(w[0]), uw);
+  }
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    store_pop<S, V>(column_at(pop + uint64_t(i) * a.stride, wrap_index(n, a.shift[i], a.n_padded)), f[i]);
+  }
+
+  if (a.periodic) {
+    const bool edge = (((a.periodic & 1u) != 0) && (x == 0 || x + 1 == a.nx)) ||
+                      (((a.periodic & 2u) != 0) && (y == 0 || y + 1 == a.ny)) ||
+                      (((a.periodic & 4u) != 0) && (z == 0 || z + 1 == a.nz));
+    if (edge) push_periodic_halo<Q, S, V>(a, pop, f, int(x), int(y), int(z));
+  }
+}
+
+// ---- data movement and evaluation kernels (not on the time-step path) ----
+
+struct CoreGeom {
+  uint32_t ncore, nx, ny, nz, sx, sy, n_padded;
+  uint64_t stride;
+  FastDiv div_z, div_y;
+};
+
+__device__ __forceinline__ uint32_t padded_of_core(const CoreGeom& g, uint32_t c)
+{
+  const uint32_t t = g.div_z.div(c);
+  const uint32_t z = c - t * g.nz;
+  const uint32_t x = g.div_y.div(t);
+  const uint32_t y = t - x * g.ny;
+  return (x + 1) * g.sx + (y + 1) * g.sy + (z + 1);
+}
+
+template <int Q>
+struct Shifts { uint32_t s[Q]; };
+
+// host layout [q][ncore] (type H) -> PS storage (type S)
+template <int Q, typename H, typename S>
+__global__ void scatter_populations(const H* src, S* pop, CoreGeom g, Shifts<Q> sh)
+{
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.ncore) return;
+  const uint32_t n = padded_of_core(g, c);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    pop[uint64_t(i) * g.stride + wrap_index(n, sh.s[i], g.n_padded)] = S(src[uint64_t(i) * g.ncore + c]);
+  }
+}
+
+template <int Q, typename H, typename S>
+__global__ void gather_populations(H* dst, const S* pop, CoreGeom g, Shifts<Q> sh)
+{
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.ncore) return;
+  const uint32_t n = padded_of_core(g, c);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    dst[uint64_t(i) * g.ncore + c] = H(pop[uint64_t(i) * g.stride + wrap_index(n, sh.s[i], g.n_padded)]);
+  }
+}
+
+// Equilibrium from per-cell (rho, u), iniEquilibrium semantics.
+template <int Q, typename S>
+__global__ void ini_equilibrium_kernel(const double* rho_u, const uint8_t* ids, S* pop, CoreGeom g,
+                                       Shifts<Q> sh, double hfx, double hfy, double hfz, int forced)
+{
+  using V = typename StorageTraits<S>::V;
+  using L = Lattice<Q>;
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.ncore) return;
+  const int id = ids ? ids[c] : 1;
+  if (id == 0) return;
+  const V rho = V(rho_u[c]);
+  V u[3] = {V(rho_u[uint64_t(g.ncore) + c]), V(rho_u[2 * uint64_t(g.ncore) + c]),
+            V(rho_u[3 * uint64_t(g.ncore) + c])};
+  if (forced && id == 1) {
+    u[0] -= V(hfx);
+    u[1] -= V(hfy);
+    u[2] -= V(hfz);
+  }
+  const V drho = rho - V(1);
+  const V usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  const V A = drho - V(1.5) * rho * usq;
+  V f[Q];
+  f[0] = weight<V, Q>(0) * A;
+#pragma unroll
+  for (int i = 1; i <= L::half; ++i) {
+    V even, odd;
+    equilibrium_pair<Q, V>(i, A, rho, u, even, odd);
+    f[i] = even + odd;
+    f[i + L::half] = even - odd;
+  }
+  const uint32_t n = padded_of_core(g, c);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) pop[uint64_t(i) * g.stride + wrap_index(n, sh.s[i], g.n_padded)] = S(f[i]);
+}
+
+// Per-cell rho, u in FP64 ([4][ncore]); u includes F/2 on forced bulk cells.
+template <int Q, typename S>
+__global__ void moments_kernel(double* out, const uint8_t* ids, const S* pop, CoreGeom g, Shifts<Q> sh,
+                               double hfx, double hfy, double hfz, int forced)
+{
+  using V = typename StorageTraits<S>::V;
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.ncore) return;
+  const uint32_t n = padded_of_core(g, c);
+  V f[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) f[i] = V(pop[uint64_t(i) * g.stride + wrap_index(n, sh.s[i], g.n_padded)]);
+  V drho, j[3];
+  moments<Q, V>(f, drho, j);
+  const V rho = V(1) + drho;
+  double u[3] = {double(j[0] / rho), double(j[1] / rho), double(j[2] / rho)};
+  const int id = ids ? ids[c] : 1;
+  if (forced && id == 1) {
+    u[0] += double(V(hfx));
+    u[1] += double(V(hfy));
+    u[2] += double(V(hfz));
+  }
+  out[c] = double(rho);
+  out[uint64_t(g.ncore) + c] = u[0];
+  out[2 * uint64_t(g.ncore) + c] = u[1];
+  out[3 * uint64_t(g.ncore) + c] = u[2];
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// First level of the FP64 conservation reduction over bulk cells: one
+// partial {rho, jx, jy, jz} per workgroup, fixed order (deterministic).
+template <int Q, typename S>
+__global__ __launch_bounds__(kBlock) void reduce_moments_kernel(double* partial, const uint8_t* ids, const S* pop,
+                                                                CoreGeom g, Shifts<Q> sh)
+{
+  using V = typename StorageTraits<S>::V;
+  double acc[4] = {0, 0, 0, 0};
+  for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < g.ncore; c += gridDim.x * kBlock) {
+    if (ids && ids[c] != 1) continue;
+    const uint32_t n = padded_of_core(g, c);
+    V f[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] = V(pop[uint64_t(i) * g.stride + wrap_index(n, sh.s[i], g.n_padded)]);
+    V drho, j[3];
+    moments<Q, V>(f, drho, j);
+    acc[0] += 1.0 + double(drho);
+    acc[1] += double(j[0]);
+    acc[2] += double(j[1]);
+    acc[3] += double(j[2]);
+  }
+  __shared__ double lds[4][kBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double s = wave_sum(acc[k]);
+    if (lane == 0) lds[k][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double s = 0;
+    for (int w = 0; w < kBlock / 64; ++w) s += lds[threadIdx.x][w];
+    partial[blockIdx.x * 4 + threadIdx.x] = s;
+  }
+}
+
+// Second level: one workgroup sums the partials in a fixed order.
+__global__ __launch_bounds__(kBlock) void reduce_partials_kernel(double* out, const double* partial, uint32_t nparts)
+{
+  __shared__ double lds[4][kBlock / 64];
+  double acc[4] = {0, 0, 0, 0};
+  for (uint32_t p = threadIdx.x; p < nparts; p += kBlock)
+    for (int k = 0; k < 4; ++k) acc[k] += partial[p * 4 + k];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double s = wave_sum(acc[k]);
+    if (lane == 0) lds[k][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double s = 0;
+    for (int w = 0; w < kBlock / 64; ++w) s += lds[threadIdx.x][w];
+    out[threadIdx.x] = s;
+  }
+}
+
+}  // namespace lbmh

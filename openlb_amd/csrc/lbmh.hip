@@ -1,0 +1,543 @@
+// C ABI of the MI355X block-lattice backend (declarations in include/lbmh.h).
+//
+// Host side of one OpenLB block on one GPU: allocation of the population
+// columns, the per-cell dynamics ids and the wall table, the PS shifts (kept
+// on the host and passed to every launch), launches of the fused time step,
+// and the data-movement / evaluation kernels.  Errors are returned as status
+// codes with a per-thread message; nothing throws across the boundary.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lbm_kernels.hpp"
+#include "lbmh.h"
+
+using namespace lbmh;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg)
+{
+  g_error = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* where)
+{
+  return fail(LBMH_E_DEVICE, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define LBMH_HIP(call)                                   \
+  do {                                                   \
+    hipError_t e_ = (call);                              \
+    if (e_ != hipSuccess) return hip_fail(e_, #call);    \
+  } while (0)
+
+size_t storage_size(int s) { return s == LBMH_F64 ? 8 : (s == LBMH_F32 ? 4 : 2); }
+
+// RAII device buffer for temporaries.
+struct DeviceBuffer {
+  void* p = nullptr;
+  ~DeviceBuffer() { if (p) (void)hipFree(p); }
+};
+
+}  // namespace
+
+struct lbmh_lattice {
+  int device = 0;
+  int q = 19;
+  int n[3] = {0, 0, 0};
+  int periodic[3] = {0, 0, 0};
+  int storage = LBMH_F32;
+  int kind = LBMH_BGK;
+  uint32_t size[3] = {0, 0, 0};
+  uint32_t n_padded = 0;
+  uint64_t stride = 0;
+  uint32_t ncore = 0;
+  int64_t steps = 0;
+  int64_t shift[27] = {};
+  int64_t dist[27] = {};
+  void* d_pop = nullptr;
+  uint8_t* d_ids = nullptr;
+  double* d_walls = nullptr;
+  double* d_partials = nullptr;
+  uint32_t n_partials = 0;
+  bool uniform_bulk = false;   // every core cell has id 1: skip the id read
+  bool xcd_remap = true;
+  lbmh_bulk_params params{};
+  double walls[kMaxWalls][4] = {};
+  size_t footprint = 0;
+};
+
+namespace {
+
+CoreGeom core_geom(const lbmh_lattice* L)
+{
+  CoreGeom g;
+  g.ncore = L->ncore;
+  g.nx = L->n[0];
+  g.ny = L->n[1];
+  g.nz = L->n[2];
+  g.sx = L->size[1] * L->size[2];
+  g.sy = L->size[2];
+  g.n_padded = L->n_padded;
+  g.stride = L->stride;
+  g.div_z = FastDiv(uint32_t(L->n[2]));
+  g.div_y = FastDiv(uint32_t(L->n[1]));
+  return g;
+}
+
+template <int Q>
+Shifts<Q> shifts_of(const lbmh_lattice* L)
+{
+  Shifts<Q> s;
+  for (int i = 0; i < Q; ++i) s.s[i] = uint32_t(L->shift[i]);
+  return s;
+}
+
+template <typename V>
+BulkParams<V> bulk_params(const lbmh_lattice* L)
+{
+  const lbmh_bulk_params& p = L->params;
+  BulkParams<V> b;
+  const double omega = p.omega;
+  const double tau = 1.0 / omega;
+  b.omega = V(omega);
+  b.omega2 = V(1.0 / (p.magic / (tau - 0.5) + 0.5));
+  b.guo = V(1.0 - omega * 0.5);
+  for (int d = 0; d < 3; ++d) {
+    b.force[d] = V(p.force[d]);
+    b.half_force[d] = V(p.force[d] * 0.5);
+  }
+  if (L->q == 19) {
+    // constants of OpenLB's generated D3Q19 Smagorinsky kernel
+    // (src/cse/dynamics/d7a4bdcd...cse.h:110)
+    b.sm_a = V(3.00000046417339);
+    b.sm_b = V(p.smago * p.smago / std::sqrt(2.0));
+    b.sm_k = V(0.0277777691819762);
+  } else {
+    // collisionLES.h:74-80: tau_eff = tau/2 + sqrt(tau^2 + 18 sqrt2 C^2 |Pi|/rho)/2
+    b.sm_a = V(0.5);
+    b.sm_b = V(18.0 * std::sqrt(2.0) * p.smago * p.smago);
+    b.sm_k = V(1.0);
+  }
+  b.tau = V(tau);
+  b.half_tau = V(0.5 * tau);
+  return b;
+}
+
+template <int Q, typename S, int KIND>
+int launch_kind(lbmh_lattice* L, hipStream_t stream)
+{
+  using V = typename StorageTraits<S>::V;
+  StepArgs<Q, V> a;
+  a.pop = L->d_pop;
+  a.stride = L->stride;
+  a.n_padded = L->n_padded;
+  a.ids = L->d_ids;
+  a.ncore = L->ncore;
+  a.nx = L->n[0];
+  a.ny = L->n[1];
+  a.nz = L->n[2];
+  a.sx = L->size[1] * L->size[2];
+  a.sy = L->size[2];
+  a.div_z = FastDiv(uint32_t(L->n[2]));
+  a.div_y = FastDiv(uint32_t(L->n[1]));
+  a.periodic = uint32_t(L->periodic[0] | (L->periodic[1] << 1) | (L->periodic[2] << 2));
+  a.xcd_remap = L->xcd_remap ? 1u : 0u;
+  a.walls = L->d_walls;
+  a.bp = bulk_params<V>(L);
+  const uint32_t grid = (L->ncore + kBlock - 1) / kBlock;
+  for (int i = 0; i < Q; ++i) a.shift[i] = uint32_t(L->shift[i]);
+  if (L->uniform_bulk) {
+    hipLaunchKernelGGL((collide_stream_kernel<Q, S, KIND, false>), dim3(grid), dim3(kBlock), 0, stream, a);
+  } else {
+    hipLaunchKernelGGL((collide_stream_kernel<Q, S, KIND, true>), dim3(grid), dim3(kBlock), 0, stream, a);
+  }
+  LBMH_HIP(hipGetLastError());
+  // stream: shift_i -= c_i . projection (blockLattice.hh:317-339)
+  const int64_t N = L->n_padded;
+  for (int i = 0; i < Q; ++i) {
+    int64_t s = (L->shift[i] - L->dist[i]) % N;
+    if (s < 0) s += N;
+    L->shift[i] = s;
+  }
+  L->steps += 1;
+  return LBMH_OK;
+}
+
+template <int Q, typename S>
+int launch_step(lbmh_lattice* L, hipStream_t stream)
+{
+  switch (L->kind) {
+  case LBMH_BGK: return launch_kind<Q, S, kBGK>(L, stream);
+  case LBMH_FORCED_BGK: return launch_kind<Q, S, kForcedBGK>(L, stream);
+  case LBMH_TRT: return launch_kind<Q, S, kTRT>(L, stream);
+  case LBMH_FORCED_TRT: return launch_kind<Q, S, kForcedTRT>(L, stream);
+  case LBMH_SMAGORINSKY: return launch_kind<Q, S, kSmagorinsky>(L, stream);
+  default: return fail(LBMH_E_INVALID, "unknown bulk kind");
+  }
+}
+
+template <int Q>
+int step_q(lbmh_lattice* L, hipStream_t stream)
+{
+  switch (L->storage) {
+  case LBMH_F64: return launch_step<Q, double>(L, stream);
+  case LBMH_F32: return launch_step<Q, float>(L, stream);
+  default: return launch_step<Q, _Float16>(L, stream);
+  }
+}
+
+// Dispatch of a functor over (Q, storage type).
+template <typename F>
+int with_types(const lbmh_lattice* L, F&& f)
+{
+  if (L->q == 19) {
+    if (L->storage == LBMH_F64) return f(std::integral_constant<int, 19>(), double());
+    if (L->storage == LBMH_F32) return f(std::integral_constant<int, 19>(), float());
+    return f(std::integral_constant<int, 19>(), _Float16());
+  }
+  if (L->storage == LBMH_F64) return f(std::integral_constant<int, 27>(), double());
+  if (L->storage == LBMH_F32) return f(std::integral_constant<int, 27>(), float());
+  return f(std::integral_constant<int, 27>(), _Float16());
+}
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev)
+  {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard()
+  {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+uint32_t grid_for(uint32_t n, uint32_t block) { return (n + block - 1) / block; }
+
+}  // namespace
+
+extern "C" {
+
+const char* lbmh_last_error(void) { return g_error.c_str(); }
+
+int lbmh_abi_version(void) { return LBMH_ABI_VERSION; }
+
+int lbmh_device_count(int* count)
+{
+  if (!count) return fail(LBMH_E_INVALID, "count is null");
+  LBMH_HIP(hipGetDeviceCount(count));
+  return LBMH_OK;
+}
+
+int lbmh_device_arch(int device, char* name, size_t len)
+{
+  if (!name || len == 0) return fail(LBMH_E_INVALID, "name buffer is null");
+  hipDeviceProp_t prop;
+  LBMH_HIP(hipGetDeviceProperties(&prop, device));
+  std::snprintf(name, len, "%s", prop.gcnArchName);
+  return LBMH_OK;
+}
+
+int lbmh_lattice_create(const lbmh_lattice_desc* desc, lbmh_lattice** out)
+{
+  if (!desc || !out) return fail(LBMH_E_INVALID, "null argument");
+  *out = nullptr;
+  if (desc->q != 19 && desc->q != 27) return fail(LBMH_E_INVALID, "q must be 19 or 27");
+  if (desc->storage < LBMH_F64 || desc->storage > LBMH_F16) return fail(LBMH_E_INVALID, "unknown storage type");
+  if (desc->bulk_kind < LBMH_BGK || desc->bulk_kind > LBMH_SMAGORINSKY) return fail(LBMH_E_INVALID, "unknown bulk kind");
+  if (desc->q == 27 && desc->bulk_kind == LBMH_FORCED_TRT) {
+    // keep the kernel set to the dynamics OpenLB defines for both descriptors
+  }
+  for (int d = 0; d < 3; ++d) {
+    if (desc->nx < 1 || desc->ny < 1 || desc->nz < 1) return fail(LBMH_E_INVALID, "extent must be >= 1");
+  }
+  const uint64_t sx = uint64_t(desc->nx) + 2, sy = uint64_t(desc->ny) + 2, sz = uint64_t(desc->nz) + 2;
+  const uint64_t npad = sx * sy * sz;
+  // 32-bit cell indices, with room for index + shift < 2^32 (CellID is uint32 in
+  // OpenLB, src/core/blockStructure.h:36,80-82)
+  if (npad >= (uint64_t(1) << 31)) return fail(LBMH_E_INVALID, "padded cell count must be below 2^31");
+  auto* L = new (std::nothrow) lbmh_lattice();
+  if (!L) return fail(LBMH_E_MEMORY, "host allocation failed");
+  L->device = desc->device;
+  L->q = desc->q;
+  L->n[0] = desc->nx;
+  L->n[1] = desc->ny;
+  L->n[2] = desc->nz;
+  for (int d = 0; d < 3; ++d) L->periodic[d] = desc->periodic[d] ? 1 : 0;
+  L->storage = desc->storage;
+  L->kind = desc->bulk_kind;
+  L->size[0] = uint32_t(sx);
+  L->size[1] = uint32_t(sy);
+  L->size[2] = uint32_t(sz);
+  L->n_padded = uint32_t(npad);
+  L->ncore = uint32_t(uint64_t(desc->nx) * desc->ny * desc->nz);
+  // columns start on 256-byte boundaries
+  const uint64_t align = 256 / storage_size(L->storage);
+  L->stride = (npad + align - 1) / align * align;
+  for (int i = 0; i < L->q; ++i) {
+    L->shift[i] = 0;
+    L->dist[i] = L->q == 19 ? neighbor_distance<19>(i, int64_t(sy * sz), int64_t(sz))
+                            : neighbor_distance<27>(i, int64_t(sy * sz), int64_t(sz));
+  }
+  L->params.omega = 1.0;
+  for (int w = 0; w < kMaxWalls; ++w) L->walls[w][0] = 1.0;
+
+  DeviceGuard guard(L->device);
+  const size_t pop_bytes = size_t(L->stride) * L->q * storage_size(L->storage);
+  L->n_partials = 1024;
+  hipError_t e = hipMalloc(&L->d_pop, pop_bytes);
+  if (e == hipSuccess) e = hipMalloc((void**)&L->d_ids, L->ncore);
+  if (e == hipSuccess) e = hipMalloc((void**)&L->d_walls, sizeof(double) * 4 * kMaxWalls);
+  if (e == hipSuccess) e = hipMalloc((void**)&L->d_partials, sizeof(double) * 4 * (L->n_partials + 1));
+  if (e == hipSuccess) e = hipMemset(L->d_pop, 0, pop_bytes);
+  if (e == hipSuccess) e = hipMemset(L->d_ids, 0, L->ncore);
+  if (e == hipSuccess) e = hipMemcpy(L->d_walls, L->walls, sizeof(L->walls), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    const int rc = hip_fail(e, "lbmh_lattice_create");
+    lbmh_lattice_destroy(L);
+    return rc == LBMH_E_DEVICE && e == hipErrorOutOfMemory ? fail(LBMH_E_MEMORY, g_error) : rc;
+  }
+  L->footprint = pop_bytes + L->ncore + sizeof(double) * 4 * (kMaxWalls + L->n_partials + 1);
+  *out = L;
+  return LBMH_OK;
+}
+
+int lbmh_lattice_destroy(lbmh_lattice* L)
+{
+  if (!L) return LBMH_OK;
+  DeviceGuard guard(L->device);
+  if (L->d_pop) (void)hipFree(L->d_pop);
+  if (L->d_ids) (void)hipFree(L->d_ids);
+  if (L->d_walls) (void)hipFree(L->d_walls);
+  if (L->d_partials) (void)hipFree(L->d_partials);
+  delete L;
+  return LBMH_OK;
+}
+
+int lbmh_set_dynamics_ids(lbmh_lattice* L, const uint8_t* ids)
+{
+  if (!L || !ids) return fail(LBMH_E_INVALID, "null argument");
+  DeviceGuard guard(L->device);
+  bool uniform = true;
+  for (uint32_t c = 0; c < L->ncore; ++c) {
+    if (ids[c] != 1) {
+      uniform = false;
+      break;
+    }
+  }
+  LBMH_HIP(hipMemcpy(L->d_ids, ids, L->ncore, hipMemcpyHostToDevice));
+  L->uniform_bulk = uniform;
+  return LBMH_OK;
+}
+
+int lbmh_set_bulk_params(lbmh_lattice* L, const lbmh_bulk_params* p)
+{
+  if (!L || !p) return fail(LBMH_E_INVALID, "null argument");
+  if (!(p->omega > 0.0 && p->omega < 2.0)) return fail(LBMH_E_INVALID, "omega must lie in (0, 2)");
+  if ((L->kind == LBMH_TRT || L->kind == LBMH_FORCED_TRT) && !(p->magic > 0.0))
+    return fail(LBMH_E_INVALID, "TRT magic parameter must be positive");
+  if (L->kind == LBMH_SMAGORINSKY && !(p->smago >= 0.0))
+    return fail(LBMH_E_INVALID, "Smagorinsky constant must be non-negative");
+  L->params = *p;
+  return LBMH_OK;
+}
+
+int lbmh_set_wall(lbmh_lattice* L, int id, double rho, const double u[3])
+{
+  if (!L || !u) return fail(LBMH_E_INVALID, "null argument");
+  if (id < LBMH_ID_FIRST_WALL || id >= kMaxWalls) return fail(LBMH_E_INVALID, "wall id must be in [3, 255]");
+  DeviceGuard guard(L->device);
+  L->walls[id][0] = rho;
+  for (int d = 0; d < 3; ++d) L->walls[id][d + 1] = u[d];
+  LBMH_HIP(hipMemcpy(L->d_walls + 4 * id, L->walls[id], sizeof(double) * 4, hipMemcpyHostToDevice));
+  return LBMH_OK;
+}
+
+int lbmh_populations_upload(lbmh_lattice* L, const void* host, int host_dtype)
+{
+  if (!L || !host) return fail(LBMH_E_INVALID, "null argument");
+  if (host_dtype < LBMH_F64 || host_dtype > LBMH_F16) return fail(LBMH_E_INVALID, "unknown host dtype");
+  DeviceGuard guard(L->device);
+  const size_t bytes = size_t(L->q) * L->ncore * storage_size(host_dtype);
+  DeviceBuffer staging;
+  LBMH_HIP(hipMalloc(&staging.p, bytes));
+  LBMH_HIP(hipMemcpy(staging.p, host, bytes, hipMemcpyHostToDevice));
+  const CoreGeom g = core_geom(L);
+  const uint32_t grid = grid_for(L->ncore, 256);
+  const int rc = with_types(L, [&](auto qc, auto s) -> int {
+    constexpr int Q = decltype(qc)::value;
+    using S = decltype(s);
+    const Shifts<Q> sh = shifts_of<Q>(L);
+    S* pop = static_cast<S*>(L->d_pop);
+    if (host_dtype == LBMH_F64)
+      hipLaunchKernelGGL((scatter_populations<Q, double, S>), dim3(grid), dim3(256), 0, 0, (const double*)staging.p, pop, g, sh);
+    else if (host_dtype == LBMH_F32)
+      hipLaunchKernelGGL((scatter_populations<Q, float, S>), dim3(grid), dim3(256), 0, 0, (const float*)staging.p, pop, g, sh);
+    else
+      hipLaunchKernelGGL((scatter_populations<Q, _Float16, S>), dim3(grid), dim3(256), 0, 0, (const _Float16*)staging.p, pop, g, sh);
+    LBMH_HIP(hipGetLastError());
+    return LBMH_OK;
+  });
+  if (rc != LBMH_OK) return rc;
+  LBMH_HIP(hipDeviceSynchronize());
+  return LBMH_OK;
+}
+
+int lbmh_populations_download(lbmh_lattice* L, void* host, int host_dtype)
+{
+  if (!L || !host) return fail(LBMH_E_INVALID, "null argument");
+  if (host_dtype < LBMH_F64 || host_dtype > LBMH_F16) return fail(LBMH_E_INVALID, "unknown host dtype");
+  DeviceGuard guard(L->device);
+  const size_t bytes = size_t(L->q) * L->ncore * storage_size(host_dtype);
+  DeviceBuffer staging;
+  LBMH_HIP(hipMalloc(&staging.p, bytes));
+  const CoreGeom g = core_geom(L);
+  const uint32_t grid = grid_for(L->ncore, 256);
+  LBMH_HIP(hipDeviceSynchronize());
+  const int rc = with_types(L, [&](auto qc, auto s) -> int {
+    constexpr int Q = decltype(qc)::value;
+    using S = decltype(s);
+    const Shifts<Q> sh = shifts_of<Q>(L);
+    const S* pop = static_cast<const S*>(L->d_pop);
+    if (host_dtype == LBMH_F64)
+      hipLaunchKernelGGL((gather_populations<Q, double, S>), dim3(grid), dim3(256), 0, 0, (double*)staging.p, pop, g, sh);
+    else if (host_dtype == LBMH_F32)
+      hipLaunchKernelGGL((gather_populations<Q, float, S>), dim3(grid), dim3(256), 0, 0, (float*)staging.p, pop, g, sh);
+    else
+      hipLaunchKernelGGL((gather_populations<Q, _Float16, S>), dim3(grid), dim3(256), 0, 0, (_Float16*)staging.p, pop, g, sh);
+    LBMH_HIP(hipGetLastError());
+    return LBMH_OK;
+  });
+  if (rc != LBMH_OK) return rc;
+  LBMH_HIP(hipMemcpy(host, staging.p, bytes, hipMemcpyDeviceToHost));
+  return LBMH_OK;
+}
+
+int lbmh_ini_equilibrium(lbmh_lattice* L, const double* rho_u)
+{
+  if (!L || !rho_u) return fail(LBMH_E_INVALID, "null argument");
+  DeviceGuard guard(L->device);
+  const size_t bytes = size_t(4) * L->ncore * sizeof(double);
+  DeviceBuffer staging;
+  LBMH_HIP(hipMalloc(&staging.p, bytes));
+  LBMH_HIP(hipMemcpy(staging.p, rho_u, bytes, hipMemcpyHostToDevice));
+  const CoreGeom g = core_geom(L);
+  const uint32_t grid = grid_for(L->ncore, 256);
+  const int forced = (L->kind == LBMH_FORCED_BGK || L->kind == LBMH_FORCED_TRT) ? 1 : 0;
+  const double* hf = L->params.force;
+  const int rc = with_types(L, [&](auto qc, auto s) -> int {
+    constexpr int Q = decltype(qc)::value;
+    using S = decltype(s);
+    hipLaunchKernelGGL((ini_equilibrium_kernel<Q, S>), dim3(grid), dim3(256), 0, 0, (const double*)staging.p,
+                       L->uniform_bulk ? nullptr : L->d_ids, static_cast<S*>(L->d_pop), g, shifts_of<Q>(L),
+                       hf[0] * 0.5, hf[1] * 0.5, hf[2] * 0.5, forced);
+    LBMH_HIP(hipGetLastError());
+    return LBMH_OK;
+  });
+  if (rc != LBMH_OK) return rc;
+  LBMH_HIP(hipDeviceSynchronize());
+  return LBMH_OK;
+}
+
+int lbmh_step(lbmh_lattice* L, int64_t n_steps, void* stream)
+{
+  if (!L) return fail(LBMH_E_INVALID, "null lattice");
+  if (n_steps < 0) return fail(LBMH_E_INVALID, "negative step count");
+  DeviceGuard guard(L->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  for (int64_t k = 0; k < n_steps; ++k) {
+    const int rc = L->q == 19 ? step_q<19>(L, s) : step_q<27>(L, s);
+    if (rc != LBMH_OK) return rc;
+  }
+  return LBMH_OK;
+}
+
+int lbmh_get_step_count(const lbmh_lattice* L, int64_t* steps)
+{
+  if (!L || !steps) return fail(LBMH_E_INVALID, "null argument");
+  *steps = L->steps;
+  return LBMH_OK;
+}
+
+int lbmh_get_shifts(const lbmh_lattice* L, int64_t* shifts)
+{
+  if (!L || !shifts) return fail(LBMH_E_INVALID, "null argument");
+  for (int i = 0; i < L->q; ++i) shifts[i] = L->shift[i];
+  return LBMH_OK;
+}
+
+int lbmh_moments(lbmh_lattice* L, double* rho_u)
+{
+  if (!L || !rho_u) return fail(LBMH_E_INVALID, "null argument");
+  DeviceGuard guard(L->device);
+  const size_t bytes = size_t(4) * L->ncore * sizeof(double);
+  DeviceBuffer out;
+  LBMH_HIP(hipMalloc(&out.p, bytes));
+  const CoreGeom g = core_geom(L);
+  const uint32_t grid = grid_for(L->ncore, 256);
+  const int forced = (L->kind == LBMH_FORCED_BGK || L->kind == LBMH_FORCED_TRT) ? 1 : 0;
+  const double* hf = L->params.force;
+  LBMH_HIP(hipDeviceSynchronize());
+  const int rc = with_types(L, [&](auto qc, auto s) -> int {
+    constexpr int Q = decltype(qc)::value;
+    using S = decltype(s);
+    hipLaunchKernelGGL((moments_kernel<Q, S>), dim3(grid), dim3(256), 0, 0, (double*)out.p,
+                       L->uniform_bulk ? nullptr : L->d_ids, static_cast<const S*>(L->d_pop), g, shifts_of<Q>(L),
+                       hf[0] * 0.5, hf[1] * 0.5, hf[2] * 0.5, forced);
+    LBMH_HIP(hipGetLastError());
+    return LBMH_OK;
+  });
+  if (rc != LBMH_OK) return rc;
+  LBMH_HIP(hipMemcpy(rho_u, out.p, bytes, hipMemcpyDeviceToHost));
+  return LBMH_OK;
+}
+
+int lbmh_reduce_moments(lbmh_lattice* L, double out[4])
+{
+  if (!L || !out) return fail(LBMH_E_INVALID, "null argument");
+  DeviceGuard guard(L->device);
+  const CoreGeom g = core_geom(L);
+  const uint32_t blocks = std::min<uint32_t>(L->n_partials, grid_for(L->ncore, kBlock));
+  LBMH_HIP(hipDeviceSynchronize());
+  const int rc = with_types(L, [&](auto qc, auto s) -> int {
+    constexpr int Q = decltype(qc)::value;
+    using S = decltype(s);
+    hipLaunchKernelGGL((reduce_moments_kernel<Q, S>), dim3(blocks), dim3(kBlock), 0, 0, L->d_partials,
+                       L->uniform_bulk ? nullptr : L->d_ids, static_cast<const S*>(L->d_pop), g, shifts_of<Q>(L));
+    LBMH_HIP(hipGetLastError());
+    return LBMH_OK;
+  });
+  if (rc != LBMH_OK) return rc;
+  double* result = L->d_partials + 4 * L->n_partials;
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(kBlock), 0, 0, result, L->d_partials, blocks);
+  LBMH_HIP(hipGetLastError());
+  LBMH_HIP(hipMemcpy(out, result, sizeof(double) * 4, hipMemcpyDeviceToHost));
+  return LBMH_OK;
+}
+
+int lbmh_synchronize(lbmh_lattice* L)
+{
+  if (!L) return fail(LBMH_E_INVALID, "null lattice");
+  DeviceGuard guard(L->device);
+  LBMH_HIP(hipDeviceSynchronize());
+  return LBMH_OK;
+}
+
+int lbmh_lattice_footprint(const lbmh_lattice* L, size_t* bytes)
+{
+  if (!L || !bytes) return fail(LBMH_E_INVALID, "null argument");
+  *bytes = L->footprint;
+  return LBMH_OK;
+}
+
+}  // extern "C"

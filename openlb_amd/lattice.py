@@ -1,0 +1,148 @@
+"""Python front end of one device block lattice (thin layer over the C ABI).
+
+The names follow OpenLB's block/super-lattice API so that tests read like the
+reference's example programs: ``define_dynamics`` (``SuperLattice::defineDynamics``,
+``src/core/superLattice.h:214-233``), ``set_parameters`` (``setParameter<OMEGA>``,
+``:318-336``), ``ini_equilibrium`` (``:344-352``) and ``collide_and_stream`` (``:380``).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import abi
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+_NP_DTYPE = {abi.F64: np.float64, abi.F32: np.float32, abi.F16: np.float16}
+
+
+class BlockLattice:
+    """A padded D3Q19/D3Q27 block with periodic-shift population columns in HBM."""
+
+    def __init__(self, q: int, shape, periodic=(0, 0, 0), storage: str = "f32",
+                 kind: str = "bgk", device: int = 0):
+        L = abi.load()
+        self.q = int(q)
+        self.shape = tuple(int(n) for n in shape)
+        self.periodic = tuple(int(bool(p)) for p in periodic)
+        self.storage = storage
+        self.kind = kind
+        desc = abi.LatticeDesc()
+        desc.q = self.q
+        desc.nx, desc.ny, desc.nz = self.shape
+        for d in range(3):
+            desc.periodic[d] = self.periodic[d]
+        desc.storage = abi.STORAGE[storage]
+        desc.bulk_kind = abi.KINDS[kind]
+        desc.device = device
+        h = ctypes.c_void_p()
+        abi.check(L.lbmh_lattice_create(ctypes.byref(desc), ctypes.byref(h)))
+        self._h = h
+
+    @classmethod
+    def from_case(cls, case, storage: str = "f32", device: int = 0):
+        lat = cls(case.q, case.shape, case.periodic, storage, case.kind, device)
+        lat.define_dynamics(case.ids)
+        lat.set_parameters(omega=case.omega, magic=case.magic, smago=case.smago, force=case.force)
+        for wid, (rho, u) in case.walls.items():
+            lat.set_wall(int(wid), rho, u)
+        return lat
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            abi.load().lbmh_lattice_destroy(h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def ncore(self) -> int:
+        return int(np.prod(self.shape))
+
+    def define_dynamics(self, ids: np.ndarray):
+        ids = np.ascontiguousarray(ids, dtype=np.uint8)
+        if ids.shape != self.shape:
+            raise ValueError(f"ids shape {ids.shape} != lattice shape {self.shape}")
+        abi.check(abi.load().lbmh_set_dynamics_ids(self._h, _ptr(ids)))
+
+    def set_parameters(self, omega: float, magic: float = 0.25, smago: float = 0.0,
+                       force=(0.0, 0.0, 0.0)):
+        p = abi.BulkParams()
+        p.omega, p.magic, p.smago = float(omega), float(magic), float(smago)
+        for d in range(3):
+            p.force[d] = float(force[d])
+        abi.check(abi.load().lbmh_set_bulk_params(self._h, ctypes.byref(p)))
+
+    def set_wall(self, wall_id: int, rho: float, u):
+        u3 = (ctypes.c_double * 3)(*map(float, u))
+        abi.check(abi.load().lbmh_set_wall(self._h, int(wall_id), float(rho), u3))
+
+    def upload_populations(self, f: np.ndarray, dtype: str = "f64"):
+        code = abi.STORAGE[dtype]
+        a = np.ascontiguousarray(f, dtype=_NP_DTYPE[code]).reshape(self.q, -1)
+        if a.shape[1] != self.ncore:
+            raise ValueError("population array does not match the lattice")
+        abi.check(abi.load().lbmh_populations_upload(self._h, _ptr(a), code))
+
+    def download_populations(self, dtype: str = "f64") -> np.ndarray:
+        code = abi.STORAGE[dtype]
+        out = np.empty((self.q,) + self.shape, dtype=_NP_DTYPE[code])
+        abi.check(abi.load().lbmh_populations_download(self._h, _ptr(out), code))
+        return out
+
+    def ini_equilibrium(self, rho_u: np.ndarray):
+        a = np.ascontiguousarray(rho_u, dtype=np.float64)
+        if a.shape != (4,) + self.shape:
+            raise ValueError("rho_u must have shape (4, nx, ny, nz)")
+        abi.check(abi.load().lbmh_ini_equilibrium(self._h, _ptr(a)))
+
+    def collide_and_stream(self, steps: int = 1, stream: int | None = None):
+        abi.check(abi.load().lbmh_step(self._h, int(steps), ctypes.c_void_p(stream or 0)))
+
+    def synchronize(self):
+        abi.check(abi.load().lbmh_synchronize(self._h))
+
+    def step_count(self) -> int:
+        s = ctypes.c_int64(0)
+        abi.check(abi.load().lbmh_get_step_count(self._h, ctypes.byref(s)))
+        return s.value
+
+    def shifts(self) -> np.ndarray:
+        out = np.zeros(self.q, dtype=np.int64)
+        abi.check(abi.load().lbmh_get_shifts(self._h, _ptr(out)))
+        return out
+
+    def moments(self) -> np.ndarray:
+        out = np.empty((4,) + self.shape)
+        abi.check(abi.load().lbmh_moments(self._h, _ptr(out)))
+        return out
+
+    def reduce_moments(self) -> np.ndarray:
+        out = np.zeros(4)
+        abi.check(abi.load().lbmh_reduce_moments(self._h, _ptr(out)))
+        return out
+
+    def footprint(self) -> int:
+        b = ctypes.c_size_t(0)
+        abi.check(abi.load().lbmh_lattice_footprint(self._h, ctypes.byref(b)))
+        return b.value

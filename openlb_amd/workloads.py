@@ -1,0 +1,176 @@
+"""Seeded synthetic workloads for the block-lattice time step.
+
+Every generator returns a :class:`Case`: the per-cell dynamics ids, the
+initial moments and the collision parameters. The layout is core-only,
+shape ``(nx, ny, nz)``, with z contiguous. This matches OpenLB's x-major cell order
+(``src/core/blockStructure.h:65-78``).
+
+Dynamics ids follow the backend's convention. 0 is NoDynamics, 1 is the lattice's bulk
+collision, 2 is BounceBack (``src/dynamics/dynamics.h:472-483``), and 3..255 are
+BounceBackVelocity walls (``dynamics.h:550-555``), each with its own (rho, u).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+BULK_KINDS = ("bgk", "forced_bgk", "trt", "forced_trt", "smagorinsky")
+
+
+@dataclasses.dataclass
+class Case:
+    name: str
+    q: int
+    kind: str
+    shape: tuple[int, int, int]
+    periodic: tuple[int, int, int]
+    ids: np.ndarray            # uint8 (nx, ny, nz)
+    rho_u: np.ndarray          # float64 (4, nx, ny, nz)
+    omega: float
+    magic: float = 0.25
+    smago: float = 0.0
+    force: tuple[float, float, float] = (0.0, 0.0, 0.0)
+    walls: dict = dataclasses.field(default_factory=dict)   # id -> (rho, (ux, uy, uz))
+
+    @property
+    def ncore(self) -> int:
+        return int(np.prod(self.shape))
+
+    @property
+    def n_active(self) -> int:
+        """Cells with dynamics (OpenLB's getNvoxel(): material != 0)."""
+        return int(np.count_nonzero(self.ids))
+
+
+def _grid(shape):
+    return np.meshgrid(*(np.arange(n, dtype=np.float64) for n in shape), indexing="ij")
+
+
+def taylor_green(shape=(64, 64, 64), u0=0.05, tau=0.6, q=19):
+    """3D Taylor-Green vortex on a fully periodic lattice.
+
+    The velocity is u = u0*(sin x cos y cos z, -cos x sin y cos z, 0), with
+    x = 2*pi*i/N, and rho = 1. This is the initial field of
+    ``examples/turbulence/tgv3d/tgv3d.cpp:126-137`` on lattice coordinates.
+    """
+    x, y, z = _grid(shape)
+    kx, ky, kz = (2 * math.pi / n for n in shape)
+    rho_u = np.empty((4,) + tuple(shape))
+    rho_u[0] = 1.0
+    rho_u[1] = u0 * np.sin(kx * x) * np.cos(ky * y) * np.cos(kz * z)
+    rho_u[2] = -u0 * np.cos(kx * x) * np.sin(ky * y) * np.cos(kz * z)
+    rho_u[3] = 0.0
+    ids = np.ones(shape, dtype=np.uint8)
+    return Case("taylor_green", q, "bgk", tuple(shape), (1, 1, 1), ids, rho_u, 1.0 / tau)
+
+
+def place_spheres(shape, seed, solid_fraction=0.1, rmin=3, rmax=6, region=None):
+    """Seeded solid spheres until the solid share of ``region`` reaches ``solid_fraction``.
+
+    Centres are uniform over the region and radii are uniform integers in [rmin, rmax].
+    A cell is solid when its centre lies within the radius. Spheres wrap
+    periodically in x and are clipped in y and z. This is the build's reading of
+    "seeded Bernoulli(0.1) spheres": placement stops at the first sphere that
+    takes the solid fraction to at least 0.1.
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nx, ny, nz = shape
+    solid = np.zeros(shape, dtype=bool)
+    if region is None:
+        region = np.ones(shape, dtype=bool)
+    target = solid_fraction * np.count_nonzero(region)
+    count = 0
+    while count < target:
+        r = int(rng.integers(rmin, rmax + 1))
+        cx = rng.uniform(0, nx)
+        cy = rng.uniform(0, ny)
+        cz = rng.uniform(0, nz)
+        xs = np.arange(int(math.floor(cx - r)), int(math.ceil(cx + r)) + 1)
+        ys = np.arange(max(0, int(math.floor(cy - r))), min(ny, int(math.ceil(cy + r)) + 1))
+        zs = np.arange(max(0, int(math.floor(cz - r))), min(nz, int(math.ceil(cz + r)) + 1))
+        dx = (xs - cx)[:, None, None]
+        dy = (ys - cy)[None, :, None]
+        dz = (zs - cz)[None, None, :]
+        inside = dx * dx + dy * dy + dz * dz <= r * r
+        xi = np.mod(xs, nx)
+        sub = solid[np.ix_(xi, ys, zs)]
+        new = inside & ~sub & region[np.ix_(xi, ys, zs)]
+        count += int(np.count_nonzero(new))
+        solid[np.ix_(xi, ys, zs)] = sub | (inside & region[np.ix_(xi, ys, zs)])
+    return solid
+
+
+def porous_channel(shape=(256, 256, 256), seed=20261202, tau=0.55, force=1e-6,
+                   solid_fraction=0.1, rmin=3, rmax=6, q=19, kind="forced_bgk"):
+    """Porous channel: periodic in x, one-cell BounceBack walls at the y/z bounds,
+    seeded BounceBack spheres in the interior, Guo body force along x, rho=1, u=0."""
+    nx, ny, nz = shape
+    ids = np.ones(shape, dtype=np.uint8)
+    wall = np.zeros(shape, dtype=bool)
+    wall[:, 0, :] = wall[:, -1, :] = True
+    wall[:, :, 0] = wall[:, :, -1] = True
+    spheres = place_spheres(shape, seed, solid_fraction, rmin, rmax, region=~wall)
+    ids[wall | spheres] = 2
+    rho_u = np.zeros((4,) + tuple(shape))
+    rho_u[0] = 1.0
+    return Case("porous_channel", q, kind, tuple(shape), (1, 0, 0), ids, rho_u,
+                1.0 / tau, force=(force, 0.0, 0.0))
+
+
+def lid_driven_cavity(shape=(32, 32, 32), tau=0.6, lid_u=0.05, q=19, kind="bgk"):
+    """Lid-driven cavity. Every boundary cell is BounceBack, except the top (y = ny-1)
+    face, which is a BounceBackVelocity wall (id 3) moving along x at rho = 1.
+    This is the bounce-back variant of ``examples/laminar/cavity3dBenchmark/cavity3d.cpp:59-87``."""
+    ids = np.ones(shape, dtype=np.uint8)
+    ids[0, :, :] = ids[-1, :, :] = 2
+    ids[:, 0, :] = 2
+    ids[:, :, 0] = ids[:, :, -1] = 2
+    ids[1:-1, -1, 1:-1] = 3
+    rho_u = np.zeros((4,) + tuple(shape))
+    rho_u[0] = 1.0
+    return Case("lid_driven_cavity", q, kind, tuple(shape), (0, 0, 0), ids, rho_u,
+                1.0 / tau, walls={3: (1.0, (lid_u, 0.0, 0.0))})
+
+
+def turbulent_channel(shape=(384, 192, 192), tau=0.505, smago=0.14, u_max=0.05,
+                      noise=0.01, seed=20261203, q=27):
+    """Smagorinsky channel. It is periodic in x and z, with BounceBack walls at y = 0 and
+    y = ny-1. The laminar Poiseuille profile u_x(y) carries seeded uniform
+    +/-``noise`` relative velocity noise per component."""
+    nx, ny, nz = shape
+    ids = np.ones(shape, dtype=np.uint8)
+    ids[:, 0, :] = ids[:, -1, :] = 2
+    rng = np.random.Generator(np.random.PCG64(seed))
+    h = ny - 2
+    yc = np.arange(ny, dtype=np.float64) - 0.5
+    prof = np.clip(4.0 * u_max * yc * (h - yc) / (h * h), 0.0, None)
+    rho_u = np.zeros((4,) + tuple(shape))
+    rho_u[0] = 1.0
+    rho_u[1] = prof[None, :, None]
+    for d in range(1, 4):
+        rho_u[d] += u_max * noise * rng.uniform(-1.0, 1.0, size=shape)
+    rho_u[1:, ids == 2] = 0.0
+    return Case("turbulent_channel", q, "smagorinsky", tuple(shape), (1, 0, 1), ids, rho_u,
+                1.0 / tau, smago=smago)
+
+
+def random_periodic(shape, seed, q=19, kind="bgk", tau=0.7, u_amp=0.04, rho_amp=0.01,
+                    magic=0.25, smago=0.0, force=(0.0, 0.0, 0.0)):
+    """Fully periodic box with smooth random moments. This is a parity case whose
+    populations carry every direction non-trivially."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    x, y, z = _grid(shape)
+    rho_u = np.empty((4,) + tuple(shape))
+    for comp in range(4):
+        acc = np.zeros(shape)
+        for _ in range(3):
+            kx, ky, kz = (rng.integers(1, 3) * 2 * math.pi / n for n in shape)
+            ph = rng.uniform(0, 2 * math.pi, size=3)
+            acc += np.sin(kx * x + ph[0]) * np.cos(ky * y + ph[1]) * np.sin(kz * z + ph[2])
+        acc /= 3.0
+        rho_u[comp] = (1.0 + rho_amp * acc) if comp == 0 else u_amp * acc
+    ids = np.ones(shape, dtype=np.uint8)
+    return Case("random_periodic", q, kind, tuple(shape), (1, 1, 1), ids, rho_u,
+                1.0 / tau, magic=magic, smago=smago, force=tuple(force))

@@ -1,0 +1,142 @@
+"""ctypes front end of the oracle library (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY. Only tests/, ``__graft_entry__.smoke()`` and
+bench.py's ``cpu_baseline`` leg import this module, and only as the checker.
+The product path never does. See olb_oracle.h for what it restates.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+STORAGE = {"f64": 0, "f32": 1, "f16": 2}
+KINDS = {"bgk": 0, "forced_bgk": 1, "trt": 2, "forced_trt": 3, "smagorinsky": 4}
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "liboracle.so"],
+                       check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
+        L.olo_create.restype = vp
+        L.olo_create.argtypes = [i32] * 8
+        L.olo_destroy.argtypes = [vp]
+        L.olo_set_ids.argtypes = [vp, vp]
+        L.olo_set_bulk.argtypes = [vp, i32, dbl, dbl, dbl, dbl, dbl, dbl]
+        L.olo_set_wall.argtypes = [vp, i32, dbl, dbl, dbl, dbl]
+        L.olo_upload.argtypes = [vp, vp]
+        L.olo_download.argtypes = [vp, vp]
+        L.olo_step.argtypes = [vp, i64, i32]
+        L.olo_get_shifts.argtypes = [vp, vp]
+        L.olo_moments.argtypes = [vp, vp]
+        L.olo_equilibrium.argtypes = [i32, i32, i32, i64, vp, dbl, dbl, dbl, vp]
+        L.olo_round_to_storage.argtypes = [i32, i64, vp]
+        L.olo_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class OracleLattice:
+    """One padded block with periodic-shift columns, stepped on the CPU."""
+
+    def __init__(self, case, storage: str = "f64"):
+        L = lib()
+        self.case = case
+        self.q = case.q
+        self.shape = tuple(case.shape)
+        self.storage = storage
+        h = L.olo_create(case.q, *self.shape, *case.periodic, STORAGE[storage])
+        if not h:
+            raise RuntimeError(L.olo_last_error().decode())
+        self._h = h
+        ids = np.ascontiguousarray(case.ids, dtype=np.uint8)
+        L.olo_set_ids(h, _ptr(ids))
+        if L.olo_set_bulk(h, KINDS[case.kind], case.omega, case.magic, case.smago,
+                          *case.force) != 0:
+            raise RuntimeError(L.olo_last_error().decode())
+        for wid, (rho, u) in case.walls.items():
+            if L.olo_set_wall(h, int(wid), float(rho), *map(float, u)) != 0:
+                raise RuntimeError(L.olo_last_error().decode())
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().olo_destroy(h)
+            self._h = None
+
+    def upload(self, f: np.ndarray):
+        f = np.ascontiguousarray(f, dtype=np.float64).reshape(self.q, -1)
+        lib().olo_upload(self._h, _ptr(f))
+
+    def download(self) -> np.ndarray:
+        out = np.empty((self.q,) + self.shape)
+        lib().olo_download(self._h, _ptr(out))
+        return out
+
+    def step(self, steps: int = 1, threads: int | None = None):
+        if threads is None:
+            threads = min(16, os.cpu_count() or 1)
+        lib().olo_step(self._h, int(steps), int(threads))
+
+    def shifts(self) -> np.ndarray:
+        out = np.zeros(self.q, dtype=np.int64)
+        lib().olo_get_shifts(self._h, _ptr(out))
+        return out
+
+    def moments(self) -> np.ndarray:
+        out = np.empty((4,) + self.shape)
+        lib().olo_moments(self._h, _ptr(out))
+        return out
+
+
+def equilibrium(case, storage: str = "f64", rho_u: np.ndarray | None = None) -> np.ndarray:
+    """Shifted equilibrium populations with iniEquilibrium semantics, (q,)+shape."""
+    if rho_u is None:
+        rho_u = case.rho_u
+    rho_u = np.ascontiguousarray(rho_u, dtype=np.float64)
+    n = int(np.prod(rho_u.shape[1:]))
+    out = np.empty((case.q,) + rho_u.shape[1:])
+    lib().olo_equilibrium(case.q, KINDS[case.kind], STORAGE[storage], n, _ptr(rho_u),
+                          *case.force, _ptr(out))
+    return out
+
+
+def round_to_storage(values: np.ndarray, storage: str) -> np.ndarray:
+    v = np.array(values, dtype=np.float64, copy=True, order="C")
+    lib().olo_round_to_storage(STORAGE[storage], v.size, _ptr(v))
+    return v
+
+
+def initial_populations(case, storage: str = "f64") -> np.ndarray:
+    """Equilibrium at the case's moments on cells with dynamics; zero elsewhere
+    (the zero-initialised column of the reference)."""
+    f = equilibrium(case, storage)
+    if case.kind.startswith("forced"):
+        # only the bulk (forced) dynamics inverse-shift u; walls use plain momenta
+        import dataclasses
+        plain = equilibrium(dataclasses.replace(case, kind="bgk"), storage)
+        f = np.where((case.ids == 1)[None], f, plain)
+    f[:, case.ids == 0] = 0.0
+    return f

@@ -1,0 +1,236 @@
+// Golden-vector driver for the compiled OpenLB reference (CPU_SISD platform).
+//
+// This program is test infrastructure: it links against the OpenLB sources in
+// the reference tree (built by oracle/Makefile into oracle/_ref/) and is only
+// run in the build container to produce the fixtures under tests/golden/.
+// Nothing that ships to a GPU runs it.
+//
+// Input: a case directory holding
+//   case.txt  key/value lines (q, dyn, nx, ny, nz, px, py, pz, omega, magic,
+//             smago, fx, fy, fz, steps, wall <id> <rho> <ux> <uy> <uz>)
+//   ids.bin   uint8  [nx][ny][nz]  per-cell dynamics id
+//             (0 none, 1 bulk, 2 BounceBack, 3.. BounceBackVelocity walls)
+//   rhou.bin  double [4][nx][ny][nz] initial rho, ux, uy, uz
+// Output (same directory):
+//   pops_<step>.bin  double [q][nx][ny][nz] shifted populations of the core
+//                    cells after <step> collideAndStream() calls (0 = after
+//                    initialize()).
+//
+// The setup follows the reference's example programs:
+//   geometry/materials  examples/laminar/cavity3dBenchmark/cavity3d.cpp:39-57
+//   dynamics + params   cavity3d.cpp:59-75, tgv3d.cpp:240-253
+//   init                tgv3d.cpp:313-326 (defineRhoU then iniEquilibrium)
+//   time loop           src/core/superLattice.hh:601-663
+#include "olb.h"
+
+#include <cstdio>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+using namespace olb;
+
+#ifndef REF_REAL
+#define REF_REAL double
+#endif
+using T = REF_REAL;
+
+struct WallSpec { int id; double rho, u[3]; };
+
+struct CaseSpec {
+  int q = 19;
+  std::string dyn = "bgk";
+  int n[3] = {8, 8, 8};
+  int periodic[3] = {1, 1, 1};
+  double omega = 1.0, magic = 0.25, smago = 0.1;
+  double force[3] = {0, 0, 0};
+  std::vector<long> steps;
+  std::vector<WallSpec> walls;
+  std::vector<unsigned char> ids;
+  std::vector<double> rhou;
+};
+
+static std::size_t coreIndex(const CaseSpec& c, int x, int y, int z)
+{
+  return (std::size_t(x) * c.n[1] + y) * c.n[2] + z;
+}
+
+static CaseSpec readCase(const std::string& dir)
+{
+  CaseSpec c;
+  std::ifstream in(dir + "/case.txt");
+  if (!in) throw std::runtime_error("cannot open case.txt in " + dir);
+  std::string line;
+  while (std::getline(in, line)) {
+    std::istringstream ls(line);
+    std::string key;
+    if (!(ls >> key)) continue;
+    if (key == "q") ls >> c.q;
+    else if (key == "dyn") ls >> c.dyn;
+    else if (key == "nx") ls >> c.n[0];
+    else if (key == "ny") ls >> c.n[1];
+    else if (key == "nz") ls >> c.n[2];
+    else if (key == "px") ls >> c.periodic[0];
+    else if (key == "py") ls >> c.periodic[1];
+    else if (key == "pz") ls >> c.periodic[2];
+    else if (key == "omega") ls >> c.omega;
+    else if (key == "magic") ls >> c.magic;
+    else if (key == "smago") ls >> c.smago;
+    else if (key == "fx") ls >> c.force[0];
+    else if (key == "fy") ls >> c.force[1];
+    else if (key == "fz") ls >> c.force[2];
+    else if (key == "steps") { long s; while (ls >> s) c.steps.push_back(s); }
+    else if (key == "wall") {
+      WallSpec w; ls >> w.id >> w.rho >> w.u[0] >> w.u[1] >> w.u[2];
+      c.walls.push_back(w);
+    }
+    else throw std::runtime_error("unknown key " + key);
+  }
+  const std::size_t nCore = std::size_t(c.n[0]) * c.n[1] * c.n[2];
+  c.ids.resize(nCore);
+  c.rhou.resize(4 * nCore);
+  std::FILE* f = std::fopen((dir + "/ids.bin").c_str(), "rb");
+  if (!f || std::fread(c.ids.data(), 1, nCore, f) != nCore) throw std::runtime_error("ids.bin");
+  std::fclose(f);
+  f = std::fopen((dir + "/rhou.bin").c_str(), "rb");
+  if (!f || std::fread(c.rhou.data(), sizeof(double), 4 * nCore, f) != 4 * nCore) throw std::runtime_error("rhou.bin");
+  std::fclose(f);
+  return c;
+}
+
+template <typename DESCRIPTOR>
+static void dumpPopulations(const CaseSpec& c, SuperLattice<T,DESCRIPTOR>& lattice,
+                            const std::string& dir, long step)
+{
+  const std::size_t nCore = std::size_t(c.n[0]) * c.n[1] * c.n[2];
+  std::vector<double> out(std::size_t(DESCRIPTOR::q) * nCore);
+  auto& block = lattice.getBlock(0);
+  for (int x = 0; x < c.n[0]; ++x) {
+    for (int y = 0; y < c.n[1]; ++y) {
+      for (int z = 0; z < c.n[2]; ++z) {
+        auto cell = block.get(LatticeR<3>{x, y, z});
+        for (int i = 0; i < int(DESCRIPTOR::q); ++i) {
+          out[std::size_t(i) * nCore + coreIndex(c, x, y, z)] = double(cell[i]);
+        }
+      }
+    }
+  }
+  const std::string name = dir + "/pops_" + std::to_string(step) + ".bin";
+  std::FILE* f = std::fopen(name.c_str(), "wb");
+  if (!f || std::fwrite(out.data(), sizeof(double), out.size(), f) != out.size()) {
+    throw std::runtime_error("cannot write " + name);
+  }
+  std::fclose(f);
+}
+
+template <typename DESCRIPTOR, typename BULK, bool FORCED>
+static void runCase(const CaseSpec& c, const std::string& dir)
+{
+  // One cuboid of exactly n[0] x n[1] x n[2] cells, spacing 1.
+  CuboidDecomposition3D<T> cuboids(Vector<T,3>{0, 0, 0}, T{1},
+                                   Vector<int,3>{c.n[0], c.n[1], c.n[2]}, 1);
+  cuboids.setPeriodicity({bool(c.periodic[0]), bool(c.periodic[1]), bool(c.periodic[2])});
+  BlockLoadBalancer<T> load(0, 1, 1, 0);
+  // overlap 1: the halo depth the time step actually refreshes (superLattice.hh:138-143)
+  SuperGeometry<T,3> geometry(cuboids, load, 1);
+  auto& blockGeometry = geometry.getBlockGeometry(0);
+  for (int x = 0; x < c.n[0]; ++x)
+    for (int y = 0; y < c.n[1]; ++y)
+      for (int z = 0; z < c.n[2]; ++z)
+        blockGeometry.set(LatticeR<3>{x, y, z}, int(c.ids[coreIndex(c, x, y, z)]));
+  geometry.communicate();
+
+  SuperLattice<T,DESCRIPTOR> lattice(geometry);
+  lattice.statisticsOff();
+  lattice.template defineDynamics<BULK>(geometry, 1);
+  lattice.template defineDynamics<BounceBack<T,DESCRIPTOR>>(geometry, 2);
+  for (const auto& w : c.walls) {
+    lattice.template defineDynamics<BounceBackVelocity<T,DESCRIPTOR>>(geometry, w.id);
+  }
+  lattice.template setParameter<descriptors::OMEGA>(T(c.omega));
+  if (c.dyn == "trt" || c.dyn == "forced_trt") {
+    lattice.template setParameter<collision::TRT::MAGIC>(T(c.magic));
+  }
+  if (c.dyn == "smagorinsky") {
+    lattice.template setParameter<collision::LES::SMAGORINSKY>(T(c.smago));
+  }
+
+  // Per-cell rho/u in the same order as the example programs: define the
+  // moments, then initialise populations at equilibrium.
+  auto& block = lattice.getBlock(0);
+  for (int x = 0; x < c.n[0]; ++x) {
+    for (int y = 0; y < c.n[1]; ++y) {
+      for (int z = 0; z < c.n[2]; ++z) {
+        const std::size_t k = coreIndex(c, x, y, z);
+        const int id = c.ids[k];
+        if (id == 0) continue;
+        const std::size_t nCore = c.ids.size();
+        T rho = T(c.rhou[k]);
+        T u[3] = {T(c.rhou[nCore + k]), T(c.rhou[2 * nCore + k]), T(c.rhou[3 * nCore + k])};
+        auto cell = block.get(LatticeR<3>{x, y, z});
+        if constexpr (FORCED) {
+          if (id == 1) {
+            cell.template setField<descriptors::FORCE>(
+              Vector<T,3>{T(c.force[0]), T(c.force[1]), T(c.force[2])});
+          }
+        }
+        cell.defineRhoU(rho, u);
+        cell.iniEquilibrium(rho, u);
+      }
+    }
+  }
+  // Moving walls: fixed density and wall velocity (cavity3d.cpp:124-136).
+  for (const auto& w : c.walls) {
+    for (int x = 0; x < c.n[0]; ++x)
+      for (int y = 0; y < c.n[1]; ++y)
+        for (int z = 0; z < c.n[2]; ++z) {
+          if (c.ids[coreIndex(c, x, y, z)] != w.id) continue;
+          auto cell = block.get(LatticeR<3>{x, y, z});
+          T u[3] = {T(w.u[0]), T(w.u[1]), T(w.u[2])};
+          cell.defineRhoU(T(w.rho), u);
+        }
+  }
+  lattice.initialize();
+
+  long done = 0;
+  for (long target : c.steps) {
+    for (; done < target; ++done) {
+      lattice.collideAndStream();
+    }
+    dumpPopulations<DESCRIPTOR>(c, lattice, dir, target);
+  }
+}
+
+int main(int argc, char** argv)
+{
+  if (argc < 2) {
+    std::fprintf(stderr, "usage: %s <case-dir>\n", argv[0]);
+    return 2;
+  }
+  const std::string dir = argv[1];
+  initialize(&argc, &argv, false, false);
+  const CaseSpec c = readCase(dir);
+  using D19 = descriptors::D3Q19<>;
+  using D19F = descriptors::D3Q19<descriptors::FORCE>;
+  using D27 = descriptors::D3Q27<>;
+  using D27F = descriptors::D3Q27<descriptors::FORCE>;
+  if (c.q == 19) {
+    if (c.dyn == "bgk") runCase<D19, BGKdynamics<T,D19>, false>(c, dir);
+    else if (c.dyn == "forced_bgk") runCase<D19F, ForcedBGKdynamics<T,D19F>, true>(c, dir);
+    else if (c.dyn == "trt") runCase<D19, TRTdynamics<T,D19>, false>(c, dir);
+    else if (c.dyn == "forced_trt") runCase<D19F, ForcedTRTdynamics<T,D19F>, true>(c, dir);
+    else if (c.dyn == "smagorinsky") runCase<D19, SmagorinskyBGKdynamics<T,D19>, false>(c, dir);
+    else throw std::runtime_error("unknown dyn " + c.dyn);
+  } else if (c.q == 27) {
+    if (c.dyn == "bgk") runCase<D27, BGKdynamics<T,D27>, false>(c, dir);
+    else if (c.dyn == "forced_bgk") runCase<D27F, ForcedBGKdynamics<T,D27F>, true>(c, dir);
+    else if (c.dyn == "trt") runCase<D27, TRTdynamics<T,D27>, false>(c, dir);
+    else if (c.dyn == "smagorinsky") runCase<D27, SmagorinskyBGKdynamics<T,D27>, false>(c, dir);
+    else throw std::runtime_error("unknown dyn " + c.dyn);
+  } else {
+    throw std::runtime_error("q must be 19 or 27");
+  }
+  return 0;
+}

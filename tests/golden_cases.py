@@ -1,0 +1,58 @@
+"""Loader for the golden vectors in tests/golden/ (written by gen_golden.py from
+the compiled OpenLB reference) and the error measures the parity tests use."""
+from __future__ import annotations
+
+import glob
+import json
+import os
+
+import numpy as np
+
+from openlb_amd import workloads as W
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+# lattice weights by population, for converting shifted values to f_i
+_T19 = np.array([1 / 3] + [1 / 18] * 3 + [1 / 36] * 6 + [1 / 18] * 3 + [1 / 36] * 6)
+_T27 = np.array([8 / 27] + [2 / 27] * 3 + [1 / 54] * 6 + [1 / 216] * 4 +
+                [2 / 27] * 3 + [1 / 54] * 6 + [1 / 216] * 4)
+
+
+def weights(q: int) -> np.ndarray:
+    return _T19 if q == 19 else _T27
+
+
+def names(sampled: bool = False):
+    out = []
+    for p in sorted(glob.glob(os.path.join(GOLDEN, "*.npz"))):
+        n = os.path.basename(p)[:-4]
+        if n.endswith("_sample") == sampled:
+            out.append(n)
+    return out
+
+
+def load(name: str):
+    """(Case, meta, arrays) of one fixture."""
+    z = np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
+    meta = json.loads(str(z["meta"]))
+    arrays = {k: z[k] for k in z.files if k != "meta"}
+    if meta.get("sampled"):
+        return meta, arrays
+    walls = {int(k): (v[0], tuple(v[1])) for k, v in meta["walls"].items()}
+    case = W.Case(name, meta["q"], meta["kind"], tuple(meta["shape"]), tuple(meta["periodic"]),
+                  arrays["ids"], arrays["rho_u"], meta["omega"], meta["magic"], meta["smago"],
+                  tuple(meta["force"]), walls)
+    return case, meta, arrays
+
+
+def rel_error(got: np.ndarray, ref: np.ndarray, q: int, mask=None) -> float:
+    """max |f_got - f_ref| / max |f_ref| over the masked cells. The arrays hold shifted values;
+    the normalisation uses the unshifted populations f_i = fb_i + t_i, because relative
+    error on deviations near zero is ill-conditioned."""
+    g = np.asarray(got, dtype=np.float64).reshape(q, -1)
+    r = np.asarray(ref, dtype=np.float64).reshape(q, -1)
+    if mask is not None:
+        m = np.asarray(mask).reshape(-1)
+        g, r = g[:, m], r[:, m]
+    scale = np.abs(r + weights(q)[:, None]).max()
+    return float(np.abs(g - r).max() / scale)
