@@ -1,0 +1,231 @@
+"""Throughput of the block-lattice time step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload porous_channel]
+                    [--shape 256 256 256] [--storage f32] [--cpu-seconds 15]
+
+The default workload is the D3Q19 porous channel at 256^3 in FP32. It is periodic in
+x, with BounceBack walls at the y/z bounds and seeded BounceBack spheres (solid
+fraction 0.1, radius 3-6). The bulk cells run Guo-forced BGK (F = 1e-6 along x,
+tau = 0.55). One step is one ``collideAndStream()`` of the whole block: one launch of
+the fused collide-stream kernel. The inputs are resident in HBM before the timed
+region starts.
+
+The output is one JSON line on stdout. ``value`` is MLUPS, i.e. updated cells
+(cells with dynamics, OpenLB's getNvoxel()) times steps per second, over the timed
+steps. That region is bracketed by device synchronisations. ``roofline`` prices
+the kernel's algorithmic traffic, 2*Q*sizeof(stored population) bytes per cell
+update, against its average per-launch duration measured with HIP events on the
+launch stream. ``cpu_baseline`` is the oracle (the CPU restatement of OpenLB's time
+step, OpenMP over x-planes) timed on the host cores on a bounded sample of the same
+workload.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor peak
+STORE_BYTES = {"f64": 8, "f32": 4, "f16": 2}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+class HipEvents:
+    """hipEvent timing on a given stream (libamdhip64 through ctypes)."""
+
+    def __init__(self):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
+                                                 ctypes.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def create(self, n):
+        evs = []
+        for _ in range(n):
+            e = ctypes.c_void_p()
+            if self.hip.hipEventCreate(ctypes.byref(e)) != 0:
+                raise RuntimeError("hipEventCreate failed")
+            evs.append(e)
+        return evs
+
+    def record(self, ev, stream=None):
+        if self.hip.hipEventRecord(ev, ctypes.c_void_p(stream or 0)) != 0:
+            raise RuntimeError("hipEventRecord failed")
+
+    def elapsed_ms(self, a, b):
+        self.hip.hipEventSynchronize(b)
+        ms = ctypes.c_float(0)
+        if self.hip.hipEventElapsedTime(ctypes.byref(ms), a, b) != 0:
+            raise RuntimeError("hipEventElapsedTime failed")
+        return float(ms.value)
+
+    def destroy(self, evs):
+        for e in evs:
+            self.hip.hipEventDestroy(e)
+
+
+def make_case(args):
+    from openlb_amd import workloads as W
+    if args.workload == "porous_channel":
+        return W.porous_channel(tuple(args.shape), tau=0.55, force=1e-6)
+    if args.workload == "taylor_green":
+        return W.taylor_green(tuple(args.shape), u0=0.05, tau=0.6)
+    if args.workload == "turbulent_channel":
+        return W.turbulent_channel(tuple(args.shape))
+    if args.workload == "lid_driven_cavity":
+        return W.lid_driven_cavity(tuple(args.shape))
+    raise SystemExit(f"unknown workload {args.workload}")
+
+
+def cpu_baseline(case, storage, seconds):
+    """The oracle on the host cores, on a bounded sample of the same workload."""
+    from oracle import oracle as O
+    from openlb_amd import workloads as W
+    threads = min(16, os.cpu_count() or 1)
+    # same generator, a smaller block: MLUPS is normalised per cell update
+    shape = tuple(min(n, 128) for n in case.shape)
+    if case.name == "porous_channel":
+        sample = W.porous_channel(shape, tau=0.55, force=case.force[0])
+    elif case.name == "taylor_green":
+        sample = W.taylor_green(shape, u0=0.05, tau=1.0 / case.omega)
+    elif case.name == "turbulent_channel":
+        sample = W.turbulent_channel(shape)
+    else:
+        sample = W.lid_driven_cavity(shape)
+    lat = O.OracleLattice(sample, storage)
+    lat.upload(O.initial_populations(sample, storage))
+    lat.step(2, threads)  # warm-up
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        lat.step(2, threads)
+        steps += 2
+        dt = time.perf_counter() - t0
+        if dt >= seconds:
+            break
+    mlups = sample.n_active * steps / dt / 1e6
+    return {"value": round(mlups, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"{sample.name} {shape[0]}x{shape[1]}x{shape[2]} {storage}, {steps} steps "
+                      f"in {dt:.1f} s, oracle/liboracle.so (OpenMP, {threads} threads)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="porous_channel")
+    ap.add_argument("--shape", type=int, nargs=3, default=[256, 256, 256])
+    ap.add_argument("--storage", default="f32", choices=["f64", "f32", "f16"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build runs a lattice on a single MI355X and has no "
+              "multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    from openlb_amd import abi
+    from openlb_amd.lattice import BlockLattice
+
+    case = make_case(args)
+    log(f"workload {case.name} {case.shape} {args.storage}: {case.n_active} cells with dynamics")
+    lat = BlockLattice.from_case(case, args.storage)
+    lat.ini_equilibrium(case.rho_u)
+    lat.synchronize()
+    ev = HipEvents()
+
+    for _ in range(args.warmup):
+        lat.collide_and_stream(1)
+    lat.synchronize()
+
+    # timed region: K steps, device-synchronised on both sides; per-launch
+    # events on the launch stream (the null stream)
+    marks = ev.create(2 * args.steps)
+    lat.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ev.record(marks[2 * k])
+        lat.collide_and_stream(1)
+        ev.record(marks[2 * k + 1])
+    lat.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = [ev.elapsed_ms(marks[2 * k], marks[2 * k + 1]) for k in range(args.steps)]
+    ev.destroy(marks)
+    moments = lat.reduce_moments()
+    lat.close()
+    if not np.isfinite(moments).all():
+        raise SystemExit("non-finite populations after the timed steps")
+
+    cells = case.n_active
+    mlups = cells * args.steps / wall / 1e6
+    avg_ms = float(np.mean(kernel_ms))
+    bytes_per_launch = 2 * case.q * STORE_BYTES[args.storage] * cells
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    log(f"{mlups:.0f} MLUPS, kernel avg {avg_ms:.4f} ms (min {min(kernel_ms):.4f}), "
+        f"{achieved:.0f} GB/s algorithmic")
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(case, args.storage, args.cpu_seconds)
+        log(f"cpu baseline {cpu['value']} MLUPS")
+
+    out = {
+        "metric": "MLUPS (D3Q19 BGK fp32 256^3 porous channel) + % HBM roofline",
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"f32": "f32", "f64": "f64", "f16": "f16-storage/f32-math"}[args.storage],
+        "data": "synthetic (seeded)",
+        "config": {
+            "workload": f"{case.name}_d3q{case.q}_{case.kind}",
+            "shape": list(case.shape),
+            "storage": args.storage,
+            "tau": round(1.0 / case.omega, 6),
+            "force": list(case.force),
+            "cells_with_dynamics": cells,
+            "solid_cells": int(np.count_nonzero(case.ids == 2)),
+            "parallelism": "single GPU",
+            "device": abi.device_arch(0),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": f"collide_stream_kernel<{case.q},{args.storage},{case.kind}>",
+            "kernel_avg_ms": round(avg_ms, 5),
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

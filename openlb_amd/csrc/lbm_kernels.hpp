@@ -23,10 +23,24 @@
 
 namespace lbmh {
 
+// Stored type S, arithmetic type V, and the power-of-two factor the stored
+// deviation carries.  FP16 keeps fb_i * 2^12: the scaling is exact, it moves
+// FP16's subnormal range (absolute precision 6e-8) below 1.5e-8 so that small
+// deviations keep 11 significant bits, and |fb_i| <= 16 stays below FP16's
+// maximum (f_i >= 0 bounds fb_i >= -1/3 from below).
 template <typename S> struct StorageTraits;
-template <> struct StorageTraits<double> { using V = double; };
-template <> struct StorageTraits<float> { using V = float; };
-template <> struct StorageTraits<_Float16> { using V = float; };
+template <> struct StorageTraits<double> {
+  using V = double;
+  static constexpr double scale = 1.0, inv_scale = 1.0;
+};
+template <> struct StorageTraits<float> {
+  using V = float;
+  static constexpr float scale = 1.0f, inv_scale = 1.0f;
+};
+template <> struct StorageTraits<_Float16> {
+  using V = float;
+  static constexpr float scale = 4096.0f, inv_scale = 1.0f / 4096.0f;
+};
 
 constexpr int kBlock = 256;
 constexpr int kMaxWalls = 256;
@@ -63,9 +77,9 @@ __device__ __forceinline__ uint32_t remap_xcd(uint32_t b, uint32_t nblocks)
 }
 
 template <typename S, typename V>
-__device__ __forceinline__ V load_pop(const S* p) { return V(*p); }
+__device__ __forceinline__ V load_pop(const S* p) { return V(*p) * V(StorageTraits<S>::inv_scale); }
 template <typename S, typename V>
-__device__ __forceinline__ void store_pop(S* p, V v) { *p = S(v); }
+__device__ __forceinline__ void store_pop(S* p, V v) { *p = S(v * V(StorageTraits<S>::scale)); }
 
 // Element k of a column through a 32-bit byte offset from the column base, so
 // the base stays in scalar registers and each address costs one VGPR.
@@ -195,7 +209,9 @@ __global__ void scatter_populations(const H* src, S* pop, CoreGeom g, Shifts<Q> 
   const uint32_t n = padded_of_core(g, c);
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
-    pop[uint64_t(i) * g.stride + wrap_index(n, sh.s[i], g.n_padded)] = S(src[uint64_t(i) * g.ncore + c]);
+    using V = typename StorageTraits<S>::V;
+    store_pop<S, V>(pop + uint64_t(i) * g.stride + wrap_index(n, sh.s[i], g.n_padded),
+                    V(src[uint64_t(i) * g.ncore + c]));
   }
 }
 
@@ -207,7 +223,8 @@ __global__ void gather_populations(H* dst, const S* pop, CoreGeom g, Shifts<Q> s
   const uint32_t n = padded_of_core(g, c);
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
-    dst[uint64_t(i) * g.ncore + c] = H(pop[uint64_t(i) * g.stride + wrap_index(n, sh.s[i], g.n_padded)]);
+    using V = typename StorageTraits<S>::V;
+    dst[uint64_t(i) * g.ncore + c] = H(load_pop<S, V>(pop + uint64_t(i) * g.stride + wrap_index(n, sh.s[i], g.n_padded)));
   }
 }
 
@@ -244,7 +261,7 @@ __global__ void ini_equilibrium_kernel(const double* rho_u, const uint8_t* ids, 
   }
   const uint32_t n = padded_of_core(g, c);
 #pragma unroll
-  for (int i = 0; i < Q; ++i) pop[uint64_t(i) * g.stride + wrap_index(n, sh.s[i], g.n_padded)] = S(f[i]);
+  for (int i = 0; i < Q; ++i) store_pop<S, V>(pop + uint64_t(i) * g.stride + wrap_index(n, sh.s[i], g.n_padded), f[i]);
 }
 
 // Per-cell rho, u in FP64 ([4][ncore]); u includes F/2 on forced bulk cells.
@@ -258,7 +275,7 @@ __global__ void moments_kernel(double* out, const uint8_t* ids, const S* pop, Co
   const uint32_t n = padded_of_core(g, c);
   V f[Q];
 #pragma unroll
-  for (int i = 0; i < Q; ++i) f[i] = V(pop[uint64_t(i) * g.stride + wrap_index(n, sh.s[i], g.n_padded)]);
+  for (int i = 0; i < Q; ++i) f[i] = load_pop<S, V>(pop + uint64_t(i) * g.stride + wrap_index(n, sh.s[i], g.n_padded));
   V drho, j[3];
   moments<Q, V>(f, drho, j);
   const V rho = V(1) + drho;
@@ -295,7 +312,7 @@ __global__ __launch_bounds__(kBlock) void reduce_moments_kernel(double* partial,
     const uint32_t n = padded_of_core(g, c);
     V f[Q];
 #pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] = V(pop[uint64_t(i) * g.stride + wrap_index(n, sh.s[i], g.n_padded)]);
+    for (int i = 0; i < Q; ++i) f[i] = load_pop<S, V>(pop + uint64_t(i) * g.stride + wrap_index(n, sh.s[i], g.n_padded));
     V drho, j[3];
     moments<Q, V>(f, drho, j);
     acc[0] += 1.0 + double(drho);

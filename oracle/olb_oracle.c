@@ -135,8 +135,9 @@ static uint16_t float_to_half(float f)
 #define SFX f16
 #define STORE_T uint16_t
 #define V float
-#define LOAD(p) half_to_float(p)
-#define STOREV(v) float_to_half(v)
+/* FP16 keeps fb_i * 2^12 (exact power-of-two scaling, same as the device) */
+#define LOAD(p) (half_to_float(p) * (1.0f / 4096.0f))
+#define STOREV(v) float_to_half((v) * 4096.0f)
 #include "olb_oracle_impl.h"
 #undef SFX
 #undef STORE_T
@@ -335,7 +336,8 @@ int olo_round_to_storage(int storage, int64_t n, double* values)
 {
   for (int64_t k = 0; k < n; ++k) {
     if (storage == OLO_F32) values[k] = (double)(float)values[k];
-    else if (storage == OLO_F16) values[k] = (double)half_to_float(float_to_half((float)values[k]));
+    else if (storage == OLO_F16)
+      values[k] = (double)(half_to_float(float_to_half((float)values[k] * 4096.0f)) * (1.0f / 4096.0f));
   }
   return 0;
 }
