@@ -78,6 +78,32 @@ class HipEvents:
             self.hip.hipEventDestroy(e)
 
 
+def calibrate_copy(mib: int, repeats: int = 5):
+    """Device-to-device copies of a known size: the reference point that scales the
+    FETCH_SIZE / WRITE_SIZE counters of the step kernel to HBM bytes."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    nbytes = mib * 1024 * 1024
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), nbytes) or hip.hipMalloc(ctypes.byref(b), nbytes):
+        raise SystemExit("hipMalloc failed")
+    hip.hipMemset(a, 1, nbytes)
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        if hip.hipMemcpy(b, a, nbytes, 3) != 0:  # hipMemcpyDeviceToDevice
+            raise SystemExit("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    dt = (time.perf_counter() - t0) / repeats
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"calibration": "d2d copy", "bytes_read": nbytes, "bytes_written": nbytes,
+                      "repeats": repeats, "copy_gbs": round(2 * nbytes / dt / 1e9, 1)}), flush=True)
+
+
 def make_case(args):
     from openlb_amd import workloads as W
     if args.workload == "porous_channel":
@@ -133,6 +159,10 @@ def main():
     ap.add_argument("--storage", default="f32", choices=["f64", "f32", "f16"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default launch shape instead of tuning it first")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -144,11 +174,17 @@ def main():
     from openlb_amd import abi
     from openlb_amd.lattice import BlockLattice
 
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
+
     case = make_case(args)
     log(f"workload {case.name} {case.shape} {args.storage}: {case.n_active} cells with dynamics")
     lat = BlockLattice.from_case(case, args.storage)
     lat.ini_equilibrium(case.rho_u)
     lat.synchronize()
+    launch = lat.autotune(trial_steps=5) if args.autotune else lat.launch_config()
+    log(f"launch config {launch}")
     ev = HipEvents()
 
     for _ in range(args.warmup):
@@ -210,6 +246,7 @@ def main():
             "solid_cells": int(np.count_nonzero(case.ids == 2)),
             "parallelism": "single GPU",
             "device": abi.device_arch(0),
+            "launch": launch,
         },
         "roofline": {
             "bound": "hbm",

@@ -140,6 +140,28 @@ int lbmh_reduce_moments(lbmh_lattice* lattice, double out[4]);
  * gpu::cuda::device::synchronize (src/core/platform/gpu/cuda/device.hh:64-69). */
 int lbmh_synchronize(lbmh_lattice* lattice);
 
+/* Launch shape of the time-step kernel.  No OpenLB counterpart: the CUDA
+ * platform launches every kernel with 32-thread blocks (src/core/platform/gpu/
+ * cuda/operator.hh:422-429).  lds_bytes reserves dynamic LDS per 256-thread
+ * workgroup purely to cap the resident workgroups per CU (160 KB LDS per CU):
+ * fewer waves in flight keep fewer DRAM rows open for the 2*Q concurrent
+ * population streams.  Defaults: lds_bytes 40960 (4 workgroups per CU),
+ * xcd_remap 0. */
+typedef struct lbmh_launch_config {
+  int lds_bytes;    /* 0 .. 65536                                             */
+  int xcd_remap;    /* 1: each XCD walks one contiguous range of cells        */
+} lbmh_launch_config;
+
+int lbmh_set_launch_config(lbmh_lattice* lattice, const lbmh_launch_config* config);
+int lbmh_get_launch_config(const lbmh_lattice* lattice, lbmh_launch_config* config);
+
+/* Time candidate launch shapes on a scratch lattice of the same extent,
+ * dynamics ids and parameters (the lattice's own populations are untouched),
+ * keep the fastest on the lattice and return it in *chosen (may be NULL).
+ * trial_steps >= 1 timed steps per candidate.  Needs device memory for one
+ * more copy of the lattice. */
+int lbmh_autotune(lbmh_lattice* lattice, int trial_steps, lbmh_launch_config* chosen);
+
 /* Bytes of device memory the lattice holds. */
 int lbmh_lattice_footprint(const lbmh_lattice* lattice, size_t* bytes);
 

@@ -24,7 +24,8 @@ EXPORTS = (
     "lbmh_populations_download", "lbmh_ini_equilibrium", "lbmh_step",
     "lbmh_get_step_count", "lbmh_get_shifts", "lbmh_moments", "lbmh_reduce_moments",
     "lbmh_synchronize", "lbmh_lattice_footprint", "lbmh_abi_version", "lbmh_device_count",
-    "lbmh_device_arch", "lbmh_last_error",
+    "lbmh_device_arch", "lbmh_last_error", "lbmh_set_launch_config", "lbmh_get_launch_config",
+    "lbmh_autotune",
 )
 
 
@@ -46,6 +47,10 @@ class BulkParams(ctypes.Structure):
         ("smago", ctypes.c_double),
         ("force", ctypes.c_double * 3),
     ]
+
+
+class LaunchConfig(ctypes.Structure):
+    _fields_ = [("lds_bytes", ctypes.c_int), ("xcd_remap", ctypes.c_int)]
 
 
 class LbmhError(RuntimeError):
@@ -85,6 +90,9 @@ def load() -> ctypes.CDLL:
     L.lbmh_lattice_footprint.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
     L.lbmh_device_count.argtypes = [ctypes.POINTER(i32)]
     L.lbmh_device_arch.argtypes = [i32, ctypes.c_char_p, ctypes.c_size_t]
+    L.lbmh_set_launch_config.argtypes = [vp, ctypes.POINTER(LaunchConfig)]
+    L.lbmh_get_launch_config.argtypes = [vp, ctypes.POINTER(LaunchConfig)]
+    L.lbmh_autotune.argtypes = [vp, i32, ctypes.POINTER(LaunchConfig)]
     L.lbmh_last_error.restype = ctypes.c_char_p
     for name in EXPORTS:
         if name != "lbmh_last_error":

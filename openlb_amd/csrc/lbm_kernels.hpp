@@ -18,6 +18,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "fastdiv.hpp"
 #include "lbm_collision.hpp"
 
@@ -64,8 +66,11 @@ struct StepArgs {
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
-// b % 8).  Remap so that every XCD walks one contiguous range of cell chunks;
-// the partially shared cache lines at chunk edges then stay in one L2.
+// b % 8).  Optionally remap so that every XCD walks one contiguous range of
+// cell chunks (partially shared cache lines at chunk edges stay in one L2).
+// Measured on MI355X this loses ~3% on the 19-column pattern: eight separate
+// windows mean eight times as many concurrent DRAM streams, so it is off by
+// default and left to the tuner.
 __device__ __forceinline__ uint32_t remap_xcd(uint32_t b, uint32_t nblocks)
 {
   constexpr uint32_t kXcd = 8;
@@ -86,7 +91,8 @@ __device__ __forceinline__ void store_pop(S* p, V v) { *p = S(v * V(StorageTrait
 template <typename S>
 __device__ __forceinline__ S* column_at(S* column, uint32_t k)
 {
-  return reinterpret_cast<S*>(reinterpret_cast<char*>(column) + uint32_t(k * uint32_t(sizeof(S))));
+  using Byte = std::conditional_t<std::is_const_v<S>, const char, char>;
+  return reinterpret_cast<S*>(reinterpret_cast<Byte*>(column) + uint32_t(k * uint32_t(sizeof(S))));
 }
 
 __device__ __forceinline__ uint32_t wrap_index(uint32_t n, uint32_t shift, uint32_t N)
@@ -141,15 +147,21 @@ __global__ __launch_bounds__(kBlock) void collide_stream_kernel(StepArgs<Q, type
   if (a.xcd_remap) b = remap_xcd(b, gridDim.x);
   const uint32_t c = b * kBlock + threadIdx.x;
   if (c >= a.ncore) return;
-  int id = kIdBulk;
-  if constexpr (IDS) {
-    id = a.ids[c];
-    if (id == 0) return;
-  }
   const uint32_t t = a.div_z.div(c);
   const uint32_t z = c - t * a.nz;
   const uint32_t x = a.div_y.div(t);
   const uint32_t y = t - x * a.ny;
+  const bool edge = a.periodic &&
+                    ((((a.periodic & 1u) != 0) && (x == 0 || x + 1 == a.nx)) ||
+                     (((a.periodic & 2u) != 0) && (y == 0 || y + 1 == a.ny)) ||
+                     (((a.periodic & 4u) != 0) && (z == 0 || z + 1 == a.nz)));
+  int id = kIdBulk;
+  if constexpr (IDS) {
+    id = a.ids[c];
+    // NoDynamics cells are not collided, but on a periodic face their
+    // populations still cross into the overlap like every other core cell's
+    if (id == 0 && !edge) return;
+  }
   const uint32_t n = (x + 1) * a.sx + (y + 1) * a.sy + (z + 1);
   S* pop = static_cast<S*>(a.pop);
 
@@ -158,26 +170,22 @@ __global__ __launch_bounds__(kBlock) void collide_stream_kernel(StepArgs<Q, type
   for (int i = 0; i < Q; ++i) {
     f[i] = load_pop<S, V>(column_at(pop + uint64_t(i) * a.stride, wrap_index(n, a.shift[i], a.n_padded)));
   }
-  if (!IDS || id == kIdBulk) {
-    collide_bulk<Q, V, KIND>(f, a.bp);
-  } else if (id == 2) {
-    revert<Q, V>(f);
-  } else {
-    const double* w = a.walls + 4 * id;
-    const V uw[3] = {V(w[1]), V(w[2]), V(w[3])};
-    moving_wall<Q, V>(f, V(w[0]), uw);
-  }
+  if (!IDS || id != 0) {
+    if (!IDS || id == kIdBulk) {
+      collide_bulk<Q, V, KIND>(f, a.bp);
+    } else if (id == 2) {
+      revert<Q, V>(f);
+    } else {
+      const double* w = a.walls + 4 * id;
+      const V uw[3] = {V(w[1]), V(w[2]), V(w[3])};
+      moving_wall<Q, V>(f, V(w[0]), uw);
+    }
 #pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    store_pop<S, V>(column_at(pop + uint64_t(i) * a.stride, wrap_index(n, a.shift[i], a.n_padded)), f[i]);
+    for (int i = 0; i < Q; ++i) {
+      store_pop<S, V>(column_at(pop + uint64_t(i) * a.stride, wrap_index(n, a.shift[i], a.n_padded)), f[i]);
+    }
   }
-
-  if (a.periodic) {
-    const bool edge = (((a.periodic & 1u) != 0) && (x == 0 || x + 1 == a.nx)) ||
-                      (((a.periodic & 2u) != 0) && (y == 0 || y + 1 == a.ny)) ||
-                      (((a.periodic & 4u) != 0) && (z == 0 || z + 1 == a.nz));
-    if (edge) push_periodic_halo<Q, S, V>(a, pop, f, int(x), int(y), int(z));
-  }
+  if (edge) push_periodic_halo<Q, S, V>(a, pop, f, int(x), int(y), int(z));
 }
 
 // ---- data movement and evaluation kernels (not on the time-step path) ----

@@ -69,7 +69,7 @@ struct lbmh_lattice {
   double* d_partials = nullptr;
   uint32_t n_partials = 0;
   bool uniform_bulk = false;   // every core cell has id 1: skip the id read
-  bool xcd_remap = true;
+  lbmh_launch_config launch{40960, 0};
   lbmh_bulk_params params{};
   double walls[kMaxWalls][4] = {};
   size_t footprint = 0;
@@ -150,15 +150,17 @@ int launch_kind(lbmh_lattice* L, hipStream_t stream)
   a.div_z = FastDiv(uint32_t(L->n[2]));
   a.div_y = FastDiv(uint32_t(L->n[1]));
   a.periodic = uint32_t(L->periodic[0] | (L->periodic[1] << 1) | (L->periodic[2] << 2));
-  a.xcd_remap = L->xcd_remap ? 1u : 0u;
+  a.xcd_remap = L->launch.xcd_remap ? 1u : 0u;
   a.walls = L->d_walls;
   a.bp = bulk_params<V>(L);
   const uint32_t grid = (L->ncore + kBlock - 1) / kBlock;
   for (int i = 0; i < Q; ++i) a.shift[i] = uint32_t(L->shift[i]);
+  // the dynamic LDS is never touched: it only limits resident workgroups per CU
+  const unsigned lds = unsigned(L->launch.lds_bytes);
   if (L->uniform_bulk) {
-    hipLaunchKernelGGL((collide_stream_kernel<Q, S, KIND, false>), dim3(grid), dim3(kBlock), 0, stream, a);
+    hipLaunchKernelGGL((collide_stream_kernel<Q, S, KIND, false>), dim3(grid), dim3(kBlock), lds, stream, a);
   } else {
-    hipLaunchKernelGGL((collide_stream_kernel<Q, S, KIND, true>), dim3(grid), dim3(kBlock), 0, stream, a);
+    hipLaunchKernelGGL((collide_stream_kernel<Q, S, KIND, true>), dim3(grid), dim3(kBlock), lds, stream, a);
   }
   LBMH_HIP(hipGetLastError());
   // stream: shift_i -= c_i . projection (blockLattice.hh:317-339)
@@ -530,6 +532,83 @@ int lbmh_synchronize(lbmh_lattice* L)
   if (!L) return fail(LBMH_E_INVALID, "null lattice");
   DeviceGuard guard(L->device);
   LBMH_HIP(hipDeviceSynchronize());
+  return LBMH_OK;
+}
+
+int lbmh_set_launch_config(lbmh_lattice* L, const lbmh_launch_config* cfg)
+{
+  if (!L || !cfg) return fail(LBMH_E_INVALID, "null argument");
+  if (cfg->lds_bytes < 0 || cfg->lds_bytes > 65536) return fail(LBMH_E_INVALID, "lds_bytes must be in [0, 65536]");
+  L->launch = *cfg;
+  L->launch.xcd_remap = cfg->xcd_remap ? 1 : 0;
+  return LBMH_OK;
+}
+
+int lbmh_get_launch_config(const lbmh_lattice* L, lbmh_launch_config* cfg)
+{
+  if (!L || !cfg) return fail(LBMH_E_INVALID, "null argument");
+  *cfg = L->launch;
+  return LBMH_OK;
+}
+
+int lbmh_autotune(lbmh_lattice* L, int trial_steps, lbmh_launch_config* chosen)
+{
+  if (!L) return fail(LBMH_E_INVALID, "null lattice");
+  if (trial_steps < 1) return fail(LBMH_E_INVALID, "trial_steps must be >= 1");
+  DeviceGuard guard(L->device);
+  lbmh_lattice_desc desc{};
+  desc.q = L->q;
+  desc.nx = L->n[0];
+  desc.ny = L->n[1];
+  desc.nz = L->n[2];
+  for (int d = 0; d < 3; ++d) desc.periodic[d] = L->periodic[d];
+  desc.storage = L->storage;
+  desc.bulk_kind = L->kind;
+  desc.device = L->device;
+  lbmh_lattice* T = nullptr;
+  int rc = lbmh_lattice_create(&desc, &T);
+  if (rc != LBMH_OK) return rc;
+  struct Cleanup {
+    lbmh_lattice* t;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~Cleanup()
+    {
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      lbmh_lattice_destroy(t);
+    }
+  } cleanup{T};
+  LBMH_HIP(hipMemcpy(T->d_ids, L->d_ids, L->ncore, hipMemcpyDeviceToDevice));
+  LBMH_HIP(hipMemcpy(T->d_walls, L->d_walls, sizeof(double) * 4 * kMaxWalls, hipMemcpyDeviceToDevice));
+  T->uniform_bulk = L->uniform_bulk;
+  T->params = L->params;
+  LBMH_HIP(hipEventCreate(&cleanup.e0));
+  LBMH_HIP(hipEventCreate(&cleanup.e1));
+  static const int kLds[] = {0, 24576, 32768, 40960, 49152, 65536};
+  float best_ms = 0.f;
+  lbmh_launch_config best = L->launch;
+  bool have = false;
+  for (int remap = 0; remap <= 1; ++remap) {
+    for (int lds : kLds) {
+      T->launch = lbmh_launch_config{lds, remap};
+      rc = lbmh_step(T, 2, nullptr);
+      if (rc != LBMH_OK) return rc;
+      LBMH_HIP(hipEventRecord(cleanup.e0, nullptr));
+      rc = lbmh_step(T, trial_steps, nullptr);
+      if (rc != LBMH_OK) return rc;
+      LBMH_HIP(hipEventRecord(cleanup.e1, nullptr));
+      LBMH_HIP(hipEventSynchronize(cleanup.e1));
+      float ms = 0.f;
+      LBMH_HIP(hipEventElapsedTime(&ms, cleanup.e0, cleanup.e1));
+      if (!have || ms < best_ms) {
+        best_ms = ms;
+        best = T->launch;
+        have = true;
+      }
+    }
+  }
+  L->launch = best;
+  if (chosen) *chosen = best;
   return LBMH_OK;
 }
 

@@ -1,0 +1,64 @@
+"""Host-side checks of the C ABI (no GPU needed). The library loads, exports every
+function include/lbmh.h declares, and rejects invalid arguments before touching the
+device."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from openlb_amd import abi
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "lbmh.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(lbmh_\w+)\s*\(", text, re.M)))
+
+
+def test_header_and_binding_agree():
+    assert declared_functions() == sorted(abi.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = abi.load()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+
+
+def test_abi_version():
+    assert abi.load().lbmh_abi_version() == 1
+
+
+@pytest.mark.parametrize("field,value", [("q", 20), ("nx", 0), ("storage", 7), ("bulk_kind", 9)])
+def test_create_rejects_invalid_descriptor(field, value):
+    lib = abi.load()
+    desc = abi.LatticeDesc()
+    desc.q, desc.nx, desc.ny, desc.nz = 19, 8, 8, 8
+    desc.storage, desc.bulk_kind = abi.F32, 0
+    setattr(desc, field, value)
+    h = ctypes.c_void_p()
+    rc = lib.lbmh_lattice_create(ctypes.byref(desc), ctypes.byref(h))
+    assert rc == abi.E_INVALID
+    assert not h.value
+    assert lib.lbmh_last_error().decode()
+
+
+def test_create_rejects_cell_index_overflow():
+    """Padded cell count must stay below 2^31 (32-bit cell ids, OpenLB CellID uint32)."""
+    lib = abi.load()
+    desc = abi.LatticeDesc()
+    desc.q, desc.nx, desc.ny, desc.nz = 19, 1300, 1300, 1300
+    h = ctypes.c_void_p()
+    assert lib.lbmh_lattice_create(ctypes.byref(desc), ctypes.byref(h)) == abi.E_INVALID
+    assert "2^31" in lib.lbmh_last_error().decode()
+
+
+def test_null_arguments_are_rejected():
+    lib = abi.load()
+    assert lib.lbmh_step(None, 1, None) == abi.E_INVALID
+    assert lib.lbmh_set_dynamics_ids(None, None) == abi.E_INVALID
+    assert lib.lbmh_lattice_destroy(None) == abi.OK
+    with pytest.raises(abi.LbmhError):
+        abi.check(lib.lbmh_step(None, 1, None))

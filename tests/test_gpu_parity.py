@@ -76,10 +76,13 @@ def test_gpu_fp16_storage_velocity_error(gpu_available, name):
     ref.upload(arrays[f"f_{s}"].astype(np.float64))
     mref = ref.moments()
     fluid = case.ids == 1
-    umax = np.abs(mref[1:, fluid]).max()
+    # characteristic velocity: the larger of the initial and the final flow
+    # (decaying periodic modes end far below their start, driven flows above)
+    u_char = max(np.abs(mref[1:, fluid]).max(), np.abs(case.rho_u[1:, fluid]).max(),
+                 max((abs(v) for _, u in case.walls.values() for v in u), default=0.0))
     uerr = np.abs(mom[1:, fluid] - mref[1:, fluid]).max()
-    print(f"{name} f16: |du|max/|u|max = {uerr / umax:.3e}", flush=True)
-    assert uerr <= 1e-3 * umax
+    print(f"{name} f16: |du|max/u_char = {uerr / u_char:.3e}", flush=True)
+    assert uerr <= 1e-3 * u_char
 
 
 def test_gpu_tgv64_fp64_1000_steps(gpu_available):
@@ -220,3 +223,27 @@ def test_full_size_porous_channel_conserves_mass(gpu_available):
     assert np.isfinite(m1).all()
     assert abs(m1[0] - m0[0]) <= 1e-6 * m0[0]
     assert m1[1] > 0 and abs(m1[2]) < 1e-3 * m1[1] and abs(m1[3]) < 1e-3 * m1[1]
+
+
+def test_launch_configs_give_identical_results(gpu_available):
+    """Launch shape is scheduling only: every configuration must give bit-identical
+    populations. The autotuner must leave the lattice's own state untouched."""
+    case = W.porous_channel((40, 24, 30), seed=12, force=1e-5, rmin=1, rmax=3)
+    f0 = O.initial_populations(case, "f32")
+    results = []
+    for lds, remap in [(0, 0), (40960, 0), (65536, 1), (24576, 1)]:
+        with BlockLattice.from_case(case, "f32") as lat:
+            lat.upload_populations(f0, "f64")
+            lat.set_launch_config(lds, remap)
+            assert lat.launch_config() == {"lds_bytes": lds, "xcd_remap": remap}
+            lat.collide_and_stream(25)
+            results.append(lat.download_populations("f32"))
+    for r in results[1:]:
+        np.testing.assert_array_equal(r, results[0])
+    with BlockLattice.from_case(case, "f32") as lat:
+        lat.upload_populations(f0, "f64")
+        before = lat.download_populations("f32")
+        chosen = lat.autotune(trial_steps=2)
+        assert lat.launch_config() == chosen
+        assert lat.step_count() == 0
+        np.testing.assert_array_equal(lat.download_populations("f32"), before)
