@@ -159,8 +159,10 @@ def main():
     ap.add_argument("--storage", default="f32", choices=["f64", "f32", "f16"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
-                    help="keep the default launch shape instead of tuning it first")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time candidate launch shapes first and keep the fastest "
+                         "(default: the library's measured default shape)")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()

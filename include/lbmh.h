@@ -145,11 +145,14 @@ int lbmh_synchronize(lbmh_lattice* lattice);
  * cuda/operator.hh:422-429).  lds_bytes reserves dynamic LDS per 256-thread
  * workgroup purely to cap the resident workgroups per CU (160 KB LDS per CU):
  * fewer waves in flight keep fewer DRAM rows open for the 2*Q concurrent
- * population streams.  Defaults: lds_bytes 40960 (4 workgroups per CU),
- * xcd_remap 0. */
+ * population streams.  xcd_group G > 0 hands each XCD runs of G consecutive
+ * 256-cell chunks inside a window of 8*G chunks, so a cache line two chunks
+ * share is fetched into one L2 instead of two.  Defaults: lds_bytes 40960
+ * (4 workgroups per CU) and xcd_group 4 for FP64/FP32 storage, lds_bytes 0
+ * and xcd_group 8 for FP16 storage. */
 typedef struct lbmh_launch_config {
   int lds_bytes;    /* 0 .. 65536                                             */
-  int xcd_remap;    /* 1: each XCD walks one contiguous range of cells        */
+  int xcd_group;    /* 0 (dispatch order) .. 64                               */
 } lbmh_launch_config;
 
 int lbmh_set_launch_config(lbmh_lattice* lattice, const lbmh_launch_config* config);

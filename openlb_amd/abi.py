@@ -50,7 +50,7 @@ class BulkParams(ctypes.Structure):
 
 
 class LaunchConfig(ctypes.Structure):
-    _fields_ = [("lds_bytes", ctypes.c_int), ("xcd_remap", ctypes.c_int)]
+    _fields_ = [("lds_bytes", ctypes.c_int), ("xcd_group", ctypes.c_int)]
 
 
 class LbmhError(RuntimeError):

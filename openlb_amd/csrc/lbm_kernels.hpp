@@ -60,25 +60,27 @@ struct StepArgs {
   uint32_t sx, sy;          // padded strides of x and y
   FastDiv div_z, div_y;
   uint32_t periodic;        // bit d set: axis d periodic
-  uint32_t xcd_remap;       // map consecutive chunks onto one XCD
+  uint32_t xcd_group;       // runs of consecutive chunks per XCD (0: identity)
   const double* walls;      // [kMaxWalls][4] rho, ux, uy, uz
   BulkParams<V> bp;
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
-// b % 8).  Optionally remap so that every XCD walks one contiguous range of
-// cell chunks (partially shared cache lines at chunk edges stay in one L2).
-// Measured on MI355X this loses ~3% on the 19-column pattern: eight separate
-// windows mean eight times as many concurrent DRAM streams, so it is off by
-// default and left to the tuner.
-__device__ __forceinline__ uint32_t remap_xcd(uint32_t b, uint32_t nblocks)
+// b % 8), so neighbouring cell chunks land in different L2s and the cache line
+// a PS-shifted column shares between two chunks is fetched twice (measured:
+// ~12% extra HBM reads at 256 cells per chunk).  Grouped remap: XCD x takes
+// runs of `group` consecutive chunks, while the eight XCDs together stay
+// inside a window of 8*group chunks, so the number of concurrent DRAM streams
+// does not grow the way it does when each XCD walks its own eighth of the
+// lattice.  Blocks past the last whole window keep the identity mapping, which
+// keeps the map a bijection for any grid size.
+__device__ __forceinline__ uint32_t remap_xcd(uint32_t b, uint32_t nblocks, uint32_t group)
 {
   constexpr uint32_t kXcd = 8;
-  const uint32_t per = nblocks / kXcd, rem = nblocks % kXcd;
-  const uint32_t xcd = b % kXcd, k = b / kXcd;
-  // XCDs below rem own per+1 chunks
-  const uint32_t start = xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per;
-  return start + k;
+  const uint32_t window = kXcd * group;
+  if (b >= (nblocks / window) * window) return b;
+  const uint32_t x = b % kXcd, k = b / kXcd;
+  return (k / group) * window + x * group + (k % group);
 }
 
 template <typename S, typename V>
@@ -144,7 +146,7 @@ __global__ __launch_bounds__(kBlock) void collide_stream_kernel(StepArgs<Q, type
 {
   using V = typename StorageTraits<S>::V;
   uint32_t b = blockIdx.x;
-  if (a.xcd_remap) b = remap_xcd(b, gridDim.x);
+  if (a.xcd_group) b = remap_xcd(b, gridDim.x, a.xcd_group);
   const uint32_t c = b * kBlock + threadIdx.x;
   if (c >= a.ncore) return;
   const uint32_t t = a.div_z.div(c);

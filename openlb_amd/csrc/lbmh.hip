@@ -69,7 +69,7 @@ struct lbmh_lattice {
   double* d_partials = nullptr;
   uint32_t n_partials = 0;
   bool uniform_bulk = false;   // every core cell has id 1: skip the id read
-  lbmh_launch_config launch{40960, 0};
+  lbmh_launch_config launch{49152, 4};
   lbmh_bulk_params params{};
   double walls[kMaxWalls][4] = {};
   size_t footprint = 0;
@@ -150,7 +150,7 @@ int launch_kind(lbmh_lattice* L, hipStream_t stream)
   a.div_z = FastDiv(uint32_t(L->n[2]));
   a.div_y = FastDiv(uint32_t(L->n[1]));
   a.periodic = uint32_t(L->periodic[0] | (L->periodic[1] << 1) | (L->periodic[2] << 2));
-  a.xcd_remap = L->launch.xcd_remap ? 1u : 0u;
+  a.xcd_group = uint32_t(L->launch.xcd_group);
   a.walls = L->d_walls;
   a.bp = bulk_params<V>(L);
   const uint32_t grid = (L->ncore + kBlock - 1) / kBlock;
@@ -292,6 +292,9 @@ int lbmh_lattice_create(const lbmh_lattice_desc* desc, lbmh_lattice** out)
                             : neighbor_distance<27>(i, int64_t(sy * sz), int64_t(sz));
   }
   L->params.omega = 1.0;
+  // measured best shapes on MI355X for the 256^3 D3Q19 step: FP16 columns
+  // want full occupancy, FP32/FP64 three to four workgroups per CU
+  L->launch = desc->storage == LBMH_F16 ? lbmh_launch_config{0, 8} : lbmh_launch_config{40960, 4};
   for (int w = 0; w < kMaxWalls; ++w) L->walls[w][0] = 1.0;
 
   DeviceGuard guard(L->device);
@@ -539,8 +542,8 @@ int lbmh_set_launch_config(lbmh_lattice* L, const lbmh_launch_config* cfg)
 {
   if (!L || !cfg) return fail(LBMH_E_INVALID, "null argument");
   if (cfg->lds_bytes < 0 || cfg->lds_bytes > 65536) return fail(LBMH_E_INVALID, "lds_bytes must be in [0, 65536]");
+  if (cfg->xcd_group < 0 || cfg->xcd_group > 64) return fail(LBMH_E_INVALID, "xcd_group must be in [0, 64]");
   L->launch = *cfg;
-  L->launch.xcd_remap = cfg->xcd_remap ? 1 : 0;
   return LBMH_OK;
 }
 
@@ -585,12 +588,13 @@ int lbmh_autotune(lbmh_lattice* L, int trial_steps, lbmh_launch_config* chosen)
   LBMH_HIP(hipEventCreate(&cleanup.e0));
   LBMH_HIP(hipEventCreate(&cleanup.e1));
   static const int kLds[] = {0, 24576, 32768, 40960, 49152, 65536};
+  static const int kGroup[] = {0, 2, 4, 8, 16};
   float best_ms = 0.f;
   lbmh_launch_config best = L->launch;
   bool have = false;
-  for (int remap = 0; remap <= 1; ++remap) {
+  for (int group : kGroup) {
     for (int lds : kLds) {
-      T->launch = lbmh_launch_config{lds, remap};
+      T->launch = lbmh_launch_config{lds, group};
       rc = lbmh_step(T, 2, nullptr);
       if (rc != LBMH_OK) return rc;
       LBMH_HIP(hipEventRecord(cleanup.e0, nullptr));

@@ -142,20 +142,20 @@ class BlockLattice:
         abi.check(abi.load().lbmh_reduce_moments(self._h, _ptr(out)))
         return out
 
-    def set_launch_config(self, lds_bytes: int, xcd_remap: int = 0):
-        cfg = abi.LaunchConfig(int(lds_bytes), int(xcd_remap))
+    def set_launch_config(self, lds_bytes: int, xcd_group: int = 0):
+        cfg = abi.LaunchConfig(int(lds_bytes), int(xcd_group))
         abi.check(abi.load().lbmh_set_launch_config(self._h, ctypes.byref(cfg)))
 
     def launch_config(self) -> dict:
         cfg = abi.LaunchConfig()
         abi.check(abi.load().lbmh_get_launch_config(self._h, ctypes.byref(cfg)))
-        return {"lds_bytes": cfg.lds_bytes, "xcd_remap": cfg.xcd_remap}
+        return {"lds_bytes": cfg.lds_bytes, "xcd_group": cfg.xcd_group}
 
     def autotune(self, trial_steps: int = 10) -> dict:
         """Pick the fastest launch shape on a scratch copy (populations untouched)."""
         cfg = abi.LaunchConfig()
         abi.check(abi.load().lbmh_autotune(self._h, int(trial_steps), ctypes.byref(cfg)))
-        return {"lds_bytes": cfg.lds_bytes, "xcd_remap": cfg.xcd_remap}
+        return {"lds_bytes": cfg.lds_bytes, "xcd_group": cfg.xcd_group}
 
     def footprint(self) -> int:
         b = ctypes.c_size_t(0)

@@ -231,11 +231,11 @@ def test_launch_configs_give_identical_results(gpu_available):
     case = W.porous_channel((40, 24, 30), seed=12, force=1e-5, rmin=1, rmax=3)
     f0 = O.initial_populations(case, "f32")
     results = []
-    for lds, remap in [(0, 0), (40960, 0), (65536, 1), (24576, 1)]:
+    for lds, group in [(0, 0), (40960, 0), (65536, 8), (24576, 3), (49152, 1)]:
         with BlockLattice.from_case(case, "f32") as lat:
             lat.upload_populations(f0, "f64")
-            lat.set_launch_config(lds, remap)
-            assert lat.launch_config() == {"lds_bytes": lds, "xcd_remap": remap}
+            lat.set_launch_config(lds, group)
+            assert lat.launch_config() == {"lds_bytes": lds, "xcd_group": group}
             lat.collide_and_stream(25)
             results.append(lat.download_populations("f32"))
     for r in results[1:]:
