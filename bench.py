@@ -104,6 +104,19 @@ def calibrate_copy(mib: int, repeats: int = 5):
                       "repeats": repeats, "copy_gbs": round(2 * nbytes / dt / 1e9, 1)}), flush=True)
 
 
+def measured_traffic(kernel_name, case, launch):
+    """HBM bytes per launch of this kernel, workload and launch shape from the
+    committed counter runs (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE scaled
+    by a 1 GiB device copy), or None if that combination was not profiled."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    shape = "x".join(str(n) for n in case.shape)
+    key = f"{kernel_name}|{case.name}|{shape}|lds{launch['lds_bytes']}|xcd{launch['xcd_group']}"
+    entry = json.load(open(path)).get(key)
+    return None if entry is None else entry["traffic_bytes"]
+
+
 def make_case(args):
     from openlb_amd import workloads as W
     if args.workload == "porous_channel":
@@ -213,6 +226,8 @@ def main():
 
     cells = case.n_active
     mlups = cells * args.steps / wall / 1e6
+    kernel_name = f"collide_stream_kernel<{case.q},{args.storage},{case.kind}>"
+    traffic = measured_traffic(kernel_name, case, launch)
     avg_ms = float(np.mean(kernel_ms))
     bytes_per_launch = 2 * case.q * STORE_BYTES[args.storage] * cells
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
@@ -256,8 +271,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": f"collide_stream_kernel<{case.q},{args.storage},{case.kind}>",
+            "traffic": traffic,
+            "kernel": kernel_name,
             "kernel_avg_ms": round(avg_ms, 5),
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },

@@ -36,17 +36,35 @@ FLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(LIB_DIR, exist_ok=True)
-    tmp = LIB_PATH + ".tmp"
-    cmd = [HIPCC, *FLAGS, *SOURCES, "-o", tmp]
+HOST_DIR = os.path.join(PKG, "host")
+EXAMPLES = {"cavity3d_benchmark": os.path.join(HOST_DIR, "examples", "cavity3d_benchmark.cpp")}
+
+
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
         sys.stderr.write(proc.stdout + proc.stderr)
-        raise RuntimeError(f"hipcc failed with exit code {proc.returncode}")
+        raise RuntimeError(f"{cmd[0]} failed with exit code {proc.returncode}")
+
+
+def build_examples(verbose: bool = False):
+    """C++ programs on the host layer (openlb_amd/host/olb_hip.h), linked to liblbmh.so."""
+    for name, src in EXAMPLES.items():
+        out = os.path.join(LIB_DIR, name)
+        _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}", f"-I{HOST_DIR}",
+              src, "-o", out + ".tmp", f"-L{LIB_DIR}", "-llbmh", "-Wl,-rpath,$ORIGIN",
+              "-L/opt/rocm/lib", "-Wl,-rpath-link,/opt/rocm/lib"], verbose)
+        os.replace(out + ".tmp", out)
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(LIB_DIR, exist_ok=True)
+    tmp = LIB_PATH + ".tmp"
+    _run([HIPCC, *FLAGS, *SOURCES, "-o", tmp], verbose)
     os.replace(tmp, LIB_PATH)
+    build_examples(verbose)
     return LIB_PATH
 
 

@@ -1,0 +1,307 @@
+// olb_hip.h -- C++ host layer of the MI355X block-lattice backend.
+//
+// Header-only.  Mirrors the part of OpenLB's API that drives the time step
+// (names, argument meaning, error behaviour), on top of the C ABI in lbmh.h:
+//
+//   SuperLattice<T,DESCRIPTOR>(extent, periodicity)      superLattice.h:84-101
+//   defineDynamics<DYNAMICS>(indicator)                  superLattice.h:214-233
+//   setParameter<PARAMETER>(value)                       superLattice.h:318-336
+//   defineRhoU / iniEquilibrium(indicator, rho, u)       superLattice.h:344-352
+//   collideAndStream()                                   superLattice.h:380
+//   setProcessingContext(Evaluation | Simulation)        superLattice.h:154-164
+//   get(x, y, z)[iPop] on the host mirror                superLattice.h:169-176
+//   statisticsOff()                                      superLattice.h:397-403
+//
+// Dynamics are tags carrying the C ABI's bulk kind; parameters are tags of
+// the same names as OpenLB's fields.  HIP errors surface as std::runtime_error
+// (like gpu::cuda::device::check, src/core/platform/gpu/cuda/device.hh:48-62),
+// invalid requests as std::invalid_argument / std::logic_error.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <functional>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "lbmh.h"
+
+namespace olb {
+namespace hip {
+
+// ---- descriptors (src/descriptor/definition/common.h:120-166, 313-366) ----
+namespace descriptors {
+struct D3Q19 { static constexpr int d = 3, q = 19; };
+struct D3Q27 { static constexpr int d = 3, q = 27; };
+// parameter tags (descriptors::OMEGA, descriptors::FORCE)
+struct OMEGA {};
+struct FORCE {};
+}  // namespace descriptors
+
+namespace collision {
+struct TRT { struct MAGIC {}; };
+namespace LES { struct SMAGORINSKY {}; }
+}  // namespace collision
+
+// ---- dynamics tags (src/dynamics/dynamics.h, smagorinskyBGKdynamics.h) ----
+enum class DynamicsRole { Bulk, BounceBack, BounceBackVelocity, None };
+
+template <typename T, typename D> struct BGKdynamics { static constexpr int kind = LBMH_BGK; static constexpr DynamicsRole role = DynamicsRole::Bulk; };
+template <typename T, typename D> struct ForcedBGKdynamics { static constexpr int kind = LBMH_FORCED_BGK; static constexpr DynamicsRole role = DynamicsRole::Bulk; };
+template <typename T, typename D> struct TRTdynamics { static constexpr int kind = LBMH_TRT; static constexpr DynamicsRole role = DynamicsRole::Bulk; };
+template <typename T, typename D> struct ForcedTRTdynamics { static constexpr int kind = LBMH_FORCED_TRT; static constexpr DynamicsRole role = DynamicsRole::Bulk; };
+template <typename T, typename D> struct SmagorinskyBGKdynamics { static constexpr int kind = LBMH_SMAGORINSKY; static constexpr DynamicsRole role = DynamicsRole::Bulk; };
+template <typename T, typename D> struct BounceBack { static constexpr DynamicsRole role = DynamicsRole::BounceBack; };
+template <typename T, typename D> struct BounceBackVelocity { static constexpr DynamicsRole role = DynamicsRole::BounceBackVelocity; };
+template <typename T, typename D> struct NoDynamics { static constexpr DynamicsRole role = DynamicsRole::None; };
+
+enum class ProcessingContext { Evaluation, Simulation };
+
+using Indicator = std::function<bool(int, int, int)>;
+using ScalarField = std::function<double(int, int, int)>;
+using VectorField = std::function<std::array<double, 3>(int, int, int)>;
+
+namespace detail {
+inline void check(int rc)
+{
+  if (rc == LBMH_OK) return;
+  const std::string msg = lbmh_last_error();
+  if (rc == LBMH_E_INVALID) throw std::invalid_argument(msg);
+  throw std::runtime_error(msg);
+}
+
+template <typename T> constexpr int storage_of()
+{
+  if constexpr (std::is_same_v<T, double>) return LBMH_F64;
+  else if constexpr (std::is_same_v<T, float>) return LBMH_F32;
+  else return LBMH_F16;
+}
+
+template <typename DYN> struct kind_of { static constexpr int value = -1; };
+template <template <typename, typename> class DYN, typename T, typename D>
+struct kind_of<DYN<T, D>> {
+  static constexpr int value = []() {
+    if constexpr (DYN<T, D>::role == DynamicsRole::Bulk) return DYN<T, D>::kind;
+    else return -1;
+  }();
+};
+}  // namespace detail
+
+// Host view of one cell of the evaluation mirror (cell[iPop] as in cpu::Cell).
+template <typename T>
+class ConstCellView {
+ public:
+  ConstCellView(const std::vector<double>& pop, std::size_t n, std::size_t c) : _pop(pop), _n(n), _c(c) {}
+  T operator[](unsigned iPop) const { return T(_pop[iPop * _n + _c]); }
+
+ private:
+  const std::vector<double>& _pop;
+  std::size_t _n, _c;
+};
+
+// One block of OpenLB's super lattice, resident on one MI355X.  Storage type
+// follows T (double / float); FP16 storage with FP32 arithmetic is selected
+// with T = float and storage = LBMH_F16.
+template <typename T, typename DESCRIPTOR>
+class SuperLattice {
+ public:
+  static constexpr int q = DESCRIPTOR::q;
+
+  SuperLattice(std::array<int, 3> extent, std::array<bool, 3> periodicity, int device = 0,
+               int storage = detail::storage_of<T>())
+      : _extent(extent), _periodic(periodicity), _device(device), _storage(storage),
+        _ids(std::size_t(extent[0]) * extent[1] * extent[2], LBMH_ID_NONE)
+  {
+    if (storage == LBMH_F16 && !std::is_same_v<T, float>) {
+      throw std::invalid_argument("FP16 storage computes in FP32: use T = float");
+    }
+  }
+
+  SuperLattice(const SuperLattice&) = delete;
+  SuperLattice& operator=(const SuperLattice&) = delete;
+  ~SuperLattice() { lbmh_lattice_destroy(_handle); }
+
+  std::size_t ncore() const { return _ids.size(); }
+  const std::array<int, 3>& extent() const { return _extent; }
+
+  /// Set dynamics of the indicated cells (SuperLattice::defineDynamics).
+  template <typename DYNAMICS>
+  void defineDynamics(const Indicator& indicator)
+  {
+    requireSetup("defineDynamics");
+    uint8_t id = LBMH_ID_NONE;
+    if constexpr (DYNAMICS::role == DynamicsRole::Bulk) {
+      if (_kind >= 0 && _kind != DYNAMICS::kind) {
+        throw std::logic_error("one bulk dynamics per block lattice");
+      }
+      _kind = DYNAMICS::kind;
+      id = LBMH_ID_BULK;
+    } else if constexpr (DYNAMICS::role == DynamicsRole::BounceBack) {
+      id = LBMH_ID_BOUNCE_BACK;
+    } else if constexpr (DYNAMICS::role == DynamicsRole::BounceBackVelocity) {
+      id = uint8_t(LBMH_ID_FIRST_WALL + _wallsDefined);
+      if (id == 0) throw std::logic_error("at most 253 moving-wall groups per block");
+      _pendingWall = id;
+      ++_wallsDefined;
+    }
+    forEach([&](int x, int y, int z, std::size_t c) {
+      if (indicator(x, y, z)) _ids[c] = id;
+    });
+  }
+
+  /// Update PARAMETER of the bulk dynamics (SuperLattice::setParameter).
+  template <typename PARAMETER>
+  void setParameter(T value)
+  {
+    if constexpr (std::is_same_v<PARAMETER, descriptors::OMEGA>) _params.omega = double(value);
+    else if constexpr (std::is_same_v<PARAMETER, collision::TRT::MAGIC>) _params.magic = double(value);
+    else if constexpr (std::is_same_v<PARAMETER, collision::LES::SMAGORINSKY>) _params.smago = double(value);
+    else static_assert(!sizeof(PARAMETER), "unsupported parameter");
+    if (_handle) detail::check(lbmh_set_bulk_params(_handle, &_params));
+  }
+
+  template <typename PARAMETER>
+  void setParameter(std::array<T, 3> value)
+  {
+    static_assert(std::is_same_v<PARAMETER, descriptors::FORCE>, "vector parameter must be FORCE");
+    for (int d = 0; d < 3; ++d) _params.force[d] = double(value[d]);
+    if (_handle) detail::check(lbmh_set_bulk_params(_handle, &_params));
+  }
+
+  /// Fixed density and velocity of the BounceBackVelocity cells defined last
+  /// (defineRhoU / defineU on their material, cavity3d.cpp:124-136).
+  void defineWall(T rho, std::array<T, 3> u)
+  {
+    if (!_pendingWall) throw std::logic_error("defineWall needs a BounceBackVelocity group first");
+    _walls.push_back({int(_pendingWall), double(rho), {double(u[0]), double(u[1]), double(u[2])}});
+    if (_handle) applyWalls();
+  }
+
+  /// Equilibrium at (rho, u) on the indicated cells (SuperLattice::iniEquilibrium).
+  void iniEquilibrium(const Indicator& indicator, const ScalarField& rho, const VectorField& u)
+  {
+    const std::size_t n = ncore();
+    if (_rhoU.empty()) {
+      _rhoU.assign(4 * n, 0.0);
+      for (std::size_t c = 0; c < n; ++c) _rhoU[c] = 1.0;
+    }
+    forEach([&](int x, int y, int z, std::size_t c) {
+      if (!indicator(x, y, z)) return;
+      _rhoU[c] = rho(x, y, z);
+      const auto v = u(x, y, z);
+      for (int d = 0; d < 3; ++d) _rhoU[(d + 1) * n + c] = v[d];
+    });
+    if (_handle) detail::check(lbmh_ini_equilibrium(_handle, _rhoU.data()));
+  }
+
+  /// Allocate the device block and upload ids, parameters and the initial
+  /// state (SuperLattice::initialize + setProcessingContext(Simulation)).
+  void initialize()
+  {
+    if (_handle) return;
+    lbmh_lattice_desc desc{};
+    desc.q = q;
+    desc.nx = _extent[0];
+    desc.ny = _extent[1];
+    desc.nz = _extent[2];
+    for (int d = 0; d < 3; ++d) desc.periodic[d] = _periodic[d] ? 1 : 0;
+    desc.storage = _storage;
+    desc.bulk_kind = _kind >= 0 ? _kind : LBMH_BGK;
+    desc.device = _device;
+    detail::check(lbmh_lattice_create(&desc, &_handle));
+    detail::check(lbmh_set_dynamics_ids(_handle, _ids.data()));
+    detail::check(lbmh_set_bulk_params(_handle, &_params));
+    applyWalls();
+    if (!_rhoU.empty()) detail::check(lbmh_ini_equilibrium(_handle, _rhoU.data()));
+  }
+
+  /// One time step (SuperLattice::collideAndStream), asynchronous.
+  void collideAndStream(void* stream = nullptr)
+  {
+    if (!_handle) initialize();
+    detail::check(lbmh_step(_handle, 1, stream));
+    _mirrorValid = false;
+  }
+
+  /// Evaluation: download populations to the host mirror; Simulation: upload it.
+  void setProcessingContext(ProcessingContext context)
+  {
+    if (!_handle) initialize();
+    _mirror.resize(std::size_t(q) * ncore());
+    if (context == ProcessingContext::Evaluation) {
+      detail::check(lbmh_populations_download(_handle, _mirror.data(), LBMH_F64));
+      _mirrorValid = true;
+    } else {
+      if (!_mirrorValid) throw std::logic_error("no host populations to upload");
+      detail::check(lbmh_populations_upload(_handle, _mirror.data(), LBMH_F64));
+    }
+  }
+
+  ConstCellView<T> get(int x, int y, int z) const
+  {
+    if (!_mirrorValid) throw std::logic_error("call setProcessingContext(Evaluation) first");
+    return ConstCellView<T>(_mirror, ncore(), index(x, y, z));
+  }
+
+  /// Mutable host mirror, [q][nx][ny][nz], shifted values.
+  std::vector<double>& populations() { return _mirror; }
+
+  void synchronize() const { if (_handle) detail::check(lbmh_synchronize(_handle)); }
+  void statisticsOff() {}
+
+  /// FP64 {sum rho, sum j} over bulk cells.
+  std::array<double, 4> conservedMoments() const
+  {
+    std::array<double, 4> out{};
+    detail::check(lbmh_reduce_moments(_handle, out.data()));
+    return out;
+  }
+
+  std::size_t index(int x, int y, int z) const
+  {
+    return (std::size_t(x) * _extent[1] + y) * _extent[2] + z;
+  }
+
+  lbmh_lattice* handle() const { return _handle; }
+
+ private:
+  struct Wall { int id; double rho; double u[3]; };
+
+  template <typename F>
+  void forEach(F&& f) const
+  {
+    for (int x = 0; x < _extent[0]; ++x)
+      for (int y = 0; y < _extent[1]; ++y)
+        for (int z = 0; z < _extent[2]; ++z) f(x, y, z, index(x, y, z));
+  }
+
+  void requireSetup(const char* what) const
+  {
+    if (_handle) throw std::logic_error(std::string(what) + " after initialize()");
+  }
+
+  void applyWalls()
+  {
+    for (const auto& w : _walls) detail::check(lbmh_set_wall(_handle, w.id, w.rho, w.u));
+  }
+
+  std::array<int, 3> _extent;
+  std::array<bool, 3> _periodic;
+  int _device;
+  int _storage;
+  int _kind = -1;
+  std::vector<uint8_t> _ids;
+  std::vector<double> _rhoU;
+  std::vector<double> _mirror;
+  bool _mirrorValid = false;
+  lbmh_bulk_params _params{1.0, 0.25, 0.0, {0.0, 0.0, 0.0}};
+  std::vector<Wall> _walls;
+  uint8_t _pendingWall = 0;
+  int _wallsDefined = 0;
+  lbmh_lattice* _handle = nullptr;
+};
+
+}  // namespace hip
+}  // namespace olb

@@ -134,6 +134,23 @@ def lid_driven_cavity(shape=(32, 32, 32), tau=0.6, lid_u=0.05, q=19, kind="bgk")
                 1.0 / tau, walls={3: (1.0, (lid_u, 0.0, 0.0))})
 
 
+def cavity_benchmark(size=100, q=19):
+    """The cavity of OpenLB's benchmark harness in its bounce-back-lid variant
+    (examples/laminar/cavity3dBenchmark/cavity3d.cpp:39-57, 85-87, 154-250):
+    (size+1)^3 cells, BounceBack boundary, the whole top face (y = size) a
+    BounceBackVelocity lid at lattice velocity 0.01, BGK with
+    tau = 1/2 + 3 * 1e-5 * size (lattice velocity 0.01, Re 1000)."""
+    n = size + 1
+    ids = np.full((n, n, n), 2, dtype=np.uint8)
+    ids[1:-1, 1:-1, 1:-1] = 1
+    ids[:, -1, :] = 3
+    rho_u = np.zeros((4, n, n, n))
+    rho_u[0] = 1.0
+    tau = 0.5 + 3.0 * 1e-5 * size
+    return Case("cavity_benchmark", q, "bgk", (n, n, n), (0, 0, 0), ids, rho_u, 1.0 / tau,
+                walls={3: (1.0, (0.01, 0.0, 0.0))})
+
+
 def turbulent_channel(shape=(384, 192, 192), tau=0.505, smago=0.14, u_max=0.05,
                       noise=0.01, seed=20261203, q=27):
     """Smagorinsky channel. It is periodic in x and z, with BounceBack walls at y = 0 and

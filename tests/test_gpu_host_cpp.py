@@ -1,0 +1,48 @@
+"""The C++ host layer (openlb_amd/host/olb_hip.h) drives the C ABI the way
+OpenLB's benchmark harness drives SuperLattice. This runs its port of
+examples/laminar/cavity3dBenchmark and checks its populations against the
+oracle on the same cavity."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import golden_cases as G
+from oracle import oracle as O
+from openlb_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                   "openlb_amd", "_lib", "cavity3d_benchmark")
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_cpp_cavity_benchmark_matches_oracle(gpu_available, storage, tmp_path):
+    size, steps = 24, 50
+    dump = tmp_path / "pop.bin"
+    cmd = [EXE, "--size", str(size), "--steps", str(steps), "--dump", str(dump)]
+    if storage == "f64":
+        cmd.append("--fp64")
+    out = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    fields = [s.strip() for s in out.stdout.strip().splitlines()[-1].split(",")]
+    assert fields[:5] == [str(size), str(steps), "1", "1", "1"]
+    assert float(fields[5]) > 0
+    case = W.cavity_benchmark(size)
+    got = np.fromfile(dump, dtype=np.float64).reshape((19,) + case.shape)
+    ora = O.OracleLattice(case, storage)
+    ora.upload(O.initial_populations(case, storage))
+    ora.step(10 + steps, threads=8)
+    err = G.rel_error(got, ora.download(), 19, case.ids == 1)
+    print(f"C++ host layer cavity {storage}: rel {err:.3e}", flush=True)
+    assert err <= (1e-13 if storage == "f64" else 1e-5)
+
+
+def test_cpp_layer_reports_errors_as_exceptions(gpu_available):
+    """A misuse surfaces as an exception message and a non-zero exit, not a crash."""
+    out = subprocess.run([EXE, "--size", "4", "--steps", "1", "--tau", "0.4"], capture_output=True,
+                         text=True, timeout=60)
+    assert out.returncode == 1
+    assert "omega must lie in (0, 2)" in out.stderr
