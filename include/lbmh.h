@@ -165,6 +165,68 @@ int lbmh_get_launch_config(const lbmh_lattice* lattice, lbmh_launch_config* conf
  * more copy of the lattice. */
 int lbmh_autotune(lbmh_lattice* lattice, int trial_steps, lbmh_launch_config* chosen);
 
+/* ---- Ensembles: many independent lattices advanced by one launch per step ----
+ *
+ * No single OpenLB counterpart.  The reference runs uncertainty-quantification
+ * samples one SuperLattice at a time (examples/uncertaintyQuantification/
+ * cavity3d/cavity3d.cpp:110-148); an ensemble holds them all on one GPU.  Each
+ * member is one block lattice (same semantics as lbmh_lattice: padded PS
+ * columns, dynamics ids 0..15, per-member bulk parameters and walls).  Members
+ * share one descriptor, storage type and bulk kind; extents, periodicity,
+ * parameters and step budgets differ.  Device memory comes from two arenas
+ * sized at creation, so adding and removing members never allocates or frees
+ * device memory.  lbmh_ensemble_add / _remove / _set_steps wait for all work
+ * queued on the device before they change the member table. */
+typedef struct lbmh_ensemble lbmh_ensemble;
+
+typedef struct lbmh_ensemble_desc {
+  int q;                    /* 19 or 27                                        */
+  int storage;              /* LBMH_F64 / LBMH_F32 / LBMH_F16                  */
+  int bulk_kind;            /* LBMH_BGK ... LBMH_SMAGORINSKY                   */
+  int device;
+  int max_members;          /* member slots                                    */
+  size_t population_bytes;  /* population arena                                */
+  size_t cell_capacity;     /* id arena (cells over all members)               */
+} lbmh_ensemble_desc;
+
+int lbmh_ensemble_create(const lbmh_ensemble_desc* desc, lbmh_ensemble** out);
+int lbmh_ensemble_destroy(lbmh_ensemble* ensemble);
+
+/* Device bytes one member of this extent takes from the arenas (populations,
+ * ids); the admission check of a serving loop. */
+int lbmh_ensemble_member_bytes(const lbmh_ensemble* ensemble, const int extent[3], size_t* population_bytes,
+                               size_t* id_bytes);
+
+/* Add a member: extent, periodicity, ids [nx][ny][nz] (values 0..15), bulk
+ * parameters, and the number of steps it should run (< 0: until removed).
+ * Populations start at zero (f_i = t_i).  Returns LBMH_E_MEMORY when the
+ * arenas or the slots are full. */
+int lbmh_ensemble_add(lbmh_ensemble* ensemble, const int extent[3], const int periodic[3], const uint8_t* ids,
+                      const lbmh_bulk_params* params, int64_t steps, int* member);
+int lbmh_ensemble_remove(lbmh_ensemble* ensemble, int member);
+int lbmh_ensemble_set_wall(lbmh_ensemble* ensemble, int member, int id, double rho, const double u[3]);
+int lbmh_ensemble_set_steps(lbmh_ensemble* ensemble, int member, int64_t steps);
+/* steps completed and steps still to run (< 0: unbounded) */
+int lbmh_ensemble_status(lbmh_ensemble* ensemble, int member, int64_t* steps_done, int64_t* steps_left);
+
+int lbmh_ensemble_ini_equilibrium(lbmh_ensemble* ensemble, int member, const double* rho_u);
+int lbmh_ensemble_upload(lbmh_ensemble* ensemble, int member, const void* host, int host_dtype);
+int lbmh_ensemble_download(lbmh_ensemble* ensemble, int member, void* host, int host_dtype);
+int lbmh_ensemble_moments(lbmh_ensemble* ensemble, int member, double* rho_u);
+
+/* Advance every member that still has steps by up to n_steps steps: one
+ * launch of the fused step kernel over all members' cells and one launch of
+ * the stream/count kernel per step, asynchronously on `stream`.  The _graph
+ * variant captures the step loop in a HIP graph (rebuilt when membership
+ * changes) and replays it. */
+int lbmh_ensemble_step(lbmh_ensemble* ensemble, int64_t n_steps, void* stream);
+int lbmh_ensemble_step_graph(lbmh_ensemble* ensemble, int64_t n_steps, void* stream);
+
+int lbmh_ensemble_set_launch_config(lbmh_ensemble* ensemble, const lbmh_launch_config* config);
+/* members, cells in the launch, arena bytes in use and capacity */
+int lbmh_ensemble_usage(const lbmh_ensemble* ensemble, int* members, size_t* cells, size_t* bytes_used,
+                        size_t* bytes_capacity);
+
 /* Bytes of device memory the lattice holds. */
 int lbmh_lattice_footprint(const lbmh_lattice* lattice, size_t* bytes);
 

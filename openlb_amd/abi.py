@@ -26,6 +26,11 @@ EXPORTS = (
     "lbmh_synchronize", "lbmh_lattice_footprint", "lbmh_abi_version", "lbmh_device_count",
     "lbmh_device_arch", "lbmh_last_error", "lbmh_set_launch_config", "lbmh_get_launch_config",
     "lbmh_autotune",
+    "lbmh_ensemble_create", "lbmh_ensemble_destroy", "lbmh_ensemble_member_bytes", "lbmh_ensemble_add",
+    "lbmh_ensemble_remove", "lbmh_ensemble_set_wall", "lbmh_ensemble_set_steps", "lbmh_ensemble_status",
+    "lbmh_ensemble_ini_equilibrium", "lbmh_ensemble_upload", "lbmh_ensemble_download",
+    "lbmh_ensemble_moments", "lbmh_ensemble_step", "lbmh_ensemble_step_graph",
+    "lbmh_ensemble_set_launch_config", "lbmh_ensemble_usage",
 )
 
 
@@ -51,6 +56,14 @@ class BulkParams(ctypes.Structure):
 
 class LaunchConfig(ctypes.Structure):
     _fields_ = [("lds_bytes", ctypes.c_int), ("xcd_group", ctypes.c_int)]
+
+
+class EnsembleDesc(ctypes.Structure):
+    _fields_ = [
+        ("q", ctypes.c_int), ("storage", ctypes.c_int), ("bulk_kind", ctypes.c_int),
+        ("device", ctypes.c_int), ("max_members", ctypes.c_int),
+        ("population_bytes", ctypes.c_size_t), ("cell_capacity", ctypes.c_size_t),
+    ]
 
 
 class LbmhError(RuntimeError):
@@ -93,6 +106,25 @@ def load() -> ctypes.CDLL:
     L.lbmh_set_launch_config.argtypes = [vp, ctypes.POINTER(LaunchConfig)]
     L.lbmh_get_launch_config.argtypes = [vp, ctypes.POINTER(LaunchConfig)]
     L.lbmh_autotune.argtypes = [vp, i32, ctypes.POINTER(LaunchConfig)]
+    i3 = ctypes.c_int * 3
+    szp = ctypes.POINTER(ctypes.c_size_t)
+    L.lbmh_ensemble_create.argtypes = [ctypes.POINTER(EnsembleDesc), ctypes.POINTER(vp)]
+    L.lbmh_ensemble_destroy.argtypes = [vp]
+    L.lbmh_ensemble_member_bytes.argtypes = [vp, i3, szp, szp]
+    L.lbmh_ensemble_add.argtypes = [vp, i3, i3, vp, ctypes.POINTER(BulkParams), i64,
+                                    ctypes.POINTER(i32)]
+    L.lbmh_ensemble_remove.argtypes = [vp, i32]
+    L.lbmh_ensemble_set_wall.argtypes = [vp, i32, i32, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
+    L.lbmh_ensemble_set_steps.argtypes = [vp, i32, i64]
+    L.lbmh_ensemble_status.argtypes = [vp, i32, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+    L.lbmh_ensemble_ini_equilibrium.argtypes = [vp, i32, vp]
+    L.lbmh_ensemble_upload.argtypes = [vp, i32, vp, i32]
+    L.lbmh_ensemble_download.argtypes = [vp, i32, vp, i32]
+    L.lbmh_ensemble_moments.argtypes = [vp, i32, vp]
+    L.lbmh_ensemble_step.argtypes = [vp, i64, vp]
+    L.lbmh_ensemble_step_graph.argtypes = [vp, i64, vp]
+    L.lbmh_ensemble_set_launch_config.argtypes = [vp, ctypes.POINTER(LaunchConfig)]
+    L.lbmh_ensemble_usage.argtypes = [vp, ctypes.POINTER(i32), szp, szp, szp]
     L.lbmh_last_error.restype = ctypes.c_char_p
     for name in EXPORTS:
         if name != "lbmh_last_error":

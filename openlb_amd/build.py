@@ -16,7 +16,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 LIB_DIR = os.path.join(PKG, "_lib")
 LIB_PATH = os.path.join(LIB_DIR, "liblbmh.so")
-SOURCES = [os.path.join(PKG, "csrc", "lbmh.hip")]
+SOURCES = [os.path.join(PKG, "csrc", "lbmh.hip"), os.path.join(PKG, "csrc", "lbmh_ensemble.hip")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

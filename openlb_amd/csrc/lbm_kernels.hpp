@@ -139,15 +139,12 @@ __device__ __forceinline__ void push_periodic_halo(const StepArgs<Q, V>& a, S* p
   }
 }
 
-// One time step of a block: collide every core cell in place, refresh the
+// Time step of core cell c of one block: collide in place, refresh the
 // periodic overlap.  The caller advances the shifts afterwards.
 template <int Q, typename S, int KIND, bool IDS>
-__global__ __launch_bounds__(kBlock) void collide_stream_kernel(StepArgs<Q, typename StorageTraits<S>::V> a)
+__device__ __forceinline__ void collide_stream_cell(const StepArgs<Q, typename StorageTraits<S>::V>& a, uint32_t c)
 {
   using V = typename StorageTraits<S>::V;
-  uint32_t b = blockIdx.x;
-  if (a.xcd_group) b = remap_xcd(b, gridDim.x, a.xcd_group);
-  const uint32_t c = b * kBlock + threadIdx.x;
   if (c >= a.ncore) return;
   const uint32_t t = a.div_z.div(c);
   const uint32_t z = c - t * a.nz;
@@ -188,6 +185,15 @@ __global__ __launch_bounds__(kBlock) void collide_stream_kernel(StepArgs<Q, type
     }
   }
   if (edge) push_periodic_halo<Q, S, V>(a, pop, f, int(x), int(y), int(z));
+}
+
+// One time step of a block, one lane per core cell.
+template <int Q, typename S, int KIND, bool IDS>
+__global__ __launch_bounds__(kBlock) void collide_stream_kernel(StepArgs<Q, typename StorageTraits<S>::V> a)
+{
+  uint32_t b = blockIdx.x;
+  if (a.xcd_group) b = remap_xcd(b, gridDim.x, a.xcd_group);
+  collide_stream_cell<Q, S, KIND, IDS>(a, b * kBlock + threadIdx.x);
 }
 
 // ---- data movement and evaluation kernels (not on the time-step path) ----
@@ -346,7 +352,8 @@ __global__ __launch_bounds__(kBlock) void reduce_moments_kernel(double* partial,
 }
 
 // Second level: one workgroup sums the partials in a fixed order.
-__global__ __launch_bounds__(kBlock) void reduce_partials_kernel(double* out, const double* partial, uint32_t nparts)
+static __global__ __launch_bounds__(kBlock) void reduce_partials_kernel(double* out, const double* partial,
+                                                                        uint32_t nparts)
 {
   __shared__ double lds[4][kBlock / 64];
   double acc[4] = {0, 0, 0, 0};

@@ -15,39 +15,11 @@
 
 #include "lbm_kernels.hpp"
 #include "lbmh.h"
+#include "lbmh_common.hpp"
 
 using namespace lbmh;
 
-namespace {
-
-thread_local std::string g_error;
-
-int fail(int code, const std::string& msg)
-{
-  g_error = msg;
-  return code;
-}
-
-int hip_fail(hipError_t e, const char* where)
-{
-  return fail(LBMH_E_DEVICE, std::string(where) + ": " + hipGetErrorString(e));
-}
-
-#define LBMH_HIP(call)                                   \
-  do {                                                   \
-    hipError_t e_ = (call);                              \
-    if (e_ != hipSuccess) return hip_fail(e_, #call);    \
-  } while (0)
-
-size_t storage_size(int s) { return s == LBMH_F64 ? 8 : (s == LBMH_F32 ? 4 : 2); }
-
-// RAII device buffer for temporaries.
-struct DeviceBuffer {
-  void* p = nullptr;
-  ~DeviceBuffer() { if (p) (void)hipFree(p); }
-};
-
-}  // namespace
+thread_local std::string lbmh::g_error;
 
 struct lbmh_lattice {
   int device = 0;
@@ -104,32 +76,7 @@ Shifts<Q> shifts_of(const lbmh_lattice* L)
 template <typename V>
 BulkParams<V> bulk_params(const lbmh_lattice* L)
 {
-  const lbmh_bulk_params& p = L->params;
-  BulkParams<V> b;
-  const double omega = p.omega;
-  const double tau = 1.0 / omega;
-  b.omega = V(omega);
-  b.omega2 = V(1.0 / (p.magic / (tau - 0.5) + 0.5));
-  b.guo = V(1.0 - omega * 0.5);
-  for (int d = 0; d < 3; ++d) {
-    b.force[d] = V(p.force[d]);
-    b.half_force[d] = V(p.force[d] * 0.5);
-  }
-  if (L->q == 19) {
-    // constants of OpenLB's generated D3Q19 Smagorinsky kernel
-    // (src/cse/dynamics/d7a4bdcd...cse.h:110)
-    b.sm_a = V(3.00000046417339);
-    b.sm_b = V(p.smago * p.smago / std::sqrt(2.0));
-    b.sm_k = V(0.0277777691819762);
-  } else {
-    // collisionLES.h:74-80: tau_eff = tau/2 + sqrt(tau^2 + 18 sqrt2 C^2 |Pi|/rho)/2
-    b.sm_a = V(0.5);
-    b.sm_b = V(18.0 * std::sqrt(2.0) * p.smago * p.smago);
-    b.sm_k = V(1.0);
-  }
-  b.tau = V(tau);
-  b.half_tau = V(0.5 * tau);
-  return b;
+  return derive_bulk_params<V>(L->q, L->params);
 }
 
 template <int Q, typename S, int KIND>
@@ -211,20 +158,6 @@ int with_types(const lbmh_lattice* L, F&& f)
   return f(std::integral_constant<int, 27>(), _Float16());
 }
 
-struct DeviceGuard {
-  int prev = -1;
-  explicit DeviceGuard(int dev)
-  {
-    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
-    if (prev != dev) (void)hipSetDevice(dev);
-  }
-  ~DeviceGuard()
-  {
-    if (prev >= 0) (void)hipSetDevice(prev);
-  }
-};
-
-uint32_t grid_for(uint32_t n, uint32_t block) { return (n + block - 1) / block; }
 
 }  // namespace
 
@@ -348,11 +281,7 @@ int lbmh_set_dynamics_ids(lbmh_lattice* L, const uint8_t* ids)
 int lbmh_set_bulk_params(lbmh_lattice* L, const lbmh_bulk_params* p)
 {
   if (!L || !p) return fail(LBMH_E_INVALID, "null argument");
-  if (!(p->omega > 0.0 && p->omega < 2.0)) return fail(LBMH_E_INVALID, "omega must lie in (0, 2)");
-  if ((L->kind == LBMH_TRT || L->kind == LBMH_FORCED_TRT) && !(p->magic > 0.0))
-    return fail(LBMH_E_INVALID, "TRT magic parameter must be positive");
-  if (L->kind == LBMH_SMAGORINSKY && !(p->smago >= 0.0))
-    return fail(LBMH_E_INVALID, "Smagorinsky constant must be non-negative");
+  if (!valid_bulk_params(L->kind, *p)) return LBMH_E_INVALID;
   L->params = *p;
   return LBMH_OK;
 }

@@ -1,0 +1,675 @@
+// C ABI of lattice ensembles (declarations in include/lbmh.h).
+//
+// Members live in two device arenas (populations, dynamics ids) carved at
+// creation, so admitting or retiring a member never calls hipMalloc/hipFree
+// (hipFree would also wait for the whole device).  The member table and the
+// chunk -> member map are device arrays; the host keeps the arena bookkeeping
+// and rebuilds the map when membership changes.  Shifts and step counters are
+// advanced on the device, so a step loop can be captured in a HIP graph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <new>
+#include <vector>
+
+#include "lbm_ensemble.hpp"
+#include "lbmh.h"
+#include "lbmh_common.hpp"
+
+using namespace lbmh;
+
+namespace {
+
+constexpr uint64_t kAlign = 256;
+
+uint64_t align_up(uint64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+// First-fit allocator over [0, capacity) with coalescing of free ranges.
+class Arena {
+ public:
+  explicit Arena(uint64_t capacity = 0) : capacity_(capacity)
+  {
+    if (capacity) free_[0] = capacity;
+  }
+  bool alloc(uint64_t bytes, uint64_t* off)
+  {
+    bytes = align_up(bytes);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= bytes) {
+        *off = it->first;
+        const uint64_t rest = it->second - bytes, start = it->first + bytes;
+        free_.erase(it);
+        if (rest) free_[start] = rest;
+        used_ += bytes;
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(uint64_t off, uint64_t bytes)
+  {
+    bytes = align_up(bytes);
+    used_ -= bytes;
+    auto it = free_.emplace(off, bytes).first;
+    auto next = std::next(it);
+    if (next != free_.end() && it->first + it->second == next->first) {
+      it->second += next->second;
+      free_.erase(next);
+    }
+    if (it != free_.begin()) {
+      auto prev = std::prev(it);
+      if (prev->first + prev->second == it->first) {
+        prev->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  uint64_t used() const { return used_; }
+  uint64_t capacity() const { return capacity_; }
+
+ private:
+  uint64_t capacity_ = 0, used_ = 0;
+  std::map<uint64_t, uint64_t> free_;
+};
+
+struct Member {
+  bool active = false;
+  int n[3] = {0, 0, 0};
+  int periodic[3] = {0, 0, 0};
+  uint64_t pop_off = 0, pop_bytes = 0;   // bytes in the population arena
+  uint64_t ids_off = 0, ids_bytes = 0;
+  uint32_t ncore = 0;
+  lbmh_bulk_params params{};
+};
+
+}  // namespace
+
+struct lbmh_ensemble {
+  int device = 0, q = 19, storage = LBMH_F32, kind = LBMH_BGK;
+  int max_members = 0;
+  void* d_pop = nullptr;
+  uint8_t* d_ids = nullptr;
+  double* d_walls = nullptr;
+  void* d_table = nullptr;
+  uint32_t* d_chunk = nullptr;
+  uint32_t chunk_capacity = 0, total_chunks = 0;
+  size_t entry_bytes = 0;
+  Arena pop_arena, id_arena;
+  std::vector<Member> members;
+  std::vector<unsigned char> h_table;   // host mirror of the member table
+  lbmh_launch_config launch{0, 4};
+  uint64_t version = 1;
+  hipStream_t capture_stream = nullptr;
+  hipGraphExec_t graph = nullptr;
+  uint64_t graph_version = 0;
+  int64_t graph_steps = 0;
+};
+
+namespace {
+
+template <int Q>
+EnsembleEntry<Q>& entry(lbmh_ensemble* E, int m)
+{
+  return *reinterpret_cast<EnsembleEntry<Q>*>(E->h_table.data() + size_t(m) * E->entry_bytes);
+}
+
+template <typename F>
+int with_q(const lbmh_ensemble* E, F&& f)
+{
+  return E->q == 19 ? f(std::integral_constant<int, 19>()) : f(std::integral_constant<int, 27>());
+}
+
+template <typename F>
+int with_types(const lbmh_ensemble* E, F&& f)
+{
+  if (E->q == 19) {
+    if (E->storage == LBMH_F64) return f(std::integral_constant<int, 19>(), double());
+    if (E->storage == LBMH_F32) return f(std::integral_constant<int, 19>(), float());
+    return f(std::integral_constant<int, 19>(), _Float16());
+  }
+  if (E->storage == LBMH_F64) return f(std::integral_constant<int, 27>(), double());
+  if (E->storage == LBMH_F32) return f(std::integral_constant<int, 27>(), float());
+  return f(std::integral_constant<int, 27>(), _Float16());
+}
+
+int pull_table(lbmh_ensemble* E)
+{
+  LBMH_HIP(hipDeviceSynchronize());
+  LBMH_HIP(hipMemcpy(E->h_table.data(), E->d_table, E->h_table.size(), hipMemcpyDeviceToHost));
+  return LBMH_OK;
+}
+
+// Assign chunk ranges to active members in slot order and upload table + map.
+int push_table(lbmh_ensemble* E)
+{
+  std::vector<uint32_t> map;
+  map.reserve(E->total_chunks);
+  const int rc = with_q(E, [&](auto qc) -> int {
+    constexpr int Q = decltype(qc)::value;
+    for (int m = 0; m < E->max_members; ++m) {
+      EnsembleEntry<Q>& e = entry<Q>(E, m);
+      if (!E->members[m].active) {
+        e.chunk_begin = 0;
+        e.steps_left = 0;
+        continue;
+      }
+      e.chunk_begin = uint32_t(map.size());
+      const uint32_t chunks = grid_for(e.ncore, kBlock);
+      map.insert(map.end(), chunks, uint32_t(m));
+    }
+    return LBMH_OK;
+  });
+  if (rc != LBMH_OK) return rc;
+  if (map.size() > E->chunk_capacity) return fail(LBMH_E_MEMORY, "chunk map capacity exceeded");
+  E->total_chunks = uint32_t(map.size());
+  LBMH_HIP(hipMemcpy(E->d_table, E->h_table.data(), E->h_table.size(), hipMemcpyHostToDevice));
+  if (!map.empty()) LBMH_HIP(hipMemcpy(E->d_chunk, map.data(), map.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  E->version += 1;
+  return LBMH_OK;
+}
+
+int check_member(const lbmh_ensemble* E, int m)
+{
+  if (!E) return fail(LBMH_E_INVALID, "null ensemble");
+  if (m < 0 || m >= E->max_members || !E->members[m].active) return fail(LBMH_E_INVALID, "no such member");
+  return LBMH_OK;
+}
+
+template <int Q>
+CoreGeom geom_of(const EnsembleEntry<Q>& e)
+{
+  CoreGeom g;
+  g.ncore = e.ncore;
+  g.nx = e.nx;
+  g.ny = e.ny;
+  g.nz = e.nz;
+  g.sx = e.sx;
+  g.sy = e.sy;
+  g.n_padded = e.n_padded;
+  g.stride = e.stride;
+  g.div_z = e.div_z;
+  g.div_y = e.div_y;
+  return g;
+}
+
+template <int Q>
+Shifts<Q> shifts_of(const EnsembleEntry<Q>& e)
+{
+  Shifts<Q> s;
+  for (int i = 0; i < Q; ++i) s.s[i] = e.shift[i];
+  return s;
+}
+
+int launch_steps(lbmh_ensemble* E, int64_t n, hipStream_t stream)
+{
+  const unsigned lds = unsigned(E->launch.lds_bytes);
+  const uint32_t group = uint32_t(E->launch.xcd_group);
+  const uint32_t adv_grid = grid_for(uint32_t(E->max_members), 256);
+  return with_types(E, [&](auto qc, auto s) -> int {
+    constexpr int Q = decltype(qc)::value;
+    using S = decltype(s);
+    auto* table = static_cast<EnsembleEntry<Q>*>(E->d_table);
+    S* arena = static_cast<S*>(E->d_pop);
+    for (int64_t k = 0; k < n; ++k) {
+      if (E->total_chunks) {
+        switch (E->kind) {
+        case LBMH_BGK:
+          hipLaunchKernelGGL((ensemble_step_kernel<Q, S, kBGK>), dim3(E->total_chunks), dim3(kBlock), lds, stream,
+                             table, E->d_chunk, arena, E->d_ids, E->d_walls, group);
+          break;
+        case LBMH_FORCED_BGK:
+          hipLaunchKernelGGL((ensemble_step_kernel<Q, S, kForcedBGK>), dim3(E->total_chunks), dim3(kBlock), lds,
+                             stream, table, E->d_chunk, arena, E->d_ids, E->d_walls, group);
+          break;
+        case LBMH_TRT:
+          hipLaunchKernelGGL((ensemble_step_kernel<Q, S, kTRT>), dim3(E->total_chunks), dim3(kBlock), lds, stream,
+                             table, E->d_chunk, arena, E->d_ids, E->d_walls, group);
+          break;
+        case LBMH_FORCED_TRT:
+          hipLaunchKernelGGL((ensemble_step_kernel<Q, S, kForcedTRT>), dim3(E->total_chunks), dim3(kBlock), lds,
+                             stream, table, E->d_chunk, arena, E->d_ids, E->d_walls, group);
+          break;
+        default:
+          hipLaunchKernelGGL((ensemble_step_kernel<Q, S, kSmagorinsky>), dim3(E->total_chunks), dim3(kBlock), lds,
+                             stream, table, E->d_chunk, arena, E->d_ids, E->d_walls, group);
+          break;
+        }
+        LBMH_HIP(hipGetLastError());
+      }
+      hipLaunchKernelGGL((ensemble_advance_kernel<Q>), dim3(adv_grid), dim3(256), 0, stream, table,
+                         uint32_t(E->max_members));
+      LBMH_HIP(hipGetLastError());
+    }
+    return LBMH_OK;
+  });
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbmh_ensemble_create(const lbmh_ensemble_desc* d, lbmh_ensemble** out)
+{
+  if (!d || !out) return fail(LBMH_E_INVALID, "null argument");
+  *out = nullptr;
+  if (d->q != 19 && d->q != 27) return fail(LBMH_E_INVALID, "q must be 19 or 27");
+  if (d->storage < LBMH_F64 || d->storage > LBMH_F16) return fail(LBMH_E_INVALID, "unknown storage type");
+  if (d->bulk_kind < LBMH_BGK || d->bulk_kind > LBMH_SMAGORINSKY) return fail(LBMH_E_INVALID, "unknown bulk kind");
+  if (d->max_members < 1 || d->max_members > (1 << 20)) return fail(LBMH_E_INVALID, "max_members out of range");
+  if (d->population_bytes < kAlign || d->cell_capacity < 1) return fail(LBMH_E_INVALID, "arena sizes must be positive");
+  auto* E = new (std::nothrow) lbmh_ensemble();
+  if (!E) return fail(LBMH_E_MEMORY, "host allocation failed");
+  E->device = d->device;
+  E->q = d->q;
+  E->storage = d->storage;
+  E->kind = d->bulk_kind;
+  E->max_members = d->max_members;
+  E->entry_bytes = align_up(E->q == 19 ? sizeof(EnsembleEntry<19>) : sizeof(EnsembleEntry<27>));
+  E->members.resize(size_t(E->max_members));
+  E->h_table.assign(E->entry_bytes * size_t(E->max_members), 0);
+  E->pop_arena = Arena(align_up(d->population_bytes));
+  E->id_arena = Arena(align_up(d->cell_capacity) + kAlign * size_t(E->max_members));
+  E->chunk_capacity = uint32_t(d->cell_capacity / kBlock + size_t(E->max_members) + 1);
+  E->launch = d->storage == LBMH_F16 ? lbmh_launch_config{0, 8} : lbmh_launch_config{40960, 4};
+  DeviceGuard guard(E->device);
+  hipError_t e = hipMalloc(&E->d_pop, E->pop_arena.capacity());
+  if (e == hipSuccess) e = hipMalloc((void**)&E->d_ids, E->id_arena.capacity());
+  if (e == hipSuccess) e = hipMalloc((void**)&E->d_walls, sizeof(double) * 4 * kEnsembleWalls * E->max_members);
+  if (e == hipSuccess) e = hipMalloc(&E->d_table, E->h_table.size());
+  if (e == hipSuccess) e = hipMalloc((void**)&E->d_chunk, sizeof(uint32_t) * E->chunk_capacity);
+  if (e == hipSuccess) e = hipMemcpy(E->d_table, E->h_table.data(), E->h_table.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&E->capture_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    const int rc = hip_fail(e, "lbmh_ensemble_create");
+    lbmh_ensemble_destroy(E);
+    return e == hipErrorOutOfMemory ? fail(LBMH_E_MEMORY, g_error) : rc;
+  }
+  *out = E;
+  return LBMH_OK;
+}
+
+int lbmh_ensemble_destroy(lbmh_ensemble* E)
+{
+  if (!E) return LBMH_OK;
+  DeviceGuard guard(E->device);
+  (void)hipDeviceSynchronize();
+  if (E->graph) (void)hipGraphExecDestroy(E->graph);
+  if (E->capture_stream) (void)hipStreamDestroy(E->capture_stream);
+  if (E->d_pop) (void)hipFree(E->d_pop);
+  if (E->d_ids) (void)hipFree(E->d_ids);
+  if (E->d_walls) (void)hipFree(E->d_walls);
+  if (E->d_table) (void)hipFree(E->d_table);
+  if (E->d_chunk) (void)hipFree(E->d_chunk);
+  delete E;
+  return LBMH_OK;
+}
+
+int lbmh_ensemble_member_bytes(const lbmh_ensemble* E, const int extent[3], size_t* pop_bytes, size_t* id_bytes)
+{
+  if (!E || !extent) return fail(LBMH_E_INVALID, "null argument");
+  for (int d = 0; d < 3; ++d)
+    if (extent[d] < 1) return fail(LBMH_E_INVALID, "extent must be >= 1");
+  const uint64_t npad = uint64_t(extent[0] + 2) * (extent[1] + 2) * (extent[2] + 2);
+  const uint64_t elem = storage_size(E->storage);
+  const uint64_t stride = align_up(npad * elem) / elem;
+  if (pop_bytes) *pop_bytes = size_t(align_up(stride * elem * E->q));
+  if (id_bytes) *id_bytes = size_t(align_up(uint64_t(extent[0]) * extent[1] * extent[2]));
+  return LBMH_OK;
+}
+
+int lbmh_ensemble_add(lbmh_ensemble* E, const int extent[3], const int periodic[3], const uint8_t* ids,
+                      const lbmh_bulk_params* params, int64_t steps, int* member)
+{
+  if (!E || !extent || !periodic || !ids || !params || !member) return fail(LBMH_E_INVALID, "null argument");
+  *member = -1;
+  for (int d = 0; d < 3; ++d)
+    if (extent[d] < 1) return fail(LBMH_E_INVALID, "extent must be >= 1");
+  const uint64_t sx = uint64_t(extent[0]) + 2, sy = uint64_t(extent[1]) + 2, sz = uint64_t(extent[2]) + 2;
+  const uint64_t npad = sx * sy * sz;
+  if (npad >= (uint64_t(1) << 31)) return fail(LBMH_E_INVALID, "padded cell count must be below 2^31");
+  const uint32_t ncore = uint32_t(uint64_t(extent[0]) * extent[1] * extent[2]);
+  for (uint32_t c = 0; c < ncore; ++c)
+    if (ids[c] >= kEnsembleWalls) return fail(LBMH_E_INVALID, "ensemble dynamics ids must be below 16");
+  if (!valid_bulk_params(E->kind, *params)) return LBMH_E_INVALID;
+  int slot = -1;
+  for (int m = 0; m < E->max_members; ++m)
+    if (!E->members[m].active) {
+      slot = m;
+      break;
+    }
+  if (slot < 0) return fail(LBMH_E_MEMORY, "no free member slot");
+  size_t pop_bytes = 0, id_bytes = 0;
+  lbmh_ensemble_member_bytes(E, extent, &pop_bytes, &id_bytes);
+  Member mem;
+  if (!E->pop_arena.alloc(pop_bytes, &mem.pop_off)) return fail(LBMH_E_MEMORY, "population arena full");
+  if (!E->id_arena.alloc(id_bytes, &mem.ids_off)) {
+    E->pop_arena.release(mem.pop_off, pop_bytes);
+    return fail(LBMH_E_MEMORY, "id arena full");
+  }
+  mem.pop_bytes = pop_bytes;
+  mem.ids_bytes = id_bytes;
+  mem.active = true;
+  mem.ncore = ncore;
+  mem.params = *params;
+  for (int d = 0; d < 3; ++d) {
+    mem.n[d] = extent[d];
+    mem.periodic[d] = periodic[d] ? 1 : 0;
+  }
+  DeviceGuard guard(E->device);
+  auto undo = [&](int rc) {
+    E->pop_arena.release(mem.pop_off, mem.pop_bytes);
+    E->id_arena.release(mem.ids_off, mem.ids_bytes);
+    return rc;
+  };
+  int rc = pull_table(E);
+  if (rc != LBMH_OK) return undo(rc);
+  hipError_t e = hipMemset(static_cast<char*>(E->d_pop) + mem.pop_off, 0, pop_bytes);
+  if (e == hipSuccess) e = hipMemcpy(E->d_ids + mem.ids_off, ids, ncore, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    std::vector<double> walls(4 * kEnsembleWalls, 0.0);
+    for (int w = 0; w < kEnsembleWalls; ++w) walls[4 * w] = 1.0;
+    e = hipMemcpy(E->d_walls + size_t(slot) * 4 * kEnsembleWalls, walls.data(), walls.size() * sizeof(double),
+                  hipMemcpyHostToDevice);
+  }
+  if (e != hipSuccess) return undo(hip_fail(e, "lbmh_ensemble_add"));
+  const uint64_t elem = storage_size(E->storage);
+  with_q(E, [&](auto qc) -> int {
+    constexpr int Q = decltype(qc)::value;
+    EnsembleEntry<Q>& en = entry<Q>(E, slot);
+    std::memset(&en, 0, sizeof(en));
+    en.pop_offset = mem.pop_off / elem;
+    en.stride = align_up(npad * elem) / elem;
+    en.ids_offset = mem.ids_off;
+    en.n_padded = uint32_t(npad);
+    en.ncore = ncore;
+    en.nx = uint32_t(extent[0]);
+    en.ny = uint32_t(extent[1]);
+    en.nz = uint32_t(extent[2]);
+    en.sx = uint32_t(sy * sz);
+    en.sy = uint32_t(sz);
+    en.div_z = FastDiv(uint32_t(extent[2]));
+    en.div_y = FastDiv(uint32_t(extent[1]));
+    en.periodic = uint32_t(mem.periodic[0] | (mem.periodic[1] << 1) | (mem.periodic[2] << 2));
+    en.steps_left = steps;
+    en.steps_done = 0;
+    en.bp = derive_bulk_params<double>(E->q, *params);
+    for (int i = 0; i < Q; ++i) {
+      const int64_t dist = neighbor_distance<Q>(i, int64_t(sy * sz), int64_t(sz));
+      int64_t dm = dist % int64_t(npad);
+      if (dm < 0) dm += int64_t(npad);
+      en.dmod[i] = uint32_t(dm);
+      en.shift[i] = 0;
+    }
+    return LBMH_OK;
+  });
+  E->members[slot] = mem;
+  rc = push_table(E);
+  if (rc != LBMH_OK) {
+    E->members[slot].active = false;
+    return undo(rc);
+  }
+  *member = slot;
+  return LBMH_OK;
+}
+
+int lbmh_ensemble_remove(lbmh_ensemble* E, int m)
+{
+  int rc = check_member(E, m);
+  if (rc != LBMH_OK) return rc;
+  DeviceGuard guard(E->device);
+  rc = pull_table(E);
+  if (rc != LBMH_OK) return rc;
+  Member& mem = E->members[m];
+  E->pop_arena.release(mem.pop_off, mem.pop_bytes);
+  E->id_arena.release(mem.ids_off, mem.ids_bytes);
+  mem = Member();
+  with_q(E, [&](auto qc) -> int {
+    constexpr int Q = decltype(qc)::value;
+    std::memset(&entry<Q>(E, m), 0, sizeof(EnsembleEntry<Q>));
+    return LBMH_OK;
+  });
+  return push_table(E);
+}
+
+int lbmh_ensemble_set_wall(lbmh_ensemble* E, int m, int id, double rho, const double u[3])
+{
+  int rc = check_member(E, m);
+  if (rc != LBMH_OK) return rc;
+  if (!u) return fail(LBMH_E_INVALID, "null velocity");
+  if (id < LBMH_ID_FIRST_WALL || id >= kEnsembleWalls) return fail(LBMH_E_INVALID, "ensemble wall id must be in [3, 15]");
+  DeviceGuard guard(E->device);
+  const double w[4] = {rho, u[0], u[1], u[2]};
+  LBMH_HIP(hipDeviceSynchronize());
+  LBMH_HIP(hipMemcpy(E->d_walls + (size_t(m) * kEnsembleWalls + id) * 4, w, sizeof(w), hipMemcpyHostToDevice));
+  return LBMH_OK;
+}
+
+int lbmh_ensemble_set_steps(lbmh_ensemble* E, int m, int64_t steps)
+{
+  int rc = check_member(E, m);
+  if (rc != LBMH_OK) return rc;
+  DeviceGuard guard(E->device);
+  rc = pull_table(E);
+  if (rc != LBMH_OK) return rc;
+  return with_q(E, [&](auto qc) -> int {
+    constexpr int Q = decltype(qc)::value;
+    EnsembleEntry<Q>& en = entry<Q>(E, m);
+    en.steps_left = steps;
+    char* dst = static_cast<char*>(E->d_table) + size_t(m) * E->entry_bytes + offsetof(EnsembleEntry<Q>, steps_left);
+    LBMH_HIP(hipMemcpy(dst, &en.steps_left, sizeof(int64_t), hipMemcpyHostToDevice));
+    return LBMH_OK;
+  });
+}
+
+int lbmh_ensemble_status(lbmh_ensemble* E, int m, int64_t* done, int64_t* left)
+{
+  int rc = check_member(E, m);
+  if (rc != LBMH_OK) return rc;
+  DeviceGuard guard(E->device);
+  rc = pull_table(E);
+  if (rc != LBMH_OK) return rc;
+  return with_q(E, [&](auto qc) -> int {
+    constexpr int Q = decltype(qc)::value;
+    const EnsembleEntry<Q>& en = entry<Q>(E, m);
+    if (done) *done = en.steps_done;
+    if (left) *left = en.steps_left;
+    return LBMH_OK;
+  });
+}
+
+int lbmh_ensemble_ini_equilibrium(lbmh_ensemble* E, int m, const double* rho_u)
+{
+  int rc = check_member(E, m);
+  if (rc != LBMH_OK) return rc;
+  if (!rho_u) return fail(LBMH_E_INVALID, "null moments");
+  DeviceGuard guard(E->device);
+  rc = pull_table(E);
+  if (rc != LBMH_OK) return rc;
+  const Member& mem = E->members[m];
+  DeviceBuffer staging;
+  const size_t bytes = size_t(4) * mem.ncore * sizeof(double);
+  LBMH_HIP(hipMalloc(&staging.p, bytes));
+  LBMH_HIP(hipMemcpy(staging.p, rho_u, bytes, hipMemcpyHostToDevice));
+  const int forced = (E->kind == LBMH_FORCED_BGK || E->kind == LBMH_FORCED_TRT) ? 1 : 0;
+  rc = with_types(E, [&](auto qc, auto s) -> int {
+    constexpr int Q = decltype(qc)::value;
+    using S = decltype(s);
+    const EnsembleEntry<Q>& en = entry<Q>(E, m);
+    S* pop = static_cast<S*>(E->d_pop) + en.pop_offset;
+    hipLaunchKernelGGL((ini_equilibrium_kernel<Q, S>), dim3(grid_for(mem.ncore, 256)), dim3(256), 0, 0,
+                       (const double*)staging.p, E->d_ids + mem.ids_off, pop, geom_of(en), shifts_of(en),
+                       mem.params.force[0] * 0.5, mem.params.force[1] * 0.5, mem.params.force[2] * 0.5, forced);
+    LBMH_HIP(hipGetLastError());
+    return LBMH_OK;
+  });
+  if (rc != LBMH_OK) return rc;
+  LBMH_HIP(hipDeviceSynchronize());
+  return LBMH_OK;
+}
+
+int lbmh_ensemble_upload(lbmh_ensemble* E, int m, const void* host, int dtype)
+{
+  int rc = check_member(E, m);
+  if (rc != LBMH_OK) return rc;
+  if (!host || dtype < LBMH_F64 || dtype > LBMH_F16) return fail(LBMH_E_INVALID, "bad host buffer or dtype");
+  DeviceGuard guard(E->device);
+  rc = pull_table(E);
+  if (rc != LBMH_OK) return rc;
+  const Member& mem = E->members[m];
+  DeviceBuffer staging;
+  const size_t bytes = size_t(E->q) * mem.ncore * storage_size(dtype);
+  LBMH_HIP(hipMalloc(&staging.p, bytes));
+  LBMH_HIP(hipMemcpy(staging.p, host, bytes, hipMemcpyHostToDevice));
+  rc = with_types(E, [&](auto qc, auto s) -> int {
+    constexpr int Q = decltype(qc)::value;
+    using S = decltype(s);
+    const EnsembleEntry<Q>& en = entry<Q>(E, m);
+    S* pop = static_cast<S*>(E->d_pop) + en.pop_offset;
+    const dim3 grid(grid_for(mem.ncore, 256));
+    if (dtype == LBMH_F64)
+      hipLaunchKernelGGL((scatter_populations<Q, double, S>), grid, dim3(256), 0, 0, (const double*)staging.p, pop, geom_of(en), shifts_of(en));
+    else if (dtype == LBMH_F32)
+      hipLaunchKernelGGL((scatter_populations<Q, float, S>), grid, dim3(256), 0, 0, (const float*)staging.p, pop, geom_of(en), shifts_of(en));
+    else
+      hipLaunchKernelGGL((scatter_populations<Q, _Float16, S>), grid, dim3(256), 0, 0, (const _Float16*)staging.p, pop, geom_of(en), shifts_of(en));
+    LBMH_HIP(hipGetLastError());
+    return LBMH_OK;
+  });
+  if (rc != LBMH_OK) return rc;
+  LBMH_HIP(hipDeviceSynchronize());
+  return LBMH_OK;
+}
+
+int lbmh_ensemble_download(lbmh_ensemble* E, int m, void* host, int dtype)
+{
+  int rc = check_member(E, m);
+  if (rc != LBMH_OK) return rc;
+  if (!host || dtype < LBMH_F64 || dtype > LBMH_F16) return fail(LBMH_E_INVALID, "bad host buffer or dtype");
+  DeviceGuard guard(E->device);
+  rc = pull_table(E);
+  if (rc != LBMH_OK) return rc;
+  const Member& mem = E->members[m];
+  DeviceBuffer staging;
+  const size_t bytes = size_t(E->q) * mem.ncore * storage_size(dtype);
+  LBMH_HIP(hipMalloc(&staging.p, bytes));
+  rc = with_types(E, [&](auto qc, auto s) -> int {
+    constexpr int Q = decltype(qc)::value;
+    using S = decltype(s);
+    const EnsembleEntry<Q>& en = entry<Q>(E, m);
+    const S* pop = static_cast<const S*>(E->d_pop) + en.pop_offset;
+    const dim3 grid(grid_for(mem.ncore, 256));
+    if (dtype == LBMH_F64)
+      hipLaunchKernelGGL((gather_populations<Q, double, S>), grid, dim3(256), 0, 0, (double*)staging.p, pop, geom_of(en), shifts_of(en));
+    else if (dtype == LBMH_F32)
+      hipLaunchKernelGGL((gather_populations<Q, float, S>), grid, dim3(256), 0, 0, (float*)staging.p, pop, geom_of(en), shifts_of(en));
+    else
+      hipLaunchKernelGGL((gather_populations<Q, _Float16, S>), grid, dim3(256), 0, 0, (_Float16*)staging.p, pop, geom_of(en), shifts_of(en));
+    LBMH_HIP(hipGetLastError());
+    return LBMH_OK;
+  });
+  if (rc != LBMH_OK) return rc;
+  LBMH_HIP(hipMemcpy(host, staging.p, bytes, hipMemcpyDeviceToHost));
+  return LBMH_OK;
+}
+
+int lbmh_ensemble_moments(lbmh_ensemble* E, int m, double* rho_u)
+{
+  int rc = check_member(E, m);
+  if (rc != LBMH_OK) return rc;
+  if (!rho_u) return fail(LBMH_E_INVALID, "null output");
+  DeviceGuard guard(E->device);
+  rc = pull_table(E);
+  if (rc != LBMH_OK) return rc;
+  const Member& mem = E->members[m];
+  DeviceBuffer out;
+  const size_t bytes = size_t(4) * mem.ncore * sizeof(double);
+  LBMH_HIP(hipMalloc(&out.p, bytes));
+  const int forced = (E->kind == LBMH_FORCED_BGK || E->kind == LBMH_FORCED_TRT) ? 1 : 0;
+  rc = with_types(E, [&](auto qc, auto s) -> int {
+    constexpr int Q = decltype(qc)::value;
+    using S = decltype(s);
+    const EnsembleEntry<Q>& en = entry<Q>(E, m);
+    const S* pop = static_cast<const S*>(E->d_pop) + en.pop_offset;
+    hipLaunchKernelGGL((moments_kernel<Q, S>), dim3(grid_for(mem.ncore, 256)), dim3(256), 0, 0, (double*)out.p,
+                       E->d_ids + mem.ids_off, pop, geom_of(en), shifts_of(en), mem.params.force[0] * 0.5,
+                       mem.params.force[1] * 0.5, mem.params.force[2] * 0.5, forced);
+    LBMH_HIP(hipGetLastError());
+    return LBMH_OK;
+  });
+  if (rc != LBMH_OK) return rc;
+  LBMH_HIP(hipMemcpy(rho_u, out.p, bytes, hipMemcpyDeviceToHost));
+  return LBMH_OK;
+}
+
+int lbmh_ensemble_step(lbmh_ensemble* E, int64_t n, void* stream)
+{
+  if (!E) return fail(LBMH_E_INVALID, "null ensemble");
+  if (n < 0) return fail(LBMH_E_INVALID, "negative step count");
+  DeviceGuard guard(E->device);
+  return launch_steps(E, n, static_cast<hipStream_t>(stream));
+}
+
+int lbmh_ensemble_step_graph(lbmh_ensemble* E, int64_t n, void* stream)
+{
+  if (!E) return fail(LBMH_E_INVALID, "null ensemble");
+  if (n < 0) return fail(LBMH_E_INVALID, "negative step count");
+  DeviceGuard guard(E->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  constexpr int64_t kGraphSteps = 32;
+  if (n >= kGraphSteps && (!E->graph || E->graph_version != E->version)) {
+    if (E->graph) {
+      LBMH_HIP(hipGraphExecDestroy(E->graph));
+      E->graph = nullptr;
+    }
+    LBMH_HIP(hipStreamBeginCapture(E->capture_stream, hipStreamCaptureModeThreadLocal));
+    const int rc = launch_steps(E, kGraphSteps, E->capture_stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(E->capture_stream, &g);
+    if (rc != LBMH_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    const hipError_t ei = hipGraphInstantiate(&E->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return hip_fail(ei, "hipGraphInstantiate");
+    E->graph_version = E->version;
+    E->graph_steps = kGraphSteps;
+  }
+  int64_t done = 0;
+  if (E->graph && E->graph_version == E->version) {
+    for (; done + E->graph_steps <= n; done += E->graph_steps) LBMH_HIP(hipGraphLaunch(E->graph, s));
+  }
+  return launch_steps(E, n - done, s);
+}
+
+int lbmh_ensemble_set_launch_config(lbmh_ensemble* E, const lbmh_launch_config* cfg)
+{
+  if (!E || !cfg) return fail(LBMH_E_INVALID, "null argument");
+  if (cfg->lds_bytes < 0 || cfg->lds_bytes > 65536) return fail(LBMH_E_INVALID, "lds_bytes must be in [0, 65536]");
+  if (cfg->xcd_group < 0 || cfg->xcd_group > 64) return fail(LBMH_E_INVALID, "xcd_group must be in [0, 64]");
+  E->launch = *cfg;
+  E->version += 1;
+  return LBMH_OK;
+}
+
+int lbmh_ensemble_usage(const lbmh_ensemble* E, int* members, size_t* cells, size_t* used, size_t* capacity)
+{
+  if (!E) return fail(LBMH_E_INVALID, "null ensemble");
+  int count = 0;
+  size_t ncells = 0;
+  for (const Member& m : E->members)
+    if (m.active) {
+      ++count;
+      ncells += m.ncore;
+    }
+  if (members) *members = count;
+  if (cells) *cells = ncells;
+  if (used) *used = size_t(E->pop_arena.used() + E->id_arena.used());
+  if (capacity) *capacity = size_t(E->pop_arena.capacity() + E->id_arena.capacity());
+  return LBMH_OK;
+}
+
+}  // extern "C"

@@ -151,6 +151,22 @@ def cavity_benchmark(size=100, q=19):
                 walls={3: (1.0, (0.01, 0.0, 0.0))})
 
 
+def cavity_ensemble(count=512, seed=20261204, lo=16, hi=96, tau=(0.52, 1.2), lid=(0.02, 0.1), q=19):
+    """Config-4 ensemble: independent lid-driven cavities. Each edge is drawn
+    log-uniform in [lo, hi] per axis, tau uniform in ``tau``, and the lid velocity
+    uniform in ``lid`` along x (lid_driven_cavity layout: BounceBack box, moving top face)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    cases = []
+    for k in range(count):
+        shape = tuple(int(round(math.exp(rng.uniform(math.log(lo), math.log(hi))))) for _ in range(3))
+        t = float(rng.uniform(*tau))
+        u = float(rng.uniform(*lid))
+        c = lid_driven_cavity(shape, tau=t, lid_u=u, q=q)
+        c.name = f"cavity_{k}"
+        cases.append(c)
+    return cases
+
+
 def turbulent_channel(shape=(384, 192, 192), tau=0.505, smago=0.14, u_max=0.05,
                       noise=0.01, seed=20261203, q=27):
     """Smagorinsky channel. It is periodic in x and z, with BounceBack walls at y = 0 and

@@ -1,0 +1,144 @@
+"""GPU tests of lattice ensembles (C ABI lbmh_ensemble_*).
+
+Each member runs exactly the single-block cell body with the same derived
+parameters, so a member must be bit-identical to the same lattice advanced
+on its own by lbmh_step. The single-lattice path is pinned to the oracle and
+to OpenLB by test_gpu_parity.py, so bit-identity extends that pin to every
+member."""
+import numpy as np
+import pytest
+
+import golden_cases as G
+from oracle import oracle as O
+from openlb_amd import workloads as W
+from openlb_amd.ensemble import Ensemble
+from openlb_amd.lattice import BlockLattice
+
+pytestmark = pytest.mark.gpu
+
+
+def small_members(seed=3, count=7):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    cases = []
+    for k in range(count):
+        shape = tuple(int(v) for v in rng.integers(4, 23, size=3))
+        c = W.lid_driven_cavity(shape, tau=float(rng.uniform(0.52, 1.2)), lid_u=float(rng.uniform(0.02, 0.1)))
+        cases.append(c)
+    # a periodic member with NoDynamics holes and a second wall id
+    p = W.random_periodic((9, 5, 13), seed=seed, tau=0.8)
+    p.ids = p.ids.copy()
+    p.ids[2, :, 3] = 0
+    p.ids[4, 1, :] = 5
+    p.walls = {5: (1.02, (0.01, -0.02, 0.005))}
+    cases.append(p)
+    return cases
+
+
+def standalone(case, storage, f0, steps):
+    with BlockLattice.from_case(case, storage) as lat:
+        lat.upload_populations(f0, "f64")
+        lat.collide_and_stream(steps)
+        return lat.download_populations("f64")
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16"])
+def test_members_match_standalone_lattices(gpu_available, storage):
+    cases = small_members()
+    with Ensemble(19, storage, "bgk", max_members=16, population_bytes=64 << 20, cell_capacity=1 << 20) as ens:
+        members, f0s = [], []
+        for c in cases:
+            m = ens.add(c)
+            f0 = O.initial_populations(c, storage)
+            f0s.append(f0)
+            ens.upload(m, f0)
+            members.append(m)
+        ens.step(17)
+        for m, c, f0 in zip(members, cases, f0s):
+            assert ens.status(m) == (17, -1)
+            np.testing.assert_array_equal(ens.download(m), standalone(c, storage, f0, 17))
+
+
+def test_step_budgets_stop_members_individually(gpu_available):
+    cases = small_members(seed=5, count=3)[:3]
+    budgets = [4, 11, -1]
+    with Ensemble(19, "f32", "bgk", max_members=4, population_bytes=16 << 20, cell_capacity=1 << 18) as ens:
+        ms = [ens.add(c, steps=b) for c, b in zip(cases, budgets)]
+        f0s = []
+        for m, c in zip(ms, cases):
+            f0 = O.initial_populations(c, "f32")
+            ens.upload(m, f0)
+            f0s.append(f0)
+        ens.step(20)
+        assert [ens.status(m) for m in ms] == [(4, 0), (11, 0), (20, -1)]
+        for m, c, f0, b in zip(ms, cases, f0s, [4, 11, 20]):
+            np.testing.assert_array_equal(ens.download(m), standalone(c, "f32", f0, b))
+        # a finished member can be given more steps
+        ens.set_steps(ms[0], 3)
+        ens.step(5)
+        assert ens.status(ms[0]) == (7, 0)
+
+
+def test_graph_replay_matches_eager_steps(gpu_available):
+    """The captured step loop is replayed several times, with eager work and
+    synchronisations between replays. Every replay must match eager stepping bit for bit."""
+    cases = small_members(seed=7, count=5)
+    with Ensemble(19, "f32", "bgk", max_members=8, population_bytes=32 << 20, cell_capacity=1 << 20) as a, \
+            Ensemble(19, "f32", "bgk", max_members=8, population_bytes=32 << 20, cell_capacity=1 << 20) as b:
+        for c in cases:
+            f0 = O.initial_populations(c, "f32")
+            a.upload(a.add(c), f0)
+            b.upload(b.add(c), f0)
+        for replay in range(3):
+            a.step(70, graph=True)     # two graph replays of 32 steps plus 6 plain steps
+            b.step(70, graph=False)
+            for m in range(len(cases)):
+                np.testing.assert_array_equal(a.download(m), b.download(m), err_msg=f"replay {replay}")
+                assert a.status(m) == b.status(m) == (70 * (replay + 1), -1)
+        # membership change invalidates the captured graph
+        extra = W.lid_driven_cavity((6, 7, 8), tau=0.7, lid_u=0.05)
+        f0 = O.initial_populations(extra, "f32")
+        ma, mb = a.add(extra), b.add(extra)
+        a.upload(ma, f0)
+        b.upload(mb, f0)
+        a.step(40, graph=True)
+        b.step(40)
+        for m in range(len(cases) + 1):
+            np.testing.assert_array_equal(a.download(m), b.download(m))
+
+
+def test_add_remove_reuses_arena(gpu_available):
+    cases = small_members(seed=9, count=6)
+    with Ensemble(19, "f64", "bgk", max_members=6, population_bytes=48 << 20, cell_capacity=1 << 18) as ens:
+        ms = [ens.add(c) for c in cases[:5]]
+        used = ens.usage()["bytes_used"]
+        ens.remove(ms[1])
+        ens.remove(ms[3])
+        assert ens.usage()["members"] == 3
+        assert ens.usage()["bytes_used"] < used
+        # refill the freed slots with other extents
+        new = [W.lid_driven_cavity((7, 9, 5), tau=0.9, lid_u=0.03), cases[5]]
+        for c in new:
+            m = ens.add(c)
+            f0 = O.initial_populations(c, "f64")
+            ens.upload(m, f0)
+            ens.step(12)
+            np.testing.assert_array_equal(ens.download(m), standalone(c, "f64", f0, 12))
+        with pytest.raises(Exception):
+            for _ in range(10):
+                ens.add(W.lid_driven_cavity((60, 60, 60)))
+
+
+def test_member_matches_oracle(gpu_available):
+    c = W.lid_driven_cavity((14, 11, 17), tau=0.58, lid_u=0.09)
+    f0 = O.initial_populations(c, "f64")
+    with Ensemble(19, "f64", "bgk", max_members=2, population_bytes=8 << 20, cell_capacity=1 << 16) as ens:
+        m = ens.add(c)
+        ens.upload(m, f0)
+        ens.step(60, graph=True)
+        got = ens.download(m)
+        mom = ens.moments(m)
+    ora = O.OracleLattice(c, "f64")
+    ora.upload(f0)
+    ora.step(60, threads=4)
+    assert G.rel_error(got, ora.download(), 19, c.ids == 1) <= 1e-13
+    np.testing.assert_allclose(mom[:, c.ids == 1], ora.moments()[:, c.ids == 1], atol=1e-13)
