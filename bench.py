@@ -104,6 +104,75 @@ def calibrate_copy(mib: int, repeats: int = 5):
                       "repeats": repeats, "copy_gbs": round(2 * nbytes / dt / 1e9, 1)}), flush=True)
 
 
+def ensemble_bench(count=512, steps=2000, seed=20261204):
+    """Config 4: `count` independent D3Q19 BGK lid-driven cavities (edges log-uniform
+    in [16, 96], tau in [0.52, 1.2], lid velocity in [0.02, 0.1]), each run for `steps`
+    steps. The plain baseline gives every cavity its own lattice and launches its steps
+    one after another, as the reference runs UQ samples one SuperLattice at a time. The
+    ensemble advances all members with one launch per step and replays the step loop
+    from a HIP graph. Velocity error: the FP16-stored ensemble against the FP64 ensemble,
+    relative to each cavity's lid velocity (the FP64 path is pinned to OpenLB at 1e-12)."""
+    from openlb_amd import workloads as W
+    from openlb_amd.ensemble import Ensemble
+    from openlb_amd.lattice import BlockLattice
+    cases = W.cavity_ensemble(count, seed=seed)
+    cells = sum(c.ncore for c in cases)
+    lups = cells * steps
+    out = {"workload": f"cavity_ensemble_d3q19_bgk x{count}", "members": count, "steps_each": steps,
+           "cells": cells, "seed": seed}
+
+    lats = []
+    for c in cases:
+        lat = BlockLattice.from_case(c, "f32")
+        lat.ini_equilibrium(c.rho_u)
+        lats.append(lat)
+    lats[0].synchronize()
+    t0 = time.perf_counter()
+    for lat in lats:
+        lat.collide_and_stream(steps)
+    lats[0].synchronize()
+    plain = time.perf_counter() - t0
+    for lat in lats:
+        lat.close()
+    del lats
+    out["plain_f32"] = {"seconds": round(plain, 3), "solves_per_s": round(count / plain, 2),
+                        "mlups": round(lups / plain / 1e6, 1)}
+    log(f"ensemble: plain per-lattice FP32 launches {plain:.2f} s")
+
+    final = {}
+    for storage in ("f32", "f16", "f64"):
+        pop = sum(Ensemble.bytes_for(19, storage, c.shape)[0] for c in cases)
+        ids = sum(Ensemble.bytes_for(19, storage, c.shape)[1] for c in cases)
+        with Ensemble(19, storage, "bgk", max_members=count, population_bytes=pop + (1 << 20),
+                      cell_capacity=ids + (1 << 20)) as ens:
+            members = [ens.add(c, steps=steps) for c in cases]
+            for m, c in zip(members, cases):
+                ens.ini_equilibrium(m, c.rho_u)
+            ens.step(0, graph=True)
+            t0 = time.perf_counter()
+            ens.step(steps, graph=True)
+            done0 = ens.status(members[0])[0]
+            dt = time.perf_counter() - t0     # status() synchronises the device
+            if done0 != steps:
+                raise SystemExit(f"ensemble member ran {done0} steps, expected {steps}")
+            out[f"ensemble_{storage}"] = {"seconds": round(dt, 3), "solves_per_s": round(count / dt, 2),
+                                          "mlups": round(lups / dt / 1e6, 1)}
+            log(f"ensemble {storage}: {dt:.2f} s")
+            if storage in ("f16", "f64"):
+                final[storage] = [ens.moments(m)[1:] for m in members]
+    errs = []
+    for c, a, b in zip(cases, final["f16"], final["f64"]):
+        fluid = c.ids == 1
+        errs.append(float(np.abs(a[:, fluid] - b[:, fluid]).max() / c.walls[3][1][0]))
+    out["f16_velocity_error_vs_f64"] = {"max": round(max(errs), 6), "median": round(float(np.median(errs)), 6),
+                                        "relative_to": "lid velocity"}
+    out["speedup_f16_ensemble_vs_plain_f32"] = round(out["ensemble_f16"]["solves_per_s"] /
+                                                     out["plain_f32"]["solves_per_s"], 2)
+    out["speedup_f32_ensemble_vs_plain_f32"] = round(out["ensemble_f32"]["solves_per_s"] /
+                                                     out["plain_f32"]["solves_per_s"], 2)
+    return out
+
+
 def measured_traffic(kernel_name, case, launch):
     """HBM bytes per launch of this kernel, workload and launch shape from the
     committed counter runs (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE scaled
@@ -176,6 +245,10 @@ def main():
                     help="time candidate launch shapes first and keep the fastest "
                          "(default: the library's measured default shape)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
+    ap.add_argument("--no-ensemble", dest="ensemble", action="store_false",
+                    help="skip the 512-lattice ensemble measurement (config 4)")
+    ap.add_argument("--ensemble-members", type=int, default=512)
+    ap.add_argument("--ensemble-steps", type=int, default=2000)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -234,6 +307,11 @@ def main():
     log(f"{mlups:.0f} MLUPS, kernel avg {avg_ms:.4f} ms (min {min(kernel_ms):.4f}), "
         f"{achieved:.0f} GB/s algorithmic")
 
+    ensemble = None
+    if args.ensemble:
+        log("ensemble (config 4) ...")
+        ensemble = ensemble_bench(args.ensemble_members, args.ensemble_steps)
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -277,6 +355,7 @@ def main():
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },
         "cpu_baseline": cpu,
+        "ensemble": ensemble,
     }
     print(json.dumps(out), flush=True)
 

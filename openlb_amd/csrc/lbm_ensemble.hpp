@@ -33,18 +33,31 @@ struct EnsembleEntry {
   uint32_t dmod[Q];         // (c_i . projection) mod n_padded
 };
 
+// Sizes of the arenas, so that a workgroup can refuse an entry that does not fit.
+struct EnsembleLimits {
+  uint64_t pop_elements;    // elements in the population arena
+  uint64_t id_bytes;        // bytes in the id arena
+  uint32_t members;         // table slots
+};
+
 template <int Q, typename S, int KIND>
 __global__ __launch_bounds__(kBlock) void ensemble_step_kernel(const EnsembleEntry<Q>* __restrict__ table,
                                                                const uint32_t* __restrict__ chunk_lattice,
                                                                S* __restrict__ arena, const uint8_t* __restrict__ ids,
-                                                               const double* __restrict__ walls, uint32_t xcd_group)
+                                                               const double* __restrict__ walls, uint32_t xcd_group,
+                                                               EnsembleLimits limits)
 {
   using V = typename StorageTraits<S>::V;
   uint32_t b = blockIdx.x;
   if (xcd_group) b = remap_xcd(b, gridDim.x, xcd_group);
   const uint32_t lat = chunk_lattice[b];
+  if (lat >= limits.members) return;
   const EnsembleEntry<Q>& e = table[lat];
   if (e.steps_left == 0) return;
+  // uniform guard: every column and the id range of the entry inside the arenas
+  if (e.stride < e.n_padded || e.pop_offset + uint64_t(Q) * e.stride > limits.pop_elements ||
+      e.ids_offset + e.ncore > limits.id_bytes)
+    return;
   StepArgs<Q, V> a;
   a.pop = arena + e.pop_offset;
   a.stride = e.stride;

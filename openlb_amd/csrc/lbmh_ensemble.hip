@@ -212,28 +212,30 @@ int launch_steps(lbmh_ensemble* E, int64_t n, hipStream_t stream)
     using S = decltype(s);
     auto* table = static_cast<EnsembleEntry<Q>*>(E->d_table);
     S* arena = static_cast<S*>(E->d_pop);
+    const EnsembleLimits limits{E->pop_arena.capacity() / sizeof(S), E->id_arena.capacity(),
+                                uint32_t(E->max_members)};
     for (int64_t k = 0; k < n; ++k) {
       if (E->total_chunks) {
         switch (E->kind) {
         case LBMH_BGK:
           hipLaunchKernelGGL((ensemble_step_kernel<Q, S, kBGK>), dim3(E->total_chunks), dim3(kBlock), lds, stream,
-                             table, E->d_chunk, arena, E->d_ids, E->d_walls, group);
+                             table, E->d_chunk, arena, E->d_ids, E->d_walls, group, limits);
           break;
         case LBMH_FORCED_BGK:
           hipLaunchKernelGGL((ensemble_step_kernel<Q, S, kForcedBGK>), dim3(E->total_chunks), dim3(kBlock), lds,
-                             stream, table, E->d_chunk, arena, E->d_ids, E->d_walls, group);
+                             stream, table, E->d_chunk, arena, E->d_ids, E->d_walls, group, limits);
           break;
         case LBMH_TRT:
           hipLaunchKernelGGL((ensemble_step_kernel<Q, S, kTRT>), dim3(E->total_chunks), dim3(kBlock), lds, stream,
-                             table, E->d_chunk, arena, E->d_ids, E->d_walls, group);
+                             table, E->d_chunk, arena, E->d_ids, E->d_walls, group, limits);
           break;
         case LBMH_FORCED_TRT:
           hipLaunchKernelGGL((ensemble_step_kernel<Q, S, kForcedTRT>), dim3(E->total_chunks), dim3(kBlock), lds,
-                             stream, table, E->d_chunk, arena, E->d_ids, E->d_walls, group);
+                             stream, table, E->d_chunk, arena, E->d_ids, E->d_walls, group, limits);
           break;
         default:
           hipLaunchKernelGGL((ensemble_step_kernel<Q, S, kSmagorinsky>), dim3(E->total_chunks), dim3(kBlock), lds,
-                             stream, table, E->d_chunk, arena, E->d_ids, E->d_walls, group);
+                             stream, table, E->d_chunk, arena, E->d_ids, E->d_walls, group, limits);
           break;
         }
         LBMH_HIP(hipGetLastError());
@@ -266,7 +268,8 @@ int lbmh_ensemble_create(const lbmh_ensemble_desc* d, lbmh_ensemble** out)
   E->storage = d->storage;
   E->kind = d->bulk_kind;
   E->max_members = d->max_members;
-  E->entry_bytes = align_up(E->q == 19 ? sizeof(EnsembleEntry<19>) : sizeof(EnsembleEntry<27>));
+  // the device indexes table[m] with this exact stride
+  E->entry_bytes = E->q == 19 ? sizeof(EnsembleEntry<19>) : sizeof(EnsembleEntry<27>);
   E->members.resize(size_t(E->max_members));
   E->h_table.assign(E->entry_bytes * size_t(E->max_members), 0);
   E->pop_arena = Arena(align_up(d->population_bytes));
