@@ -208,6 +208,9 @@ int lbmh_ensemble_set_wall(lbmh_ensemble* ensemble, int member, int id, double r
 int lbmh_ensemble_set_steps(lbmh_ensemble* ensemble, int member, int64_t steps);
 /* steps completed and steps still to run (< 0: unbounded) */
 int lbmh_ensemble_status(lbmh_ensemble* ensemble, int member, int64_t* steps_done, int64_t* steps_left);
+/* the same for every slot at once (arrays of max_members entries; inactive
+ * slots report 0 / 0); one device round trip */
+int lbmh_ensemble_status_all(lbmh_ensemble* ensemble, int64_t* steps_done, int64_t* steps_left);
 
 int lbmh_ensemble_ini_equilibrium(lbmh_ensemble* ensemble, int member, const double* rho_u);
 int lbmh_ensemble_upload(lbmh_ensemble* ensemble, int member, const void* host, int host_dtype);

@@ -30,7 +30,7 @@ EXPORTS = (
     "lbmh_ensemble_remove", "lbmh_ensemble_set_wall", "lbmh_ensemble_set_steps", "lbmh_ensemble_status",
     "lbmh_ensemble_ini_equilibrium", "lbmh_ensemble_upload", "lbmh_ensemble_download",
     "lbmh_ensemble_moments", "lbmh_ensemble_step", "lbmh_ensemble_step_graph",
-    "lbmh_ensemble_set_launch_config", "lbmh_ensemble_usage",
+    "lbmh_ensemble_set_launch_config", "lbmh_ensemble_usage", "lbmh_ensemble_status_all",
 )
 
 
@@ -125,6 +125,7 @@ def load() -> ctypes.CDLL:
     L.lbmh_ensemble_step_graph.argtypes = [vp, i64, vp]
     L.lbmh_ensemble_set_launch_config.argtypes = [vp, ctypes.POINTER(LaunchConfig)]
     L.lbmh_ensemble_usage.argtypes = [vp, ctypes.POINTER(i32), szp, szp, szp]
+    L.lbmh_ensemble_status_all.argtypes = [vp, vp, vp]
     L.lbmh_last_error.restype = ctypes.c_char_p
     for name in EXPORTS:
         if name != "lbmh_last_error":

@@ -482,6 +482,24 @@ int lbmh_ensemble_status(lbmh_ensemble* E, int m, int64_t* done, int64_t* left)
   });
 }
 
+int lbmh_ensemble_status_all(lbmh_ensemble* E, int64_t* done, int64_t* left)
+{
+  if (!E || !done || !left) return fail(LBMH_E_INVALID, "null argument");
+  DeviceGuard guard(E->device);
+  const int rc = pull_table(E);
+  if (rc != LBMH_OK) return rc;
+  return with_q(E, [&](auto qc) -> int {
+    constexpr int Q = decltype(qc)::value;
+    for (int m = 0; m < E->max_members; ++m) {
+      const EnsembleEntry<Q>& en = entry<Q>(E, m);
+      const bool on = E->members[m].active;
+      done[m] = on ? en.steps_done : 0;
+      left[m] = on ? en.steps_left : 0;
+    }
+    return LBMH_OK;
+  });
+}
+
 int lbmh_ensemble_ini_equilibrium(lbmh_ensemble* E, int m, const double* rho_u)
 {
   int rc = check_member(E, m);

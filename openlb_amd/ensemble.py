@@ -33,6 +33,7 @@ class Ensemble:
         abi.check(abi.load().lbmh_ensemble_create(ctypes.byref(d), ctypes.byref(h)))
         self._h = h
         self.q, self.storage, self.kind = q, storage, kind
+        self.max_members = max_members
         self.shapes: dict[int, tuple] = {}
 
     @staticmethod
@@ -100,6 +101,13 @@ class Ensemble:
         abi.check(abi.load().lbmh_ensemble_status(self._h, int(member), ctypes.byref(done),
                                                   ctypes.byref(left)))
         return done.value, left.value
+
+    def status_all(self) -> tuple[np.ndarray, np.ndarray]:
+        """(steps done, steps left) of every slot, one device round trip."""
+        done = np.zeros(self.max_members, dtype=np.int64)
+        left = np.zeros(self.max_members, dtype=np.int64)
+        abi.check(abi.load().lbmh_ensemble_status_all(self._h, _ptr(done), _ptr(left)))
+        return done, left
 
     def ini_equilibrium(self, member: int, rho_u: np.ndarray):
         a = np.ascontiguousarray(rho_u, dtype=np.float64)

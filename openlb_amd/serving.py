@@ -1,0 +1,191 @@
+"""Continuous-batching solve server on one GPU.
+
+Requests are independent D3Q19 solves. Each has its own extent, step count,
+driving velocity and storage type (FP32 or FP16). Two ensembles (one per storage
+type) hold the running solves on the device.
+
+The loop repeats:
+
+1. Admit every request that has arrived, in arrival order, while its footprint
+   fits the ensemble's arenas and a slot is free. Admission is checked against the
+   HBM budget the arenas were sized to; a request that does not fit waits at the
+   head of the queue.
+2. Advance every ensemble that has running solves by one tick of ``tick_steps`` steps
+   (a HIP-graph replay). Each solve carries its own step budget, so it stops at
+   exactly its requested step count even in mid-tick.
+3. Retire the finished solves: optionally fetch their moments, free their slots and
+   arena space, and record their latency.
+
+Every solve in the mix runs the same per-cell code on the same data as it would
+alone, so its result is bit-identical (tests/test_gpu_serving.py).
+"""
+from __future__ import annotations
+
+import collections
+import dataclasses
+import math
+import time
+
+import numpy as np
+
+from . import abi, workloads
+from .ensemble import Ensemble
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    arrival: float          # seconds after the start of the stream
+    case: workloads.Case
+    steps: int
+    storage: str            # "f32" or "f16"
+
+    @property
+    def lups(self) -> int:
+        return self.case.ncore * self.steps
+
+
+@dataclasses.dataclass
+class Result:
+    rid: int
+    storage: str
+    cells: int
+    steps: int
+    arrival: float
+    admitted: float
+    finished: float
+    moments: np.ndarray | None = None
+    populations: np.ndarray | None = None
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+def channel_request(rid, arrival, size, steps, u_in, storage, tau=0.6):
+    """A D3Q19 square-duct channel of size^3 cells. It is periodic in x, with BounceBack
+    walls at the y/z bounds. The flow is driven by a Guo body force that would give a
+    plane-Poiseuille centre velocity of u_in: F = 8 nu u_in / H^2, H = size - 2."""
+    nu = (tau - 0.5) / 3
+    h = max(size - 2, 1)
+    force = 8 * nu * u_in / (h * h)
+    case = workloads.porous_channel((size, size, size), seed=rid, tau=tau, force=force, solid_fraction=0.0)
+    case.name = f"channel_{rid}"
+    case.rho_u = None     # rest state (rho 1, u 0); built at admission to keep the queue small
+    return Request(rid, arrival, case, int(steps), storage)
+
+
+def initial_moments(case) -> np.ndarray:
+    if case.rho_u is not None:
+        return case.rho_u
+    rho_u = np.zeros((4,) + tuple(case.shape))
+    rho_u[0] = 1.0
+    return rho_u
+
+
+def poisson_channel_stream(count=400, rate=200.0, seed=20261205, size=(24, 128), steps=(500, 5000),
+                           u_mean=0.05, u_std=0.01, storages=("f32", "f16"), tau=0.6):
+    """Config-5 request stream. Arrivals are Poisson at `rate` per second. Sizes are
+    log-uniform in `size`, step counts uniform in `steps`, and inflow velocities
+    normal(u_mean, u_std^2) as UQ samples. The storage type is drawn uniformly from `storages`."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t = 0.0
+    out = []
+    for rid in range(count):
+        t += float(rng.exponential(1.0 / rate))
+        s = int(round(math.exp(rng.uniform(math.log(size[0]), math.log(size[1])))))
+        n = int(rng.integers(steps[0], steps[1] + 1))
+        u = float(rng.normal(u_mean, u_std))
+        st = storages[int(rng.integers(0, len(storages)))]
+        out.append(channel_request(rid, t, s, n, u, st, tau))
+    return out
+
+
+class SolveServer:
+    def __init__(self, budget_bytes: int, storages=("f32", "f16"), max_members: int = 1024,
+                 tick_steps: int = 64, keep_moments: bool = False, keep_populations: bool = False,
+                 device: int = 0):
+        self.tick_steps = int(tick_steps)
+        self.keep_moments = keep_moments
+        self.keep_populations = keep_populations
+        share = budget_bytes // len(storages)
+        self.ens = {}
+        for st in storages:
+            # id arena: one byte per cell; a member's populations take at least
+            # 2*19 bytes per cell, so cells <= pop bytes / 38
+            cells = share // (19 * {"f32": 4, "f16": 2, "f64": 8}[st]) + (1 << 20)
+            self.ens[st] = Ensemble(19, st, "forced_bgk", max_members=max_members,
+                                    population_bytes=share, cell_capacity=cells, device=device)
+        self.max_active = 0
+
+    def close(self):
+        for e in self.ens.values():
+            e.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _try_admit(self, req: Request):
+        try:
+            m = self.ens[req.storage].add(req.case, steps=req.steps)
+        except abi.LbmhError as exc:
+            if exc.code == abi.E_MEMORY:
+                return None
+            raise
+        self.ens[req.storage].ini_equilibrium(m, initial_moments(req.case))
+        return m
+
+    def run(self, requests, realtime: bool = True) -> list[Result]:
+        """Serve the stream. With realtime=True a request becomes visible at its
+        arrival time on the wall clock. Otherwise every request is present at time zero."""
+        queue = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        active = {}      # (storage, member) -> (request, admitted)
+        results = []
+        t0 = time.perf_counter()
+        while queue or active:
+            now = time.perf_counter() - t0
+            while queue and (not realtime or queue[0].arrival <= now):
+                req = queue[0]
+                m = self._try_admit(req)
+                if m is None:
+                    break
+                active[(req.storage, m)] = (req, time.perf_counter() - t0)
+                queue.popleft()
+            self.max_active = max(self.max_active, len(active))
+            if not active:
+                if queue:
+                    time.sleep(max(0.0, queue[0].arrival - (time.perf_counter() - t0)))
+                continue
+            running = {st for st, _ in active}
+            for st in running:
+                self.ens[st].step(self.tick_steps, graph=True)
+            for st in running:
+                done, left = self.ens[st].status_all()     # waits for the tick
+                finished = time.perf_counter() - t0
+                for (s2, m), (req, adm) in list(active.items()):
+                    if s2 != st or left[m] != 0:
+                        continue
+                    if done[m] != req.steps:
+                        raise RuntimeError(f"request {req.rid} ran {done[m]} of {req.steps} steps")
+                    res = Result(req.rid, st, req.case.ncore, req.steps, req.arrival, adm, finished)
+                    if self.keep_moments:
+                        res.moments = self.ens[st].moments(m)
+                    if self.keep_populations:
+                        res.populations = self.ens[st].download(m)
+                    self.ens[st].remove(m)
+                    del active[(st, m)]
+                    results.append(res)
+        return results
+
+
+def summarize(results: list[Result], makespan: float | None = None) -> dict:
+    lat = np.array([r.latency for r in results])
+    lups = sum(r.cells * r.steps for r in results)
+    span = makespan if makespan is not None else max(r.finished for r in results)
+    return {"requests": len(results), "p50_s": round(float(np.percentile(lat, 50)), 4),
+            "p99_s": round(float(np.percentile(lat, 99)), 4), "max_s": round(float(lat.max()), 4),
+            "makespan_s": round(span, 3), "aggregate_mlups": round(lups / span / 1e6, 1),
+            "solves_per_s": round(len(results) / span, 2)}
