@@ -180,6 +180,47 @@ def ensemble_bench(count=512, steps=2000, seed=20261204):
     return out
 
 
+def turbulent_bench(shape=(384, 192, 192), steps=5000):
+    """Config 3: D3Q27 Smagorinsky-BGK channel (C = 0.14, tau = 0.505). It is periodic in x
+    and z, with BounceBack walls at y, a laminar profile and seeded +/-1% velocity noise.
+    It runs `steps` steps in FP64, FP32 and FP16 storage. It reports throughput and the
+    max / L2 errors of rho and u of the FP32 and FP16 runs against the FP64 run. The FP64
+    device path is pinned to OpenLB at 1e-12; D3Q27 Smagorinsky is the generic path, whose
+    golden case matches at 2e-15."""
+    from openlb_amd import workloads as W
+    from openlb_amd.lattice import BlockLattice
+    case = W.turbulent_channel(tuple(shape))
+    fluid = case.ids == 1
+    out = {"workload": "turbulent_channel_d3q27_smagorinsky", "shape": list(shape), "steps": steps,
+           "tau": round(1 / case.omega, 6), "smagorinsky": case.smago}
+    mom = {}
+    for storage in ("f64", "f32", "f16"):
+        with BlockLattice.from_case(case, storage) as lat:
+            lat.ini_equilibrium(case.rho_u)
+            lat.synchronize()
+            t0 = time.perf_counter()
+            lat.collide_and_stream(steps)
+            lat.synchronize()
+            dt = time.perf_counter() - t0
+            m = lat.moments()
+        mom[storage] = m[:, fluid]
+        out[storage] = {"seconds": round(dt, 3), "mlups": round(case.n_active * steps / dt / 1e6, 1),
+                        "hbm_gbs": round(2 * 27 * STORE_BYTES[storage] * case.n_active * steps / dt / 1e9, 1)}
+        log(f"turbulent channel {storage}: {dt:.2f} s")
+        if not np.isfinite(m).all():
+            out[storage]["non_finite"] = True
+    ref = mom["f64"]
+    umax = float(np.abs(ref[1:]).max())
+    for storage in ("f32", "f16"):
+        d = mom[storage] - ref
+        out[storage]["errors_vs_f64"] = {
+            "rho_max": float(np.abs(d[0]).max()), "rho_l2": float(np.sqrt((d[0] ** 2).mean())),
+            "u_max_over_umax": float(np.abs(d[1:]).max() / umax),
+            "u_l2_relative": float(np.sqrt((d[1:] ** 2).sum() / (ref[1:] ** 2).sum()))}
+    out["u_max_f64"] = umax
+    return out
+
+
 def serving_bench(count=400, rate=200.0, budget_gb=200.0, tick_steps=64):
     """Config 5: a Poisson stream of D3Q19 channel solves at `rate` req/s (sizes
     log-uniform in [24^3, 128^3], 500-5000 steps, inflow velocity N(0.05, 0.01^2), FP32 or
@@ -285,6 +326,9 @@ def main():
                     help="skip the 512-lattice ensemble measurement (config 4)")
     ap.add_argument("--ensemble-members", type=int, default=512)
     ap.add_argument("--ensemble-steps", type=int, default=2000)
+    ap.add_argument("--no-turbulent", dest="turbulent", action="store_false",
+                    help="skip the D3Q27 Smagorinsky channel precision run (config 3)")
+    ap.add_argument("--turbulent-steps", type=int, default=5000)
     ap.add_argument("--no-serving", dest="serving", action="store_false",
                     help="skip the continuous-batching serving measurement (config 5)")
     ap.add_argument("--serving-requests", type=int, default=400)
@@ -352,6 +396,11 @@ def main():
         log("ensemble (config 4) ...")
         ensemble = ensemble_bench(args.ensemble_members, args.ensemble_steps)
 
+    turbulent = None
+    if args.turbulent:
+        log("turbulent channel (config 3) ...")
+        turbulent = turbulent_bench(steps=args.turbulent_steps)
+
     served = None
     if args.serving:
         log("serving (config 5) ...")
@@ -401,6 +450,7 @@ def main():
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },
         "cpu_baseline": cpu,
+        "turbulent_channel": turbulent,
         "ensemble": ensemble,
         "serving": served,
     }
