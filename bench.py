@@ -276,21 +276,42 @@ def make_case(args):
     raise SystemExit(f"unknown workload {args.workload}")
 
 
-def cpu_baseline(case, storage, seconds):
-    """The oracle on the host cores, on a bounded sample of the same workload."""
-    from oracle import oracle as O
+def cpu_sample(case):
+    """The same generator at 128^3 at most: MLUPS is normalised per cell update."""
     from openlb_amd import workloads as W
-    threads = min(16, os.cpu_count() or 1)
-    # same generator, a smaller block: MLUPS is normalised per cell update
     shape = tuple(min(n, 128) for n in case.shape)
     if case.name == "porous_channel":
-        sample = W.porous_channel(shape, tau=0.55, force=case.force[0])
-    elif case.name == "taylor_green":
-        sample = W.taylor_green(shape, u0=0.05, tau=1.0 / case.omega)
-    elif case.name == "turbulent_channel":
-        sample = W.turbulent_channel(shape)
-    else:
-        sample = W.lid_driven_cavity(shape)
+        return W.porous_channel(shape, tau=0.55, force=case.force[0])
+    if case.name == "taylor_green":
+        return W.taylor_green(shape, u0=0.05, tau=1.0 / case.omega)
+    if case.name == "turbulent_channel":
+        return W.turbulent_channel(shape)
+    return W.lid_driven_cavity(shape)
+
+
+def cpu_baseline(case, storage, seconds):
+    """OpenLB's own CPU path (CPU_SIMD + OpenMP, compiled from the reference sources
+    into oracle/_ref/olb_ref_simd_f32) on the host cores, on a bounded sample of the same
+    workload. If that binary is absent or the storage is not FP32, the oracle is timed
+    instead (kind "port"; about 8x slower than OpenLB's SIMD path on the same cores)."""
+    from oracle import compare_cpu
+    threads = min(16, os.cpu_count() or 1)
+    sample = cpu_sample(case)
+    shape = sample.shape
+    if storage == "f32" and os.path.exists(compare_cpu.REF_SIMD):
+        probe = compare_cpu.time_reference(sample, 10, threads)
+        steps = max(10, int(seconds * probe["mlups"] * 1e6 / sample.n_active))
+        res = compare_cpu.time_reference(sample, steps, threads)
+        return {"value": round(res["mlups"], 2), "unit": "MLUPS", "cores": threads, "kind": "reference",
+                "sample": f"{sample.name} {shape[0]}x{shape[1]}x{shape[2]} {storage}, {steps} steps in "
+                          f"{res['seconds']:.1f} s, OpenLB 1.8 CPU_SIMD (AVX2) + OpenMP, {threads} threads, "
+                          f"oracle/_ref/olb_ref_simd_f32"}
+    return cpu_baseline_port(sample, storage, seconds, threads)
+
+
+def cpu_baseline_port(sample, storage, seconds, threads):
+    from oracle import oracle as O
+    shape = sample.shape
     lat = O.OracleLattice(sample, storage)
     lat.upload(O.initial_populations(sample, storage))
     lat.step(2, threads)  # warm-up
