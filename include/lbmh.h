@@ -136,6 +136,17 @@ int lbmh_moments(lbmh_lattice* lattice, double* rho_u);
  * (src/core/platform/gpu/cuda/fieldReduction.hh:152-170). */
 int lbmh_reduce_moments(lbmh_lattice* lattice, double out[4]);
 
+/* Lattice statistics in FP64 over the cells whose collision reports one:
+ * bulk cells (rho and |u|^2, u with the F/2 shift when forced) and moving
+ * walls (the wall's rho and |u_w|^2); NoDynamics and BounceBack cells count
+ * none.  out = {cells counted, average rho, average energy 0.5*<|u|^2>,
+ * max |u|}.  Evaluated on the current populations, i.e. the values the next
+ * collideAndStream would accumulate.  Replaces LatticeStatistics'
+ * Aggregatable + reset() (src/core/latticeStatistics.h:49-71,
+ * src/core/latticeStatistics.hh:75-90), which OpenLB fills during collision
+ * when statistics are enabled. */
+int lbmh_statistics(lbmh_lattice* lattice, double out[4]);
+
 /* Wait for all work queued on the lattice's device.  Replaces
  * gpu::cuda::device::synchronize (src/core/platform/gpu/cuda/device.hh:64-69). */
 int lbmh_synchronize(lbmh_lattice* lattice);

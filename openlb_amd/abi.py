@@ -22,7 +22,7 @@ EXPORTS = (
     "lbmh_lattice_create", "lbmh_lattice_destroy", "lbmh_set_dynamics_ids",
     "lbmh_set_bulk_params", "lbmh_set_wall", "lbmh_populations_upload",
     "lbmh_populations_download", "lbmh_ini_equilibrium", "lbmh_step",
-    "lbmh_get_step_count", "lbmh_get_shifts", "lbmh_moments", "lbmh_reduce_moments",
+    "lbmh_get_step_count", "lbmh_get_shifts", "lbmh_moments", "lbmh_reduce_moments", "lbmh_statistics",
     "lbmh_synchronize", "lbmh_lattice_footprint", "lbmh_abi_version", "lbmh_device_count",
     "lbmh_device_arch", "lbmh_last_error", "lbmh_set_launch_config", "lbmh_get_launch_config",
     "lbmh_autotune",
@@ -99,6 +99,7 @@ def load() -> ctypes.CDLL:
     L.lbmh_get_shifts.argtypes = [vp, vp]
     L.lbmh_moments.argtypes = [vp, vp]
     L.lbmh_reduce_moments.argtypes = [vp, vp]
+    L.lbmh_statistics.argtypes = [vp, vp]
     L.lbmh_synchronize.argtypes = [vp]
     L.lbmh_lattice_footprint.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
     L.lbmh_device_count.argtypes = [ctypes.POINTER(i32)]

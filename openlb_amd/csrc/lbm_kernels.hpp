@@ -351,6 +351,103 @@ __global__ __launch_bounds__(kBlock) void reduce_moments_kernel(double* partial,
   }
 }
 
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  return v;
+}
+
+// Lattice statistics of OpenLB (LatticeStatistics::Aggregatable, src/core/
+// latticeStatistics.h:49-71): over the cells whose collision reports a
+// statistic -- bulk cells {rho, |u|^2} (u with the F/2 shift when forced) and
+// moving walls {rho_w, |u_w|^2} (collision.h:343); BounceBack reports none.
+// Partial per workgroup: {count, sum rho, sum |u|^2, max |u|^2}.
+template <int Q, typename S>
+__global__ __launch_bounds__(kBlock) void statistics_kernel(double* partial, const uint8_t* ids, const S* pop,
+                                                            CoreGeom g, Shifts<Q> sh, const double* walls,
+                                                            double hfx, double hfy, double hfz, int forced)
+{
+  using V = typename StorageTraits<S>::V;
+  double acc[4] = {0, 0, 0, 0};
+  for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < g.ncore; c += gridDim.x * kBlock) {
+    const int id = ids ? ids[c] : 1;
+    if (id == 0 || id == 2) continue;
+    double rho, usq;
+    if (id >= 3) {
+      const double* w = walls + 4 * id;
+      rho = w[0];
+      usq = w[1] * w[1] + w[2] * w[2] + w[3] * w[3];
+    } else {
+      const uint32_t n = padded_of_core(g, c);
+      V f[Q];
+#pragma unroll
+      for (int i = 0; i < Q; ++i) f[i] = load_pop<S, V>(pop + uint64_t(i) * g.stride + wrap_index(n, sh.s[i], g.n_padded));
+      V drho, j[3];
+      moments<Q, V>(f, drho, j);
+      const V r = V(1) + drho;
+      V u[3] = {j[0] / r, j[1] / r, j[2] / r};
+      if (forced) {
+        u[0] += V(hfx);
+        u[1] += V(hfy);
+        u[2] += V(hfz);
+      }
+      rho = double(r);
+      usq = double(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    }
+    acc[0] += 1.0;
+    acc[1] += rho;
+    acc[2] += usq;
+    acc[3] = fmax(acc[3], usq);
+  }
+  __shared__ double lds[4][kBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double s = wave_sum(acc[k]);
+    if (lane == 0) lds[k][wave] = s;
+  }
+  {
+    const double m = wave_max(acc[3]);
+    if (lane == 0) lds[3][wave] = m;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w)
+      s = threadIdx.x == 3 ? fmax(s, lds[3][w]) : s + lds[threadIdx.x][w];
+    partial[blockIdx.x * 4 + threadIdx.x] = s;
+  }
+}
+
+static __global__ __launch_bounds__(kBlock) void statistics_partials_kernel(double* out, const double* partial,
+                                                                            uint32_t nparts)
+{
+  __shared__ double lds[4][kBlock / 64];
+  double acc[4] = {0, 0, 0, 0};
+  for (uint32_t p = threadIdx.x; p < nparts; p += kBlock) {
+    for (int k = 0; k < 3; ++k) acc[k] += partial[p * 4 + k];
+    acc[3] = fmax(acc[3], partial[p * 4 + 3]);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double s = wave_sum(acc[k]);
+    if (lane == 0) lds[k][wave] = s;
+  }
+  {
+    const double m = wave_max(acc[3]);
+    if (lane == 0) lds[3][wave] = m;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w)
+      s = threadIdx.x == 3 ? fmax(s, lds[3][w]) : s + lds[threadIdx.x][w];
+    out[threadIdx.x] = s;
+  }
+}
+
 // Second level: one workgroup sums the partials in a fixed order.
 static __global__ __launch_bounds__(kBlock) void reduce_partials_kernel(double* out, const double* partial,
                                                                         uint32_t nparts)

@@ -459,6 +459,37 @@ int lbmh_reduce_moments(lbmh_lattice* L, double out[4])
   return LBMH_OK;
 }
 
+int lbmh_statistics(lbmh_lattice* L, double out[4])
+{
+  if (!L || !out) return fail(LBMH_E_INVALID, "null argument");
+  DeviceGuard guard(L->device);
+  const CoreGeom g = core_geom(L);
+  const uint32_t blocks = std::min<uint32_t>(L->n_partials, grid_for(L->ncore, kBlock));
+  const int forced = (L->kind == LBMH_FORCED_BGK || L->kind == LBMH_FORCED_TRT) ? 1 : 0;
+  const double* hf = L->params.force;
+  LBMH_HIP(hipDeviceSynchronize());
+  const int rc = with_types(L, [&](auto qc, auto s) -> int {
+    constexpr int Q = decltype(qc)::value;
+    using S = decltype(s);
+    hipLaunchKernelGGL((statistics_kernel<Q, S>), dim3(blocks), dim3(kBlock), 0, 0, L->d_partials,
+                       L->uniform_bulk ? nullptr : L->d_ids, static_cast<const S*>(L->d_pop), g, shifts_of<Q>(L),
+                       static_cast<const double*>(L->d_walls), hf[0] * 0.5, hf[1] * 0.5, hf[2] * 0.5, forced);
+    LBMH_HIP(hipGetLastError());
+    return LBMH_OK;
+  });
+  if (rc != LBMH_OK) return rc;
+  double* result = L->d_partials + 4 * L->n_partials;
+  hipLaunchKernelGGL(statistics_partials_kernel, dim3(1), dim3(kBlock), 0, 0, result, L->d_partials, blocks);
+  LBMH_HIP(hipGetLastError());
+  double acc[4];
+  LBMH_HIP(hipMemcpy(acc, result, sizeof(double) * 4, hipMemcpyDeviceToHost));
+  out[0] = acc[0];
+  out[1] = acc[0] > 0 ? acc[1] / acc[0] : 0.0;
+  out[2] = acc[0] > 0 ? 0.5 * acc[2] / acc[0] : 0.0;
+  out[3] = std::sqrt(acc[3]);
+  return LBMH_OK;
+}
+
 int lbmh_synchronize(lbmh_lattice* L)
 {
   if (!L) return fail(LBMH_E_INVALID, "null lattice");

@@ -23,7 +23,7 @@
 using namespace olb::hip;
 
 template <typename T>
-int run(int size, long steps, double tau, const char* dump)
+int run(int size, long steps, double tau, const char* dump, bool stats)
 {
   using DESCRIPTOR = descriptors::D3Q19;
   const int n = size + 1;
@@ -53,6 +53,12 @@ int run(int size, long steps, double tau, const char* dump)
 
   lattice.setProcessingContext(ProcessingContext::Evaluation);
   std::cout << size << ", " << steps << ", " << 1 << ", " << 1 << ", " << 1 << ", " << mlups << std::endl;
+  if (stats) {
+    const auto st = lattice.getStatistics();
+    std::printf("statistics %zu %.17g %.17g %.17g\n", st.getNumCells(), st.getAverageRho(), st.getAverageEnergy(),
+                st.getMaxU());
+    std::fflush(stdout);
+  }
   if (dump) {
     std::FILE* f = std::fopen(dump, "wb");
     if (!f) throw std::runtime_error(std::string("cannot open ") + dump);
@@ -69,6 +75,7 @@ int main(int argc, char** argv)
   long steps = 100;
   bool fp64 = false;
   const char* dump = nullptr;
+  bool stats = false;
   double tau = -1;
   for (int i = 1; i < argc; ++i) {
     if (!std::strcmp(argv[i], "--size") && i + 1 < argc) size = std::atoi(argv[++i]);
@@ -76,15 +83,16 @@ int main(int argc, char** argv)
     else if (!std::strcmp(argv[i], "--fp64")) fp64 = true;
     else if (!std::strcmp(argv[i], "--dump") && i + 1 < argc) dump = argv[++i];
     else if (!std::strcmp(argv[i], "--tau") && i + 1 < argc) tau = std::atof(argv[++i]);
+    else if (!std::strcmp(argv[i], "--stats")) stats = true;
     else if (!std::strcmp(argv[i], "--no-results")) continue;
     else {
-      std::fprintf(stderr, "usage: %s [--size N] [--steps K] [--fp64] [--tau T] [--dump FILE]\n", argv[0]);
+      std::fprintf(stderr, "usage: %s [--size N] [--steps K] [--fp64] [--tau T] [--dump FILE] [--stats]\n", argv[0]);
       return 2;
     }
   }
   if (tau <= 0) tau = 0.5 + 3.0 * 1e-5 * size;  // nu = nu_phys * dt / dx^2 with u_lattice 0.01
   try {
-    return fp64 ? run<double>(size, steps, tau, dump) : run<float>(size, steps, tau, dump);
+    return fp64 ? run<double>(size, steps, tau, dump, stats) : run<float>(size, steps, tau, dump, stats);
   } catch (const std::exception& e) {
     std::fprintf(stderr, "error: %s\n", e.what());
     return 1;

@@ -250,6 +250,27 @@ class SuperLattice {
 
   void synchronize() const { if (_handle) detail::check(lbmh_synchronize(_handle)); }
   void statisticsOff() {}
+  void statisticsOn() {}
+
+  /// Lattice statistics, computed on demand on the device in FP64 (no cost in
+  /// the time step).  OpenLB's getStatistics() after step k reports the
+  /// populations the collision of step k saw; this reports the current ones,
+  /// i.e. what step k+1 will see (src/core/latticeStatistics.h:49-71).
+  struct Statistics {
+    std::size_t numCells;
+    double averageRho, averageEnergy, maxU;
+    double getAverageRho() const { return averageRho; }
+    double getAverageEnergy() const { return averageEnergy; }
+    double getMaxU() const { return maxU; }
+    std::size_t getNumCells() const { return numCells; }
+  };
+
+  Statistics getStatistics() const
+  {
+    double out[4];
+    detail::check(lbmh_statistics(_handle, out));
+    return Statistics{std::size_t(out[0]), out[1], out[2], out[3]};
+  }
 
   /// FP64 {sum rho, sum j} over bulk cells.
   std::array<double, 4> conservedMoments() const

@@ -142,6 +142,15 @@ class BlockLattice:
         abi.check(abi.load().lbmh_reduce_moments(self._h, _ptr(out)))
         return out
 
+    def statistics(self) -> dict:
+        """OpenLB's lattice statistics on the current populations (src/core/
+        latticeStatistics.hh:75-90): cells counted, average rho, average
+        energy 0.5*<|u|^2> and max |u| over bulk cells and moving walls."""
+        out = np.zeros(4)
+        abi.check(abi.load().lbmh_statistics(self._h, _ptr(out)))
+        return {"cells": int(out[0]), "average_rho": float(out[1]), "average_energy": float(out[2]),
+                "max_u": float(out[3])}
+
     def set_launch_config(self, lds_bytes: int, xcd_group: int = 0):
         cfg = abi.LaunchConfig(int(lds_bytes), int(xcd_group))
         abi.check(abi.load().lbmh_set_launch_config(self._h, ctypes.byref(cfg)))

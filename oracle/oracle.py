@@ -140,3 +140,29 @@ def initial_populations(case, storage: str = "f64") -> np.ndarray:
         f = np.where((case.ids == 1)[None], f, plain)
     f[:, case.ids == 0] = 0.0
     return f
+
+
+def statistics(case, rho_u: np.ndarray) -> np.ndarray:
+    """OpenLB's lattice statistics of one state: {cells, average rho, average
+    energy 0.5*<|u|^2>, max |u|} (LatticeStatistics::Aggregatable and reset(),
+    src/core/latticeStatistics.h:49-71, latticeStatistics.hh:75-90).
+
+    ``rho_u`` holds the per-cell moments as OracleLattice.moments() returns
+    them (bulk u with the F/2 shift when forced).  Bulk cells report their
+    own rho and |u|^2; a moving wall reports the rho and velocity it was
+    defined with (src/dynamics/collision.h:343-345); NoDynamics and
+    BounceBack report nothing (collision.h:52, 309).  Sums in FP64."""
+    ids = case.ids
+    bulk = ids == 1
+    rho = [rho_u[0][bulk]]
+    usq = [(rho_u[1:, bulk] ** 2).sum(axis=0)]
+    for wid, (wrho, wu) in case.walls.items():
+        n = int((ids == wid).sum())
+        rho.append(np.full(n, float(wrho)))
+        usq.append(np.full(n, float(np.dot(wu, wu))))
+    rho = np.concatenate(rho)
+    usq = np.concatenate(usq)
+    n = rho.size
+    if n == 0:
+        return np.zeros(4)
+    return np.array([n, rho.sum() / n, 0.5 * usq.sum() / n, np.sqrt(usq.max())])

@@ -210,12 +210,26 @@ static void runCase(const CaseSpec& c, const std::string& dir)
     return;
   }
 
+  // Statistics mode: OpenLB accumulates the statistics of step k during its
+  // collision, i.e. from the populations after k-1 steps; the first step's
+  // averages are OpenLB's placeholders (latticeStatistics.hh:64-72).
+  const bool stats = std::getenv("OLB_REF_STATS") != nullptr;
+  if (stats) lattice.statisticsOn();
   long done = 0;
   for (long target : c.steps) {
     for (; done < target; ++done) {
       lattice.collideAndStream();
     }
     dumpPopulations<DESCRIPTOR>(c, lattice, dir, target);
+    if (stats) {
+      auto& st = lattice.getStatistics();
+      const std::string name = dir + "/stats_" + std::to_string(target) + ".txt";
+      std::FILE* f = std::fopen(name.c_str(), "w");
+      if (!f) throw std::runtime_error("cannot write " + name);
+      std::fprintf(f, "%zu %.17g %.17g %.17g\n", std::size_t(st.getNumCells()), double(st.getAverageRho()),
+                   double(st.getAverageEnergy()), double(st.getMaxU()));
+      std::fclose(f);
+    }
   }
 }
 

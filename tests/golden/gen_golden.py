@@ -11,7 +11,8 @@ It stores OpenLB's populations as ``<name>.npz``. The populations are shifted,
 f_i - t_i, on the core cells, in [q][nx][ny][nz] layout. Snapshot 0 is the state
 after ``SuperLattice::initialize()``, and snapshot k is the state after k calls of
 ``collideAndStream()``. A ``manifest.json`` records the parameters and a SHA-256
-of every array.
+of every array, and ``statistics.json`` OpenLB's lattice statistics of each
+fixture's last snapshot (``--statistics-only`` writes just that file).
 """
 from __future__ import annotations
 
@@ -157,5 +158,30 @@ def main():
         json.dump(manifest, fh, indent=1, sort_keys=True)
 
 
+def write_statistics():
+    """statistics.json: OpenLB's lattice statistics of each fixture's last
+    snapshot.  OpenLB accumulates the statistics of step k+1 during its
+    collision, i.e. from the populations after k steps, so the reference runs
+    k+1 steps with statistics on (OLB_REF_STATS) and reports those of step
+    k+1 as the statistics of snapshot k."""
+    out = {}
+    for name, case, steps, precision in cases():
+        k = steps[-1]
+        exe = os.path.join(ROOT, "oracle", "_ref", f"olb_ref_{precision}")
+        with tempfile.TemporaryDirectory() as d:
+            write_case_dir(d, case, [k + 1])
+            subprocess.run([exe, d], check=True, cwd=d, stdout=subprocess.DEVNULL,
+                           env=dict(os.environ, OLB_REF_STATS="1"))
+            with open(os.path.join(d, f"stats_{k + 1}.txt")) as fh:
+                cells, rho, energy, umax = fh.read().split()
+        out[name] = {"snapshot": k, "cells": int(cells), "average_rho": float(rho),
+                     "average_energy": float(energy), "max_u": float(umax)}
+        print(f"{name}: statistics of snapshot {k}", flush=True)
+    with open(os.path.join(HERE, "statistics.json"), "w") as fh:
+        json.dump(out, fh, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
-    main()
+    if "--statistics-only" not in sys.argv:
+        main()
+    write_statistics()

@@ -56,3 +56,17 @@ def rel_error(got: np.ndarray, ref: np.ndarray, q: int, mask=None) -> float:
         g, r = g[:, m], r[:, m]
     scale = np.abs(r + weights(q)[:, None]).max()
     return float(np.abs(g - r).max() / scale)
+
+
+def statistics(name: str) -> dict:
+    """OpenLB's lattice statistics of the fixture's last snapshot (statistics.json)."""
+    with open(os.path.join(GOLDEN, "statistics.json")) as fh:
+        return json.load(fh)[name]
+
+
+def statistics_error(got, ref: dict) -> float:
+    """Largest relative difference of {average rho, average energy, max |u|};
+    the cell counts must agree exactly."""
+    assert int(got[0]) == ref["cells"], (got[0], ref["cells"])
+    keys = ("average_rho", "average_energy", "max_u")
+    return max(abs(float(g) - ref[k]) / max(abs(ref[k]), 1e-300) for g, k in zip(got[1:], keys))

@@ -142,3 +142,45 @@ def test_member_matches_oracle(gpu_available):
     ora.step(60, threads=4)
     assert G.rel_error(got, ora.download(), 19, c.ids == 1) <= 1e-13
     np.testing.assert_allclose(mom[:, c.ids == 1], ora.moments()[:, c.ids == 1], atol=1e-13)
+
+
+def test_nan_in_one_member_leaves_the_others_untouched(gpu_available):
+    """Members share one launch but no memory: a diverged member (NaN seeded in
+    one cell) must not leak into its neighbours in the arena, which stay
+    bit-identical to their standalone runs; the diverged member shows the NaN
+    in its moments."""
+    cases = small_members(seed=11, count=5)
+    bad = 2
+    with Ensemble(19, "f32", "bgk", max_members=16, population_bytes=64 << 20, cell_capacity=1 << 20) as ens:
+        members, f0s = [], []
+        for k, c in enumerate(cases):
+            m = ens.add(c)
+            f0 = O.initial_populations(c, "f32")
+            f0s.append(f0)
+            seeded = f0.copy()
+            if k == bad:
+                centre = tuple(n // 2 for n in c.shape)
+                seeded[(slice(None),) + centre] = np.nan
+            ens.upload(m, seeded)
+            members.append(m)
+        ens.step(32, graph=True)
+        for k, (m, c, f0) in enumerate(zip(members, cases, f0s)):
+            got = ens.download(m)
+            if k == bad:
+                assert np.isnan(got).any()
+                assert np.isnan(ens.moments(m)).any()
+            else:
+                assert np.isfinite(got).all()
+                np.testing.assert_array_equal(got, standalone(c, "f32", f0, 32))
+
+
+def test_statistics_flag_a_diverged_lattice(gpu_available):
+    """A NaN population shows up in the lattice statistics (average rho, max |u|)."""
+    case = W.lid_driven_cavity((10, 9, 12), tau=0.6, lid_u=0.05)
+    f0 = O.initial_populations(case, "f32")
+    f0[:, 4, 4, 5] = np.nan
+    with BlockLattice.from_case(case, "f32") as lat:
+        lat.upload_populations(f0, "f64")
+        lat.collide_and_stream(3)
+        st = lat.statistics()
+    assert np.isnan(st["average_rho"]) and np.isnan(st["average_energy"])

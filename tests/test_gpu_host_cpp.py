@@ -22,12 +22,13 @@ EXE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
 def test_cpp_cavity_benchmark_matches_oracle(gpu_available, storage, tmp_path):
     size, steps = 24, 50
     dump = tmp_path / "pop.bin"
-    cmd = [EXE, "--size", str(size), "--steps", str(steps), "--dump", str(dump)]
+    cmd = [EXE, "--size", str(size), "--steps", str(steps), "--dump", str(dump), "--stats"]
     if storage == "f64":
         cmd.append("--fp64")
     out = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
-    fields = [s.strip() for s in out.stdout.strip().splitlines()[-1].split(",")]
+    lines = out.stdout.strip().splitlines()
+    fields = [s.strip() for s in [ln for ln in lines if "," in ln][-1].split(",")]
     assert fields[:5] == [str(size), str(steps), "1", "1", "1"]
     assert float(fields[5]) > 0
     case = W.cavity_benchmark(size)
@@ -38,6 +39,12 @@ def test_cpp_cavity_benchmark_matches_oracle(gpu_available, storage, tmp_path):
     err = G.rel_error(got, ora.download(), 19, case.ids == 1)
     print(f"C++ host layer cavity {storage}: rel {err:.3e}", flush=True)
     assert err <= (1e-13 if storage == "f64" else 1e-5)
+    # getStatistics() of the C++ layer: statistics of the dumped state
+    st = [float(v) for v in [ln for ln in lines if ln.startswith("statistics ")][-1].split()[1:]]
+    ora.upload(got)
+    ref = O.statistics(case, ora.moments())
+    assert st[0] == ref[0]
+    np.testing.assert_allclose(st[1:], ref[1:], rtol=1e-12 if storage == "f64" else 1e-6)
 
 
 def test_cpp_layer_reports_errors_as_exceptions(gpu_available):

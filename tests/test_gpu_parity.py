@@ -247,3 +247,41 @@ def test_launch_configs_give_identical_results(gpu_available):
         assert lat.launch_config() == chosen
         assert lat.step_count() == 0
         np.testing.assert_array_equal(lat.download_populations("f32"), before)
+
+
+STATS_TOL = {"f64": 1e-12, "f32": 3e-5}
+
+
+@pytest.mark.parametrize("name", G.names())
+def test_gpu_statistics_match_reference(gpu_available, name):
+    """Device FP64 statistics of the fixture's last snapshot against the values
+    OpenLB reported for it (cells, average rho, average energy, max |u|)."""
+    case, meta, arrays = G.load(name)
+    ref = G.statistics(name)
+    with BlockLattice.from_case(case, meta["precision"]) as lat:
+        lat.upload_populations(arrays[f"f_{ref['snapshot']}"].astype(np.float64), "f64")
+        st = lat.statistics()
+    got = [st["cells"], st["average_rho"], st["average_energy"], st["max_u"]]
+    err = G.statistics_error(got, ref)
+    print(f"{name}: statistics rel {err:.3e}", flush=True)
+    assert err <= STATS_TOL[meta["precision"]]
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16"])
+def test_gpu_statistics_after_steps_match_oracle(gpu_available, storage):
+    """Statistics after stepping, on a case with holes (NoDynamics cells), a
+    BounceBack obstacle and a moving wall, against the oracle's."""
+    case = _with_holes(W.lid_driven_cavity((11, 9, 70), tau=0.6, lid_u=0.1))
+    f0 = O.initial_populations(case, storage)
+    with BlockLattice.from_case(case, storage) as lat:
+        lat.upload_populations(f0, "f64")
+        lat.collide_and_stream(15)
+        st = lat.statistics()
+        f = lat.download_populations("f64")
+    ora = O.OracleLattice(case, storage)
+    ora.upload(f)
+    ref = O.statistics(case, ora.moments())
+    got = np.array([st["cells"], st["average_rho"], st["average_energy"], st["max_u"]])
+    assert got[0] == ref[0]
+    tol = {"f64": 1e-13, "f32": 1e-6, "f16": 1e-6}[storage]
+    np.testing.assert_allclose(got[1:], ref[1:], rtol=tol)

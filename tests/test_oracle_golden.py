@@ -37,6 +37,41 @@ def test_oracle_matches_reference(name):
         assert err_all <= TOL[precision], f"{name} step {s} (all cells): {err_all:.3e}"
 
 
+# OpenLB accumulates statistics in the lattice's own precision (T) cell by cell:
+# in FP32 the running sum over ~3000 cells carries a few 1e-6 of rounding
+STATS_TOL = {"f64": 1e-12, "f32": 3e-5}
+
+
+@pytest.mark.parametrize("name", G.names())
+def test_oracle_statistics_match_reference(name):
+    """Lattice statistics (cells, average rho, average energy, max |u|) of the
+    fixture's last snapshot against what OpenLB reported for it."""
+    case, meta, arrays = G.load(name)
+    ref = G.statistics(name)
+    lat = O.OracleLattice(case, meta["precision"])
+    lat.upload(arrays[f"f_{ref['snapshot']}"].astype(np.float64))
+    got = O.statistics(case, lat.moments())
+    err = G.statistics_error(got, ref)
+    assert err <= STATS_TOL[meta["precision"]], f"{name}: {err:.3e}"
+
+
+def test_oracle_statistics_definitions():
+    """Hand-built state: two bulk cells, one BounceBack, one NoDynamics and one
+    moving wall, so every term of the statistics is known in closed form."""
+    from openlb_amd import workloads as W
+    ids = np.array([1, 1, 2, 0, 3], dtype=np.uint8).reshape(5, 1, 1)
+    case = W.Case("t", 19, "bgk", (5, 1, 1), (1, 1, 1), ids, np.zeros((4, 5, 1, 1)), 1.0, 0.25, 0.0,
+                  (0.0, 0.0, 0.0), {3: (1.2, (0.1, 0.0, 0.0))})
+    rho_u = np.zeros((4, 5, 1, 1))
+    rho_u[0, :, 0, 0] = [0.9, 1.1, 7.0, 9.0, 5.0]
+    rho_u[1:, 0, 0, 0] = [0.3, 0.0, 0.4]     # |u|^2 = 0.25
+    rho_u[1:, 1, 0, 0] = [0.0, 0.1, 0.0]     # |u|^2 = 0.01
+    rho_u[1:, 2, 0, 0] = [5.0, 5.0, 5.0]     # BounceBack: ignored
+    got = O.statistics(case, rho_u)
+    np.testing.assert_allclose(got, [3, (0.9 + 1.1 + 1.2) / 3, 0.5 * (0.25 + 0.01 + 0.01) / 3, 0.5],
+                               rtol=1e-15)
+
+
 @pytest.mark.parametrize("name", G.names())
 def test_oracle_equilibrium_matches_reference_init(name):
     """iniEquilibrium: the oracle's second-order equilibrium against OpenLB's
