@@ -33,6 +33,7 @@ class Case:
     smago: float = 0.0
     force: tuple[float, float, float] = (0.0, 0.0, 0.0)
     walls: dict = dataclasses.field(default_factory=dict)   # id -> (rho, (ux, uy, uz))
+    local: dict = dataclasses.field(default_factory=dict)   # id -> "velocity" | "pressure"
 
     @property
     def ncore(self) -> int:
@@ -149,6 +150,66 @@ def cavity_benchmark(size=100, q=19):
     tau = 0.5 + 3.0 * 1e-5 * size
     return Case("cavity_benchmark", q, "bgk", (n, n, n), (0, 0, 0), ids, rho_u, 1.0 / tau,
                 walls={3: (1.0, (0.01, 0.0, 0.0))})
+
+
+def local_velocity_cavity(shape=(32, 32, 32), tau=0.6, lid_u=0.05, q=19):
+    """Lid-driven cavity with OpenLB's default boundary of the benchmark harness:
+    LocalVelocity (regularized) boundaries on the whole box, at rest on the walls
+    (material 2) and moving along x on the lid (material 3).  The lid is the top
+    face (y = ny-1) without its edges and corners, the cells OpenLB's
+    ``rename(2, 3, 1, lid)`` selects (cavity3d.cpp:39-57, 68-76, 113-131;
+    blockGeometry.hh:537-565)."""
+    ids = np.full(shape, 2, dtype=np.uint8)
+    ids[1:-1, 1:-1, 1:-1] = 1
+    ids[1:-1, -1, 1:-1] = 3
+    rho_u = np.zeros((4,) + tuple(shape))
+    rho_u[0] = 1.0
+    return Case("local_velocity_cavity", q, "bgk", tuple(shape), (0, 0, 0), ids, rho_u, 1.0 / tau,
+                walls={2: (1.0, (0.0, 0.0, 0.0)), 3: (1.0, (lid_u, 0.0, 0.0))},
+                local={2: "velocity", 3: "velocity"})
+
+
+def cavity_benchmark_local(size=100, q=19):
+    """OpenLB's benchmark cavity in its default configuration (LID_DRIVEN without
+    LID_DRIVEN_BOUNCE_BACK, cavity3d.cpp:32-36, 68-76): (size+1)^3 cells,
+    LocalVelocity boundaries, lid velocity 0.01, tau = 1/2 + 3e-5 * size."""
+    c = local_velocity_cavity((size + 1,) * 3, tau=0.5 + 3.0 * 1e-5 * size, lid_u=0.01, q=q)
+    c.name = "cavity_benchmark_local"
+    return c
+
+
+def velocity_channel(shape=(24, 12, 10), u_in=0.03, rho_out=1.0, tau=0.6, q=19):
+    """Duct with a LocalVelocity inlet (x = 0, material 3, uniform u_in along x),
+    a LocalPressure outlet (x = nx-1, material 4, density rho_out) and BounceBack
+    walls (material 2) on the y and z faces, including the inlet's and outlet's
+    rims.  Starts at rest."""
+    ids = np.full(shape, 2, dtype=np.uint8)
+    ids[:, 1:-1, 1:-1] = 1
+    ids[0, 1:-1, 1:-1] = 3
+    ids[-1, 1:-1, 1:-1] = 4
+    rho_u = np.zeros((4,) + tuple(shape))
+    rho_u[0] = 1.0
+    return Case("velocity_channel", q, "bgk", tuple(shape), (0, 0, 0), ids, rho_u, 1.0 / tau,
+                walls={3: (1.0, (u_in, 0.0, 0.0)), 4: (rho_out, (0.0, 0.0, 0.0))},
+                local={3: "velocity", 4: "pressure"})
+
+
+def step_cavity(shape=(14, 12, 16), block=(6, 5, 7), tau=0.6, lid_u=0.05, q=19):
+    """Lid-driven cavity whose lower corner holds a solid block of wall material
+    (x < block[0], y < block[1], z < block[2]) with LocalVelocity boundaries on
+    every wall: the block's convex edges and corner give OpenLB's internal edge
+    and corner boundary types next to the box's flat faces, external edges and
+    corners."""
+    c = local_velocity_cavity(shape, tau=tau, lid_u=lid_u, q=q)
+    bx, by, bz = block
+    c.ids[:bx, :by, :bz] = 2
+    # the block interior is surrounded by wall cells only: no dynamics there,
+    # like a cell OpenLB's innerClean() resets to material 0
+    inner = np.zeros(shape, dtype=bool)
+    inner[1:bx - 1, 1:by - 1, 1:bz - 1] = True
+    c.ids[inner] = 0
+    c.name = "step_cavity"
+    return c
 
 
 def cavity_ensemble(count=512, seed=20261204, lo=16, hi=96, tau=(0.52, 1.2), lid=(0.02, 0.1), q=19):

@@ -19,6 +19,25 @@ struct olo_lattice {
   int kind;
   double omega, magic, smago, force[3];
   double wall_rho[256], wall_u[256][3];
+  /* local boundaries, per padded cell: kind (OLK_*; 0 = none), discrete
+   * normal or (plane, n1, n2), and the defined rho / u */
+  uint8_t* lkind;
+  int8_t* lnorm;
+  double* lval;
+  int64_t n_edge, n_corner;
+  int64_t *edge_cells, *corner_cells;
+};
+
+/* per-cell local boundary kinds */
+enum {
+  OLK_NONE = 0,
+  OLK_FLAT_VELOCITY = 1,  /* CombinedRLB + RegularizedVelocityBoundaryTuple   momenta/aliases.h:127-132 */
+  OLK_FLAT_PRESSURE = 2,  /* CombinedRLB + RegularizedPressureBoundaryTuple   momenta/aliases.h:135-141 */
+  OLK_OUTER_EDGE = 3,     /* BGK + FixedVelocityBoundaryTuple, OuterVelocityEdgeProcessor3D */
+  OLK_OUTER_CORNER = 4,   /* BGK + FixedVelocityBoundaryTuple, OuterVelocityCornerProcessor3D */
+  OLK_INNER_EDGE = 5,     /* CombinedRLB + InnerEdgeVelocityTuple3D          momenta/aliases.h:204-209 */
+  OLK_INNER_CORNER = 6,   /* CombinedRLB + InnerCornerVelocityTuple3D        momenta/aliases.h:212-217 */
+  OLK_DEAD = 7            /* boundary-material cell without a matching type: no dynamics */
 };
 
 static char g_err[256];
@@ -145,6 +164,188 @@ static uint16_t float_to_half(float f)
 #undef LOAD
 #undef STOREV
 
+/* ---- boundary classification (computeBoundaryTypeAndNormal,
+ * src/geometry/discreteNormals.hh:174-1190).  OpenLB tests neighbour patterns
+ * in a fixed order: flat faces, external corners, internal corners, external
+ * edges, internal edges.  The first match of the first four groups is kept;
+ * an internal-edge match replaces whatever was found before.  Each rule
+ * lists the neighbours it tests: 'W' wall (neither fluid nor outside),
+ * 'F' fluid, 'N' not fluid.  The resulting normal points out of the fluid. */
+typedef struct { signed char dx, dy, dz; char pred; } olo_cond;
+typedef struct { signed char type, n[3], overwrite, ncond; olo_cond c[8]; } olo_rule;
+static const olo_rule RULES[46] = {
+  {0, {-1, 0, 0}, 0, 5, {{0, 0, 1, 'W'}, {0, 0, -1, 'W'}, {0, 1, 0, 'W'}, {0, -1, 0, 'W'}, {1, 0, 0, 'F'}}},
+  {0, {1, 0, 0}, 0, 5, {{0, 0, 1, 'W'}, {0, 0, -1, 'W'}, {0, 1, 0, 'W'}, {0, -1, 0, 'W'}, {-1, 0, 0, 'F'}}},
+  {0, {0, -1, 0}, 0, 5, {{0, 0, 1, 'W'}, {0, 0, -1, 'W'}, {1, 0, 0, 'W'}, {-1, 0, 0, 'W'}, {0, 1, 0, 'F'}}},
+  {0, {0, 1, 0}, 0, 5, {{0, 0, 1, 'W'}, {0, 0, -1, 'W'}, {1, 0, 0, 'W'}, {-1, 0, 0, 'W'}, {0, -1, 0, 'F'}}},
+  {0, {0, 0, -1}, 0, 5, {{1, 0, 0, 'W'}, {-1, 0, 0, 'W'}, {0, 1, 0, 'W'}, {0, -1, 0, 'W'}, {0, 0, 1, 'F'}}},
+  {0, {0, 0, 1}, 0, 5, {{1, 0, 0, 'W'}, {-1, 0, 0, 'W'}, {0, 1, 0, 'W'}, {0, -1, 0, 'W'}, {0, 0, -1, 'F'}}},
+  {1, {-1, -1, -1}, 0, 7, {{1, 0, 0, 'W'}, {0, 0, 1, 'W'}, {1, 0, 1, 'W'}, {0, 1, 0, 'W'}, {1, 1, 0, 'W'}, {0, 1, 1, 'W'}, {1, 1, 1, 'F'}}},
+  {1, {-1, 1, -1}, 0, 7, {{1, 0, 0, 'W'}, {0, 0, 1, 'W'}, {1, 0, 1, 'W'}, {0, -1, 0, 'W'}, {1, -1, 0, 'W'}, {0, -1, 1, 'W'}, {1, -1, 1, 'F'}}},
+  {1, {-1, 1, 1}, 0, 7, {{1, 0, 0, 'W'}, {0, 0, -1, 'W'}, {1, 0, -1, 'W'}, {0, -1, 0, 'W'}, {1, -1, 0, 'W'}, {0, -1, -1, 'W'}, {1, -1, -1, 'F'}}},
+  {1, {-1, -1, 1}, 0, 7, {{1, 0, 0, 'W'}, {0, 0, -1, 'W'}, {1, 0, -1, 'W'}, {0, 1, 0, 'W'}, {1, 1, 0, 'W'}, {0, 1, -1, 'W'}, {1, 1, -1, 'F'}}},
+  {1, {1, 1, 1}, 0, 7, {{-1, 0, 0, 'W'}, {0, 0, -1, 'W'}, {-1, 0, -1, 'W'}, {0, -1, 0, 'W'}, {0, -1, -1, 'W'}, {-1, -1, 0, 'W'}, {-1, -1, -1, 'F'}}},
+  {1, {1, -1, 1}, 0, 7, {{-1, 0, 0, 'W'}, {0, 0, -1, 'W'}, {-1, 0, -1, 'W'}, {0, 1, 0, 'W'}, {0, 1, -1, 'W'}, {-1, 1, 0, 'W'}, {-1, 1, -1, 'F'}}},
+  {1, {1, -1, -1}, 0, 7, {{-1, 0, 0, 'W'}, {0, 0, 1, 'W'}, {-1, 0, 1, 'W'}, {0, 1, 0, 'W'}, {0, 1, 1, 'W'}, {-1, 1, 0, 'W'}, {-1, 1, 1, 'F'}}},
+  {1, {1, 1, -1}, 0, 7, {{-1, 0, 0, 'W'}, {0, 0, 1, 'W'}, {-1, 0, 1, 'W'}, {0, -1, 0, 'W'}, {0, -1, 1, 'W'}, {-1, -1, 0, 'W'}, {-1, -1, 1, 'F'}}},
+  {2, {1, 1, 1}, 0, 6, {{-1, 0, 0, 'F'}, {0, 0, -1, 'F'}, {0, 0, 1, 'W'}, {1, 0, 0, 'W'}, {0, -1, 0, 'F'}, {0, 1, 0, 'W'}}},
+  {2, {1, -1, 1}, 0, 6, {{-1, 0, 0, 'F'}, {0, 0, -1, 'F'}, {0, 0, 1, 'W'}, {1, 0, 0, 'W'}, {0, 1, 0, 'F'}, {0, -1, 0, 'W'}}},
+  {2, {1, -1, -1}, 0, 6, {{-1, 0, 0, 'F'}, {0, 0, 1, 'F'}, {0, 0, -1, 'W'}, {1, 0, 0, 'W'}, {0, 1, 0, 'F'}, {0, -1, 0, 'W'}}},
+  {2, {1, 1, -1}, 0, 6, {{-1, 0, 0, 'F'}, {0, 0, 1, 'F'}, {0, 0, -1, 'W'}, {1, 0, 0, 'W'}, {0, -1, 0, 'F'}, {0, 1, 0, 'W'}}},
+  {2, {-1, 1, 1}, 0, 6, {{1, 0, 0, 'F'}, {0, 0, -1, 'F'}, {-1, 0, 0, 'W'}, {0, 0, 1, 'W'}, {0, -1, 0, 'F'}, {0, 1, 0, 'W'}}},
+  {2, {-1, -1, 1}, 0, 6, {{1, 0, 0, 'F'}, {0, 0, -1, 'F'}, {-1, 0, 0, 'W'}, {0, 0, 1, 'W'}, {0, 1, 0, 'F'}, {0, -1, 0, 'W'}}},
+  {2, {-1, 1, -1}, 0, 6, {{1, 0, 0, 'F'}, {0, 0, 1, 'F'}, {-1, 0, 0, 'W'}, {0, 0, -1, 'W'}, {0, -1, 0, 'F'}, {0, 1, 0, 'W'}}},
+  {2, {-1, -1, -1}, 0, 6, {{1, 0, 0, 'F'}, {0, 0, 1, 'F'}, {-1, 0, 0, 'W'}, {0, 0, -1, 'W'}, {0, 1, 0, 'F'}, {0, -1, 0, 'W'}}},
+  {3, {0, 1, -1}, 0, 7, {{-1, 0, 0, 'W'}, {1, 0, 0, 'W'}, {0, 0, 1, 'W'}, {1, 0, 1, 'N'}, {-1, 0, 1, 'N'}, {0, -1, 1, 'F'}, {0, -1, 0, 'W'}}},
+  {3, {0, -1, -1}, 0, 7, {{-1, 0, 0, 'W'}, {1, 0, 0, 'W'}, {0, 0, 1, 'W'}, {1, 0, 1, 'N'}, {-1, 0, 1, 'N'}, {0, 1, 1, 'F'}, {0, 1, 0, 'W'}}},
+  {3, {0, -1, 1}, 0, 7, {{-1, 0, 0, 'W'}, {1, 0, 0, 'W'}, {0, 0, -1, 'W'}, {1, 0, -1, 'N'}, {-1, 0, -1, 'N'}, {0, 1, -1, 'F'}, {0, 1, 0, 'N'}}},
+  {3, {0, 1, 1}, 0, 7, {{-1, 0, 0, 'W'}, {1, 0, 0, 'W'}, {0, 0, -1, 'W'}, {1, 0, -1, 'N'}, {-1, 0, -1, 'N'}, {0, -1, -1, 'F'}, {0, -1, 0, 'N'}}},
+  {3, {-1, 0, -1}, 0, 7, {{0, 1, 0, 'W'}, {0, -1, 0, 'W'}, {0, 0, 1, 'W'}, {1, 0, 1, 'F'}, {1, 0, 0, 'W'}, {-1, 0, 0, 'N'}, {0, 0, -1, 'N'}}},
+  {3, {1, 0, -1}, 0, 7, {{0, 1, 0, 'W'}, {0, -1, 0, 'W'}, {0, 0, 1, 'W'}, {-1, 0, 1, 'F'}, {-1, 0, 0, 'W'}, {1, 0, 0, 'N'}, {0, 0, -1, 'N'}}},
+  {3, {-1, 0, 1}, 0, 7, {{0, 1, 0, 'W'}, {0, -1, 0, 'W'}, {0, 0, -1, 'W'}, {1, 0, -1, 'F'}, {1, 0, 0, 'W'}, {-1, 0, 0, 'N'}, {0, 0, 1, 'N'}}},
+  {3, {1, 0, 1}, 0, 7, {{0, 1, 0, 'W'}, {0, -1, 0, 'W'}, {0, 0, -1, 'W'}, {-1, 0, -1, 'F'}, {-1, 0, 0, 'W'}, {1, 0, 0, 'N'}, {0, 0, 1, 'N'}}},
+  {3, {-1, -1, 0}, 0, 5, {{0, 0, 1, 'W'}, {0, 0, -1, 'W'}, {0, 1, 0, 'W'}, {1, 0, 0, 'N'}, {1, 1, 0, 'F'}}},
+  {3, {1, -1, 0}, 0, 5, {{0, 0, 1, 'W'}, {0, 0, -1, 'W'}, {0, 1, 0, 'W'}, {-1, 0, 0, 'N'}, {-1, 1, 0, 'F'}}},
+  {3, {1, 1, 0}, 0, 5, {{0, 0, 1, 'W'}, {0, 0, -1, 'W'}, {0, -1, 0, 'W'}, {-1, 0, 0, 'N'}, {-1, -1, 0, 'F'}}},
+  {3, {-1, 1, 0}, 0, 5, {{0, 0, 1, 'W'}, {0, 0, -1, 'W'}, {0, -1, 0, 'W'}, {1, 0, 0, 'N'}, {1, -1, 0, 'F'}}},
+  {4, {0, -1, -1}, 1, 8, {{-1, 0, 0, 'W'}, {1, 0, 0, 'W'}, {0, 0, -1, 'W'}, {0, 0, 1, 'F'}, {0, 1, 0, 'F'}, {0, -1, 0, 'W'}, {-1, -1, 0, 'W'}, {1, -1, 0, 'W'}}},
+  {4, {0, 1, -1}, 1, 8, {{-1, 0, 0, 'W'}, {1, 0, 0, 'W'}, {0, 0, -1, 'W'}, {0, 0, 1, 'F'}, {0, -1, 0, 'F'}, {0, 1, 0, 'W'}, {-1, 1, 0, 'W'}, {1, 1, 0, 'W'}}},
+  {4, {0, -1, 1}, 1, 8, {{1, 0, 0, 'W'}, {-1, 0, 0, 'W'}, {0, 0, 1, 'W'}, {0, 0, -1, 'F'}, {0, 1, 0, 'F'}, {-1, -1, 0, 'W'}, {0, -1, 0, 'W'}, {1, -1, 0, 'W'}}},
+  {4, {0, 1, 1}, 1, 8, {{1, 0, 0, 'W'}, {-1, 0, 0, 'W'}, {0, 0, 1, 'W'}, {0, 0, -1, 'F'}, {0, -1, 0, 'F'}, {-1, 1, 0, 'W'}, {0, 1, 0, 'W'}, {1, 1, 0, 'W'}}},
+  {4, {1, 0, 1}, 1, 8, {{-1, 0, 0, 'F'}, {1, 0, 0, 'W'}, {0, 1, 0, 'W'}, {0, -1, 0, 'W'}, {0, 0, -1, 'F'}, {0, 0, 1, 'W'}, {0, 1, 1, 'W'}, {0, -1, 1, 'W'}}},
+  {4, {1, 0, -1}, 1, 8, {{-1, 0, 0, 'F'}, {1, 0, 0, 'W'}, {0, 1, 0, 'W'}, {0, -1, 0, 'W'}, {0, 0, 1, 'F'}, {0, 0, -1, 'W'}, {0, 1, -1, 'W'}, {0, -1, -1, 'W'}}},
+  {4, {-1, 0, 1}, 1, 8, {{1, 0, 0, 'F'}, {-1, 0, 0, 'W'}, {0, -1, 0, 'W'}, {0, 1, 0, 'W'}, {0, 0, -1, 'F'}, {0, 0, 1, 'W'}, {0, 1, 1, 'W'}, {0, -1, 1, 'W'}}},
+  {4, {-1, 0, -1}, 1, 8, {{1, 0, 0, 'F'}, {-1, 0, 0, 'W'}, {0, -1, 0, 'W'}, {0, 1, 0, 'W'}, {0, 0, 1, 'F'}, {0, 0, -1, 'W'}, {0, 1, -1, 'W'}, {0, -1, -1, 'W'}}},
+  {4, {1, 1, 0}, 1, 6, {{0, -1, 0, 'F'}, {0, 0, -1, 'W'}, {0, 0, 1, 'W'}, {0, 1, 0, 'W'}, {-1, 0, 0, 'F'}, {-1, -1, 0, 'F'}}},
+  {4, {-1, 1, 0}, 1, 6, {{0, -1, 0, 'F'}, {0, 0, -1, 'W'}, {0, 0, 1, 'W'}, {0, 1, 0, 'W'}, {1, 0, 0, 'F'}, {1, -1, 0, 'F'}}},
+  {4, {1, -1, 0}, 1, 6, {{0, 1, 0, 'F'}, {0, 0, -1, 'W'}, {0, 0, 1, 'W'}, {0, -1, 0, 'W'}, {-1, 0, 0, 'F'}, {-1, 1, 0, 'F'}}},
+  {4, {-1, -1, 0}, 1, 6, {{0, 1, 0, 'F'}, {0, 0, -1, 'W'}, {0, 0, 1, 'W'}, {0, -1, 0, 'W'}, {1, 0, 0, 'F'}, {1, 1, 0, 'F'}}},
+};
+
+/* material of cell (x,y,z): periodic axes wrap, elsewhere beyond the block is outside */
+static int material_at(const int n[3], const int per[3], const uint8_t* m, int x, int y, int z)
+{
+  int r[3] = {x, y, z};
+  for (int d = 0; d < 3; ++d) {
+    if (r[d] < 0 || r[d] >= n[d]) {
+      if (!per[d]) return 0;
+      r[d] = (r[d] % n[d] + n[d]) % n[d];
+    }
+  }
+  return m[((int64_t)r[0] * n[1] + r[1]) * n[2] + r[2]];
+}
+
+static void classify_cell(const int n[3], const int per[3], const uint8_t* m, int x, int y, int z,
+                          int* type, int nrm[3])
+{
+  *type = 0;
+  nrm[0] = nrm[1] = nrm[2] = 0;
+  int found = 0;
+  for (int r = 0; r < 46; ++r) {
+    const olo_rule* R = &RULES[r];
+    if (found && !R->overwrite) continue;
+    int ok = 1;
+    for (int k = 0; k < R->ncond && ok; ++k) {
+      const int mat = material_at(n, per, m, x + R->c[k].dx, y + R->c[k].dy, z + R->c[k].dz);
+      const int fluid = mat == 1, outside = mat == 0;
+      switch (R->c[k].pred) {
+      case 'W': ok = !fluid && !outside; break;
+      case 'F': ok = fluid; break;
+      default: ok = !fluid; break;
+      }
+    }
+    if (!ok) continue;
+    *type = R->type;
+    for (int d = 0; d < 3; ++d) nrm[d] = R->n[d];
+    found = 1;
+  }
+}
+
+int olo_classify(int nx, int ny, int nz, int px, int py, int pz, const uint8_t* materials, int8_t* out)
+{
+  const int n[3] = {nx, ny, nz}, per[3] = {px, py, pz};
+  const int64_t nc = (int64_t)nx * ny * nz;
+  for (int x = 0; x < nx; ++x)
+    for (int y = 0; y < ny; ++y)
+      for (int z = 0; z < nz; ++z) {
+        const int64_t c = ((int64_t)x * ny + y) * nz + z;
+        out[c] = -1;
+        out[nc + c] = out[2 * nc + c] = out[3 * nc + c] = 0;
+        if (materials[c] == 0 || materials[c] == 1) continue;
+        int type, nrm[3];
+        classify_cell(n, per, materials, x, y, z, &type, nrm);
+        out[c] = (int8_t)type;
+        for (int d = 0; d < 3; ++d) out[(d + 1) * nc + c] = (int8_t)nrm[d];
+      }
+  return 0;
+}
+
+static int append_cell(int64_t** list, int64_t* count, int64_t cell)
+{
+  int64_t* p = (int64_t*)realloc(*list, sizeof(int64_t) * (size_t)(*count + 1));
+  if (!p) return -1;
+  p[*count] = cell;
+  *list = p;
+  *count += 1;
+  return 0;
+}
+
+/* boundary::set<LocalVelocity/LocalPressure> (setBoundary.h:83-101) with the
+ * dynamics localVelocity3D.h:47-98 / localPressure3D.h:46-58 choose per type;
+ * OuterVelocityEdge/CornerProcessor3D are registered for external edges and
+ * corners (localVelocity3D.h:101-112).  The edge processor's template
+ * arguments are (plane, n[(plane+1)%3], n[(plane+2)%3]) with plane the axis
+ * the normal does not point along (setBoundary3D.h:583-625). */
+int olo_define_local(olo_lattice* L, const uint8_t* materials, int material, int kind,
+                     double rho, double ux, double uy, double uz)
+{
+  if (kind != OLO_LOCAL_VELOCITY && kind != OLO_LOCAL_PRESSURE) { set_err("olo_define_local: kind"); return -1; }
+  if (L->kind != OLO_BGK) { set_err("olo_define_local: local boundaries need BGK bulk dynamics"); return -1; }
+  const int p = L->pad;
+  if (!L->lkind) {
+    L->lkind = (uint8_t*)calloc((size_t)L->ncells, 1);
+    L->lnorm = (int8_t*)calloc((size_t)L->ncells * 3, 1);
+    L->lval = (double*)calloc((size_t)L->ncells * 4, sizeof(double));
+    if (!L->lkind || !L->lnorm || !L->lval) { set_err("olo_define_local: out of memory"); return -1; }
+  }
+  for (int x = 0; x < L->n[0]; ++x)
+    for (int y = 0; y < L->n[1]; ++y)
+      for (int z = 0; z < L->n[2]; ++z) {
+        const int64_t c = ((int64_t)x * L->n[1] + y) * L->n[2] + z;
+        if (materials[c] != material) continue;
+        const int64_t cell = (int64_t)(x + p) * L->proj[0] + (int64_t)(y + p) * L->proj[1] + z + p;
+        int type, nrm[3], k;
+        classify_cell(L->n, L->periodic, materials, x, y, z, &type, nrm);
+        const int flat = type == 0 && (nrm[0] || nrm[1] || nrm[2]);
+        if (kind == OLO_LOCAL_PRESSURE) {
+          if (!flat) { set_err("olo_define_local: LocalPressure needs flat boundaries"); return -1; }
+          k = OLK_FLAT_PRESSURE;
+        } else if (type == 0) {
+          k = flat ? OLK_FLAT_VELOCITY : OLK_DEAD;
+        } else {
+          static const int map[5] = {0, OLK_OUTER_CORNER, OLK_INNER_CORNER, OLK_OUTER_EDGE, OLK_INNER_EDGE};
+          k = map[type];
+        }
+        int8_t* ln = &L->lnorm[3 * cell];
+        if (k == OLK_OUTER_EDGE || k == OLK_INNER_EDGE) {
+          const int plane = nrm[0] == 0 ? 0 : (nrm[1] == 0 ? 1 : 2);
+          ln[0] = (int8_t)plane;
+          ln[1] = (int8_t)nrm[(plane + 1) % 3];
+          ln[2] = (int8_t)nrm[(plane + 2) % 3];
+        } else {
+          for (int d = 0; d < 3; ++d) ln[d] = (int8_t)nrm[d];
+        }
+        L->lkind[cell] = (uint8_t)k;
+        L->lval[4 * cell + 0] = rho;
+        L->lval[4 * cell + 1] = ux;
+        L->lval[4 * cell + 2] = uy;
+        L->lval[4 * cell + 3] = uz;
+        if (k == OLK_OUTER_EDGE && append_cell(&L->edge_cells, &L->n_edge, cell) != 0) return -1;
+        if (k == OLK_OUTER_CORNER && append_cell(&L->corner_cells, &L->n_corner, cell) != 0) return -1;
+      }
+  return 0;
+}
+
 static size_t storage_bytes(int storage) { return storage == OLO_F64 ? 8 : (storage == OLO_F32 ? 4 : 2); }
 
 /* overlap-1 halo list: a cell outside the core is refreshed from the core cell
@@ -231,6 +432,11 @@ void olo_destroy(olo_lattice* L)
   free(L->ids);
   free(L->halo_dst);
   free(L->halo_src);
+  free(L->lkind);
+  free(L->lnorm);
+  free(L->lval);
+  free(L->edge_cells);
+  free(L->corner_cells);
   free(L);
 }
 
@@ -294,6 +500,26 @@ int olo_step(olo_lattice* L, int64_t steps, int threads)
     case OLO_F32: step_f32(L, threads); break;
     default: step_f16(L, threads); break;
     }
+  }
+  return 0;
+}
+
+int olo_initialize(olo_lattice* L)
+{
+  switch (L->storage) {
+  case OLO_F64: post_process_f64(L); break;
+  case OLO_F32: post_process_f32(L); break;
+  default: post_process_f16(L); break;
+  }
+  return 0;
+}
+
+int olo_statistics(const olo_lattice* L, double out[4])
+{
+  switch (L->storage) {
+  case OLO_F64: statistics_f64(L, out); break;
+  case OLO_F32: statistics_f32(L, out); break;
+  default: statistics_f16(L, out); break;
   }
   return 0;
 }

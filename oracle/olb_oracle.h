@@ -19,6 +19,15 @@
  *   Guo forcing (u += F/2)              src/dynamics/forcing.h:37-74,
  *                                       src/dynamics/momenta/elements.h:1473-1512
  *   descriptors D3Q19 / D3Q27           src/descriptor/definition/common.h:120-166,313-366
+ *   boundary type + discrete normal     src/geometry/discreteNormals.hh:174-1190
+ *   LocalVelocity / LocalPressure       src/boundary/localVelocity3D.h:36-116,
+ *                                       src/boundary/localPressure3D.h:36-66
+ *   regularized (CombinedRLB) dynamics  src/dynamics/dynamics.h:390-440,
+ *                                       src/dynamics/momenta/elements.h:54-70,325-470,528-640,1622-1765,
+ *                                       src/boundary/helper.h:31-80, src/dynamics/lbm.h:225-247,311-329
+ *   outer edge / corner post-processors src/boundary/boundaryPostProcessors3D.hh:164-292,
+ *                                       src/utilities/finiteDifference3D.h:73-215
+ *   lattice statistics                  src/core/latticeStatistics.h:49-71, .hh:75-90
  *
  * Pinning: every function here is checked against golden vectors written by
  * the compiled reference (tests/golden/), and against closed-form invariants
@@ -67,6 +76,25 @@ int olo_moments(const olo_lattice* L, double* rho_u);
  * semantics (forced kinds inverse-shift u by F/2), layout [q][n] */
 int olo_equilibrium(int q, int kind, int storage, int64_t n, const double* rho_u,
                     double fx, double fy, double fz, double* f);
+/* Local boundaries with BGK bulk dynamics (boundary::set<LocalVelocity> /
+ * <LocalPressure>): every core cell of `material` in the material map
+ * ([nx][ny][nz]; 1 fluid, 0 outside, anything else wall) is classified as
+ * OpenLB classifies it and gets the dynamics (and post-processor) OpenLB
+ * assigns to that type.  rho/u are the values defineRhoU gives the material:
+ * the wall velocity (LocalVelocity) or density and tangential velocity
+ * (LocalPressure). */
+enum { OLO_LOCAL_VELOCITY = 0, OLO_LOCAL_PRESSURE = 1 };
+int olo_define_local(olo_lattice* L, const uint8_t* materials, int material, int kind,
+                     double rho, double ux, double uy, double uz);
+/* OpenLB's computeBoundaryTypeAndNormal for every core cell that is neither
+ * fluid nor outside: out int8 [4][nx][ny][nz] = type, n_x, n_y, n_z (type -1
+ * on fluid and outside cells).  Axes marked periodic wrap; elsewhere the
+ * cells beyond the block count as outside. */
+int olo_classify(int nx, int ny, int nz, int px, int py, int pz, const uint8_t* materials, int8_t* out);
+/* SuperLattice::initialize(): run the post-stream post-processors once */
+int olo_initialize(olo_lattice* L);
+/* statistics the next collision reports: {cells, sum rho, sum |u|^2, max |u|^2} */
+int olo_statistics(const olo_lattice* L, double out[4]);
 /* round a double array through the storage type (f64 / f32 / f16) */
 int olo_round_to_storage(int storage, int64_t n, double* values);
 const char* olo_last_error(void);

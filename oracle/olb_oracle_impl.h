@@ -159,6 +159,183 @@ static void FN(moving_wall)(const olo_lattice* L, V* f, int id)
       f[i] += (V)2 * rho * u[d] * (V)L->c[i][d] * FN(tw)(L, i) * (V)3;
 }
 
+/* ---- local (regularized) boundaries ------------------------------------ */
+
+/* velocityBMRho<dir,orient>                          momenta/elements.h:54-70 */
+static V FN(bm_rho)(const olo_lattice* L, const V* f, int dir, int orient, const V* u)
+{
+  V on = 0, nrm = 0;
+  for (int i = 0; i < L->q; ++i)
+    if (L->c[i][dir] == 0) on += f[i];
+  for (int i = 0; i < L->q; ++i)
+    if (L->c[i][dir] == orient) nrm += f[i];
+  return ((V)2 * nrm + on + (V)1) / ((V)1 + (V)orient * u[dir]);
+}
+
+/* BoundaryHelpers<dir,orient>::computeStress                boundary/helper.h:31-80 */
+static void FN(bc_stress)(const olo_lattice* L, const V* f, int dir, int orient, V rho, const V* u, V* pi)
+{
+  const V usq = FN(usqr)(u);
+  V fneq[27];
+  for (int i = 0; i < L->q; ++i) {
+    fneq[i] = 0;
+    if (L->c[i][dir] == 0) fneq[i] = (i == 0) ? (V)0 : f[i] - FN(feq)(L, i, rho, u, usq);
+  }
+  for (int i = 0; i < L->q; ++i)
+    if (L->c[i][dir] == orient) fneq[i] = f[i] - FN(feq)(L, i, rho, u, usq);
+  int k = 0;
+  for (int a = 0; a < 3; ++a)
+    for (int b = a; b < 3; ++b) {
+      V s = 0;
+      for (int i = 0; i < L->q; ++i)
+        if (L->c[i][dir] == 0) s += (V)(L->c[i][a] * L->c[i][b]) * fneq[i];
+      for (int i = 0; i < L->q; ++i)
+        if (L->c[i][dir] == orient) s += (V)2 * (V)(L->c[i][a] * L->c[i][b]) * fneq[i];
+      pi[k++] = s;
+    }
+}
+
+/* lbm<D>::computeStress on shifted populations                   lbm.h:311-329 */
+static void FN(bulk_stress)(const olo_lattice* L, const V* f, V rho, const V* u, V* pi)
+{
+  int k = 0;
+  for (int a = 0; a < 3; ++a)
+    for (int b = a; b < 3; ++b) {
+      V s = 0;
+      for (int i = 0; i < L->q; ++i) s += (V)(L->c[i][a] * L->c[i][b]) * f[i];
+      s -= rho * u[a] * u[b];
+      if (a == b) s -= (V)1 / (V)3 * (rho - (V)1);
+      pi[k++] = s;
+    }
+}
+
+/* equilibrium<D>::fromPiToFneq                                   lbm.h:225-247 */
+static V FN(pi_to_fneq)(const olo_lattice* L, int i, const V* pi)
+{
+  V fneq = 0;
+  int k = 0;
+  for (int a = 0; a < 3; ++a)
+    for (int b = a; b < 3; ++b) {
+      V add = (V)(L->c[i][a] * L->c[i][b]);
+      if (a == b) add -= (V)1 / (V)3;
+      else add *= (V)2;
+      add *= pi[k++];
+      fneq += add;
+    }
+  fneq *= FN(tw)(L, i) * (V)3 * (V)3 / (V)2;
+  return fneq;
+}
+
+/* InnerEdgeDensity3D / InnerCornerDensity3D              momenta/elements.h:400-445 */
+static V FN(inner_rho)(const olo_lattice* L, const V* f, int k, const int8_t* ln, const V* u)
+{
+  if (k == OLK_INNER_EDGE) {
+    const int p = ln[0];
+    const V r1 = FN(bm_rho)(L, f, (p + 1) % 3, ln[1], u);
+    const V r2 = FN(bm_rho)(L, f, (p + 2) % 3, ln[2], u);
+    return (r1 + r2) / (V)2;
+  }
+  const V rx = FN(bm_rho)(L, f, 0, ln[0], u);
+  const V ry = FN(bm_rho)(L, f, 1, ln[1], u);
+  const V rz = FN(bm_rho)(L, f, 2, ln[2], u);
+  return (rx + ry + rz) / (V)3;
+}
+
+/* InnerEdgeStress3D / InnerCornerStress3D: the unknown populations are
+ * replaced by bounce-back of their non-equilibrium part, then bulk stress
+ *                                                   momenta/elements.h:1711-1765 */
+static void FN(inner_stress)(const olo_lattice* L, const V* f, int k, const int8_t* ln, V rho, const V* u, V* pi)
+{
+  const V usq = FN(usqr)(u);
+  V g[27];
+  for (int i = 0; i < L->q; ++i) g[i] = f[i];
+  for (int i = 0; i < L->q; ++i) {
+    int unknown;
+    if (k == OLK_INNER_EDGE) {
+      const int p = ln[0];
+      unknown = L->c[i][(p + 1) % 3] == -ln[1] && L->c[i][(p + 2) % 3] == -ln[2];
+    } else {
+      unknown = L->c[i][0] == -ln[0] && L->c[i][1] == -ln[1] && L->c[i][2] == -ln[2];
+    }
+    if (!unknown) continue;
+    const int o = L->opp[i];
+    g[i] = g[o] - FN(feq)(L, o, rho, u, usq) + FN(feq)(L, i, rho, u, usq);
+  }
+  FN(bulk_stress)(L, g, rho, u, pi);
+}
+
+/* rho and u of a local-boundary cell, from its boundary momenta */
+static void FN(local_rho_u)(const olo_lattice* L, const V* f, int64_t n, V* rho, V* u)
+{
+  const int k = L->lkind[n];
+  const int8_t* ln = &L->lnorm[3 * n];
+  const double* lv = &L->lval[4 * n];
+  for (int d = 0; d < 3; ++d) u[d] = (V)lv[1 + d];
+  switch (k) {
+  case OLK_FLAT_VELOCITY: {
+    const int dir = ln[0] ? 0 : (ln[1] ? 1 : 2);
+    *rho = FN(bm_rho)(L, f, dir, ln[dir], u);
+    break;
+  }
+  case OLK_FLAT_PRESSURE: {
+    /* FixedDensity + FixedPressureMomentum::computeU      elements.h:589-613 */
+    const int dir = ln[0] ? 0 : (ln[1] ? 1 : 2), orient = ln[dir];
+    const V r = (V)lv[0];
+    V on = 0, nrm = 0;
+    for (int i = 0; i < L->q; ++i)
+      if (L->c[i][dir] == 0) on += f[i];
+    for (int i = 0; i < L->q; ++i)
+      if (L->c[i][dir] == orient) nrm += f[i];
+    u[dir] = (V)orient * (((V)2 * nrm + on + (V)1) / r - (V)1);
+    *rho = r;
+    break;
+  }
+  case OLK_OUTER_EDGE:
+  case OLK_OUTER_CORNER: {
+    V r = 0;
+    for (int i = 0; i < L->q; ++i) r += f[i];
+    *rho = r + (V)1;
+    break;
+  }
+  case OLK_INNER_EDGE:
+  case OLK_INNER_CORNER:
+    *rho = FN(inner_rho)(L, f, k, ln, u);
+    break;
+  default:
+    *rho = (V)1;
+    u[0] = u[1] = u[2] = 0;
+    break;
+  }
+}
+
+/* CombinedRLBdynamics::collide (dynamics.h:420-432), or plain BGK with
+ * FixedVelocityBoundaryTuple on outer edges and corners; returns the
+ * statistic the collision reports (rho, |u|^2), rho < 0 for none */
+static V FN(collide_local)(const olo_lattice* L, V* f, int64_t n, V* usq_out)
+{
+  const int k = L->lkind[n];
+  const int8_t* ln = &L->lnorm[3 * n];
+  const V omega = (V)L->omega;
+  V rho, u[3];
+  if (k == OLK_DEAD || k == OLK_NONE) return (V)-1;
+  if (k != OLK_OUTER_EDGE && k != OLK_OUTER_CORNER) {
+    V pi[6];
+    FN(local_rho_u)(L, f, n, &rho, u);
+    if (k == OLK_FLAT_VELOCITY || k == OLK_FLAT_PRESSURE) {
+      const int dir = ln[0] ? 0 : (ln[1] ? 1 : 2);
+      FN(bc_stress)(L, f, dir, ln[dir], rho, u, pi);
+    } else {
+      FN(inner_stress)(L, f, k, ln, rho, u, pi);
+    }
+    const V usq = FN(usqr)(u);
+    for (int i = 0; i < L->q; ++i) f[i] = FN(feq)(L, i, rho, u, usq) + FN(pi_to_fneq)(L, i, pi);
+  }
+  FN(local_rho_u)(L, f, n, &rho, u);
+  FN(bgk)(L, f, rho, u, omega);
+  *usq_out = FN(usqr)(u);
+  return rho;
+}
+
 static void FN(collide_cell)(const olo_lattice* L, V* f, int id)
 {
   if (id == 2) { FN(revert)(L, f); return; }
@@ -195,6 +372,195 @@ static void FN(collide_cell)(const olo_lattice* L, V* f, int id)
   }
 }
 
+static void FN(load_cell)(const olo_lattice* L, int64_t n, V* f)
+{
+  const STORE_T* pop = (const STORE_T*)L->pop;
+  const int64_t N = L->ncells;
+  for (int i = 0; i < L->q; ++i) {
+    int64_t k = n + L->shift[i];
+    if (k >= N) k -= N;
+    f[i] = LOAD(pop[(int64_t)i * N + k]);
+  }
+}
+
+static void FN(store_cell)(olo_lattice* L, int64_t n, const V* f)
+{
+  STORE_T* pop = (STORE_T*)L->pop;
+  const int64_t N = L->ncells;
+  for (int i = 0; i < L->q; ++i) {
+    int64_t k = n + L->shift[i];
+    if (k >= N) k -= N;
+    pop[(int64_t)i * N + k] = STOREV(f[i]);
+  }
+}
+
+/* padded index of cell n moved by (dx,dy,dz); periodic axes wrap inside the core */
+static int64_t FN(neighbor)(const olo_lattice* L, int64_t n, int dx, int dy, int dz)
+{
+  const int p = L->pad;
+  int r[3] = {(int)(n / L->proj[0]) - p, (int)((n % L->proj[0]) / L->proj[1]) - p, (int)(n % L->proj[1]) - p};
+  const int o[3] = {dx, dy, dz};
+  for (int d = 0; d < 3; ++d) {
+    r[d] += o[d];
+    if (L->periodic[d] && (r[d] < 0 || r[d] >= L->n[d])) r[d] = (r[d] % L->n[d] + L->n[d]) % L->n[d];
+  }
+  return (int64_t)(r[0] + p) * L->proj[0] + (int64_t)(r[1] + p) * L->proj[1] + r[2] + p;
+}
+
+/* computeRho / computeU of any cell through its dynamics' momenta */
+static void FN(cell_rho_u)(const olo_lattice* L, int64_t n, V* rho, V* u)
+{
+  const int id = L->ids[n];
+  if (L->lkind && L->lkind[n]) {
+    V f[27];
+    FN(load_cell)(L, n, f);
+    FN(local_rho_u)(L, f, n, rho, u);
+    return;
+  }
+  if (id == 1) {
+    V f[27];
+    FN(load_cell)(L, n, f);
+    FN(rho_u)(L, f, L->kind == OLO_FORCED_BGK || L->kind == OLO_FORCED_TRT, rho, u);
+    return;
+  }
+  u[0] = u[1] = u[2] = 0;
+  *rho = (V)1;  /* NoDynamics: OneDensity; BounceBack: FixedDensity defined as 1 */
+  if (id >= 3) {
+    *rho = (V)L->wall_rho[id];
+    for (int d = 0; d < 3; ++d) u[d] = (V)L->wall_u[id][d];
+  }
+}
+
+/* fd::boundaryGradient / centralGradient                  finiteDifference.h:33-44 */
+static V FN(bgrad)(V u0, V u1, V u2) { return (-(V)3 * u0 + (V)4 * u1 - (V)1 * u2) / (V)2; }
+
+/* regularized reconstruction from rho, u and the velocity gradient
+ * dA_uB[a][b] = d_a u_b (boundaryPostProcessors3D.hh:190-211, 270-290) */
+static void FN(fd_reconstruct)(olo_lattice* L, int64_t n, V rho, V (*dA)[3])
+{
+  const V omega = (V)L->omega;
+  const V s = -rho / (V)3 / omega;
+  V pi[6];
+  pi[0] = (V)2 * dA[0][0] * s;
+  pi[3] = (V)2 * dA[1][1] * s;
+  pi[5] = (V)2 * dA[2][2] * s;
+  pi[1] = (dA[0][1] + dA[1][0]) * s;
+  pi[2] = (dA[0][2] + dA[2][0]) * s;
+  pi[4] = (dA[1][2] + dA[2][1]) * s;
+  V u[3];
+  for (int d = 0; d < 3; ++d) u[d] = (V)L->lval[4 * n + 1 + d];
+  const V usq = FN(usqr)(u);
+  V f[27];
+  for (int i = 0; i < L->q; ++i) f[i] = FN(feq)(L, i, rho, u, usq) + FN(pi_to_fneq)(L, i, pi);
+  FN(store_cell)(L, n, f);
+}
+
+static void FN(neighbor_u)(const olo_lattice* L, int64_t n, const int* off, V* u)
+{
+  V r;
+  FN(cell_rho_u)(L, FN(neighbor)(L, n, off[0], off[1], off[2]), &r, u);
+}
+
+static V FN(neighbor_rho)(const olo_lattice* L, int64_t n, const int* off)
+{
+  V r, u[3];
+  FN(cell_rho_u)(L, FN(neighbor)(L, n, off[0], off[1], off[2]), &r, u);
+  return r;
+}
+
+/* OuterVelocityEdgeProcessor3D<plane,n1,n2>::apply    boundaryPostProcessors3D.hh:166-238 */
+static void FN(outer_edge)(olo_lattice* L, int64_t n)
+{
+  const int8_t* ln = &L->lnorm[3 * n];
+  const int p = ln[0], n1 = ln[1], n2 = ln[2];
+  const int d1 = (p + 1) % 3, d2 = (p + 2) % 3;
+  int o[3];
+  o[0] = o[1] = o[2] = 0; o[d1] = -n1;     const V rho10 = FN(neighbor_rho)(L, n, o);
+  o[0] = o[1] = o[2] = 0; o[d2] = -n2;     const V rho01 = FN(neighbor_rho)(L, n, o);
+  o[0] = o[1] = o[2] = 0; o[d1] = -2 * n1; const V rho20 = FN(neighbor_rho)(L, n, o);
+  o[0] = o[1] = o[2] = 0; o[d2] = -2 * n2; const V rho02 = FN(neighbor_rho)(L, n, o);
+  const V rho = (V)2 / (V)3 * (rho01 + rho10) - (V)1 / (V)6 * (rho02 + rho20);
+  V dA[3][3], u0[3], u1[3], u2[3];
+  for (int d = 0; d < 3; ++d) u0[d] = (V)L->lval[4 * n + 1 + d];
+  /* along the edge: central difference */
+  o[0] = o[1] = o[2] = 0; o[p] = 1;  FN(neighbor_u)(L, n, o, u1);
+  o[0] = o[1] = o[2] = 0; o[p] = -1; FN(neighbor_u)(L, n, o, u2);
+  for (int b = 0; b < 3; ++b) dA[p][b] = (u1[b] - u2[b]) / (V)2;
+  /* into the fluid: one-sided second-order differences */
+  const int dirs[2] = {d1, d2}, nrm[2] = {n1, n2};
+  for (int k = 0; k < 2; ++k) {
+    o[0] = o[1] = o[2] = 0; o[dirs[k]] = -nrm[k];     FN(neighbor_u)(L, n, o, u1);
+    o[0] = o[1] = o[2] = 0; o[dirs[k]] = -2 * nrm[k]; FN(neighbor_u)(L, n, o, u2);
+    for (int b = 0; b < 3; ++b) dA[dirs[k]][b] = (V)(-nrm[k]) * FN(bgrad)(u0[b], u1[b], u2[b]);
+  }
+  FN(fd_reconstruct)(L, n, rho, dA);
+}
+
+/* OuterVelocityCornerProcessor3D<nx,ny,nz>::apply     boundaryPostProcessors3D.hh:242-292 */
+static void FN(outer_corner)(olo_lattice* L, int64_t n)
+{
+  const int8_t* ln = &L->lnorm[3 * n];
+  V r1[3], r2[3];
+  for (int d = 0; d < 3; ++d) {
+    int o[3] = {0, 0, 0};
+    o[d] = -ln[d];
+    r1[d] = FN(neighbor_rho)(L, n, o);
+    o[d] = -2 * ln[d];
+    r2[d] = FN(neighbor_rho)(L, n, o);
+  }
+  const V rho = (V)4 / (V)9 * (r1[2] + r1[1] + r1[0]) - (V)1 / (V)9 * (r2[2] + r2[1] + r2[0]);
+  V dA[3][3], u0[3], u1[3], u2[3];
+  for (int d = 0; d < 3; ++d) u0[d] = (V)L->lval[4 * n + 1 + d];
+  for (int a = 0; a < 3; ++a) {
+    int o[3] = {0, 0, 0};
+    o[a] = -ln[a];     FN(neighbor_u)(L, n, o, u1);
+    o[a] = -2 * ln[a]; FN(neighbor_u)(L, n, o, u2);
+    for (int b = 0; b < 3; ++b) dA[a][b] = (V)(-ln[a]) * FN(bgrad)(u0[b], u1[b], u2[b]);
+  }
+  FN(fd_reconstruct)(L, n, rho, dA);
+}
+
+/* PostStream post-processors by priority: edges (0), then corners (1)
+ *                                                   blockLattice.hh:485-490 */
+static void FN(post_process)(olo_lattice* L)
+{
+  for (int64_t k = 0; k < L->n_edge; ++k) FN(outer_edge)(L, L->edge_cells[k]);
+  for (int64_t k = 0; k < L->n_corner; ++k) FN(outer_corner)(L, L->corner_cells[k]);
+}
+
+/* statistics the next collision would report (Aggregatable, latticeStatistics.h:49-71):
+ * out = {cells, sum rho, sum |u|^2, max |u|^2}, summed in FP64 */
+static void FN(statistics)(const olo_lattice* L, double* out)
+{
+  out[0] = out[1] = out[2] = out[3] = 0;
+  const int p = L->pad;
+  const int forced = (L->kind == OLO_FORCED_BGK || L->kind == OLO_FORCED_TRT);
+  for (int x = 0; x < L->n[0]; ++x)
+    for (int y = 0; y < L->n[1]; ++y)
+      for (int z = 0; z < L->n[2]; ++z) {
+        const int64_t n = (int64_t)(x + p) * L->proj[0] + (int64_t)(y + p) * L->proj[1] + z + p;
+        const int id = L->ids[n];
+        V rho, usq, u[3], f[27];
+        if (id == 0 || (id == 2 && !(L->lkind && L->lkind[n]))) continue;
+        FN(load_cell)(L, n, f);
+        if (L->lkind && L->lkind[n]) {
+          rho = FN(collide_local)(L, f, n, &usq);
+          if (rho < 0) continue;
+        } else if (id == 1) {
+          FN(rho_u)(L, f, forced, &rho, u);
+          usq = FN(usqr)(u);
+        } else {
+          rho = (V)L->wall_rho[id];
+          for (int d = 0; d < 3; ++d) u[d] = (V)L->wall_u[id][d];
+          usq = FN(usqr)(u);
+        }
+        out[0] += 1;
+        out[1] += (double)rho;
+        out[2] += (double)usq;
+        if ((double)usq > out[3]) out[3] = (double)usq;
+      }
+}
+
 /* One collideAndStream(): collide core cells in place, refresh the overlap-1
  * halo from the periodic images, rotate every column. */
 static void FN(step)(olo_lattice* L, int threads)
@@ -219,7 +585,12 @@ static void FN(step)(olo_lattice* L, int threads)
           idx[i] = (int64_t)i * N + k;
           f[i] = LOAD(pop[idx[i]]);
         }
-        FN(collide_cell)(L, f, id);
+        if (L->lkind && L->lkind[n]) {
+          V usq;
+          (void)FN(collide_local)(L, f, n, &usq);
+        } else {
+          FN(collide_cell)(L, f, id);
+        }
         for (int i = 0; i < q; ++i) pop[idx[i]] = STOREV(f[i]);
       }
     }
@@ -241,6 +612,7 @@ static void FN(step)(olo_lattice* L, int threads)
     if (s < 0) s += N;
     L->shift[i] = s;
   }
+  FN(post_process)(L);
 }
 
 static void FN(upload)(olo_lattice* L, const double* f)

@@ -17,6 +17,7 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 STORAGE = {"f64": 0, "f32": 1, "f16": 2}
 KINDS = {"bgk": 0, "forced_bgk": 1, "trt": 2, "forced_trt": 3, "smagorinsky": 4}
+LOCAL = {"velocity": 0, "pressure": 1}
 
 _lib = None
 
@@ -49,6 +50,10 @@ def lib():
         L.olo_moments.argtypes = [vp, vp]
         L.olo_equilibrium.argtypes = [i32, i32, i32, i64, vp, dbl, dbl, dbl, vp]
         L.olo_round_to_storage.argtypes = [i32, i64, vp]
+        L.olo_define_local.argtypes = [vp, vp, i32, i32, dbl, dbl, dbl, dbl]
+        L.olo_classify.argtypes = [i32] * 6 + [vp, vp]
+        L.olo_initialize.argtypes = [vp]
+        L.olo_statistics.argtypes = [vp, vp]
         L.olo_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -76,8 +81,15 @@ class OracleLattice:
         if L.olo_set_bulk(h, KINDS[case.kind], case.omega, case.magic, case.smago,
                           *case.force) != 0:
             raise RuntimeError(L.olo_last_error().decode())
+        local = getattr(case, "local", {}) or {}
         for wid, (rho, u) in case.walls.items():
+            if wid in local:
+                continue
             if L.olo_set_wall(h, int(wid), float(rho), *map(float, u)) != 0:
+                raise RuntimeError(L.olo_last_error().decode())
+        for mid, kind in local.items():
+            rho, u = case.walls.get(mid, (1.0, (0.0, 0.0, 0.0)))
+            if L.olo_define_local(h, _ptr(ids), int(mid), LOCAL[kind], float(rho), *map(float, u)) != 0:
                 raise RuntimeError(L.olo_last_error().decode())
 
     def __del__(self):
@@ -109,6 +121,29 @@ class OracleLattice:
         out = np.empty((4,) + self.shape)
         lib().olo_moments(self._h, _ptr(out))
         return out
+
+    def initialize(self):
+        """SuperLattice::initialize(): the post-stream post-processors once."""
+        lib().olo_initialize(self._h)
+
+    def statistics(self) -> np.ndarray:
+        """{cells, average rho, average energy, max |u|} that the next collision
+        reports, boundary cells included (latticeStatistics.hh:75-90)."""
+        acc = np.zeros(4)
+        lib().olo_statistics(self._h, _ptr(acc))
+        n = acc[0]
+        if n == 0:
+            return np.zeros(4)
+        return np.array([n, acc[1] / n, 0.5 * acc[2] / n, np.sqrt(acc[3])])
+
+
+def classify(materials: np.ndarray, periodic=(0, 0, 0)) -> np.ndarray:
+    """OpenLB's boundary type and discrete normal of every wall cell,
+    int8 [4][nx][ny][nz] (type -1 on fluid and outside cells)."""
+    m = np.ascontiguousarray(materials, dtype=np.uint8)
+    out = np.empty((4,) + m.shape, dtype=np.int8)
+    lib().olo_classify(*m.shape, *map(int, periodic), _ptr(m), _ptr(out))
+    return out
 
 
 def equilibrium(case, storage: str = "f64", rho_u: np.ndarray | None = None) -> np.ndarray:

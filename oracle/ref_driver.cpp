@@ -7,7 +7,8 @@
 //
 // Input: a case directory holding
 //   case.txt  key/value lines (q, dyn, nx, ny, nz, px, py, pz, omega, magic,
-//             smago, fx, fy, fz, steps, wall <id> <rho> <ux> <uy> <uz>)
+//             smago, fx, fy, fz, steps, wall <id> <rho> <ux> <uy> <uz>,
+//             local <id> velocity|pressure)
 //   ids.bin   uint8  [nx][ny][nz]  per-cell dynamics id
 //             (0 none, 1 bulk, 2 BounceBack, 3.. BounceBackVelocity walls)
 //   rhou.bin  double [4][nx][ny][nz] initial rho, ux, uy, uz
@@ -15,6 +16,16 @@
 //   pops_<step>.bin  double [q][nx][ny][nz] shifted populations of the core
 //                    cells after <step> collideAndStream() calls (0 = after
 //                    initialize()).
+//   normals.bin      int8 [4][nx][ny][nz] (only with OLB_REF_NORMALS set):
+//                    type and discrete normal that OpenLB's
+//                    computeBoundaryTypeAndNormal gives every core cell that
+//                    is neither fluid (material 1) nor outside (material 0);
+//                    type -1 elsewhere.
+//
+// A material listed as `local <id> velocity` (`pressure`) gets OpenLB's
+// LocalVelocity (LocalPressure) boundary with the bulk dynamics mixed in, as
+// cavity3d.cpp:72-73 sets it; its `wall` line gives the velocity (density)
+// that defineRhoU assigns.
 //
 // The setup follows the reference's example programs:
 //   geometry/materials  examples/laminar/cavity3dBenchmark/cavity3d.cpp:39-57
@@ -30,6 +41,7 @@
 #include <map>
 #include <sstream>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 using namespace olb;
@@ -50,6 +62,7 @@ struct CaseSpec {
   double force[3] = {0, 0, 0};
   std::vector<long> steps;
   std::vector<WallSpec> walls;
+  std::map<int, std::string> local;
   std::vector<unsigned char> ids;
   std::vector<double> rhou;
 };
@@ -84,6 +97,11 @@ static CaseSpec readCase(const std::string& dir)
     else if (key == "fy") ls >> c.force[1];
     else if (key == "fz") ls >> c.force[2];
     else if (key == "steps") { long s; while (ls >> s) c.steps.push_back(s); }
+    else if (key == "local") {
+      int id; std::string kind; ls >> id >> kind;
+      if (kind != "velocity" && kind != "pressure") throw std::runtime_error("local kind " + kind);
+      c.local[id] = kind;
+    }
     else if (key == "wall") {
       WallSpec w; ls >> w.id >> w.rho >> w.u[0] >> w.u[1] >> w.u[2];
       c.walls.push_back(w);
@@ -127,6 +145,32 @@ static void dumpPopulations(const CaseSpec& c, SuperLattice<T,DESCRIPTOR>& latti
   std::fclose(f);
 }
 
+// Boundary type and discrete normal of every non-fluid, non-outside core
+// cell, as OpenLB's boundary setters compute them (setBoundary.h:83-101).
+static void dumpNormals(const CaseSpec& c, SuperGeometry<T,3>& geometry, const std::string& dir)
+{
+  auto fluid = geometry.getMaterialIndicator(1);
+  auto outside = geometry.getMaterialIndicator(0);
+  auto& fluidI = fluid->getBlockIndicatorF(0);
+  auto& outsideI = outside->getBlockIndicatorF(0);
+  const std::size_t nCore = c.ids.size();
+  std::vector<signed char> out(4 * nCore, 0);
+  for (int x = 0; x < c.n[0]; ++x)
+    for (int y = 0; y < c.n[1]; ++y)
+      for (int z = 0; z < c.n[2]; ++z) {
+        const std::size_t k = coreIndex(c, x, y, z);
+        out[k] = -1;
+        if (c.ids[k] == 0 || c.ids[k] == 1) continue;
+        const auto [type, normal] = computeBoundaryTypeAndNormal(fluidI, outsideI, Vector<int,3>{x, y, z});
+        out[k] = static_cast<signed char>(static_cast<int>(type));
+        for (int d = 0; d < 3; ++d) out[(d + 1) * nCore + k] = static_cast<signed char>(normal[d]);
+      }
+  const std::string name = dir + "/normals.bin";
+  std::FILE* f = std::fopen(name.c_str(), "wb");
+  if (!f || std::fwrite(out.data(), 1, out.size(), f) != out.size()) throw std::runtime_error("cannot write " + name);
+  std::fclose(f);
+}
+
 template <typename DESCRIPTOR, typename BULK, bool FORCED>
 static void runCase(const CaseSpec& c, const std::string& dir)
 {
@@ -142,14 +186,40 @@ static void runCase(const CaseSpec& c, const std::string& dir)
     for (int y = 0; y < c.n[1]; ++y)
       for (int z = 0; z < c.n[2]; ++z)
         blockGeometry.set(LatticeR<3>{x, y, z}, int(c.ids[coreIndex(c, x, y, z)]));
+  // Materials were written into the block directly, which does not flag the
+  // super geometry for communication; rename(0, 0) changes no material but
+  // sets the flag (superGeometry.hh:323-333), so communicate() then fills the
+  // overlap -- including its periodic images -- as a geometry built with
+  // SuperGeometry::rename would have it.
+  geometry.rename(0, 0);
   geometry.communicate();
+
+  if (std::getenv("OLB_REF_NORMALS")) {
+    dumpNormals(c, geometry, dir);
+  }
 
   SuperLattice<T,DESCRIPTOR> lattice(geometry);
   lattice.statisticsOff();
   lattice.template defineDynamics<BULK>(geometry, 1);
-  lattice.template defineDynamics<BounceBack<T,DESCRIPTOR>>(geometry, 2);
+  if (!c.local.count(2)) {
+    lattice.template defineDynamics<BounceBack<T,DESCRIPTOR>>(geometry, 2);
+  }
   for (const auto& w : c.walls) {
+    if (c.local.count(w.id)) continue;
     lattice.template defineDynamics<BounceBackVelocity<T,DESCRIPTOR>>(geometry, w.id);
+  }
+  for (const auto& [id, kind] : c.local) {
+    // local boundaries are generated for plain BGK bulk dynamics only (as in
+    // cavity3d.cpp), which keeps the driver's compile time down
+    if constexpr (std::is_same_v<BULK, BGKdynamics<T,DESCRIPTOR>>) {
+      if (kind == "velocity") {
+        boundary::set<boundary::LocalVelocity<T,DESCRIPTOR,BULK>>(lattice, geometry, id);
+      } else {
+        boundary::set<boundary::LocalPressure<T,DESCRIPTOR,BULK>>(lattice, geometry, id);
+      }
+    } else {
+      throw std::runtime_error("local boundaries need dyn bgk");
+    }
   }
   lattice.template setParameter<descriptors::OMEGA>(T(c.omega));
   if (c.dyn == "trt" || c.dyn == "forced_trt") {

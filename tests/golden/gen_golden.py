@@ -11,8 +11,10 @@ It stores OpenLB's populations as ``<name>.npz``. The populations are shifted,
 f_i - t_i, on the core cells, in [q][nx][ny][nz] layout. Snapshot 0 is the state
 after ``SuperLattice::initialize()``, and snapshot k is the state after k calls of
 ``collideAndStream()``. A ``manifest.json`` records the parameters and a SHA-256
-of every array, and ``statistics.json`` OpenLB's lattice statistics of each
-fixture's last snapshot (``--statistics-only`` writes just that file).
+of every array, ``statistics.json`` OpenLB's lattice statistics of each
+fixture's last snapshot, and ``normals.npz`` OpenLB's boundary classification
+of a few geometries (``--statistics-only`` / ``--normals-only`` write just
+those files).
 """
 from __future__ import annotations
 
@@ -64,7 +66,48 @@ def cases():
     out.append(("periodic27_trt_f64", c, [0, 50], "f64"))
     c = W.lid_driven_cavity((10, 8, 9), tau=0.7, lid_u=0.1, q=27)
     out.append(("cavity27_bgk_f64", c, [0, 100], "f64"))
+    # LocalVelocity / LocalPressure boundaries (regularized, with edge and
+    # corner post-processors), the default boundary of the benchmark cavity
+    c = W.local_velocity_cavity((12, 10, 14), tau=0.6, lid_u=0.08)
+    out.append(("cavity19_local_velocity_f64", c, [0, 1, 200], "f64"))
+    out.append(("cavity19_local_velocity_f32", c, [0, 100], "f32"))
+    c = W.local_velocity_cavity((10, 8, 9), tau=0.7, lid_u=0.1, q=27)
+    out.append(("cavity27_local_velocity_f64", c, [0, 100], "f64"))
+    c = W.velocity_channel((24, 12, 10), u_in=0.03, rho_out=1.0, tau=0.6)
+    out.append(("channel19_velocity_pressure_f64", c, [0, 200], "f64"))
+    c = W.step_cavity((14, 12, 16), block=(6, 5, 7), tau=0.6, lid_u=0.05)
+    out.append(("step19_local_velocity_f64", c, [0, 100], "f64"))
     return out
+
+
+def normal_cases():
+    """Geometries whose boundary classification (type and discrete normal of
+    every wall cell, OpenLB's computeBoundaryTypeAndNormal) is kept as a fixture."""
+    return [
+        ("cavity", W.local_velocity_cavity((8, 7, 9))),
+        ("channel", W.velocity_channel((10, 7, 6))),
+        ("step", W.step_cavity((14, 12, 16), block=(6, 5, 7))),
+        ("spheres", W.porous_channel((20, 14, 12), seed=7, rmin=1, rmax=2)),
+        ("spheres_closed", W.porous_channel((16, 18, 14), seed=3, rmin=1, rmax=3)),
+    ]
+
+
+def write_normals():
+    """normals.npz: for each geometry its materials and OpenLB's (type, normal)
+    per core cell, type -1 on fluid and outside cells."""
+    arrays = {}
+    for name, case in normal_cases():
+        exe = os.path.join(ROOT, "oracle", "_ref", "olb_ref_f64")
+        with tempfile.TemporaryDirectory() as d:
+            write_case_dir(d, case, [0])
+            subprocess.run([exe, d], check=True, cwd=d, stdout=subprocess.DEVNULL,
+                           env=dict(os.environ, OLB_REF_NORMALS="1"))
+            nm = np.fromfile(os.path.join(d, "normals.bin"), dtype=np.int8)
+        arrays[f"{name}_ids"] = case.ids.astype(np.uint8)
+        arrays[f"{name}_periodic"] = np.array(case.periodic, dtype=np.int8)
+        arrays[f"{name}_normals"] = nm.reshape((4,) + tuple(case.shape))
+        print(f"normals of {name}: shape {case.shape}", flush=True)
+    np.savez_compressed(os.path.join(HERE, "normals.npz"), **arrays)
 
 
 def write_case_dir(d, case, steps):
@@ -78,6 +121,8 @@ def write_case_dir(d, case, steps):
     lines.append("steps " + " ".join(str(s) for s in steps))
     for wid, (rho, u) in case.walls.items():
         lines.append(f"wall {wid} {rho!r} {u[0]!r} {u[1]!r} {u[2]!r}")
+    for mid, kind in case.local.items():
+        lines.append(f"local {mid} {kind}")
     with open(os.path.join(d, "case.txt"), "w") as fh:
         fh.write("\n".join(lines) + "\n")
 
@@ -102,6 +147,7 @@ def meta_of(case, steps, precision):
         "periodic": list(case.periodic), "omega": case.omega, "magic": case.magic,
         "smago": case.smago, "force": list(case.force),
         "walls": {str(k): [v[0], list(v[1])] for k, v in case.walls.items()},
+        "local": {str(k): v for k, v in case.local.items()},
         "steps": steps, "precision": precision,
         "source": "OpenLB 1.8 CPU_SISD, oracle/ref_driver.cpp",
     }
@@ -182,6 +228,8 @@ def write_statistics():
 
 
 if __name__ == "__main__":
-    if "--statistics-only" not in sys.argv:
+    if "--statistics-only" not in sys.argv and "--normals-only" not in sys.argv:
         main()
-    write_statistics()
+    if "--normals-only" not in sys.argv:
+        write_statistics()
+    write_normals()

@@ -26,6 +26,8 @@ def names(sampled: bool = False):
     out = []
     for p in sorted(glob.glob(os.path.join(GOLDEN, "*.npz"))):
         n = os.path.basename(p)[:-4]
+        if n == "normals":
+            continue
         if n.endswith("_sample") == sampled:
             out.append(n)
     return out
@@ -39,9 +41,10 @@ def load(name: str):
     if meta.get("sampled"):
         return meta, arrays
     walls = {int(k): (v[0], tuple(v[1])) for k, v in meta["walls"].items()}
+    local = {int(k): v for k, v in meta.get("local", {}).items()}
     case = W.Case(name, meta["q"], meta["kind"], tuple(meta["shape"]), tuple(meta["periodic"]),
                   arrays["ids"], arrays["rho_u"], meta["omega"], meta["magic"], meta["smago"],
-                  tuple(meta["force"]), walls)
+                  tuple(meta["force"]), walls, local)
     return case, meta, arrays
 
 

