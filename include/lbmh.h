@@ -136,6 +136,44 @@ int lbmh_moments(lbmh_lattice* lattice, double* rho_u);
  * (src/core/platform/gpu/cuda/fieldReduction.hh:152-170). */
 int lbmh_reduce_moments(lbmh_lattice* lattice, double out[4]);
 
+/* Local (regularized) boundaries, OpenLB's boundary::LocalVelocity and
+ * boundary::LocalPressure with BGK bulk dynamics mixed in
+ * (src/boundary/localVelocity3D.h:36-116, localPressure3D.h:36-66, set by
+ * boundary::set, src/boundary/setBoundary.h:83-101).  Every core cell that
+ * currently carries dynamics id `id` (the boundary's material) is classified
+ * from the id map -- id 1 fluid, id 0 outside, any other id wall -- as
+ * computeBoundaryTypeAndNormal does (src/geometry/discreteNormals.hh:
+ * 174-1190), and gets the dynamics OpenLB assigns to its type:
+ *   flat face      regularized velocity (pressure) boundary, CombinedRLBdynamics
+ *   outer edge     BGK at fixed velocity + OuterVelocityEdgeProcessor3D
+ *   outer corner   BGK at fixed velocity + OuterVelocityCornerProcessor3D
+ *   inner edge     CombinedRLBdynamics + InnerEdgeVelocityTuple3D
+ *   inner corner   CombinedRLBdynamics + InnerCornerVelocityTuple3D
+ * A velocity boundary cell that matches no type gets no dynamics; a pressure
+ * boundary accepts flat faces only (LBMH_E_INVALID otherwise, where OpenLB
+ * throws).  The cells are relabelled to free ids (counting down from 255), one
+ * per (type, normal); rho/u are what defineRhoU gives the material: the
+ * velocity of a velocity boundary, the density (and tangential velocity) of a
+ * pressure boundary.  The post-processors run after every step's shift, edges
+ * before corners (PostStream priorities).  Requires LBMH_BGK bulk dynamics;
+ * lbmh_set_dynamics_ids drops all local boundaries. */
+#define LBMH_LOCAL_VELOCITY 0
+#define LBMH_LOCAL_PRESSURE 1
+int lbmh_set_local_boundary(lbmh_lattice* lattice, int id, int kind, double rho, const double u[3]);
+
+/* Boundary type and discrete normal of every core cell of an id map that is
+ * neither fluid (1) nor outside (0): out int8 [4][nx][ny][nz] = type (0 flat,
+ * 1 external corner, 2 internal corner, 3 external edge, 4 internal edge; -1
+ * on fluid and outside cells) and the normal (pointing out of the fluid).
+ * Host only.  Replaces computeBoundaryTypeAndNormal
+ * (src/geometry/discreteNormals.hh:174-1190). */
+int lbmh_classify_boundary(const int extent[3], const int periodic[3], const uint8_t* ids, int8_t* out);
+
+/* SuperLattice::initialize() for the local boundaries: run the edge and
+ * corner post-processors once on the current populations
+ * (src/core/blockLattice.hh:56-67).  No-op without local boundaries. */
+int lbmh_initialize(lbmh_lattice* lattice);
+
 /* Lattice statistics in FP64 over the cells whose collision reports one:
  * bulk cells (rho and |u|^2, u with the F/2 shift when forced) and moving
  * walls (the wall's rho and |u_w|^2); NoDynamics and BounceBack cells count

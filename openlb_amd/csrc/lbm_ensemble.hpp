@@ -76,6 +76,7 @@ __global__ __launch_bounds__(kBlock) void ensemble_step_kernel(const EnsembleEnt
   a.periodic = e.periodic;
   a.xcd_group = 0;
   a.walls = walls + uint64_t(lat) * kEnsembleWalls * 4;
+  a.info = nullptr;
   a.bp.omega = V(e.bp.omega);
   a.bp.omega2 = V(e.bp.omega2);
   a.bp.guo = V(e.bp.guo);

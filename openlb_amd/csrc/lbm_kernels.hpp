@@ -22,27 +22,10 @@
 
 #include "fastdiv.hpp"
 #include "lbm_collision.hpp"
+#include "lbm_local.hpp"
+#include "lbm_storage.hpp"
 
 namespace lbmh {
-
-// Stored type S, arithmetic type V, and the power-of-two factor the stored
-// deviation carries.  FP16 keeps fb_i * 2^12: the scaling is exact, it moves
-// FP16's subnormal range (absolute precision 6e-8) below 1.5e-8 so that small
-// deviations keep 11 significant bits, and |fb_i| <= 16 stays below FP16's
-// maximum (f_i >= 0 bounds fb_i >= -1/3 from below).
-template <typename S> struct StorageTraits;
-template <> struct StorageTraits<double> {
-  using V = double;
-  static constexpr double scale = 1.0, inv_scale = 1.0;
-};
-template <> struct StorageTraits<float> {
-  using V = float;
-  static constexpr float scale = 1.0f, inv_scale = 1.0f;
-};
-template <> struct StorageTraits<_Float16> {
-  using V = float;
-  static constexpr float scale = 4096.0f, inv_scale = 1.0f / 4096.0f;
-};
 
 constexpr int kBlock = 256;
 constexpr int kMaxWalls = 256;
@@ -62,6 +45,7 @@ struct StepArgs {
   uint32_t periodic;        // bit d set: axis d periodic
   uint32_t xcd_group;       // runs of consecutive chunks per XCD (0: identity)
   const double* walls;      // [kMaxWalls][4] rho, ux, uy, uz
+  const uint32_t* info;     // [kMaxWalls] packed LocalInfo (local-boundary lattices only)
   BulkParams<V> bp;
 };
 
@@ -81,26 +65,6 @@ __device__ __forceinline__ uint32_t remap_xcd(uint32_t b, uint32_t nblocks, uint
   if (b >= (nblocks / window) * window) return b;
   const uint32_t x = b % kXcd, k = b / kXcd;
   return (k / group) * window + x * group + (k % group);
-}
-
-template <typename S, typename V>
-__device__ __forceinline__ V load_pop(const S* p) { return V(*p) * V(StorageTraits<S>::inv_scale); }
-template <typename S, typename V>
-__device__ __forceinline__ void store_pop(S* p, V v) { *p = S(v * V(StorageTraits<S>::scale)); }
-
-// Element k of a column through a 32-bit byte offset from the column base, so
-// the base stays in scalar registers and each address costs one VGPR.
-template <typename S>
-__device__ __forceinline__ S* column_at(S* column, uint32_t k)
-{
-  using Byte = std::conditional_t<std::is_const_v<S>, const char, char>;
-  return reinterpret_cast<S*>(reinterpret_cast<Byte*>(column) + uint32_t(k * uint32_t(sizeof(S))));
-}
-
-__device__ __forceinline__ uint32_t wrap_index(uint32_t n, uint32_t shift, uint32_t N)
-{
-  const uint32_t k = n + shift;
-  return k >= N ? k - N : k;
 }
 
 // Write populations that cross a periodic face into the halo cell they are
@@ -140,8 +104,9 @@ __device__ __forceinline__ void push_periodic_halo(const StepArgs<Q, V>& a, S* p
 }
 
 // Time step of core cell c of one block: collide in place, refresh the
-// periodic overlap.  The caller advances the shifts afterwards.
-template <int Q, typename S, int KIND, bool IDS>
+// periodic overlap.  The caller advances the shifts afterwards.  LOCAL adds
+// the regularized local-boundary dynamics to the ids >= 3.
+template <int Q, typename S, int KIND, bool IDS, bool LOCAL = false>
 __device__ __forceinline__ void collide_stream_cell(const StepArgs<Q, typename StorageTraits<S>::V>& a, uint32_t c)
 {
   using V = typename StorageTraits<S>::V;
@@ -176,8 +141,21 @@ __device__ __forceinline__ void collide_stream_cell(const StepArgs<Q, typename S
       revert<Q, V>(f);
     } else {
       const double* w = a.walls + 4 * id;
-      const V uw[3] = {V(w[1]), V(w[2]), V(w[3])};
-      moving_wall<Q, V>(f, V(w[0]), uw);
+      bool wall = true;
+      if constexpr (LOCAL) {
+        const LocalInfo li = unpack_local_info(a.info[id]);
+        if (li.kind != kMovingWall) {
+          wall = false;
+          if (li.kind != kDeadWall) {
+            V rho, usq;
+            collide_local<Q, V>(f, li, w, a.bp.omega, rho, usq);
+          }
+        }
+      }
+      if (wall) {
+        const V uw[3] = {V(w[1]), V(w[2]), V(w[3])};
+        moving_wall<Q, V>(f, V(w[0]), uw);
+      }
     }
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
@@ -188,12 +166,12 @@ __device__ __forceinline__ void collide_stream_cell(const StepArgs<Q, typename S
 }
 
 // One time step of a block, one lane per core cell.
-template <int Q, typename S, int KIND, bool IDS>
+template <int Q, typename S, int KIND, bool IDS, bool LOCAL = false>
 __global__ __launch_bounds__(kBlock) void collide_stream_kernel(StepArgs<Q, typename StorageTraits<S>::V> a)
 {
   uint32_t b = blockIdx.x;
   if (a.xcd_group) b = remap_xcd(b, gridDim.x, a.xcd_group);
-  collide_stream_cell<Q, S, KIND, IDS>(a, b * kBlock + threadIdx.x);
+  collide_stream_cell<Q, S, KIND, IDS, LOCAL>(a, b * kBlock + threadIdx.x);
 }
 
 // ---- data movement and evaluation kernels (not on the time-step path) ----
@@ -366,6 +344,7 @@ __device__ __forceinline__ double wave_max(double v)
 template <int Q, typename S>
 __global__ __launch_bounds__(kBlock) void statistics_kernel(double* partial, const uint8_t* ids, const S* pop,
                                                             CoreGeom g, Shifts<Q> sh, const double* walls,
+                                                            const uint32_t* info, double omega,
                                                             double hfx, double hfy, double hfz, int forced)
 {
   using V = typename StorageTraits<S>::V;
@@ -374,7 +353,19 @@ __global__ __launch_bounds__(kBlock) void statistics_kernel(double* partial, con
     const int id = ids ? ids[c] : 1;
     if (id == 0 || id == 2) continue;
     double rho, usq;
-    if (id >= 3) {
+    const LocalInfo li = info ? unpack_local_info(info[id < 3 ? 0 : id]) : LocalInfo{kMovingWall, 0, 0, 0};
+    if (id >= 3 && li.kind == kDeadWall) continue;
+    if (id >= 3 && li.kind != kMovingWall) {
+      // the statistic the local boundary's collision reports
+      const uint32_t n = padded_of_core(g, c);
+      V f[Q];
+#pragma unroll
+      for (int i = 0; i < Q; ++i) f[i] = load_pop<S, V>(pop + uint64_t(i) * g.stride + wrap_index(n, sh.s[i], g.n_padded));
+      V r, u2;
+      collide_local<Q, V>(f, li, walls + 4 * id, V(omega), r, u2);
+      rho = double(r);
+      usq = double(u2);
+    } else if (id >= 3) {
       const double* w = walls + 4 * id;
       rho = w[0];
       usq = w[1] * w[1] + w[2] * w[2] + w[3] * w[3];

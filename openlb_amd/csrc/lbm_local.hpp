@@ -1,0 +1,514 @@
+// Local (regularized) velocity and pressure boundaries, and the finite-
+// difference post-processors OpenLB attaches to their outer edges and corners.
+//
+// OpenLB assigns these per wall cell from its discrete normal
+// (src/boundary/localVelocity3D.h:36-116, localPressure3D.h:36-66):
+//   flat face        CombinedRLBdynamics<BGK, RegularizedVelocityBoundaryTuple<dir,orient>>
+//                    (pressure: RegularizedPressureBoundaryTuple<dir,orient>)
+//   outer edge       BGK with FixedVelocityBoundaryTuple + OuterVelocityEdgeProcessor3D
+//   outer corner     BGK with FixedVelocityBoundaryTuple + OuterVelocityCornerProcessor3D
+//   inner edge       CombinedRLBdynamics<BGK, InnerEdgeVelocityTuple3D<plane,n1,n2>>
+//   inner corner     CombinedRLBdynamics<BGK, InnerCornerVelocityTuple3D<nx,ny,nz>>
+// Here each such cell carries a dynamics id >= 3 whose entry holds the kind,
+// the normal (or plane and normals for edges) and the defined rho / u.  The
+// collisions run inside the fused step kernel; the edge and corner
+// post-processors run afterwards, as OpenLB's PostStream stage does, over
+// lists of those cells (edges first, priority 0, then corners, priority 1).
+//
+// These cells are a thin surface layer, so the code favours following
+// OpenLB's operation order over the pair-wise deviation form of the bulk path:
+// the equilibrium is rho t_i (1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u^2) - t_i
+// (lbm.h:50-64) and every sum runs over populations in index order.
+#pragma once
+
+#include "fastdiv.hpp"
+#include "lbm_collision.hpp"
+#include "lbm_storage.hpp"
+
+namespace lbmh {
+
+enum LocalKind : uint32_t {
+  kMovingWall = 0,     // BounceBackVelocity (per-id rho, u)
+  kFlatVelocity = 1,
+  kFlatPressure = 2,
+  kOuterEdge = 3,
+  kOuterCorner = 4,
+  kInnerEdge = 5,
+  kInnerCorner = 6,
+  kDeadWall = 7        // wall cell without a matching boundary type: no dynamics
+};
+
+// Per-id packed description: kind in bits 0-7, then three signed 4-bit
+// fields (normal components, or plane / n1 / n2 for edges) stored + 8.
+struct LocalInfo {
+  uint32_t kind;
+  int a, b, c;
+};
+
+__host__ __device__ inline uint32_t pack_local_info(uint32_t kind, int a, int b, int c)
+{
+  return kind | (uint32_t(a + 8) << 8) | (uint32_t(b + 8) << 12) | (uint32_t(c + 8) << 16);
+}
+
+__device__ __forceinline__ LocalInfo unpack_local_info(uint32_t w)
+{
+  LocalInfo r;
+  r.kind = w & 0xffu;
+  r.a = int((w >> 8) & 0xfu) - 8;
+  r.b = int((w >> 12) & 0xfu) - 8;
+  r.c = int((w >> 16) & 0xfu) - 8;
+  return r;
+}
+
+// Component d of velocity i for a run-time axis d (i is a compile-time
+// constant after unrolling, so this is a select among three constants).
+template <int Q>
+__device__ __forceinline__ int cdir(int i, int d)
+{
+  using L = Lattice<Q>;
+  return d == 0 ? L::c[i][0] : (d == 1 ? L::c[i][1] : L::c[i][2]);
+}
+
+// equilibrium<D>::secondOrder on shifted populations      lbm.h:50-64
+template <int Q, typename V>
+__device__ __forceinline__ V feq_shifted(int i, V rho, const V (&u)[3], V usq)
+{
+  using L = Lattice<Q>;
+  V cu = V(0);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) cu += V(L::c[i][d]) * u[d];
+  const V t = weight<V, Q>(i);
+  return rho * t * (V(1) + V(3) * cu + V(3) * V(3) * V(0.5) * cu * cu - V(3) * V(0.5) * usq) - t;
+}
+
+template <typename V>
+__device__ __forceinline__ V norm_sqr(const V (&u)[3])
+{
+  return u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+}
+
+// velocityBMRho<dir,orient>: density from the populations with c[dir] == 0
+// and c[dir] == orient                                  momenta/elements.h:54-70
+template <int Q, typename V>
+__device__ __forceinline__ V bm_rho(const V (&f)[Q], int dir, int orient, const V (&u)[3])
+{
+  using L = Lattice<Q>;
+  V on = V(0), nrm = V(0);
+#pragma unroll
+  for (int i = 0; i < Q; ++i)
+    if (cdir<Q>(i, dir) == 0) on += f[i];
+#pragma unroll
+  for (int i = 0; i < Q; ++i)
+    if (cdir<Q>(i, dir) == orient) nrm += f[i];
+  return (V(2) * nrm + on + V(1)) / (V(1) + V(orient) * u[dir]);
+}
+
+// BoundaryHelpers<dir,orient>::computeStress           boundary/helper.h:31-80
+template <int Q, typename V>
+__device__ __forceinline__ void boundary_stress(const V (&f)[Q], int dir, int orient, V rho, const V (&u)[3],
+                                                V (&pi)[6])
+{
+  using L = Lattice<Q>;
+  const V usq = norm_sqr(u);
+  V fneq[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const int ci = cdir<Q>(i, dir);
+    fneq[i] = (ci == 0 && i == 0) ? V(0) : f[i] - feq_shifted<Q, V>(i, rho, u, usq);
+  }
+  int k = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+#pragma unroll
+    for (int b = a; b < 3; ++b) {
+      V s = V(0);
+#pragma unroll
+      for (int i = 0; i < Q; ++i)
+        if (cdir<Q>(i, dir) == 0) s += V(L::c[i][a] * L::c[i][b]) * fneq[i];
+#pragma unroll
+      for (int i = 0; i < Q; ++i)
+        if (cdir<Q>(i, dir) == orient) s += V(2) * V(L::c[i][a] * L::c[i][b]) * fneq[i];
+      pi[k++] = s;
+    }
+  }
+}
+
+// lbm<D>::computeStress                                          lbm.h:311-329
+template <int Q, typename V>
+__device__ __forceinline__ void bulk_stress(const V (&f)[Q], V rho, const V (&u)[3], V (&pi)[6])
+{
+  using L = Lattice<Q>;
+  int k = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+#pragma unroll
+    for (int b = a; b < 3; ++b) {
+      V s = V(0);
+#pragma unroll
+      for (int i = 0; i < Q; ++i) s += V(L::c[i][a] * L::c[i][b]) * f[i];
+      s -= rho * u[a] * u[b];
+      if (a == b) s -= V(1) / V(3) * (rho - V(1));
+      pi[k++] = s;
+    }
+  }
+}
+
+// equilibrium<D>::fromPiToFneq                                  lbm.h:225-247
+template <int Q, typename V>
+__device__ __forceinline__ V pi_to_fneq(int i, const V (&pi)[6])
+{
+  using L = Lattice<Q>;
+  V fneq = V(0);
+  int k = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+#pragma unroll
+    for (int b = a; b < 3; ++b) {
+      V add = V(L::c[i][a] * L::c[i][b]);
+      if (a == b) add -= V(1) / V(3);
+      else add *= V(2);
+      add *= pi[k++];
+      fneq += add;
+    }
+  }
+  fneq *= weight<V, Q>(i) * V(3) * V(3) / V(2);
+  return fneq;
+}
+
+__device__ __forceinline__ int flat_dir(const LocalInfo& li)
+{
+  return li.a ? 0 : (li.b ? 1 : 2);
+}
+
+__device__ __forceinline__ int info_component(const LocalInfo& li, int d)
+{
+  return d == 0 ? li.a : (d == 1 ? li.b : li.c);
+}
+
+// rho and u of a local-boundary cell through its boundary momenta
+//   flat velocity  VelocityBoundaryDensity + FixedVelocityMomentumGeneric   elements.h:325-342, 528-571
+//   flat pressure  FixedDensity + FixedPressureMomentum                      elements.h:237-262, 575-640
+//   outer edges and corners: BulkDensity + fixed velocity                   aliases.h:188-193
+//   inner edges / corners: InnerEdgeDensity3D / InnerCornerDensity3D        elements.h:400-445
+template <int Q, typename V>
+__device__ __forceinline__ void local_rho_u(const V (&f)[Q], const LocalInfo& li, const double* w, V& rho, V (&u)[3])
+{
+  using L = Lattice<Q>;
+  u[0] = V(w[1]);
+  u[1] = V(w[2]);
+  u[2] = V(w[3]);
+  switch (li.kind) {
+  case kFlatVelocity: {
+    const int dir = flat_dir(li);
+    rho = bm_rho<Q, V>(f, dir, info_component(li, dir), u);
+    break;
+  }
+  case kFlatPressure: {
+    const int dir = flat_dir(li), orient = info_component(li, dir);
+    const V r = V(w[0]);
+    V on = V(0), nrm = V(0);
+#pragma unroll
+    for (int i = 0; i < Q; ++i)
+      if (cdir<Q>(i, dir) == 0) on += f[i];
+#pragma unroll
+    for (int i = 0; i < Q; ++i)
+      if (cdir<Q>(i, dir) == orient) nrm += f[i];
+    u[dir] = V(orient) * ((V(2) * nrm + on + V(1)) / r - V(1));
+    rho = r;
+    break;
+  }
+  case kOuterEdge:
+  case kOuterCorner: {
+    V s = V(0);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) s += f[i];
+    rho = s + V(1);
+    break;
+  }
+  case kInnerEdge: {
+    const V r1 = bm_rho<Q, V>(f, (li.a + 1) % 3, li.b, u);
+    const V r2 = bm_rho<Q, V>(f, (li.a + 2) % 3, li.c, u);
+    rho = (r1 + r2) / V(2);
+    break;
+  }
+  case kInnerCorner: {
+    const V rx = bm_rho<Q, V>(f, 0, li.a, u);
+    const V ry = bm_rho<Q, V>(f, 1, li.b, u);
+    const V rz = bm_rho<Q, V>(f, 2, li.c, u);
+    rho = (rx + ry + rz) / V(3);
+    break;
+  }
+  default:
+    rho = V(1);
+    u[0] = u[1] = u[2] = V(0);
+    break;
+  }
+}
+
+// InnerEdgeStress3D / InnerCornerStress3D: bounce back the non-equilibrium
+// part of the unknown populations, then the bulk stress    elements.h:1711-1765
+template <int Q, typename V>
+__device__ __forceinline__ void inner_stress(const V (&f)[Q], const LocalInfo& li, V rho, const V (&u)[3],
+                                             V (&pi)[6])
+{
+  using L = Lattice<Q>;
+  const V usq = norm_sqr(u);
+  V g[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) g[i] = f[i];
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    bool unknown;
+    if (li.kind == kInnerEdge) {
+      unknown = cdir<Q>(i, (li.a + 1) % 3) == -li.b && cdir<Q>(i, (li.a + 2) % 3) == -li.c;
+    } else {
+      unknown = L::c[i][0] == -li.a && L::c[i][1] == -li.b && L::c[i][2] == -li.c;
+    }
+    if (unknown) {
+      const int o = opposite<Q>(i);
+      g[i] = f[o] - feq_shifted<Q, V>(o, rho, u, usq) + feq_shifted<Q, V>(i, rho, u, usq);
+    }
+  }
+  bulk_stress<Q, V>(g, rho, u, pi);
+}
+
+// Collision of a local-boundary cell: CombinedRLBdynamics::collide
+// (dynamics.h:420-432) -- regularize from the boundary momenta, then BGK with
+// the momenta recomputed -- or plain BGK at (rho = sum f + 1, u fixed) on outer
+// edges and corners.  Returns the statistic the collision reports.
+template <int Q, typename V>
+__device__ __forceinline__ void collide_local(V (&f)[Q], const LocalInfo& li, const double* w, V omega, V& rho_out,
+                                              V& usq_out)
+{
+  V rho, u[3];
+  if (li.kind != kOuterEdge && li.kind != kOuterCorner) {
+    V pi[6];
+    local_rho_u<Q, V>(f, li, w, rho, u);
+    if (li.kind == kFlatVelocity || li.kind == kFlatPressure) {
+      const int dir = flat_dir(li);
+      boundary_stress<Q, V>(f, dir, info_component(li, dir), rho, u, pi);
+    } else {
+      inner_stress<Q, V>(f, li, rho, u, pi);
+    }
+    const V usq = norm_sqr(u);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] = feq_shifted<Q, V>(i, rho, u, usq) + pi_to_fneq<Q, V>(i, pi);
+  }
+  local_rho_u<Q, V>(f, li, w, rho, u);
+  const V usq = norm_sqr(u);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const V e = feq_shifted<Q, V>(i, rho, u, usq);
+    f[i] *= V(1) - omega;
+    f[i] += omega * e;
+  }
+  rho_out = rho;
+  usq_out = usq;
+}
+
+// ---- post-processors (PostStream stage) ----------------------------------
+
+template <int Q>
+struct PostArgs {
+  void* pop;
+  uint64_t stride;
+  uint32_t n_padded;
+  uint32_t shift[Q];
+  const uint8_t* ids;
+  const double* walls;      // [256][4]
+  const uint32_t* info;     // [256] packed LocalInfo
+  const uint32_t* cells;    // core indices of the cells to process
+  uint32_t count;
+  int nx, ny, nz;
+  uint32_t sx, sy;
+  uint32_t periodic;
+  FastDiv div_z, div_y;
+  double omega;
+};
+
+template <int Q, typename S, typename V>
+__device__ __forceinline__ void load_core_cell(const PostArgs<Q>& a, int x, int y, int z, V (&f)[Q])
+{
+  const S* pop = static_cast<const S*>(a.pop);
+  const uint32_t n = uint32_t(x + 1) * a.sx + uint32_t(y + 1) * a.sy + uint32_t(z + 1);
+#pragma unroll
+  for (int i = 0; i < Q; ++i)
+    f[i] = load_pop<S, V>(column_at(pop + uint64_t(i) * a.stride, wrap_index(n, a.shift[i], a.n_padded)));
+}
+
+// computeRho / computeU of the cell at core coordinates (x, y, z) through its
+// dynamics' momenta.  Periodic axes wrap; beyond a non-periodic face lies the
+// overlap, whose cells have no dynamics (OneDensity, ZeroMomentum).
+template <int Q, typename S, typename V>
+__device__ void cell_rho_u(const PostArgs<Q>& a, int x, int y, int z, V& rho, V (&u)[3])
+{
+  int r[3] = {x, y, z};
+  const int n[3] = {a.nx, a.ny, a.nz};
+  rho = V(1);
+  u[0] = u[1] = u[2] = V(0);
+  for (int d = 0; d < 3; ++d) {
+    if (r[d] < 0 || r[d] >= n[d]) {
+      if (!((a.periodic >> d) & 1u)) return;
+      r[d] = (r[d] % n[d] + n[d]) % n[d];
+    }
+  }
+  const uint32_t c = (uint32_t(r[0]) * uint32_t(a.ny) + uint32_t(r[1])) * uint32_t(a.nz) + uint32_t(r[2]);
+  const int id = a.ids[c];
+  if (id == 0 || id == 2) return;  // NoDynamics: OneDensity; BounceBack: FixedDensity 1, ZeroMomentum
+  if (id == 1) {
+    // lbm<D>::computeRhoU                                      lbm.h:266-307
+    using L = Lattice<Q>;
+    V f[Q];
+    load_core_cell<Q, S, V>(a, r[0], r[1], r[2], f);
+    V s = V(0), j[3] = {V(0), V(0), V(0)};
+#pragma unroll
+    for (int i = 0; i < Q; ++i) s += f[i];
+#pragma unroll
+    for (int i = 0; i < Q; ++i)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) j[d] += V(L::c[i][d]) * f[i];
+    rho = s + V(1);
+    u[0] = j[0] / rho;
+    u[1] = j[1] / rho;
+    u[2] = j[2] / rho;
+    return;
+  }
+  const double* w = a.walls + 4 * id;
+  const LocalInfo li = unpack_local_info(a.info[id]);
+  if (li.kind == kMovingWall) {
+    rho = V(w[0]);
+    u[0] = V(w[1]);
+    u[1] = V(w[2]);
+    u[2] = V(w[3]);
+    return;
+  }
+  if (li.kind == kDeadWall) return;
+  V f[Q];
+  load_core_cell<Q, S, V>(a, r[0], r[1], r[2], f);
+  local_rho_u<Q, V>(f, li, w, rho, u);
+}
+
+// fd::boundaryGradient                                    finiteDifference.h:40-44
+template <typename V>
+__device__ __forceinline__ V boundary_gradient(V u0, V u1, V u2)
+{
+  return (-V(3) * u0 + V(4) * u1 - V(1) * u2) / V(2);
+}
+
+// Regularized reconstruction from rho, the cell's fixed u and the velocity
+// gradient dA[a][b] = d_a u_b          boundaryPostProcessors3D.hh:190-211, 270-290
+template <int Q, typename S, typename V>
+__device__ void store_reconstruction(const PostArgs<Q>& a, int x, int y, int z, const double* w, V rho,
+                                     const V (&dA)[3][3])
+{
+  const V s = -rho / V(3) / V(a.omega);
+  V pi[6];
+  pi[0] = V(2) * dA[0][0] * s;
+  pi[3] = V(2) * dA[1][1] * s;
+  pi[5] = V(2) * dA[2][2] * s;
+  pi[1] = (dA[0][1] + dA[1][0]) * s;
+  pi[2] = (dA[0][2] + dA[2][0]) * s;
+  pi[4] = (dA[1][2] + dA[2][1]) * s;
+  const V u[3] = {V(w[1]), V(w[2]), V(w[3])};
+  const V usq = norm_sqr(u);
+  S* pop = static_cast<S*>(a.pop);
+  const uint32_t n = uint32_t(x + 1) * a.sx + uint32_t(y + 1) * a.sy + uint32_t(z + 1);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const V fi = feq_shifted<Q, V>(i, rho, u, usq) + pi_to_fneq<Q, V>(i, pi);
+    store_pop<S, V>(column_at(pop + uint64_t(i) * a.stride, wrap_index(n, a.shift[i], a.n_padded)), fi);
+  }
+}
+
+__device__ __forceinline__ void core_coords(const FastDiv& div_z, const FastDiv& div_y, int ny, int nz, uint32_t c,
+                                            int& x, int& y, int& z)
+{
+  const uint32_t t = div_z.div(c);
+  z = int(c - t * uint32_t(nz));
+  const uint32_t xx = div_y.div(t);
+  y = int(t - xx * uint32_t(ny));
+  x = int(xx);
+}
+
+// OuterVelocityEdgeProcessor3D<plane,n1,n2>::apply    boundaryPostProcessors3D.hh:166-238
+template <int Q, typename S>
+__global__ __launch_bounds__(64) void outer_edge_kernel(PostArgs<Q> a)
+{
+  using V = typename StorageTraits<S>::V;
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.count) return;
+  const uint32_t c = a.cells[k];
+  int x, y, z;
+  core_coords(a.div_z, a.div_y, a.ny, a.nz, c, x, y, z);
+  const int id = a.ids[c];
+  const double* w = a.walls + 4 * id;
+  const LocalInfo li = unpack_local_info(a.info[id]);
+  const int p = li.a, n1 = li.b, n2 = li.c;
+  const int d1 = (p + 1) % 3, d2 = (p + 2) % 3;
+  const int r0[3] = {x, y, z};
+  auto at = [&](int dd1, int dd2, int dp, V& rho, V (&u)[3]) {
+    int r[3] = {r0[0], r0[1], r0[2]};
+    r[d1] += dd1;
+    r[d2] += dd2;
+    r[p] += dp;
+    cell_rho_u<Q, S, V>(a, r[0], r[1], r[2], rho, u);
+  };
+  V rho10, rho01, rho20, rho02, tmp[3];
+  at(-n1, 0, 0, rho10, tmp);
+  at(0, -n2, 0, rho01, tmp);
+  at(-2 * n1, 0, 0, rho20, tmp);
+  at(0, -2 * n2, 0, rho02, tmp);
+  const V rho = V(2) / V(3) * (rho01 + rho10) - V(1) / V(6) * (rho02 + rho20);
+  const V u0[3] = {V(w[1]), V(w[2]), V(w[3])};
+  V dA[3][3], u1[3], u2[3], r;
+  // along the edge: centralGradient                     finiteDifference.h:33-38
+  at(0, 0, 1, r, u1);
+  at(0, 0, -1, r, u2);
+  for (int b = 0; b < 3; ++b) dA[p][b] = (u1[b] - u2[b]) / V(2);
+  // into the fluid along d1 and d2: DirectedGradients3D<.., true>  finiteDifference3D.h:73-150
+  at(-n1, 0, 0, r, u1);
+  at(-2 * n1, 0, 0, r, u2);
+  for (int b = 0; b < 3; ++b) dA[d1][b] = V(-n1) * boundary_gradient(u0[b], u1[b], u2[b]);
+  at(0, -n2, 0, r, u1);
+  at(0, -2 * n2, 0, r, u2);
+  for (int b = 0; b < 3; ++b) dA[d2][b] = V(-n2) * boundary_gradient(u0[b], u1[b], u2[b]);
+  store_reconstruction<Q, S, V>(a, x, y, z, w, rho, dA);
+}
+
+// OuterVelocityCornerProcessor3D<nx,ny,nz>::apply     boundaryPostProcessors3D.hh:242-292
+template <int Q, typename S>
+__global__ __launch_bounds__(64) void outer_corner_kernel(PostArgs<Q> a)
+{
+  using V = typename StorageTraits<S>::V;
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.count) return;
+  const uint32_t c = a.cells[k];
+  int x, y, z;
+  core_coords(a.div_z, a.div_y, a.ny, a.nz, c, x, y, z);
+  const int id = a.ids[c];
+  const double* w = a.walls + 4 * id;
+  const LocalInfo li = unpack_local_info(a.info[id]);
+  const int nrm[3] = {li.a, li.b, li.c};
+  V r1[3], r2[3], tmp[3];
+  for (int d = 0; d < 3; ++d) {
+    int o[3] = {0, 0, 0};
+    o[d] = -nrm[d];
+    cell_rho_u<Q, S, V>(a, x + o[0], y + o[1], z + o[2], r1[d], tmp);
+    o[d] = -2 * nrm[d];
+    cell_rho_u<Q, S, V>(a, x + o[0], y + o[1], z + o[2], r2[d], tmp);
+  }
+  const V rho = V(4) / V(9) * (r1[2] + r1[1] + r1[0]) - V(1) / V(9) * (r2[2] + r2[1] + r2[0]);
+  const V u0[3] = {V(w[1]), V(w[2]), V(w[3])};
+  V dA[3][3], u1[3], u2[3], r;
+  for (int d = 0; d < 3; ++d) {
+    int o[3] = {0, 0, 0};
+    o[d] = -nrm[d];
+    cell_rho_u<Q, S, V>(a, x + o[0], y + o[1], z + o[2], r, u1);
+    o[d] = -2 * nrm[d];
+    cell_rho_u<Q, S, V>(a, x + o[0], y + o[1], z + o[2], r, u2);
+    for (int b = 0; b < 3; ++b) dA[d][b] = V(-nrm[d]) * boundary_gradient(u0[b], u1[b], u2[b]);
+  }
+  store_reconstruction<Q, S, V>(a, x, y, z, w, rho, dA);
+}
+
+}  // namespace lbmh

@@ -10,9 +10,12 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
+#include "boundary_classify.hpp"
 #include "lbm_kernels.hpp"
 #include "lbmh.h"
 #include "lbmh_common.hpp"
@@ -45,6 +48,15 @@ struct lbmh_lattice {
   lbmh_bulk_params params{};
   double walls[kMaxWalls][4] = {};
   size_t footprint = 0;
+  // local (regularized) boundaries
+  std::vector<uint8_t> h_ids;        // host copy of the dynamics ids
+  bool id_defined[kMaxWalls] = {};   // ids given a wall or boundary entry
+  uint32_t info[kMaxWalls] = {};     // packed LocalInfo per id
+  uint32_t* d_info = nullptr;
+  bool has_local = false;
+  std::vector<uint32_t> h_edges, h_corners;
+  uint32_t* d_edges = nullptr;
+  uint32_t* d_corners = nullptr;
 };
 
 namespace {
@@ -79,6 +91,44 @@ BulkParams<V> bulk_params(const lbmh_lattice* L)
   return derive_bulk_params<V>(L->q, L->params);
 }
 
+// PostStream stage of the local boundaries: outer edges (priority 0), then
+// outer corners (priority 1), on the populations after the shift.
+template <int Q, typename S>
+int launch_post_processors(lbmh_lattice* L, hipStream_t stream)
+{
+  if (L->h_edges.empty() && L->h_corners.empty()) return LBMH_OK;
+  PostArgs<Q> a;
+  a.pop = L->d_pop;
+  a.stride = L->stride;
+  a.n_padded = L->n_padded;
+  for (int i = 0; i < Q; ++i) a.shift[i] = uint32_t(L->shift[i]);
+  a.ids = L->d_ids;
+  a.walls = L->d_walls;
+  a.info = L->d_info;
+  a.nx = L->n[0];
+  a.ny = L->n[1];
+  a.nz = L->n[2];
+  a.sx = L->size[1] * L->size[2];
+  a.sy = L->size[2];
+  a.periodic = uint32_t(L->periodic[0] | (L->periodic[1] << 1) | (L->periodic[2] << 2));
+  a.div_z = FastDiv(uint32_t(L->n[2]));
+  a.div_y = FastDiv(uint32_t(L->n[1]));
+  a.omega = L->params.omega;
+  if (!L->h_edges.empty()) {
+    a.cells = L->d_edges;
+    a.count = uint32_t(L->h_edges.size());
+    hipLaunchKernelGGL((outer_edge_kernel<Q, S>), dim3(grid_for(a.count, 64)), dim3(64), 0, stream, a);
+    LBMH_HIP(hipGetLastError());
+  }
+  if (!L->h_corners.empty()) {
+    a.cells = L->d_corners;
+    a.count = uint32_t(L->h_corners.size());
+    hipLaunchKernelGGL((outer_corner_kernel<Q, S>), dim3(grid_for(a.count, 64)), dim3(64), 0, stream, a);
+    LBMH_HIP(hipGetLastError());
+  }
+  return LBMH_OK;
+}
+
 template <int Q, typename S, int KIND>
 int launch_kind(lbmh_lattice* L, hipStream_t stream)
 {
@@ -99,12 +149,21 @@ int launch_kind(lbmh_lattice* L, hipStream_t stream)
   a.periodic = uint32_t(L->periodic[0] | (L->periodic[1] << 1) | (L->periodic[2] << 2));
   a.xcd_group = uint32_t(L->launch.xcd_group);
   a.walls = L->d_walls;
+  a.info = L->d_info;
   a.bp = bulk_params<V>(L);
   const uint32_t grid = (L->ncore + kBlock - 1) / kBlock;
   for (int i = 0; i < Q; ++i) a.shift[i] = uint32_t(L->shift[i]);
   // the dynamic LDS is never touched: it only limits resident workgroups per CU
   const unsigned lds = unsigned(L->launch.lds_bytes);
-  if (L->uniform_bulk) {
+  bool launched = false;
+  if constexpr (KIND == kBGK) {
+    if (L->has_local) {
+      hipLaunchKernelGGL((collide_stream_kernel<Q, S, KIND, true, true>), dim3(grid), dim3(kBlock), lds, stream, a);
+      launched = true;
+    }
+  }
+  if (launched) {
+  } else if (L->uniform_bulk) {
     hipLaunchKernelGGL((collide_stream_kernel<Q, S, KIND, false>), dim3(grid), dim3(kBlock), lds, stream, a);
   } else {
     hipLaunchKernelGGL((collide_stream_kernel<Q, S, KIND, true>), dim3(grid), dim3(kBlock), lds, stream, a);
@@ -118,6 +177,7 @@ int launch_kind(lbmh_lattice* L, hipStream_t stream)
     L->shift[i] = s;
   }
   L->steps += 1;
+  if (L->has_local) return launch_post_processors<Q, S>(L, stream);
   return LBMH_OK;
 }
 
@@ -237,6 +297,8 @@ int lbmh_lattice_create(const lbmh_lattice_desc* desc, lbmh_lattice** out)
   if (e == hipSuccess) e = hipMalloc((void**)&L->d_ids, L->ncore);
   if (e == hipSuccess) e = hipMalloc((void**)&L->d_walls, sizeof(double) * 4 * kMaxWalls);
   if (e == hipSuccess) e = hipMalloc((void**)&L->d_partials, sizeof(double) * 4 * (L->n_partials + 1));
+  if (e == hipSuccess) e = hipMalloc((void**)&L->d_info, sizeof(uint32_t) * kMaxWalls);
+  if (e == hipSuccess) e = hipMemcpy(L->d_info, L->info, sizeof(L->info), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(L->d_pop, 0, pop_bytes);
   if (e == hipSuccess) e = hipMemset(L->d_ids, 0, L->ncore);
   if (e == hipSuccess) e = hipMemcpy(L->d_walls, L->walls, sizeof(L->walls), hipMemcpyHostToDevice);
@@ -258,6 +320,9 @@ int lbmh_lattice_destroy(lbmh_lattice* L)
   if (L->d_ids) (void)hipFree(L->d_ids);
   if (L->d_walls) (void)hipFree(L->d_walls);
   if (L->d_partials) (void)hipFree(L->d_partials);
+  if (L->d_info) (void)hipFree(L->d_info);
+  if (L->d_edges) (void)hipFree(L->d_edges);
+  if (L->d_corners) (void)hipFree(L->d_corners);
   delete L;
   return LBMH_OK;
 }
@@ -275,6 +340,16 @@ int lbmh_set_dynamics_ids(lbmh_lattice* L, const uint8_t* ids)
   }
   LBMH_HIP(hipMemcpy(L->d_ids, ids, L->ncore, hipMemcpyHostToDevice));
   L->uniform_bulk = uniform;
+  // a new id map drops the local boundaries defined on the previous one
+  L->h_ids.assign(ids, ids + L->ncore);
+  L->has_local = false;
+  L->h_edges.clear();
+  L->h_corners.clear();
+  for (int w = 0; w < kMaxWalls; ++w) {
+    L->info[w] = pack_local_info(kMovingWall, 0, 0, 0);
+    L->id_defined[w] = false;
+  }
+  LBMH_HIP(hipMemcpy(L->d_info, L->info, sizeof(L->info), hipMemcpyHostToDevice));
   return LBMH_OK;
 }
 
@@ -293,7 +368,136 @@ int lbmh_set_wall(lbmh_lattice* L, int id, double rho, const double u[3])
   DeviceGuard guard(L->device);
   L->walls[id][0] = rho;
   for (int d = 0; d < 3; ++d) L->walls[id][d + 1] = u[d];
+  L->info[id] = pack_local_info(kMovingWall, 0, 0, 0);
+  L->id_defined[id] = true;
   LBMH_HIP(hipMemcpy(L->d_walls + 4 * id, L->walls[id], sizeof(double) * 4, hipMemcpyHostToDevice));
+  LBMH_HIP(hipMemcpy(L->d_info + id, &L->info[id], sizeof(uint32_t), hipMemcpyHostToDevice));
+  return LBMH_OK;
+}
+
+int lbmh_classify_boundary(const int extent[3], const int periodic[3], const uint8_t* ids, int8_t* out)
+{
+  if (!extent || !periodic || !ids || !out) return fail(LBMH_E_INVALID, "null argument");
+  if (extent[0] < 1 || extent[1] < 1 || extent[2] < 1) return fail(LBMH_E_INVALID, "extent must be >= 1");
+  const int n[3] = {extent[0], extent[1], extent[2]};
+  const int per[3] = {periodic[0] ? 1 : 0, periodic[1] ? 1 : 0, periodic[2] ? 1 : 0};
+  const int64_t nc = int64_t(n[0]) * n[1] * n[2];
+  for (int x = 0; x < n[0]; ++x)
+    for (int y = 0; y < n[1]; ++y)
+      for (int z = 0; z < n[2]; ++z) {
+        const int64_t c = (int64_t(x) * n[1] + y) * n[2] + z;
+        out[c] = -1;
+        out[nc + c] = out[2 * nc + c] = out[3 * nc + c] = 0;
+        if (ids[c] == 0 || ids[c] == 1) continue;
+        int type, nrm[3];
+        classify_boundary_cell(n, per, ids, x, y, z, type, nrm);
+        out[c] = int8_t(type);
+        for (int d = 0; d < 3; ++d) out[(d + 1) * nc + c] = int8_t(nrm[d]);
+      }
+  return LBMH_OK;
+}
+
+int lbmh_set_local_boundary(lbmh_lattice* L, int id, int kind, double rho, const double u[3])
+{
+  if (!L || !u) return fail(LBMH_E_INVALID, "null argument");
+  if (kind != LBMH_LOCAL_VELOCITY && kind != LBMH_LOCAL_PRESSURE) return fail(LBMH_E_INVALID, "unknown local boundary kind");
+  if (id < 2 || id >= kMaxWalls) return fail(LBMH_E_INVALID, "boundary id must be in [2, 255]");
+  if (L->kind != LBMH_BGK) return fail(LBMH_E_INVALID, "local boundaries need BGK bulk dynamics");
+  if (L->h_ids.size() != L->ncore) return fail(LBMH_E_INVALID, "set the dynamics ids first");
+  if (kind == LBMH_LOCAL_PRESSURE && !(rho > 0)) return fail(LBMH_E_INVALID, "pressure boundary density must be positive");
+  DeviceGuard guard(L->device);
+  const int n[3] = {L->n[0], L->n[1], L->n[2]};
+  const int per[3] = {L->periodic[0], L->periodic[1], L->periodic[2]};
+  // ids free for the boundary's per-type entries: absent from the map and not yet defined
+  bool used[kMaxWalls] = {};
+  for (uint8_t v : L->h_ids) used[v] = true;
+  for (int w = 0; w < kMaxWalls; ++w) used[w] = used[w] || L->id_defined[w];
+  used[0] = used[1] = used[2] = true;
+  std::map<std::tuple<int, int, int, int>, int> entry;  // (kind, a, b, c) -> id
+  std::vector<uint8_t> next = L->h_ids;
+  std::vector<uint32_t> edges, corners;
+  int free_id = kMaxWalls - 1;
+  for (int x = 0; x < n[0]; ++x)
+    for (int y = 0; y < n[1]; ++y)
+      for (int z = 0; z < n[2]; ++z) {
+        const uint32_t c = uint32_t((int64_t(x) * n[1] + y) * n[2] + z);
+        if (L->h_ids[c] != id) continue;
+        int type, nrm[3];
+        classify_boundary_cell(n, per, L->h_ids.data(), x, y, z, type, nrm);
+        const bool flat = type == kTypeFlat && (nrm[0] || nrm[1] || nrm[2]);
+        uint32_t k;
+        if (kind == LBMH_LOCAL_PRESSURE) {
+          // LocalPressure handles flat walls only (localPressure3D.h:46-58)
+          if (!flat) return fail(LBMH_E_INVALID, "local pressure boundary on a non-flat wall cell");
+          k = kFlatPressure;
+        } else if (type == kTypeFlat) {
+          k = flat ? kFlatVelocity : kDeadWall;
+        } else {
+          static const uint32_t map[5] = {0, kOuterCorner, kInnerCorner, kOuterEdge, kInnerEdge};
+          k = map[type];
+        }
+        int a = nrm[0], b = nrm[1], cc = nrm[2];
+        if (k == kOuterEdge || k == kInnerEdge) {
+          // (plane, n[(plane+1)%3], n[(plane+2)%3]), plane the axis the normal
+          // does not point along (setBoundary3D.h:583-625)
+          const int plane = nrm[0] == 0 ? 0 : (nrm[1] == 0 ? 1 : 2);
+          a = plane;
+          b = nrm[(plane + 1) % 3];
+          cc = nrm[(plane + 2) % 3];
+        }
+        const auto key = std::make_tuple(int(k), a, b, cc);
+        auto it = entry.find(key);
+        if (it == entry.end()) {
+          while (free_id >= 3 && used[free_id]) --free_id;
+          if (free_id < 3) return fail(LBMH_E_INVALID, "no free dynamics ids left for the boundary's cell types");
+          used[free_id] = true;
+          L->info[free_id] = pack_local_info(k, a, b, cc);
+          L->walls[free_id][0] = rho;
+          for (int d = 0; d < 3; ++d) L->walls[free_id][d + 1] = u[d];
+          L->id_defined[free_id] = true;
+          it = entry.emplace(key, free_id).first;
+        }
+        next[c] = uint8_t(it->second);
+        if (k == kOuterEdge) edges.push_back(c);
+        if (k == kOuterCorner) corners.push_back(c);
+      }
+  if (entry.empty()) return LBMH_OK;
+  L->h_ids.swap(next);
+  L->h_edges.insert(L->h_edges.end(), edges.begin(), edges.end());
+  L->h_corners.insert(L->h_corners.end(), corners.begin(), corners.end());
+  LBMH_HIP(hipMemcpy(L->d_ids, L->h_ids.data(), L->ncore, hipMemcpyHostToDevice));
+  LBMH_HIP(hipMemcpy(L->d_walls, L->walls, sizeof(L->walls), hipMemcpyHostToDevice));
+  LBMH_HIP(hipMemcpy(L->d_info, L->info, sizeof(L->info), hipMemcpyHostToDevice));
+  auto upload_list = [&](const std::vector<uint32_t>& h, uint32_t*& d) -> int {
+    if (d) {
+      LBMH_HIP(hipFree(d));
+      d = nullptr;
+    }
+    if (h.empty()) return LBMH_OK;
+    LBMH_HIP(hipMalloc((void**)&d, sizeof(uint32_t) * h.size()));
+    LBMH_HIP(hipMemcpy(d, h.data(), sizeof(uint32_t) * h.size(), hipMemcpyHostToDevice));
+    return LBMH_OK;
+  };
+  int rc = upload_list(L->h_edges, L->d_edges);
+  if (rc == LBMH_OK) rc = upload_list(L->h_corners, L->d_corners);
+  if (rc != LBMH_OK) return rc;
+  L->has_local = true;
+  L->uniform_bulk = false;
+  return LBMH_OK;
+}
+
+int lbmh_initialize(lbmh_lattice* L)
+{
+  if (!L) return fail(LBMH_E_INVALID, "null lattice");
+  if (!L->has_local) return LBMH_OK;
+  DeviceGuard guard(L->device);
+  const int rc = with_types(L, [&](auto qc, auto s) -> int {
+    constexpr int Q = decltype(qc)::value;
+    using S = decltype(s);
+    return launch_post_processors<Q, S>(L, nullptr);
+  });
+  if (rc != LBMH_OK) return rc;
+  LBMH_HIP(hipDeviceSynchronize());
   return LBMH_OK;
 }
 
@@ -473,7 +677,8 @@ int lbmh_statistics(lbmh_lattice* L, double out[4])
     using S = decltype(s);
     hipLaunchKernelGGL((statistics_kernel<Q, S>), dim3(blocks), dim3(kBlock), 0, 0, L->d_partials,
                        L->uniform_bulk ? nullptr : L->d_ids, static_cast<const S*>(L->d_pop), g, shifts_of<Q>(L),
-                       static_cast<const double*>(L->d_walls), hf[0] * 0.5, hf[1] * 0.5, hf[2] * 0.5, forced);
+                       static_cast<const double*>(L->d_walls), L->has_local ? L->d_info : nullptr,
+                       L->params.omega, hf[0] * 0.5, hf[1] * 0.5, hf[2] * 0.5, forced);
     LBMH_HIP(hipGetLastError());
     return LBMH_OK;
   });
@@ -543,7 +748,9 @@ int lbmh_autotune(lbmh_lattice* L, int trial_steps, lbmh_launch_config* chosen)
   } cleanup{T};
   LBMH_HIP(hipMemcpy(T->d_ids, L->d_ids, L->ncore, hipMemcpyDeviceToDevice));
   LBMH_HIP(hipMemcpy(T->d_walls, L->d_walls, sizeof(double) * 4 * kMaxWalls, hipMemcpyDeviceToDevice));
+  LBMH_HIP(hipMemcpy(T->d_info, L->d_info, sizeof(uint32_t) * kMaxWalls, hipMemcpyDeviceToDevice));
   T->uniform_bulk = L->uniform_bulk;
+  T->has_local = L->has_local;  // times the step kernel; the clone has no post-processor lists
   T->params = L->params;
   LBMH_HIP(hipEventCreate(&cleanup.e0));
   LBMH_HIP(hipEventCreate(&cleanup.e1));

@@ -1,16 +1,20 @@
 // Lid-driven cavity benchmark on the MI355X backend, written against the C++
 // host layer the same way OpenLB's harness is written against SuperLattice
-// (examples/laminar/cavity3dBenchmark/cavity3d.cpp:154-250, bounce-back lid
-// variant :85-87):
+// (examples/laminar/cavity3dBenchmark/cavity3d.cpp:154-250):
 //   * resolution `size`: (size+1)^3 cells, every one a voxel with dynamics;
-//   * boundary cells BounceBack, the top face (y = size) a BounceBackVelocity
-//     lid moving along x at the lattice velocity 0.01;
+//   * by default OpenLB's LID_DRIVEN configuration (:32-36, 68-76): LocalVelocity
+//     boundaries on the whole box, at rest on the walls and moving along x at
+//     the lattice velocity 0.01 on the lid (the top face y = size without its
+//     rim, :39-57); with --bounce-back the LID_DRIVEN_BOUNCE_BACK variant
+//     (:70-71): boundary cells BounceBack, the whole top face a
+//     BounceBackVelocity lid;
 //   * BGK bulk, tau from UnitConverterFromResolutionAndLatticeVelocity with
 //     physical viscosity 1e-3, velocity 1, length 1 (nu_lattice = 1e-5 size);
 //   * 10 warm-up steps, then `steps` timed steps bracketed by device syncs;
 //   * prints "size, steps, cuboids, ranks, threads, MLUPs" like the original.
 // Extra flags: --fp64 (double storage), --dump <file> (populations after the
-// run, double [q][nx][ny][nz]), --tau (override).
+// run, double [q][nx][ny][nz]), --tau (override), --stats (lattice statistics
+// after the run).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -23,7 +27,7 @@
 using namespace olb::hip;
 
 template <typename T>
-int run(int size, long steps, double tau, const char* dump, bool stats)
+int run(int size, long steps, double tau, const char* dump, bool stats, bool bounceBack)
 {
   using DESCRIPTOR = descriptors::D3Q19;
   const int n = size + 1;
@@ -33,10 +37,20 @@ int run(int size, long steps, double tau, const char* dump, bool stats)
   auto boundary = [n](int x, int y, int z) {
     return x == 0 || y == 0 || z == 0 || x == n - 1 || y == n - 1 || z == n - 1;
   };
-  lattice.template defineDynamics<BGKdynamics<T, DESCRIPTOR>>([&](int x, int y, int z) { return !boundary(x, y, z); });
-  lattice.template defineDynamics<BounceBack<T, DESCRIPTOR>>(boundary);
-  lattice.template defineDynamics<BounceBackVelocity<T, DESCRIPTOR>>([n](int, int y, int) { return y == n - 1; });
-  lattice.defineWall(T(1), {T(0.01), T(0), T(0)});
+  using BulkDynamics = BGKdynamics<T, DESCRIPTOR>;
+  lattice.template defineDynamics<BulkDynamics>([&](int x, int y, int z) { return !boundary(x, y, z); });
+  if (bounceBack) {
+    lattice.template defineDynamics<BounceBack<T, DESCRIPTOR>>(boundary);
+    lattice.template defineDynamics<BounceBackVelocity<T, DESCRIPTOR>>([n](int, int y, int) { return y == n - 1; });
+    lattice.defineWall(T(1), {T(0.01), T(0), T(0)});
+  } else {
+    auto lid = [n](int x, int y, int z) { return y == n - 1 && x > 0 && x < n - 1 && z > 0 && z < n - 1; };
+    boundary::set<boundary::LocalVelocity<T, DESCRIPTOR, BulkDynamics>>(
+      lattice, [&](int x, int y, int z) { return boundary(x, y, z) && !lid(x, y, z); });
+    lattice.defineWall(T(1), {T(0), T(0), T(0)});
+    boundary::set<boundary::LocalVelocity<T, DESCRIPTOR, BulkDynamics>>(lattice, lid);
+    lattice.defineWall(T(1), {T(0.01), T(0), T(0)});
+  }
   lattice.template setParameter<descriptors::OMEGA>(T(1.0 / tau));
 
   lattice.iniEquilibrium([](int, int, int) { return true; }, [](int, int, int) { return 1.0; },
@@ -76,6 +90,7 @@ int main(int argc, char** argv)
   bool fp64 = false;
   const char* dump = nullptr;
   bool stats = false;
+  bool bounceBack = false;
   double tau = -1;
   for (int i = 1; i < argc; ++i) {
     if (!std::strcmp(argv[i], "--size") && i + 1 < argc) size = std::atoi(argv[++i]);
@@ -84,15 +99,16 @@ int main(int argc, char** argv)
     else if (!std::strcmp(argv[i], "--dump") && i + 1 < argc) dump = argv[++i];
     else if (!std::strcmp(argv[i], "--tau") && i + 1 < argc) tau = std::atof(argv[++i]);
     else if (!std::strcmp(argv[i], "--stats")) stats = true;
+    else if (!std::strcmp(argv[i], "--bounce-back")) bounceBack = true;
     else if (!std::strcmp(argv[i], "--no-results")) continue;
     else {
-      std::fprintf(stderr, "usage: %s [--size N] [--steps K] [--fp64] [--tau T] [--dump FILE] [--stats]\n", argv[0]);
+      std::fprintf(stderr, "usage: %s [--size N] [--steps K] [--fp64] [--tau T] [--dump FILE] [--stats] [--bounce-back]\n", argv[0]);
       return 2;
     }
   }
   if (tau <= 0) tau = 0.5 + 3.0 * 1e-5 * size;  // nu = nu_phys * dt / dx^2 with u_lattice 0.01
   try {
-    return fp64 ? run<double>(size, steps, tau, dump, stats) : run<float>(size, steps, tau, dump, stats);
+    return fp64 ? run<double>(size, steps, tau, dump, stats, bounceBack) : run<float>(size, steps, tau, dump, stats, bounceBack);
   } catch (const std::exception& e) {
     std::fprintf(stderr, "error: %s\n", e.what());
     return 1;

@@ -11,6 +11,8 @@
 //   setProcessingContext(Evaluation | Simulation)        superLattice.h:154-164
 //   get(x, y, z)[iPop] on the host mirror                superLattice.h:169-176
 //   statisticsOff()                                      superLattice.h:397-403
+//   boundary::set<LocalVelocity|LocalPressure>(...)      boundary/setBoundary.h:83-101
+//   getStatistics()                                      superLattice.h:378
 //
 // Dynamics are tags carrying the C ABI's bulk kind; parameters are tags of
 // the same names as OpenLB's fields.  HIP errors surface as std::runtime_error
@@ -56,6 +58,13 @@ template <typename T, typename D> struct SmagorinskyBGKdynamics { static constex
 template <typename T, typename D> struct BounceBack { static constexpr DynamicsRole role = DynamicsRole::BounceBack; };
 template <typename T, typename D> struct BounceBackVelocity { static constexpr DynamicsRole role = DynamicsRole::BounceBackVelocity; };
 template <typename T, typename D> struct NoDynamics { static constexpr DynamicsRole role = DynamicsRole::None; };
+
+// ---- local boundaries (src/boundary/localVelocity3D.h, localPressure3D.h):
+// regularized velocity / pressure boundaries with the bulk dynamics mixed in
+namespace boundary {
+template <typename T, typename D, typename MIXIN> struct LocalVelocity { static constexpr int kind = LBMH_LOCAL_VELOCITY; };
+template <typename T, typename D, typename MIXIN> struct LocalPressure { static constexpr int kind = LBMH_LOCAL_PRESSURE; };
+}  // namespace boundary
 
 enum class ProcessingContext { Evaluation, Simulation };
 
@@ -170,11 +179,30 @@ class SuperLattice {
     if (_handle) detail::check(lbmh_set_bulk_params(_handle, &_params));
   }
 
-  /// Fixed density and velocity of the BounceBackVelocity cells defined last
-  /// (defineRhoU / defineU on their material, cavity3d.cpp:124-136).
+  /// Local boundary BC on the indicated cells (boundary::set, setBoundary.h:
+  /// 83-101); BC is boundary::LocalVelocity or boundary::LocalPressure with
+  /// the bulk dynamics as mixin.  The cells' types and normals are found from
+  /// the dynamics defined so far (bulk = fluid, no dynamics = outside) when the
+  /// lattice is initialized.  defineWall() then gives the velocity (density).
+  template <typename BC>
+  void setLocalBoundary(const Indicator& indicator)
+  {
+    requireSetup("boundary::set");
+    const uint8_t id = uint8_t(LBMH_ID_FIRST_WALL + _wallsDefined);
+    if (id == 0) throw std::logic_error("at most 253 wall and boundary groups per block");
+    ++_wallsDefined;
+    _pendingWall = id;
+    _local.push_back({int(id), BC::kind});
+    forEach([&](int x, int y, int z, std::size_t c) {
+      if (indicator(x, y, z)) _ids[c] = id;
+    });
+  }
+
+  /// Fixed density and velocity of the wall or boundary group defined last
+  /// (defineRhoU / defineU on its material, cavity3d.cpp:113-131).
   void defineWall(T rho, std::array<T, 3> u)
   {
-    if (!_pendingWall) throw std::logic_error("defineWall needs a BounceBackVelocity group first");
+    if (!_pendingWall) throw std::logic_error("defineWall needs a wall or boundary group first");
     _walls.push_back({int(_pendingWall), double(rho), {double(u[0]), double(u[1]), double(u[2])}});
     if (_handle) applyWalls();
   }
@@ -214,7 +242,15 @@ class SuperLattice {
     detail::check(lbmh_set_dynamics_ids(_handle, _ids.data()));
     detail::check(lbmh_set_bulk_params(_handle, &_params));
     applyWalls();
+    for (const auto& b : _local) {
+      Wall w{b.id, 1.0, {0.0, 0.0, 0.0}};
+      for (const auto& v : _walls)
+        if (v.id == b.id) w = v;
+      detail::check(lbmh_set_local_boundary(_handle, b.id, b.kind, w.rho, w.u));
+    }
     if (!_rhoU.empty()) detail::check(lbmh_ini_equilibrium(_handle, _rhoU.data()));
+    // initialize(): the post-processors of the local boundaries run once
+    detail::check(lbmh_initialize(_handle));
   }
 
   /// One time step (SuperLattice::collideAndStream), asynchronous.
@@ -289,6 +325,7 @@ class SuperLattice {
 
  private:
   struct Wall { int id; double rho; double u[3]; };
+  struct Local { int id; int kind; };
 
   template <typename F>
   void forEach(F&& f) const
@@ -303,9 +340,17 @@ class SuperLattice {
     if (_handle) throw std::logic_error(std::string(what) + " after initialize()");
   }
 
+  bool isLocal(int id) const
+  {
+    for (const auto& b : _local)
+      if (b.id == id) return true;
+    return false;
+  }
+
   void applyWalls()
   {
-    for (const auto& w : _walls) detail::check(lbmh_set_wall(_handle, w.id, w.rho, w.u));
+    for (const auto& w : _walls)
+      if (!isLocal(w.id)) detail::check(lbmh_set_wall(_handle, w.id, w.rho, w.u));
   }
 
   std::array<int, 3> _extent;
@@ -319,10 +364,20 @@ class SuperLattice {
   bool _mirrorValid = false;
   lbmh_bulk_params _params{1.0, 0.25, 0.0, {0.0, 0.0, 0.0}};
   std::vector<Wall> _walls;
+  std::vector<Local> _local;
   uint8_t _pendingWall = 0;
   int _wallsDefined = 0;
   lbmh_lattice* _handle = nullptr;
 };
+
+namespace boundary {
+/// boundary::set<BC>(lattice, indicator)           src/boundary/setBoundary.h:83-101
+template <typename BC, typename LATTICE>
+void set(LATTICE& lattice, const Indicator& indicator)
+{
+  lattice.template setLocalBoundary<BC>(indicator);
+}
+}  // namespace boundary
 
 }  // namespace hip
 }  // namespace olb

@@ -49,8 +49,13 @@ class BlockLattice:
         lat = cls(case.q, case.shape, case.periodic, storage, case.kind, device)
         lat.define_dynamics(case.ids)
         lat.set_parameters(omega=case.omega, magic=case.magic, smago=case.smago, force=case.force)
+        local = getattr(case, "local", {}) or {}
         for wid, (rho, u) in case.walls.items():
-            lat.set_wall(int(wid), rho, u)
+            if wid not in local:
+                lat.set_wall(int(wid), rho, u)
+        for mid, kind in local.items():
+            rho, u = case.walls.get(mid, (1.0, (0.0, 0.0, 0.0)))
+            lat.set_local_boundary(int(mid), kind, rho, u)
         return lat
 
     def close(self):
@@ -84,6 +89,22 @@ class BlockLattice:
         if ids.shape != self.shape:
             raise ValueError(f"ids shape {ids.shape} != lattice shape {self.shape}")
         abi.check(abi.load().lbmh_set_dynamics_ids(self._h, _ptr(ids)))
+
+    def set_local_boundary(self, material: int, kind: str = "velocity", rho: float = 1.0,
+                           u=(0.0, 0.0, 0.0)):
+        """boundary::set<LocalVelocity | LocalPressure>(lattice, geometry, material)
+        followed by defineRhoU on the material (src/boundary/setBoundary.h:83-101):
+        classify the cells carrying id ``material`` and give each the regularized
+        boundary dynamics of its type."""
+        kinds = {"velocity": 0, "pressure": 1}
+        uu = np.ascontiguousarray(u, dtype=np.float64)
+        abi.check(abi.load().lbmh_set_local_boundary(self._h, int(material), kinds[kind], float(rho),
+                                                     _ptr(uu)))
+
+    def initialize(self):
+        """SuperLattice::initialize(): the edge and corner post-processors of the
+        local boundaries run once (src/core/blockLattice.hh:56-67)."""
+        abi.check(abi.load().lbmh_initialize(self._h))
 
     def set_parameters(self, omega: float, magic: float = 0.25, smago: float = 0.0,
                        force=(0.0, 0.0, 0.0)):
@@ -170,3 +191,15 @@ class BlockLattice:
         b = ctypes.c_size_t(0)
         abi.check(abi.load().lbmh_lattice_footprint(self._h, ctypes.byref(b)))
         return b.value
+
+
+def classify_boundary(ids: np.ndarray, periodic=(0, 0, 0)) -> np.ndarray:
+    """Boundary type and discrete normal of every wall cell of an id map, as
+    OpenLB's computeBoundaryTypeAndNormal gives them (host-side, no GPU):
+    int8 [4][nx][ny][nz] = type (-1 on fluid and outside cells), n_x, n_y, n_z."""
+    m = np.ascontiguousarray(ids, dtype=np.uint8)
+    ext = np.array(m.shape, dtype=np.int32)
+    per = np.array([int(bool(p)) for p in periodic], dtype=np.int32)
+    out = np.empty((4,) + m.shape, dtype=np.int8)
+    abi.check(abi.load().lbmh_classify_boundary(_ptr(ext), _ptr(per), _ptr(m), _ptr(out)))
+    return out

@@ -5,6 +5,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 from openlb_amd import abi
@@ -62,3 +63,15 @@ def test_null_arguments_are_rejected():
     assert lib.lbmh_lattice_destroy(None) == abi.OK
     with pytest.raises(abi.LbmhError):
         abi.check(lib.lbmh_step(None, 1, None))
+
+
+@pytest.mark.parametrize("geometry", ["cavity", "channel", "step", "spheres", "spheres_closed"])
+def test_library_classification_matches_reference(geometry):
+    """The library's host-side boundary classification (used by
+    lbmh_set_local_boundary) against OpenLB's computeBoundaryTypeAndNormal on
+    the same materials.  Host code only: no device is touched."""
+    import golden_cases as G
+    from openlb_amd.lattice import classify_boundary
+    z = np.load(f"{G.GOLDEN}/normals.npz", allow_pickle=False)
+    got = classify_boundary(z[f"{geometry}_ids"], tuple(z[f"{geometry}_periodic"]))
+    np.testing.assert_array_equal(got, z[f"{geometry}_normals"])

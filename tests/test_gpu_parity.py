@@ -21,11 +21,6 @@ from openlb_amd.lattice import BlockLattice
 
 pytestmark = pytest.mark.gpu
 
-
-def _skip_local(case):
-    if case.local:
-        pytest.skip("local velocity/pressure boundaries are not on the device path yet")
-
 TOL = {"f64": 1e-12, "f32": 2e-5}
 
 
@@ -44,7 +39,6 @@ def run_gpu(case, storage, f0, steps):
 @pytest.mark.parametrize("name", G.names())
 def test_gpu_matches_reference(gpu_available, name):
     case, meta, arrays = G.load(name)
-    _skip_local(case)
     precision = meta["precision"]
     steps = meta["steps"][1:]
     got = run_gpu(case, precision, arrays["f_0"].astype(np.float64), steps)
@@ -58,7 +52,6 @@ def test_gpu_matches_reference(gpu_available, name):
 @pytest.mark.parametrize("name", [n for n in G.names() if n.endswith("_f64")])
 def test_gpu_fp32_storage_vs_fp64_reference(gpu_available, name):
     case, meta, arrays = G.load(name)
-    _skip_local(case)
     steps = [min(meta["steps"][-1], 200)]
     if steps[0] != meta["steps"][-1]:
         pytest.skip("reference snapshot beyond 200 steps only")
@@ -72,7 +65,6 @@ def test_gpu_fp32_storage_vs_fp64_reference(gpu_available, name):
 def test_gpu_fp16_storage_velocity_error(gpu_available, name):
     """FP16 shifted storage, FP32 collision: the velocity against OpenLB's FP64 run."""
     case, meta, arrays = G.load(name)
-    _skip_local(case)
     s = meta["steps"][-1]
     if s > 200:
         pytest.skip("reference snapshot beyond 200 steps only")
@@ -265,7 +257,6 @@ def test_gpu_statistics_match_reference(gpu_available, name):
     """Device FP64 statistics of the fixture's last snapshot against the values
     OpenLB reported for it (cells, average rho, average energy, max |u|)."""
     case, meta, arrays = G.load(name)
-    _skip_local(case)
     ref = G.statistics(name)
     with BlockLattice.from_case(case, meta["precision"]) as lat:
         lat.upload_populations(arrays[f"f_{ref['snapshot']}"].astype(np.float64), "f64")
@@ -294,3 +285,35 @@ def test_gpu_statistics_after_steps_match_oracle(gpu_available, storage):
     assert got[0] == ref[0]
     tol = {"f64": 1e-13, "f32": 1e-6, "f16": 1e-6}[storage]
     np.testing.assert_allclose(got[1:], ref[1:], rtol=tol)
+
+
+@pytest.mark.parametrize("name", [n for n in G.names() if G.load(n)[0].local])
+def test_gpu_initialize_matches_reference(gpu_available, name):
+    """Device iniEquilibrium followed by initialize() (the edge and corner
+    post-processors once) against OpenLB's populations after initialize()."""
+    case, meta, arrays = G.load(name)
+    precision = meta["precision"]
+    with BlockLattice.from_case(case, precision) as lat:
+        lat.ini_equilibrium(case.rho_u)
+        lat.initialize()
+        got = lat.download_populations("f64")
+    err = G.rel_error(got, arrays["f_0"], case.q, case.ids != 0)
+    print(f"{name}: initialize rel {err:.3e}", flush=True)
+    assert err <= (4e-15 if precision == "f64" else 1e-6)
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_gpu_local_boundaries_match_oracle_larger(gpu_available, storage):
+    """A 40x36x44 cavity with LocalVelocity walls and lid, 300 steps, all cells,
+    against the oracle; and a duct with a velocity inlet and pressure outlet."""
+    for case in (W.local_velocity_cavity((40, 36, 44), tau=0.56, lid_u=0.1),
+                 W.velocity_channel((60, 18, 22), u_in=0.04, rho_out=1.0, tau=0.58)):
+        ora = O.OracleLattice(case, storage)
+        ora.upload(O.initial_populations(case, storage))
+        ora.initialize()
+        f0 = ora.download()
+        got = run_gpu(case, storage, f0, [300])[300]
+        ora.step(300, threads=8)
+        err = G.rel_error(got, ora.download(), case.q, case.ids != 0)
+        print(f"{case.name} {storage}: rel {err:.3e}", flush=True)
+        assert err <= (1e-12 if storage == "f64" else 2e-5)
