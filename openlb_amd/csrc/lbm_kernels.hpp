@@ -103,10 +103,14 @@ __device__ __forceinline__ void push_periodic_halo(const StepArgs<Q, V>& a, S* p
   }
 }
 
+// How a pass treats the ids >= 3 that carry local-boundary dynamics: the
+// whole-lattice pass leaves them to a second pass over the list of those
+// cells, so the bulk kernel keeps its register budget (and occupancy).
+enum LocalMode : int { kLocalNone = 0, kLocalSkip = 1, kLocalCollide = 2 };
+
 // Time step of core cell c of one block: collide in place, refresh the
-// periodic overlap.  The caller advances the shifts afterwards.  LOCAL adds
-// the regularized local-boundary dynamics to the ids >= 3.
-template <int Q, typename S, int KIND, bool IDS, bool LOCAL = false>
+// periodic overlap.  The caller advances the shifts afterwards.
+template <int Q, typename S, int KIND, bool IDS, int LOCAL = kLocalNone>
 __device__ __forceinline__ void collide_stream_cell(const StepArgs<Q, typename StorageTraits<S>::V>& a, uint32_t c)
 {
   using V = typename StorageTraits<S>::V;
@@ -125,6 +129,9 @@ __device__ __forceinline__ void collide_stream_cell(const StepArgs<Q, typename S
     // NoDynamics cells are not collided, but on a periodic face their
     // populations still cross into the overlap like every other core cell's
     if (id == 0 && !edge) return;
+    if constexpr (LOCAL == kLocalSkip) {
+      if (id >= 3 && (a.info[id] & 0xffu) != kMovingWall) return;
+    }
   }
   const uint32_t n = (x + 1) * a.sx + (y + 1) * a.sy + (z + 1);
   S* pop = static_cast<S*>(a.pop);
@@ -142,7 +149,7 @@ __device__ __forceinline__ void collide_stream_cell(const StepArgs<Q, typename S
     } else {
       const double* w = a.walls + 4 * id;
       bool wall = true;
-      if constexpr (LOCAL) {
+      if constexpr (LOCAL == kLocalCollide) {
         const LocalInfo li = unpack_local_info(a.info[id]);
         if (li.kind != kMovingWall) {
           wall = false;
@@ -166,12 +173,23 @@ __device__ __forceinline__ void collide_stream_cell(const StepArgs<Q, typename S
 }
 
 // One time step of a block, one lane per core cell.
-template <int Q, typename S, int KIND, bool IDS, bool LOCAL = false>
+template <int Q, typename S, int KIND, bool IDS, int LOCAL = kLocalNone>
 __global__ __launch_bounds__(kBlock) void collide_stream_kernel(StepArgs<Q, typename StorageTraits<S>::V> a)
 {
   uint32_t b = blockIdx.x;
   if (a.xcd_group) b = remap_xcd(b, gridDim.x, a.xcd_group);
   collide_stream_cell<Q, S, KIND, IDS, LOCAL>(a, b * kBlock + threadIdx.x);
+}
+
+// The same time step for the listed local-boundary cells (the cells the
+// whole-lattice pass skipped).  The two passes touch disjoint populations.
+template <int Q, typename S, int KIND>
+__global__ __launch_bounds__(64) void local_boundary_kernel(StepArgs<Q, typename StorageTraits<S>::V> a,
+                                                            const uint32_t* cells, uint32_t count)
+{
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  collide_stream_cell<Q, S, KIND, true, kLocalCollide>(a, cells[k]);
 }
 
 // ---- data movement and evaluation kernels (not on the time-step path) ----

@@ -54,9 +54,10 @@ struct lbmh_lattice {
   uint32_t info[kMaxWalls] = {};     // packed LocalInfo per id
   uint32_t* d_info = nullptr;
   bool has_local = false;
-  std::vector<uint32_t> h_edges, h_corners;
+  std::vector<uint32_t> h_edges, h_corners, h_local;
   uint32_t* d_edges = nullptr;
   uint32_t* d_corners = nullptr;
+  uint32_t* d_local = nullptr;     // every local-boundary cell (second pass of the step)
 };
 
 namespace {
@@ -158,7 +159,13 @@ int launch_kind(lbmh_lattice* L, hipStream_t stream)
   bool launched = false;
   if constexpr (KIND == kBGK) {
     if (L->has_local) {
-      hipLaunchKernelGGL((collide_stream_kernel<Q, S, KIND, true, true>), dim3(grid), dim3(kBlock), lds, stream, a);
+      hipLaunchKernelGGL((collide_stream_kernel<Q, S, KIND, true, kLocalSkip>), dim3(grid), dim3(kBlock), lds, stream, a);
+      LBMH_HIP(hipGetLastError());
+      const uint32_t count = uint32_t(L->h_local.size());
+      if (count) {
+        hipLaunchKernelGGL((local_boundary_kernel<Q, S, KIND>), dim3(grid_for(count, 64)), dim3(64), 0, stream, a,
+                           static_cast<const uint32_t*>(L->d_local), count);
+      }
       launched = true;
     }
   }
@@ -323,6 +330,7 @@ int lbmh_lattice_destroy(lbmh_lattice* L)
   if (L->d_info) (void)hipFree(L->d_info);
   if (L->d_edges) (void)hipFree(L->d_edges);
   if (L->d_corners) (void)hipFree(L->d_corners);
+  if (L->d_local) (void)hipFree(L->d_local);
   delete L;
   return LBMH_OK;
 }
@@ -345,6 +353,7 @@ int lbmh_set_dynamics_ids(lbmh_lattice* L, const uint8_t* ids)
   L->has_local = false;
   L->h_edges.clear();
   L->h_corners.clear();
+  L->h_local.clear();
   for (int w = 0; w < kMaxWalls; ++w) {
     L->info[w] = pack_local_info(kMovingWall, 0, 0, 0);
     L->id_defined[w] = false;
@@ -415,7 +424,7 @@ int lbmh_set_local_boundary(lbmh_lattice* L, int id, int kind, double rho, const
   used[0] = used[1] = used[2] = true;
   std::map<std::tuple<int, int, int, int>, int> entry;  // (kind, a, b, c) -> id
   std::vector<uint8_t> next = L->h_ids;
-  std::vector<uint32_t> edges, corners;
+  std::vector<uint32_t> edges, corners, cells;
   int free_id = kMaxWalls - 1;
   for (int x = 0; x < n[0]; ++x)
     for (int y = 0; y < n[1]; ++y)
@@ -458,6 +467,7 @@ int lbmh_set_local_boundary(lbmh_lattice* L, int id, int kind, double rho, const
           it = entry.emplace(key, free_id).first;
         }
         next[c] = uint8_t(it->second);
+        cells.push_back(c);
         if (k == kOuterEdge) edges.push_back(c);
         if (k == kOuterCorner) corners.push_back(c);
       }
@@ -465,6 +475,7 @@ int lbmh_set_local_boundary(lbmh_lattice* L, int id, int kind, double rho, const
   L->h_ids.swap(next);
   L->h_edges.insert(L->h_edges.end(), edges.begin(), edges.end());
   L->h_corners.insert(L->h_corners.end(), corners.begin(), corners.end());
+  L->h_local.insert(L->h_local.end(), cells.begin(), cells.end());
   LBMH_HIP(hipMemcpy(L->d_ids, L->h_ids.data(), L->ncore, hipMemcpyHostToDevice));
   LBMH_HIP(hipMemcpy(L->d_walls, L->walls, sizeof(L->walls), hipMemcpyHostToDevice));
   LBMH_HIP(hipMemcpy(L->d_info, L->info, sizeof(L->info), hipMemcpyHostToDevice));
@@ -480,6 +491,7 @@ int lbmh_set_local_boundary(lbmh_lattice* L, int id, int kind, double rho, const
   };
   int rc = upload_list(L->h_edges, L->d_edges);
   if (rc == LBMH_OK) rc = upload_list(L->h_corners, L->d_corners);
+  if (rc == LBMH_OK) rc = upload_list(L->h_local, L->d_local);
   if (rc != LBMH_OK) return rc;
   L->has_local = true;
   L->uniform_bulk = false;
@@ -750,7 +762,7 @@ int lbmh_autotune(lbmh_lattice* L, int trial_steps, lbmh_launch_config* chosen)
   LBMH_HIP(hipMemcpy(T->d_walls, L->d_walls, sizeof(double) * 4 * kMaxWalls, hipMemcpyDeviceToDevice));
   LBMH_HIP(hipMemcpy(T->d_info, L->d_info, sizeof(uint32_t) * kMaxWalls, hipMemcpyDeviceToDevice));
   T->uniform_bulk = L->uniform_bulk;
-  T->has_local = L->has_local;  // times the step kernel; the clone has no post-processor lists
+  T->has_local = L->has_local;  // times the whole-lattice pass; the clone has no boundary lists
   T->params = L->params;
   LBMH_HIP(hipEventCreate(&cleanup.e0));
   LBMH_HIP(hipEventCreate(&cleanup.e1));
