@@ -221,6 +221,50 @@ def turbulent_bench(shape=(384, 192, 192), steps=5000):
     return out
 
 
+def cavity_bench(size=200, steps=200, cpu_size=100, cpu_seconds=10.0, no_cpu=False):
+    """OpenLB's own benchmark harness case (examples/laminar/cavity3dBenchmark,
+    cavity3d.cpp:154-250) in its default configuration: a (size+1)^3 lid-driven
+    cavity, D3Q19 BGK, FP32, LocalVelocity boundaries on the whole box (the
+    regularized face dynamics and the edge / corner post-processors), lid
+    velocity 0.01, tau = 1/2 + 3e-5 size; `size` 200 is the resolution of the
+    harness's cluster recipe (bwUniCluster2.0.sh:34-39).  10 warm-up steps, then
+    `steps` timed steps, MLUPS over all cells.  The bounce-back-lid variant
+    (LID_DRIVEN_BOUNCE_BACK) is timed beside it, and OpenLB's CPU_SIMD build on
+    the same case at `cpu_size`."""
+    from openlb_amd import workloads as W
+    from openlb_amd.lattice import BlockLattice
+    out = {"workload": "cavity3d_benchmark", "size": size, "steps": steps, "storage": "f32"}
+    for label, case in (("local_velocity", W.cavity_benchmark_local(size)),
+                        ("bounce_back", W.cavity_benchmark(size))):
+        with BlockLattice.from_case(case, "f32") as lat:
+            lat.ini_equilibrium(case.rho_u)
+            lat.initialize()
+            lat.collide_and_stream(10)
+            lat.synchronize()
+            t0 = time.perf_counter()
+            lat.collide_and_stream(steps)
+            lat.synchronize()
+            dt = time.perf_counter() - t0
+            st = lat.statistics()
+        mlups = case.n_active * steps / dt / 1e6
+        out[label] = {"seconds": round(dt, 3), "mlups": round(mlups, 1),
+                      "hbm_frac": round(mlups * 1e6 * 2 * 19 * 4 / (HBM_PEAK_GBS * 1e9), 4),
+                      "max_u": st["max_u"], "average_energy": st["average_energy"]}
+        log(f"cavity {label} {size}: {mlups:.0f} MLUPS")
+    if not no_cpu:
+        from oracle import compare_cpu
+        if os.path.exists(compare_cpu.REF_SIMD):
+            threads = min(16, os.cpu_count() or 1)
+            sample = W.cavity_benchmark_local(cpu_size)
+            probe = compare_cpu.time_reference(sample, 5, threads)
+            n = max(5, int(cpu_seconds * probe["mlups"] * 1e6 / sample.n_active))
+            res = compare_cpu.time_reference(sample, n, threads)
+            out["cpu_reference"] = {"mlups": round(res["mlups"], 2), "cores": threads,
+                                    "sample": f"cavity {cpu_size + 1}^3 local velocity f32, {n} steps, "
+                                              f"OpenLB 1.8 CPU_SIMD + OpenMP"}
+    return out
+
+
 def serving_bench(count=400, rate=200.0, budget_gb=200.0, tick_steps=64):
     """Config 5: a Poisson stream of D3Q19 channel solves at `rate` req/s (sizes
     log-uniform in [24^3, 128^3], 500-5000 steps, inflow velocity N(0.05, 0.01^2), FP32 or
@@ -350,6 +394,10 @@ def main():
     ap.add_argument("--no-turbulent", dest="turbulent", action="store_false",
                     help="skip the D3Q27 Smagorinsky channel precision run (config 3)")
     ap.add_argument("--turbulent-steps", type=int, default=5000)
+    ap.add_argument("--no-cavity", dest="cavity", action="store_false",
+                    help="skip OpenLB's cavity benchmark case (LocalVelocity boundaries)")
+    ap.add_argument("--cavity-size", type=int, default=200)
+    ap.add_argument("--cavity-steps", type=int, default=200)
     ap.add_argument("--no-serving", dest="serving", action="store_false",
                     help="skip the continuous-batching serving measurement (config 5)")
     ap.add_argument("--serving-requests", type=int, default=400)
@@ -422,6 +470,11 @@ def main():
         log("turbulent channel (config 3) ...")
         turbulent = turbulent_bench(steps=args.turbulent_steps)
 
+    cavity = None
+    if args.cavity:
+        log("cavity benchmark (LocalVelocity) ...")
+        cavity = cavity_bench(args.cavity_size, args.cavity_steps, no_cpu=args.no_cpu)
+
     served = None
     if args.serving:
         log("serving (config 5) ...")
@@ -473,6 +526,7 @@ def main():
         "cpu_baseline": cpu,
         "turbulent_channel": turbulent,
         "ensemble": ensemble,
+        "cavity_benchmark": cavity,
         "serving": served,
     }
     print(json.dumps(out), flush=True)

@@ -39,6 +39,8 @@ def write_case(d, case, steps):
     lines.append(f"steps {steps}")
     for wid, (rho, u) in case.walls.items():
         lines.append(f"wall {wid} {rho!r} {u[0]!r} {u[1]!r} {u[2]!r}")
+    for mid, kind in (getattr(case, "local", {}) or {}).items():
+        lines.append(f"local {mid} {kind}")
     with open(os.path.join(d, "case.txt"), "w") as fh:
         fh.write("\n".join(lines) + "\n")
 
