@@ -236,6 +236,8 @@ typedef struct lbmh_ensemble_desc {
   int max_members;          /* member slots                                    */
   size_t population_bytes;  /* population arena                                */
   size_t cell_capacity;     /* id arena (cells over all members)               */
+  size_t boundary_capacity; /* local-boundary cells over all members
+                               (0: cell_capacity / 4)                          */
 } lbmh_ensemble_desc;
 
 int lbmh_ensemble_create(const lbmh_ensemble_desc* desc, lbmh_ensemble** out);
@@ -254,6 +256,13 @@ int lbmh_ensemble_add(lbmh_ensemble* ensemble, const int extent[3], const int pe
                       const lbmh_bulk_params* params, int64_t steps, int* member);
 int lbmh_ensemble_remove(lbmh_ensemble* ensemble, int member);
 int lbmh_ensemble_set_wall(lbmh_ensemble* ensemble, int member, int id, double rho, const double u[3]);
+/* Local velocity / pressure boundary on the member's cells carrying `id`, as
+ * lbmh_set_local_boundary does for a lattice; the per-type ids come from the
+ * member's free ids 3..15.  Needs an LBMH_BGK ensemble. */
+int lbmh_ensemble_set_local_boundary(lbmh_ensemble* ensemble, int member, int id, int kind, double rho,
+                                     const double u[3]);
+/* initialize() of one member: its edge and corner post-processors once. */
+int lbmh_ensemble_initialize(lbmh_ensemble* ensemble, int member);
 int lbmh_ensemble_set_steps(lbmh_ensemble* ensemble, int member, int64_t steps);
 /* steps completed and steps still to run (< 0: unbounded) */
 int lbmh_ensemble_status(lbmh_ensemble* ensemble, int member, int64_t* steps_done, int64_t* steps_left);

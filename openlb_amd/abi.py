@@ -28,7 +28,8 @@ EXPORTS = (
     "lbmh_device_arch", "lbmh_last_error", "lbmh_set_launch_config", "lbmh_get_launch_config",
     "lbmh_autotune",
     "lbmh_ensemble_create", "lbmh_ensemble_destroy", "lbmh_ensemble_member_bytes", "lbmh_ensemble_add",
-    "lbmh_ensemble_remove", "lbmh_ensemble_set_wall", "lbmh_ensemble_set_steps", "lbmh_ensemble_status",
+    "lbmh_ensemble_remove", "lbmh_ensemble_set_wall", "lbmh_ensemble_set_local_boundary",
+    "lbmh_ensemble_initialize", "lbmh_ensemble_set_steps", "lbmh_ensemble_status",
     "lbmh_ensemble_ini_equilibrium", "lbmh_ensemble_upload", "lbmh_ensemble_download",
     "lbmh_ensemble_moments", "lbmh_ensemble_step", "lbmh_ensemble_step_graph",
     "lbmh_ensemble_set_launch_config", "lbmh_ensemble_usage", "lbmh_ensemble_status_all",
@@ -64,6 +65,7 @@ class EnsembleDesc(ctypes.Structure):
         ("q", ctypes.c_int), ("storage", ctypes.c_int), ("bulk_kind", ctypes.c_int),
         ("device", ctypes.c_int), ("max_members", ctypes.c_int),
         ("population_bytes", ctypes.c_size_t), ("cell_capacity", ctypes.c_size_t),
+        ("boundary_capacity", ctypes.c_size_t),
     ]
 
 
@@ -120,6 +122,8 @@ def load() -> ctypes.CDLL:
                                     ctypes.POINTER(i32)]
     L.lbmh_ensemble_remove.argtypes = [vp, i32]
     L.lbmh_ensemble_set_wall.argtypes = [vp, i32, i32, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
+    L.lbmh_ensemble_set_local_boundary.argtypes = [vp, i32, i32, i32, ctypes.c_double, vp]
+    L.lbmh_ensemble_initialize.argtypes = [vp, i32]
     L.lbmh_ensemble_set_steps.argtypes = [vp, i32, i64]
     L.lbmh_ensemble_status.argtypes = [vp, i32, ctypes.POINTER(i64), ctypes.POINTER(i64)]
     L.lbmh_ensemble_ini_equilibrium.argtypes = [vp, i32, vp]

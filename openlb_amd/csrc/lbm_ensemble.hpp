@@ -28,6 +28,8 @@ struct EnsembleEntry {
   uint32_t chunk_begin;     // first chunk of this lattice in the launch
   int64_t steps_left;       // 0: idle, < 0: unbounded
   int64_t steps_done;
+  uint32_t stepped;         // 1 if the last advance moved this lattice (its post-processors run)
+  uint32_t pad_;
   BulkParams<double> bp;
   uint32_t shift[Q];        // PS shifts, in [0, n_padded)
   uint32_t dmod[Q];         // (c_i . projection) mod n_padded
@@ -40,24 +42,12 @@ struct EnsembleLimits {
   uint32_t members;         // table slots
 };
 
-template <int Q, typename S, int KIND>
-__global__ __launch_bounds__(kBlock) void ensemble_step_kernel(const EnsembleEntry<Q>* __restrict__ table,
-                                                               const uint32_t* __restrict__ chunk_lattice,
-                                                               S* __restrict__ arena, const uint8_t* __restrict__ ids,
-                                                               const double* __restrict__ walls, uint32_t xcd_group,
-                                                               EnsembleLimits limits)
+// Whole-member arguments of the single-block cell body, from a table entry.
+template <int Q, typename S>
+__device__ __forceinline__ StepArgs<Q, typename StorageTraits<S>::V> member_args(
+    const EnsembleEntry<Q>& e, uint32_t lat, S* arena, const uint8_t* ids, const double* walls, const uint32_t* info)
 {
   using V = typename StorageTraits<S>::V;
-  uint32_t b = blockIdx.x;
-  if (xcd_group) b = remap_xcd(b, gridDim.x, xcd_group);
-  const uint32_t lat = chunk_lattice[b];
-  if (lat >= limits.members) return;
-  const EnsembleEntry<Q>& e = table[lat];
-  if (e.steps_left == 0) return;
-  // uniform guard: every column and the id range of the entry inside the arenas
-  if (e.stride < e.n_padded || e.pop_offset + uint64_t(Q) * e.stride > limits.pop_elements ||
-      e.ids_offset + e.ncore > limits.id_bytes)
-    return;
   StepArgs<Q, V> a;
   a.pop = arena + e.pop_offset;
   a.stride = e.stride;
@@ -76,7 +66,7 @@ __global__ __launch_bounds__(kBlock) void ensemble_step_kernel(const EnsembleEnt
   a.periodic = e.periodic;
   a.xcd_group = 0;
   a.walls = walls + uint64_t(lat) * kEnsembleWalls * 4;
-  a.info = nullptr;
+  a.info = info ? info + uint64_t(lat) * kEnsembleWalls : nullptr;
   a.bp.omega = V(e.bp.omega);
   a.bp.omega2 = V(e.bp.omega2);
   a.bp.guo = V(e.bp.guo);
@@ -89,7 +79,96 @@ __global__ __launch_bounds__(kBlock) void ensemble_step_kernel(const EnsembleEnt
   a.bp.sm_k = V(e.bp.sm_k);
   a.bp.tau = V(e.bp.tau);
   a.bp.half_tau = V(e.bp.half_tau);
-  collide_stream_cell<Q, S, KIND, true>(a, (b - e.chunk_begin) * kBlock + threadIdx.x);
+  return a;
+}
+
+// uniform guard: every column and the id range of the entry inside the arenas
+template <int Q>
+__device__ __forceinline__ bool entry_fits(const EnsembleEntry<Q>& e, const EnsembleLimits& limits)
+{
+  return !(e.stride < e.n_padded || e.pop_offset + uint64_t(Q) * e.stride > limits.pop_elements ||
+           e.ids_offset + e.ncore > limits.id_bytes);
+}
+
+// LOCAL = kLocalSkip when some member has local boundaries: their cells are
+// left to ensemble_local_kernel.
+template <int Q, typename S, int KIND, int LOCAL = kLocalNone>
+__global__ __launch_bounds__(kBlock) void ensemble_step_kernel(const EnsembleEntry<Q>* __restrict__ table,
+                                                               const uint32_t* __restrict__ chunk_lattice,
+                                                               S* __restrict__ arena, const uint8_t* __restrict__ ids,
+                                                               const double* __restrict__ walls,
+                                                               const uint32_t* __restrict__ info, uint32_t xcd_group,
+                                                               EnsembleLimits limits)
+{
+  uint32_t b = blockIdx.x;
+  if (xcd_group) b = remap_xcd(b, gridDim.x, xcd_group);
+  const uint32_t lat = chunk_lattice[b];
+  if (lat >= limits.members) return;
+  const EnsembleEntry<Q>& e = table[lat];
+  if (e.steps_left == 0) return;
+  if (!entry_fits(e, limits)) return;
+  const auto a = member_args<Q, S>(e, lat, arena, ids, walls, LOCAL == kLocalNone ? nullptr : info);
+  collide_stream_cell<Q, S, KIND, true, LOCAL>(a, (b - e.chunk_begin) * kBlock + threadIdx.x);
+}
+
+// The local-boundary cells of every stepping member: list entries are
+// (member, core cell) pairs.
+template <int Q, typename S, int KIND>
+__global__ __launch_bounds__(64) void ensemble_local_kernel(const EnsembleEntry<Q>* __restrict__ table,
+                                                            const uint2* __restrict__ cells, uint32_t count,
+                                                            S* __restrict__ arena, const uint8_t* __restrict__ ids,
+                                                            const double* __restrict__ walls,
+                                                            const uint32_t* __restrict__ info, EnsembleLimits limits)
+{
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const uint2 lc = cells[k];
+  if (lc.x >= limits.members) return;
+  const EnsembleEntry<Q>& e = table[lc.x];
+  if (e.steps_left == 0 || !entry_fits(e, limits) || lc.y >= e.ncore) return;
+  const auto a = member_args<Q, S>(e, lc.x, arena, ids, walls, info);
+  collide_stream_cell<Q, S, KIND, true, kLocalCollide>(a, lc.y);
+}
+
+// Edge (CORNER = false) or corner post-processors of the members that stepped
+// last (or of member `only` when only >= 0: initialize()).
+template <int Q, typename S, bool CORNER>
+__global__ __launch_bounds__(64) void ensemble_post_kernel(const EnsembleEntry<Q>* __restrict__ table,
+                                                           const uint2* __restrict__ cells, uint32_t count,
+                                                           S* __restrict__ arena, const uint8_t* __restrict__ ids,
+                                                           const double* __restrict__ walls,
+                                                           const uint32_t* __restrict__ info, EnsembleLimits limits,
+                                                           int only)
+{
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const uint2 lc = cells[k];
+  if (lc.x >= limits.members) return;
+  const EnsembleEntry<Q>& e = table[lc.x];
+  if (only >= 0 ? int(lc.x) != only : !e.stepped) return;
+  if (!entry_fits(e, limits) || lc.y >= e.ncore) return;
+  PostArgs<Q> a;
+  a.pop = arena + e.pop_offset;
+  a.stride = e.stride;
+  a.n_padded = e.n_padded;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) a.shift[i] = e.shift[i];
+  a.ids = ids + e.ids_offset;
+  a.walls = walls + uint64_t(lc.x) * kEnsembleWalls * 4;
+  a.info = info + uint64_t(lc.x) * kEnsembleWalls;
+  a.cells = nullptr;
+  a.count = 0;
+  a.nx = int(e.nx);
+  a.ny = int(e.ny);
+  a.nz = int(e.nz);
+  a.sx = e.sx;
+  a.sy = e.sy;
+  a.periodic = e.periodic;
+  a.div_z = e.div_z;
+  a.div_y = e.div_y;
+  a.omega = e.bp.omega;
+  if constexpr (CORNER) outer_corner_cell<Q, S>(a, lc.y);
+  else outer_edge_cell<Q, S>(a, lc.y);
 }
 
 // Streaming of every lattice that stepped: rotate its columns, count the step.
@@ -100,6 +179,7 @@ __global__ void ensemble_advance_kernel(EnsembleEntry<Q>* table, uint32_t nslots
   const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= nslots) return;
   EnsembleEntry<Q>& e = table[l];
+  e.stepped = e.steps_left != 0 ? 1u : 0u;
   if (e.steps_left == 0) return;
   for (int i = 0; i < Q; ++i) {
     const uint32_t s = e.shift[i], d = e.dmod[i];

@@ -432,12 +432,9 @@ __device__ __forceinline__ void core_coords(const FastDiv& div_z, const FastDiv&
 
 // OuterVelocityEdgeProcessor3D<plane,n1,n2>::apply    boundaryPostProcessors3D.hh:166-238
 template <int Q, typename S>
-__global__ __launch_bounds__(64) void outer_edge_kernel(PostArgs<Q> a)
+__device__ void outer_edge_cell(const PostArgs<Q>& a, uint32_t c)
 {
   using V = typename StorageTraits<S>::V;
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= a.count) return;
-  const uint32_t c = a.cells[k];
   int x, y, z;
   core_coords(a.div_z, a.div_y, a.ny, a.nz, c, x, y, z);
   const int id = a.ids[c];
@@ -477,12 +474,9 @@ __global__ __launch_bounds__(64) void outer_edge_kernel(PostArgs<Q> a)
 
 // OuterVelocityCornerProcessor3D<nx,ny,nz>::apply     boundaryPostProcessors3D.hh:242-292
 template <int Q, typename S>
-__global__ __launch_bounds__(64) void outer_corner_kernel(PostArgs<Q> a)
+__device__ void outer_corner_cell(const PostArgs<Q>& a, uint32_t c)
 {
   using V = typename StorageTraits<S>::V;
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= a.count) return;
-  const uint32_t c = a.cells[k];
   int x, y, z;
   core_coords(a.div_z, a.div_y, a.ny, a.nz, c, x, y, z);
   const int id = a.ids[c];
@@ -509,6 +503,20 @@ __global__ __launch_bounds__(64) void outer_corner_kernel(PostArgs<Q> a)
     for (int b = 0; b < 3; ++b) dA[d][b] = V(-nrm[d]) * boundary_gradient(u0[b], u1[b], u2[b]);
   }
   store_reconstruction<Q, S, V>(a, x, y, z, w, rho, dA);
+}
+
+template <int Q, typename S>
+__global__ __launch_bounds__(64) void outer_edge_kernel(PostArgs<Q> a)
+{
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < a.count) outer_edge_cell<Q, S>(a, a.cells[k]);
+}
+
+template <int Q, typename S>
+__global__ __launch_bounds__(64) void outer_corner_kernel(PostArgs<Q> a)
+{
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < a.count) outer_corner_cell<Q, S>(a, a.cells[k]);
 }
 
 }  // namespace lbmh

@@ -12,8 +12,10 @@
 #include <cstring>
 #include <map>
 #include <new>
+#include <tuple>
 #include <vector>
 
+#include "boundary_classify.hpp"
 #include "lbm_ensemble.hpp"
 #include "lbmh.h"
 #include "lbmh_common.hpp"
@@ -82,6 +84,11 @@ struct Member {
   uint64_t ids_off = 0, ids_bytes = 0;
   uint32_t ncore = 0;
   lbmh_bulk_params params{};
+  // local boundaries: host ids, per-id descriptors, cell lists (core indices)
+  std::vector<uint8_t> h_ids;
+  uint32_t info[kEnsembleWalls] = {};
+  bool id_defined[kEnsembleWalls] = {};
+  std::vector<uint32_t> local, edges, corners;
 };
 
 }  // namespace
@@ -105,6 +112,13 @@ struct lbmh_ensemble {
   hipGraphExec_t graph = nullptr;
   uint64_t graph_version = 0;
   int64_t graph_steps = 0;
+  // local boundaries: per-member id descriptors and (member, cell) lists
+  uint32_t* d_info = nullptr;
+  uint2* d_local = nullptr;
+  uint2* d_edges = nullptr;
+  uint2* d_corners = nullptr;
+  size_t boundary_capacity = 0;
+  uint32_t n_local = 0, n_edges = 0, n_corners = 0;
 };
 
 namespace {
@@ -177,6 +191,29 @@ int check_member(const lbmh_ensemble* E, int m)
   return LBMH_OK;
 }
 
+// Concatenate the local-boundary lists of the active members into the device lists.
+int rebuild_lists(lbmh_ensemble* E)
+{
+  std::vector<uint2> all[3];
+  for (int m = 0; m < E->max_members; ++m) {
+    const Member& mem = E->members[m];
+    if (!mem.active) continue;
+    const std::vector<uint32_t>* src[3] = {&mem.local, &mem.edges, &mem.corners};
+    for (int k = 0; k < 3; ++k)
+      for (uint32_t c : *src[k]) all[k].push_back(make_uint2(uint32_t(m), c));
+  }
+  for (int k = 0; k < 3; ++k)
+    if (all[k].size() > E->boundary_capacity) return fail(LBMH_E_MEMORY, "boundary list capacity exceeded");
+  uint2* dst[3] = {E->d_local, E->d_edges, E->d_corners};
+  for (int k = 0; k < 3; ++k)
+    if (!all[k].empty()) LBMH_HIP(hipMemcpy(dst[k], all[k].data(), all[k].size() * sizeof(uint2), hipMemcpyHostToDevice));
+  E->n_local = uint32_t(all[0].size());
+  E->n_edges = uint32_t(all[1].size());
+  E->n_corners = uint32_t(all[2].size());
+  E->version += 1;
+  return LBMH_OK;
+}
+
 template <int Q>
 CoreGeom geom_of(const EnsembleEntry<Q>& e)
 {
@@ -202,6 +239,25 @@ Shifts<Q> shifts_of(const EnsembleEntry<Q>& e)
   return s;
 }
 
+// PostStream stage of the local boundaries: edges, then corners, of the
+// members that stepped (only < 0) or of member `only`.
+template <int Q, typename S>
+int launch_post(lbmh_ensemble* E, const EnsembleEntry<Q>* table, S* arena, const EnsembleLimits& limits,
+                hipStream_t stream, int only)
+{
+  if (E->n_edges) {
+    hipLaunchKernelGGL((ensemble_post_kernel<Q, S, false>), dim3(grid_for(E->n_edges, 64)), dim3(64), 0, stream, table,
+                       E->d_edges, E->n_edges, arena, E->d_ids, E->d_walls, E->d_info, limits, only);
+    LBMH_HIP(hipGetLastError());
+  }
+  if (E->n_corners) {
+    hipLaunchKernelGGL((ensemble_post_kernel<Q, S, true>), dim3(grid_for(E->n_corners, 64)), dim3(64), 0, stream,
+                       table, E->d_corners, E->n_corners, arena, E->d_ids, E->d_walls, E->d_info, limits, only);
+    LBMH_HIP(hipGetLastError());
+  }
+  return LBMH_OK;
+}
+
 int launch_steps(lbmh_ensemble* E, int64_t n, hipStream_t stream)
 {
   const unsigned lds = unsigned(E->launch.lds_bytes);
@@ -214,35 +270,38 @@ int launch_steps(lbmh_ensemble* E, int64_t n, hipStream_t stream)
     S* arena = static_cast<S*>(E->d_pop);
     const EnsembleLimits limits{E->pop_arena.capacity() / sizeof(S), E->id_arena.capacity(),
                                 uint32_t(E->max_members)};
+    const bool local = E->n_local > 0;
     for (int64_t k = 0; k < n; ++k) {
       if (E->total_chunks) {
-        switch (E->kind) {
-        case LBMH_BGK:
-          hipLaunchKernelGGL((ensemble_step_kernel<Q, S, kBGK>), dim3(E->total_chunks), dim3(kBlock), lds, stream,
-                             table, E->d_chunk, arena, E->d_ids, E->d_walls, group, limits);
-          break;
-        case LBMH_FORCED_BGK:
-          hipLaunchKernelGGL((ensemble_step_kernel<Q, S, kForcedBGK>), dim3(E->total_chunks), dim3(kBlock), lds,
-                             stream, table, E->d_chunk, arena, E->d_ids, E->d_walls, group, limits);
-          break;
-        case LBMH_TRT:
-          hipLaunchKernelGGL((ensemble_step_kernel<Q, S, kTRT>), dim3(E->total_chunks), dim3(kBlock), lds, stream,
-                             table, E->d_chunk, arena, E->d_ids, E->d_walls, group, limits);
-          break;
-        case LBMH_FORCED_TRT:
-          hipLaunchKernelGGL((ensemble_step_kernel<Q, S, kForcedTRT>), dim3(E->total_chunks), dim3(kBlock), lds,
-                             stream, table, E->d_chunk, arena, E->d_ids, E->d_walls, group, limits);
-          break;
-        default:
-          hipLaunchKernelGGL((ensemble_step_kernel<Q, S, kSmagorinsky>), dim3(E->total_chunks), dim3(kBlock), lds,
-                             stream, table, E->d_chunk, arena, E->d_ids, E->d_walls, group, limits);
-          break;
+#define LBMH_ENSEMBLE_STEP(KIND_)                                                                               \
+  hipLaunchKernelGGL((ensemble_step_kernel<Q, S, KIND_>), dim3(E->total_chunks), dim3(kBlock), lds, stream, table, \
+                     E->d_chunk, arena, E->d_ids, E->d_walls, E->d_info, group, limits)
+        if (local) {
+          // local boundaries need BGK bulk dynamics (checked when they are defined)
+          hipLaunchKernelGGL((ensemble_step_kernel<Q, S, kBGK, kLocalSkip>), dim3(E->total_chunks), dim3(kBlock), lds,
+                             stream, table, E->d_chunk, arena, E->d_ids, E->d_walls, E->d_info, group, limits);
+        } else {
+          switch (E->kind) {
+          case LBMH_BGK: LBMH_ENSEMBLE_STEP(kBGK); break;
+          case LBMH_FORCED_BGK: LBMH_ENSEMBLE_STEP(kForcedBGK); break;
+          case LBMH_TRT: LBMH_ENSEMBLE_STEP(kTRT); break;
+          case LBMH_FORCED_TRT: LBMH_ENSEMBLE_STEP(kForcedTRT); break;
+          default: LBMH_ENSEMBLE_STEP(kSmagorinsky); break;
+          }
         }
+#undef LBMH_ENSEMBLE_STEP
+        LBMH_HIP(hipGetLastError());
+      }
+      if (local) {
+        hipLaunchKernelGGL((ensemble_local_kernel<Q, S, kBGK>), dim3(grid_for(E->n_local, 64)), dim3(64), 0, stream,
+                           table, E->d_local, E->n_local, arena, E->d_ids, E->d_walls, E->d_info, limits);
         LBMH_HIP(hipGetLastError());
       }
       hipLaunchKernelGGL((ensemble_advance_kernel<Q>), dim3(adv_grid), dim3(256), 0, stream, table,
                          uint32_t(E->max_members));
       LBMH_HIP(hipGetLastError());
+      const int rc = launch_post(E, table, arena, limits, stream, -1);
+      if (rc != LBMH_OK) return rc;
     }
     return LBMH_OK;
   });
@@ -282,6 +341,11 @@ int lbmh_ensemble_create(const lbmh_ensemble_desc* d, lbmh_ensemble** out)
   if (e == hipSuccess) e = hipMalloc((void**)&E->d_walls, sizeof(double) * 4 * kEnsembleWalls * E->max_members);
   if (e == hipSuccess) e = hipMalloc(&E->d_table, E->h_table.size());
   if (e == hipSuccess) e = hipMalloc((void**)&E->d_chunk, sizeof(uint32_t) * E->chunk_capacity);
+  E->boundary_capacity = d->boundary_capacity ? d->boundary_capacity : d->cell_capacity / 4 + 64;
+  if (e == hipSuccess) e = hipMalloc((void**)&E->d_info, sizeof(uint32_t) * kEnsembleWalls * E->max_members);
+  if (e == hipSuccess) e = hipMemset(E->d_info, 0, sizeof(uint32_t) * kEnsembleWalls * E->max_members);
+  for (uint2** list : {&E->d_local, &E->d_edges, &E->d_corners})
+    if (e == hipSuccess) e = hipMalloc((void**)list, sizeof(uint2) * E->boundary_capacity);
   if (e == hipSuccess) e = hipMemcpy(E->d_table, E->h_table.data(), E->h_table.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&E->capture_stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -305,6 +369,10 @@ int lbmh_ensemble_destroy(lbmh_ensemble* E)
   if (E->d_walls) (void)hipFree(E->d_walls);
   if (E->d_table) (void)hipFree(E->d_table);
   if (E->d_chunk) (void)hipFree(E->d_chunk);
+  if (E->d_info) (void)hipFree(E->d_info);
+  if (E->d_local) (void)hipFree(E->d_local);
+  if (E->d_edges) (void)hipFree(E->d_edges);
+  if (E->d_corners) (void)hipFree(E->d_corners);
   delete E;
   return LBMH_OK;
 }
@@ -376,6 +444,9 @@ int lbmh_ensemble_add(lbmh_ensemble* E, const int extent[3], const int periodic[
     e = hipMemcpy(E->d_walls + size_t(slot) * 4 * kEnsembleWalls, walls.data(), walls.size() * sizeof(double),
                   hipMemcpyHostToDevice);
   }
+  if (e == hipSuccess) e = hipMemcpy(E->d_info + size_t(slot) * kEnsembleWalls, mem.info, sizeof(mem.info),
+                                     hipMemcpyHostToDevice);
+  mem.h_ids.assign(ids, ids + ncore);
   if (e != hipSuccess) return undo(hip_fail(e, "lbmh_ensemble_add"));
   const uint64_t elem = storage_size(E->storage);
   with_q(E, [&](auto qc) -> int {
@@ -433,7 +504,9 @@ int lbmh_ensemble_remove(lbmh_ensemble* E, int m)
     std::memset(&entry<Q>(E, m), 0, sizeof(EnsembleEntry<Q>));
     return LBMH_OK;
   });
-  return push_table(E);
+  rc = push_table(E);
+  if (rc != LBMH_OK) return rc;
+  return rebuild_lists(E);
 }
 
 int lbmh_ensemble_set_wall(lbmh_ensemble* E, int m, int id, double rho, const double u[3])
@@ -446,6 +519,111 @@ int lbmh_ensemble_set_wall(lbmh_ensemble* E, int m, int id, double rho, const do
   const double w[4] = {rho, u[0], u[1], u[2]};
   LBMH_HIP(hipDeviceSynchronize());
   LBMH_HIP(hipMemcpy(E->d_walls + (size_t(m) * kEnsembleWalls + id) * 4, w, sizeof(w), hipMemcpyHostToDevice));
+  Member& mem = E->members[m];
+  mem.id_defined[id] = true;
+  mem.info[id] = pack_local_info(kMovingWall, 0, 0, 0);
+  LBMH_HIP(hipMemcpy(E->d_info + size_t(m) * kEnsembleWalls + id, &mem.info[id], sizeof(uint32_t), hipMemcpyHostToDevice));
+  return LBMH_OK;
+}
+
+int lbmh_ensemble_set_local_boundary(lbmh_ensemble* E, int m, int id, int kind, double rho, const double u[3])
+{
+  int rc = check_member(E, m);
+  if (rc != LBMH_OK) return rc;
+  if (!u) return fail(LBMH_E_INVALID, "null velocity");
+  if (kind != LBMH_LOCAL_VELOCITY && kind != LBMH_LOCAL_PRESSURE) return fail(LBMH_E_INVALID, "unknown local boundary kind");
+  if (id < 2 || id >= kEnsembleWalls) return fail(LBMH_E_INVALID, "ensemble boundary id must be in [2, 15]");
+  if (E->kind != LBMH_BGK) return fail(LBMH_E_INVALID, "local boundaries need BGK bulk dynamics");
+  if (kind == LBMH_LOCAL_PRESSURE && !(rho > 0)) return fail(LBMH_E_INVALID, "pressure boundary density must be positive");
+  Member& mem = E->members[m];
+  const int n[3] = {mem.n[0], mem.n[1], mem.n[2]};
+  bool used[kEnsembleWalls] = {};
+  for (uint8_t v : mem.h_ids) used[v] = true;
+  for (int w = 0; w < kEnsembleWalls; ++w) used[w] = used[w] || mem.id_defined[w];
+  used[0] = used[1] = used[2] = true;
+  std::map<std::tuple<int, int, int, int>, int> entry_ids;
+  std::vector<uint8_t> next = mem.h_ids;
+  std::vector<uint32_t> cells, edges, corners;
+  uint32_t info[kEnsembleWalls];
+  double walls[kEnsembleWalls][4];
+  std::memcpy(info, mem.info, sizeof(info));
+  int free_id = kEnsembleWalls - 1;
+  for (int x = 0; x < n[0]; ++x)
+    for (int y = 0; y < n[1]; ++y)
+      for (int z = 0; z < n[2]; ++z) {
+        const uint32_t c = uint32_t((int64_t(x) * n[1] + y) * n[2] + z);
+        if (mem.h_ids[c] != id) continue;
+        int type, nrm[3];
+        classify_boundary_cell(n, mem.periodic, mem.h_ids.data(), x, y, z, type, nrm);
+        const bool flat = type == kTypeFlat && (nrm[0] || nrm[1] || nrm[2]);
+        uint32_t k;
+        if (kind == LBMH_LOCAL_PRESSURE) {
+          if (!flat) return fail(LBMH_E_INVALID, "local pressure boundary on a non-flat wall cell");
+          k = kFlatPressure;
+        } else if (type == kTypeFlat) {
+          k = flat ? kFlatVelocity : kDeadWall;
+        } else {
+          static const uint32_t map[5] = {0, kOuterCorner, kInnerCorner, kOuterEdge, kInnerEdge};
+          k = map[type];
+        }
+        int a = nrm[0], b = nrm[1], cc = nrm[2];
+        if (k == kOuterEdge || k == kInnerEdge) {
+          const int plane = nrm[0] == 0 ? 0 : (nrm[1] == 0 ? 1 : 2);
+          a = plane;
+          b = nrm[(plane + 1) % 3];
+          cc = nrm[(plane + 2) % 3];
+        }
+        const auto key = std::make_tuple(int(k), a, b, cc);
+        auto it = entry_ids.find(key);
+        if (it == entry_ids.end()) {
+          while (free_id >= 3 && used[free_id]) --free_id;
+          if (free_id < 3) return fail(LBMH_E_INVALID, "no free ensemble dynamics ids left (ids 3..15)");
+          used[free_id] = true;
+          info[free_id] = pack_local_info(k, a, b, cc);
+          walls[free_id][0] = rho;
+          for (int d = 0; d < 3; ++d) walls[free_id][d + 1] = u[d];
+          it = entry_ids.emplace(key, free_id).first;
+        }
+        next[c] = uint8_t(it->second);
+        cells.push_back(c);
+        if (k == kOuterEdge) edges.push_back(c);
+        if (k == kOuterCorner) corners.push_back(c);
+      }
+  if (entry_ids.empty()) return LBMH_OK;
+  DeviceGuard guard(E->device);
+  LBMH_HIP(hipDeviceSynchronize());
+  for (const auto& kv : entry_ids) {
+    const int w = kv.second;
+    mem.id_defined[w] = true;
+    LBMH_HIP(hipMemcpy(E->d_walls + (size_t(m) * kEnsembleWalls + w) * 4, walls[w], sizeof(walls[w]),
+                       hipMemcpyHostToDevice));
+  }
+  std::memcpy(mem.info, info, sizeof(info));
+  mem.h_ids.swap(next);
+  mem.local.insert(mem.local.end(), cells.begin(), cells.end());
+  mem.edges.insert(mem.edges.end(), edges.begin(), edges.end());
+  mem.corners.insert(mem.corners.end(), corners.begin(), corners.end());
+  LBMH_HIP(hipMemcpy(E->d_info + size_t(m) * kEnsembleWalls, mem.info, sizeof(mem.info), hipMemcpyHostToDevice));
+  LBMH_HIP(hipMemcpy(E->d_ids + mem.ids_off, mem.h_ids.data(), mem.ncore, hipMemcpyHostToDevice));
+  return rebuild_lists(E);
+}
+
+int lbmh_ensemble_initialize(lbmh_ensemble* E, int m)
+{
+  int rc = check_member(E, m);
+  if (rc != LBMH_OK) return rc;
+  if (!E->n_edges && !E->n_corners) return LBMH_OK;
+  DeviceGuard guard(E->device);
+  rc = with_types(E, [&](auto qc, auto s) -> int {
+    constexpr int Q = decltype(qc)::value;
+    using S = decltype(s);
+    const EnsembleLimits limits{E->pop_arena.capacity() / sizeof(S), E->id_arena.capacity(),
+                                uint32_t(E->max_members)};
+    return launch_post<Q, S>(E, static_cast<EnsembleEntry<Q>*>(E->d_table), static_cast<S*>(E->d_pop), limits,
+                             nullptr, m);
+  });
+  if (rc != LBMH_OK) return rc;
+  LBMH_HIP(hipDeviceSynchronize());
   return LBMH_OK;
 }
 

@@ -24,11 +24,13 @@ _NP = {abi.F64: np.float64, abi.F32: np.float32, abi.F16: np.float16}
 
 class Ensemble:
     def __init__(self, q: int = 19, storage: str = "f32", kind: str = "bgk", max_members: int = 512,
-                 population_bytes: int = 1 << 30, cell_capacity: int = 1 << 24, device: int = 0):
+                 population_bytes: int = 1 << 30, cell_capacity: int = 1 << 24, device: int = 0,
+                 boundary_capacity: int = 0):
         d = abi.EnsembleDesc()
         d.q, d.storage, d.bulk_kind, d.device = q, abi.STORAGE[storage], abi.KINDS[kind], device
         d.max_members = max_members
         d.population_bytes, d.cell_capacity = int(population_bytes), int(cell_capacity)
+        d.boundary_capacity = int(boundary_capacity)
         h = ctypes.c_void_p()
         abi.check(abi.load().lbmh_ensemble_create(ctypes.byref(d), ctypes.byref(h)))
         self._h = h
@@ -83,11 +85,24 @@ class Ensemble:
         abi.check(abi.load().lbmh_ensemble_add(self._h, (ctypes.c_int * 3)(*case.shape),
                                                (ctypes.c_int * 3)(*case.periodic), _ptr(ids),
                                                ctypes.byref(p), int(steps), ctypes.byref(m)))
+        local = getattr(case, "local", {}) or {}
         for wid, (rho, u) in case.walls.items():
+            if wid in local:
+                continue
             abi.check(abi.load().lbmh_ensemble_set_wall(self._h, m.value, int(wid), float(rho),
                                                         (ctypes.c_double * 3)(*map(float, u))))
+        for mid, kind in local.items():
+            rho, u = case.walls.get(mid, (1.0, (0.0, 0.0, 0.0)))
+            abi.check(abi.load().lbmh_ensemble_set_local_boundary(
+                self._h, m.value, int(mid), {"velocity": 0, "pressure": 1}[kind], float(rho),
+                (ctypes.c_double * 3)(*map(float, u))))
         self.shapes[m.value] = tuple(case.shape)
         return m.value
+
+    def initialize(self, member: int):
+        """SuperLattice::initialize() of one member: its local boundaries' edge and
+        corner post-processors once."""
+        abi.check(abi.load().lbmh_ensemble_initialize(self._h, int(member)))
 
     def remove(self, member: int):
         abi.check(abi.load().lbmh_ensemble_remove(self._h, int(member)))
