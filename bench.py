@@ -267,8 +267,9 @@ def cavity_bench(size=200, steps=200, cpu_size=100, cpu_seconds=10.0, no_cpu=Fal
 
 def serving_bench(count=400, rate=200.0, budget_gb=200.0, tick_steps=64):
     """Config 5: a Poisson stream of D3Q19 channel solves at `rate` req/s (sizes
-    log-uniform in [24^3, 128^3], 500-5000 steps, inflow velocity N(0.05, 0.01^2), FP32 or
-    FP16 storage at random), served with continuous batching. The stream is finite
+    log-uniform in [24^3, 128^3], 500-5000 steps, a LocalVelocity inlet with inflow
+    velocity N(0.05, 0.01^2) and a LocalPressure outlet, FP32 or FP16 storage at
+    random), served with continuous batching. The stream is finite
     (`count` arrivals): 200 req/s offers ~2.3e11 cell updates per second, several times
     what one GPU sustains, so latency is dominated by queueing and processor sharing
     and is reported with the aggregate MLUPS. The arenas are sized to the stream's
@@ -286,7 +287,7 @@ def serving_bench(count=400, rate=200.0, budget_gb=200.0, tick_steps=64):
         span = time.perf_counter() - t0
         out = serving.summarize(results, span)
         out["max_concurrent"] = srv.max_active
-    out.update({"workload": "poisson_channel_d3q19_forced_bgk", "rate_per_s": rate,
+    out.update({"workload": "poisson_channel_d3q19_bgk_velocity_inlet_pressure_outlet", "rate_per_s": rate,
                 "arrival_window_s": round(reqs[-1].arrival, 3), "tick_steps": tick_steps,
                 "arena_bytes": budget, "hbm_budget_bytes": int(budget_gb * 1e9),
                 "offered_mlups": round(sum(r.lups for r in reqs) / reqs[-1].arrival / 1e6, 1),

@@ -63,13 +63,11 @@ class Result:
 
 
 def channel_request(rid, arrival, size, steps, u_in, storage, tau=0.6):
-    """A D3Q19 square-duct channel of size^3 cells. It is periodic in x, with BounceBack
-    walls at the y/z bounds. The flow is driven by a Guo body force that would give a
-    plane-Poiseuille centre velocity of u_in: F = 8 nu u_in / H^2, H = size - 2."""
-    nu = (tau - 0.5) / 3
-    h = max(size - 2, 1)
-    force = 8 * nu * u_in / (h * h)
-    case = workloads.porous_channel((size, size, size), seed=rid, tau=tau, force=force, solid_fraction=0.0)
+    """A D3Q19 square duct of size^3 cells with BounceBack walls: a LocalVelocity
+    inlet at x = 0 with inflow velocity u_in along x, a LocalPressure outlet at
+    x = size-1 at density 1 (the regularized boundaries of
+    src/boundary/localVelocity3D.h and localPressure3D.h), starting at rest."""
+    case = workloads.velocity_channel((size, size, size), u_in=u_in, rho_out=1.0, tau=tau)
     case.name = f"channel_{rid}"
     case.rho_u = None     # rest state (rho 1, u 0); built at admission to keep the queue small
     return Request(rid, arrival, case, int(steps), storage)
@@ -114,7 +112,7 @@ class SolveServer:
             # id arena: one byte per cell; a member's populations take at least
             # 2*19 bytes per cell, so cells <= pop bytes / 38
             cells = share // (19 * {"f32": 4, "f16": 2, "f64": 8}[st]) + (1 << 20)
-            self.ens[st] = Ensemble(19, st, "forced_bgk", max_members=max_members,
+            self.ens[st] = Ensemble(19, st, "bgk", max_members=max_members,
                                     population_bytes=share, cell_capacity=cells, device=device)
         self.max_active = 0
 
@@ -136,6 +134,7 @@ class SolveServer:
                 return None
             raise
         self.ens[req.storage].ini_equilibrium(m, initial_moments(req.case))
+        self.ens[req.storage].initialize(m)
         return m
 
     def run(self, requests, realtime: bool = True) -> list[Result]:

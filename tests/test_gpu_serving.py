@@ -16,6 +16,7 @@ pytestmark = pytest.mark.gpu
 def alone(req):
     with BlockLattice.from_case(req.case, req.storage) as lat:
         lat.ini_equilibrium(serving.initial_moments(req.case))
+        lat.initialize()
         lat.collide_and_stream(req.steps)
         return lat.download_populations("f64")
 
@@ -52,7 +53,7 @@ def test_admission_waits_for_memory(gpu_available):
     assert len(results) == 6
     for res in results:
         assert np.isfinite(res.moments).all()
-        # driven along +x by the body force
+        # driven along +x by the velocity inlet
         assert res.moments[1][res.moments[0] > 0].mean() > 0
     summary = serving.summarize(results)
     assert summary["requests"] == 6 and summary["p99_s"] >= summary["p50_s"] > 0
