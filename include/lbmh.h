@@ -258,7 +258,7 @@ int lbmh_ensemble_remove(lbmh_ensemble* ensemble, int member);
 int lbmh_ensemble_set_wall(lbmh_ensemble* ensemble, int member, int id, double rho, const double u[3]);
 /* Local velocity / pressure boundary on the member's cells carrying `id`, as
  * lbmh_set_local_boundary does for a lattice; the per-type ids come from the
- * member's free ids 3..15.  Needs an LBMH_BGK ensemble. */
+ * member's free ids 3..63.  Needs an LBMH_BGK ensemble. */
 int lbmh_ensemble_set_local_boundary(lbmh_ensemble* ensemble, int member, int id, int kind, double rho,
                                      const double u[3]);
 /* initialize() of one member: its edge and corner post-processors once. */

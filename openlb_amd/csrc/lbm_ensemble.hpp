@@ -15,7 +15,7 @@
 
 namespace lbmh {
 
-constexpr int kEnsembleWalls = 16;  // wall-table entries per lattice (ids 0..15)
+constexpr int kEnsembleWalls = 64;  // wall-table entries per lattice (ids 0..63)
 
 template <int Q>
 struct EnsembleEntry {

@@ -402,7 +402,7 @@ int lbmh_ensemble_add(lbmh_ensemble* E, const int extent[3], const int periodic[
   if (npad >= (uint64_t(1) << 31)) return fail(LBMH_E_INVALID, "padded cell count must be below 2^31");
   const uint32_t ncore = uint32_t(uint64_t(extent[0]) * extent[1] * extent[2]);
   for (uint32_t c = 0; c < ncore; ++c)
-    if (ids[c] >= kEnsembleWalls) return fail(LBMH_E_INVALID, "ensemble dynamics ids must be below 16");
+    if (ids[c] >= kEnsembleWalls) return fail(LBMH_E_INVALID, "ensemble dynamics ids must be below 64");
   if (!valid_bulk_params(E->kind, *params)) return LBMH_E_INVALID;
   int slot = -1;
   for (int m = 0; m < E->max_members; ++m)
@@ -514,7 +514,7 @@ int lbmh_ensemble_set_wall(lbmh_ensemble* E, int m, int id, double rho, const do
   int rc = check_member(E, m);
   if (rc != LBMH_OK) return rc;
   if (!u) return fail(LBMH_E_INVALID, "null velocity");
-  if (id < LBMH_ID_FIRST_WALL || id >= kEnsembleWalls) return fail(LBMH_E_INVALID, "ensemble wall id must be in [3, 15]");
+  if (id < LBMH_ID_FIRST_WALL || id >= kEnsembleWalls) return fail(LBMH_E_INVALID, "ensemble wall id must be in [3, 63]");
   DeviceGuard guard(E->device);
   const double w[4] = {rho, u[0], u[1], u[2]};
   LBMH_HIP(hipDeviceSynchronize());
@@ -532,7 +532,7 @@ int lbmh_ensemble_set_local_boundary(lbmh_ensemble* E, int m, int id, int kind, 
   if (rc != LBMH_OK) return rc;
   if (!u) return fail(LBMH_E_INVALID, "null velocity");
   if (kind != LBMH_LOCAL_VELOCITY && kind != LBMH_LOCAL_PRESSURE) return fail(LBMH_E_INVALID, "unknown local boundary kind");
-  if (id < 2 || id >= kEnsembleWalls) return fail(LBMH_E_INVALID, "ensemble boundary id must be in [2, 15]");
+  if (id < 2 || id >= kEnsembleWalls) return fail(LBMH_E_INVALID, "ensemble boundary id must be in [2, 63]");
   if (E->kind != LBMH_BGK) return fail(LBMH_E_INVALID, "local boundaries need BGK bulk dynamics");
   if (kind == LBMH_LOCAL_PRESSURE && !(rho > 0)) return fail(LBMH_E_INVALID, "pressure boundary density must be positive");
   Member& mem = E->members[m];
@@ -577,7 +577,7 @@ int lbmh_ensemble_set_local_boundary(lbmh_ensemble* E, int m, int id, int kind, 
         auto it = entry_ids.find(key);
         if (it == entry_ids.end()) {
           while (free_id >= 3 && used[free_id]) --free_id;
-          if (free_id < 3) return fail(LBMH_E_INVALID, "no free ensemble dynamics ids left (ids 3..15)");
+          if (free_id < 3) return fail(LBMH_E_INVALID, "no free ensemble dynamics ids left (ids 3..63)");
           used[free_id] = true;
           info[free_id] = pack_local_info(k, a, b, cc);
           walls[free_id][0] = rho;
