@@ -106,7 +106,7 @@ __device__ __forceinline__ void push_periodic_halo(const StepArgs<Q, V>& a, S* p
 // How a pass treats the ids >= 3 that carry local-boundary dynamics: the
 // whole-lattice pass leaves them to a second pass over the list of those
 // cells, so the bulk kernel keeps its register budget (and occupancy).
-enum LocalMode : int { kLocalNone = 0, kLocalSkip = 1, kLocalCollide = 2 };
+enum LocalMode : int { kLocalNone = 0, kLocalSkip = 1, kLocalCollide = 2, kLocalFlat = 3 };
 
 // Time step of core cell c of one block: collide in place, refresh the
 // periodic overlap.  The caller advances the shifts afterwards.
@@ -131,6 +131,13 @@ __device__ __forceinline__ void collide_stream_cell(const StepArgs<Q, typename S
     if (id == 0 && !edge) return;
     if constexpr (LOCAL == kLocalSkip) {
       if (id >= 3 && (a.info[id] & 0xffu) != kMovingWall) return;
+    }
+    if constexpr (LOCAL == kLocalFlat) {
+      // flat faces stay in this pass; edges, corners and the rest go to the list pass
+      if (id >= 3) {
+        const uint32_t k = a.info[id] & 0xffu;
+        if (k != kMovingWall && k != kFlatVelocity && k != kFlatPressure) return;
+      }
     }
   }
   const uint32_t n = (x + 1) * a.sx + (y + 1) * a.sy + (z + 1);
@@ -157,6 +164,13 @@ __device__ __forceinline__ void collide_stream_cell(const StepArgs<Q, typename S
             V rho, usq;
             collide_local<Q, V>(f, li, w, a.bp.omega, rho, usq);
           }
+        }
+      }
+      if constexpr (LOCAL == kLocalFlat) {
+        const LocalInfo li = unpack_local_info(a.info[id]);
+        if (li.kind != kMovingWall) {
+          wall = false;
+          collide_flat<Q, V>(f, li, w, a.bp.omega);
         }
       }
       if (wall) {

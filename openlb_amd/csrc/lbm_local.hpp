@@ -306,6 +306,71 @@ __device__ __forceinline__ void collide_local(V (&f)[Q], const LocalInfo& li, co
   usq_out = usq;
 }
 
+// Regularized flat velocity / pressure boundary with a small register
+// footprint, for the whole-lattice kernel: the same operations in the same
+// order as collide_local, with the non-equilibrium parts folded into the six
+// stress accumulators instead of kept per population.
+template <int Q, typename V>
+__device__ __forceinline__ void collide_flat(V (&f)[Q], const LocalInfo& li, const double* w, V omega)
+{
+  const int dir = flat_dir(li), orient = info_component(li, dir);
+  const bool pressure = li.kind == kFlatPressure;
+  V u[3], rho;
+  auto momenta = [&]() {
+    u[0] = V(w[1]);
+    u[1] = V(w[2]);
+    u[2] = V(w[3]);
+    V on = V(0), nrm = V(0);
+#pragma unroll
+    for (int i = 0; i < Q; ++i)
+      if (cdir<Q>(i, dir) == 0) on += f[i];
+#pragma unroll
+    for (int i = 0; i < Q; ++i)
+      if (cdir<Q>(i, dir) == orient) nrm += f[i];
+    if (pressure) {
+      rho = V(w[0]);
+      u[dir] = V(orient) * ((V(2) * nrm + on + V(1)) / rho - V(1));
+    } else {
+      rho = (V(2) * nrm + on + V(1)) / (V(1) + V(orient) * u[dir]);
+    }
+  };
+  momenta();
+  V usq = norm_sqr(u);
+  V pi[6] = {V(0), V(0), V(0), V(0), V(0), V(0)};
+  using L = Lattice<Q>;
+  // BoundaryHelpers::computeStress: on-wall populations, then the normal ones
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    if (cdir<Q>(i, dir) != 0) continue;
+    const V fneq = i == 0 ? V(0) : f[i] - feq_shifted<Q, V>(i, rho, u, usq);
+    int k = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = a; b < 3; ++b) pi[k++] += V(L::c[i][a] * L::c[i][b]) * fneq;
+  }
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    if (cdir<Q>(i, dir) != orient) continue;
+    const V fneq = f[i] - feq_shifted<Q, V>(i, rho, u, usq);
+    int k = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = a; b < 3; ++b) pi[k++] += V(2) * V(L::c[i][a] * L::c[i][b]) * fneq;
+  }
+#pragma unroll
+  for (int i = 0; i < Q; ++i) f[i] = feq_shifted<Q, V>(i, rho, u, usq) + pi_to_fneq<Q, V>(i, pi);
+  momenta();
+  usq = norm_sqr(u);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const V e = feq_shifted<Q, V>(i, rho, u, usq);
+    f[i] *= V(1) - omega;
+    f[i] += omega * e;
+  }
+}
+
 // ---- post-processors (PostStream stage) ----------------------------------
 
 template <int Q>

@@ -159,7 +159,7 @@ int launch_kind(lbmh_lattice* L, hipStream_t stream)
   bool launched = false;
   if constexpr (KIND == kBGK) {
     if (L->has_local) {
-      hipLaunchKernelGGL((collide_stream_kernel<Q, S, KIND, true, kLocalSkip>), dim3(grid), dim3(kBlock), lds, stream, a);
+      hipLaunchKernelGGL((collide_stream_kernel<Q, S, KIND, true, kLocalFlat>), dim3(grid), dim3(kBlock), lds, stream, a);
       LBMH_HIP(hipGetLastError());
       const uint32_t count = uint32_t(L->h_local.size());
       if (count) {
@@ -467,7 +467,8 @@ int lbmh_set_local_boundary(lbmh_lattice* L, int id, int kind, double rho, const
           it = entry.emplace(key, free_id).first;
         }
         next[c] = uint8_t(it->second);
-        cells.push_back(c);
+        // flat faces are collided by the whole-lattice pass, the rest by the list pass
+        if (k != kFlatVelocity && k != kFlatPressure) cells.push_back(c);
         if (k == kOuterEdge) edges.push_back(c);
         if (k == kOuterCorner) corners.push_back(c);
       }

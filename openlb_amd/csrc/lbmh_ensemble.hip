@@ -119,6 +119,7 @@ struct lbmh_ensemble {
   uint2* d_corners = nullptr;
   size_t boundary_capacity = 0;
   uint32_t n_local = 0, n_edges = 0, n_corners = 0;
+  bool has_local = false;            // some member ever had local boundaries
 };
 
 namespace {
@@ -270,7 +271,7 @@ int launch_steps(lbmh_ensemble* E, int64_t n, hipStream_t stream)
     S* arena = static_cast<S*>(E->d_pop);
     const EnsembleLimits limits{E->pop_arena.capacity() / sizeof(S), E->id_arena.capacity(),
                                 uint32_t(E->max_members)};
-    const bool local = E->n_local > 0;
+    const bool local = E->has_local;
     for (int64_t k = 0; k < n; ++k) {
       if (E->total_chunks) {
 #define LBMH_ENSEMBLE_STEP(KIND_)                                                                               \
@@ -278,7 +279,7 @@ int launch_steps(lbmh_ensemble* E, int64_t n, hipStream_t stream)
                      E->d_chunk, arena, E->d_ids, E->d_walls, E->d_info, group, limits)
         if (local) {
           // local boundaries need BGK bulk dynamics (checked when they are defined)
-          hipLaunchKernelGGL((ensemble_step_kernel<Q, S, kBGK, kLocalSkip>), dim3(E->total_chunks), dim3(kBlock), lds,
+          hipLaunchKernelGGL((ensemble_step_kernel<Q, S, kBGK, kLocalFlat>), dim3(E->total_chunks), dim3(kBlock), lds,
                              stream, table, E->d_chunk, arena, E->d_ids, E->d_walls, E->d_info, group, limits);
         } else {
           switch (E->kind) {
@@ -292,7 +293,7 @@ int launch_steps(lbmh_ensemble* E, int64_t n, hipStream_t stream)
 #undef LBMH_ENSEMBLE_STEP
         LBMH_HIP(hipGetLastError());
       }
-      if (local) {
+      if (local && E->n_local) {
         hipLaunchKernelGGL((ensemble_local_kernel<Q, S, kBGK>), dim3(grid_for(E->n_local, 64)), dim3(64), 0, stream,
                            table, E->d_local, E->n_local, arena, E->d_ids, E->d_walls, E->d_info, limits);
         LBMH_HIP(hipGetLastError());
@@ -585,7 +586,7 @@ int lbmh_ensemble_set_local_boundary(lbmh_ensemble* E, int m, int id, int kind, 
           it = entry_ids.emplace(key, free_id).first;
         }
         next[c] = uint8_t(it->second);
-        cells.push_back(c);
+        if (k != kFlatVelocity && k != kFlatPressure) cells.push_back(c);
         if (k == kOuterEdge) edges.push_back(c);
         if (k == kOuterCorner) corners.push_back(c);
       }
@@ -605,6 +606,7 @@ int lbmh_ensemble_set_local_boundary(lbmh_ensemble* E, int m, int id, int kind, 
   mem.corners.insert(mem.corners.end(), corners.begin(), corners.end());
   LBMH_HIP(hipMemcpy(E->d_info + size_t(m) * kEnsembleWalls, mem.info, sizeof(mem.info), hipMemcpyHostToDevice));
   LBMH_HIP(hipMemcpy(E->d_ids + mem.ids_off, mem.h_ids.data(), mem.ncore, hipMemcpyHostToDevice));
+  E->has_local = true;
   return rebuild_lists(E);
 }
 
