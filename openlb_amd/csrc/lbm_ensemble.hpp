@@ -131,7 +131,8 @@ __global__ __launch_bounds__(64) void ensemble_local_kernel(const EnsembleEntry<
 }
 
 // Edge (CORNER = false) or corner post-processors of the members that stepped
-// last (or of member `only` when only >= 0: initialize()).
+// last (or of member `only` when only >= 0: initialize()).  Eight lanes per
+// (member, cell) entry, as outer_post_kernel.
 template <int Q, typename S, bool CORNER>
 __global__ __launch_bounds__(64) void ensemble_post_kernel(const EnsembleEntry<Q>* __restrict__ table,
                                                            const uint2* __restrict__ cells, uint32_t count,
@@ -140,35 +141,45 @@ __global__ __launch_bounds__(64) void ensemble_post_kernel(const EnsembleEntry<Q
                                                            const uint32_t* __restrict__ info, EnsembleLimits limits,
                                                            int only)
 {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= count) return;
-  const uint2 lc = cells[k];
-  if (lc.x >= limits.members) return;
-  const EnsembleEntry<Q>& e = table[lc.x];
-  if (only >= 0 ? int(lc.x) != only : !e.stepped) return;
-  if (!entry_fits(e, limits) || lc.y >= e.ncore) return;
+  using V = typename StorageTraits<S>::V;
+  __shared__ V nb[64 / kPostLanes][kPostSlots][4];
+  const int slot = int(threadIdx.x % kPostLanes), j = int(threadIdx.x / kPostLanes);
+  const uint32_t k = blockIdx.x * (64 / kPostLanes) + uint32_t(j);
+  bool valid = k < count;
+  uint2 lc = make_uint2(0, 0);
   PostArgs<Q> a;
-  a.pop = arena + e.pop_offset;
-  a.stride = e.stride;
-  a.n_padded = e.n_padded;
+  if (valid) {
+    lc = cells[k];
+    valid = lc.x < limits.members;
+  }
+  if (valid) {
+    const EnsembleEntry<Q>& e = table[lc.x];
+    valid = (only >= 0 ? int(lc.x) == only : e.stepped != 0) && entry_fits(e, limits) && lc.y < e.ncore;
+    if (valid) {
+      a.pop = arena + e.pop_offset;
+      a.stride = e.stride;
+      a.n_padded = e.n_padded;
 #pragma unroll
-  for (int i = 0; i < Q; ++i) a.shift[i] = e.shift[i];
-  a.ids = ids + e.ids_offset;
-  a.walls = walls + uint64_t(lc.x) * kEnsembleWalls * 4;
-  a.info = info + uint64_t(lc.x) * kEnsembleWalls;
-  a.cells = nullptr;
-  a.count = 0;
-  a.nx = int(e.nx);
-  a.ny = int(e.ny);
-  a.nz = int(e.nz);
-  a.sx = e.sx;
-  a.sy = e.sy;
-  a.periodic = e.periodic;
-  a.div_z = e.div_z;
-  a.div_y = e.div_y;
-  a.omega = e.bp.omega;
-  if constexpr (CORNER) outer_corner_cell<Q, S>(a, lc.y);
-  else outer_edge_cell<Q, S>(a, lc.y);
+      for (int i = 0; i < Q; ++i) a.shift[i] = e.shift[i];
+      a.ids = ids + e.ids_offset;
+      a.walls = walls + uint64_t(lc.x) * kEnsembleWalls * 4;
+      a.info = info + uint64_t(lc.x) * kEnsembleWalls;
+      a.cells = nullptr;
+      a.count = 0;
+      a.nx = int(e.nx);
+      a.ny = int(e.ny);
+      a.nz = int(e.nz);
+      a.sx = e.sx;
+      a.sy = e.sy;
+      a.periodic = e.periodic;
+      a.div_z = e.div_z;
+      a.div_y = e.div_y;
+      a.omega = e.bp.omega;
+    }
+  }
+  if (valid && slot < kPostSlots) post_gather<Q, S, CORNER>(a, lc.y, slot, nb[j][slot]);
+  __syncthreads();
+  if (valid && slot == 0) post_finish<Q, S, CORNER>(a, lc.y, nb[j]);
 }
 
 // Streaming of every lattice that stepped: rotate its columns, count the step.

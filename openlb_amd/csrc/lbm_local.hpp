@@ -495,9 +495,58 @@ __device__ __forceinline__ void core_coords(const FastDiv& div_z, const FastDiv&
   x = int(xx);
 }
 
-// OuterVelocityEdgeProcessor3D<plane,n1,n2>::apply    boundaryPostProcessors3D.hh:166-238
-template <int Q, typename S>
-__device__ void outer_edge_cell(const PostArgs<Q>& a, uint32_t c)
+// The edge and corner post-processors each read six neighbours (rho and u
+// through the neighbour's own momenta).  A cell is handled by eight lanes:
+// six evaluate one neighbour each, so the dependent loads of the neighbours
+// overlap, and one lane then rebuilds the cell.
+//   edge (plane p, n1, n2)  OuterVelocityEdgeProcessor3D::apply, boundaryPostProcessors3D.hh:166-238
+//     slots: -n1 e_d1, -2 n1 e_d1, -n2 e_d2, -2 n2 e_d2, +e_p, -e_p  (d1 = p+1, d2 = p+2 mod 3)
+//   corner (nx, ny, nz)     OuterVelocityCornerProcessor3D::apply, boundaryPostProcessors3D.hh:242-292
+//     slots: -n_d e_d and -2 n_d e_d for d = x, y, z
+constexpr int kPostLanes = 8;
+constexpr int kPostSlots = 6;
+
+__device__ __forceinline__ void post_offset(const LocalInfo& li, bool corner, int slot, int (&o)[3])
+{
+  o[0] = o[1] = o[2] = 0;
+  if (corner) {
+    const int d = slot >> 1, n = info_component(li, d);
+    o[d] = (slot & 1) ? -2 * n : -n;
+    return;
+  }
+  const int p = li.a, d1 = (p + 1) % 3, d2 = (p + 2) % 3;
+  switch (slot) {
+  case 0: o[d1] = -li.b; break;
+  case 1: o[d1] = -2 * li.b; break;
+  case 2: o[d2] = -li.c; break;
+  case 3: o[d2] = -2 * li.c; break;
+  case 4: o[p] = 1; break;
+  default: o[p] = -1; break;
+  }
+}
+
+// rho and u of neighbour `slot` of post-processed cell c
+template <int Q, typename S, bool CORNER>
+__device__ void post_gather(const PostArgs<Q>& a, uint32_t c, int slot, typename StorageTraits<S>::V (&out)[4])
+{
+  using V = typename StorageTraits<S>::V;
+  int x, y, z;
+  core_coords(a.div_z, a.div_y, a.ny, a.nz, c, x, y, z);
+  const LocalInfo li = unpack_local_info(a.info[a.ids[c]]);
+  int o[3];
+  post_offset(li, CORNER, slot, o);
+  V rho, u[3];
+  cell_rho_u<Q, S, V>(a, x + o[0], y + o[1], z + o[2], rho, u);
+  out[0] = rho;
+  out[1] = u[0];
+  out[2] = u[1];
+  out[3] = u[2];
+}
+
+// Density and velocity gradient from the six neighbour evaluations, then the
+// regularized reconstruction of cell c
+template <int Q, typename S, bool CORNER>
+__device__ void post_finish(const PostArgs<Q>& a, uint32_t c, const typename StorageTraits<S>::V (*nb)[4])
 {
   using V = typename StorageTraits<S>::V;
   int x, y, z;
@@ -505,83 +554,39 @@ __device__ void outer_edge_cell(const PostArgs<Q>& a, uint32_t c)
   const int id = a.ids[c];
   const double* w = a.walls + 4 * id;
   const LocalInfo li = unpack_local_info(a.info[id]);
-  const int p = li.a, n1 = li.b, n2 = li.c;
-  const int d1 = (p + 1) % 3, d2 = (p + 2) % 3;
-  const int r0[3] = {x, y, z};
-  auto at = [&](int dd1, int dd2, int dp, V& rho, V (&u)[3]) {
-    int r[3] = {r0[0], r0[1], r0[2]};
-    r[d1] += dd1;
-    r[d2] += dd2;
-    r[p] += dp;
-    cell_rho_u<Q, S, V>(a, r[0], r[1], r[2], rho, u);
-  };
-  V rho10, rho01, rho20, rho02, tmp[3];
-  at(-n1, 0, 0, rho10, tmp);
-  at(0, -n2, 0, rho01, tmp);
-  at(-2 * n1, 0, 0, rho20, tmp);
-  at(0, -2 * n2, 0, rho02, tmp);
-  const V rho = V(2) / V(3) * (rho01 + rho10) - V(1) / V(6) * (rho02 + rho20);
   const V u0[3] = {V(w[1]), V(w[2]), V(w[3])};
-  V dA[3][3], u1[3], u2[3], r;
-  // along the edge: centralGradient                     finiteDifference.h:33-38
-  at(0, 0, 1, r, u1);
-  at(0, 0, -1, r, u2);
-  for (int b = 0; b < 3; ++b) dA[p][b] = (u1[b] - u2[b]) / V(2);
-  // into the fluid along d1 and d2: DirectedGradients3D<.., true>  finiteDifference3D.h:73-150
-  at(-n1, 0, 0, r, u1);
-  at(-2 * n1, 0, 0, r, u2);
-  for (int b = 0; b < 3; ++b) dA[d1][b] = V(-n1) * boundary_gradient(u0[b], u1[b], u2[b]);
-  at(0, -n2, 0, r, u1);
-  at(0, -2 * n2, 0, r, u2);
-  for (int b = 0; b < 3; ++b) dA[d2][b] = V(-n2) * boundary_gradient(u0[b], u1[b], u2[b]);
+  V rho, dA[3][3];
+  if constexpr (CORNER) {
+    rho = V(4) / V(9) * (nb[4][0] + nb[2][0] + nb[0][0]) - V(1) / V(9) * (nb[5][0] + nb[3][0] + nb[1][0]);
+    for (int d = 0; d < 3; ++d) {
+      const int n = info_component(li, d);
+      for (int b = 0; b < 3; ++b) dA[d][b] = V(-n) * boundary_gradient(u0[b], nb[2 * d][1 + b], nb[2 * d + 1][1 + b]);
+    }
+  } else {
+    const int p = li.a, n1 = li.b, n2 = li.c, d1 = (p + 1) % 3, d2 = (p + 2) % 3;
+    // rho10 = slot 0, rho20 = slot 1, rho01 = slot 2, rho02 = slot 3
+    rho = V(2) / V(3) * (nb[2][0] + nb[0][0]) - V(1) / V(6) * (nb[3][0] + nb[1][0]);
+    for (int b = 0; b < 3; ++b) {
+      dA[p][b] = (nb[4][1 + b] - nb[5][1 + b]) / V(2);                      // centralGradient
+      dA[d1][b] = V(-n1) * boundary_gradient(u0[b], nb[0][1 + b], nb[1][1 + b]);
+      dA[d2][b] = V(-n2) * boundary_gradient(u0[b], nb[2][1 + b], nb[3][1 + b]);
+    }
+  }
   store_reconstruction<Q, S, V>(a, x, y, z, w, rho, dA);
 }
 
-// OuterVelocityCornerProcessor3D<nx,ny,nz>::apply     boundaryPostProcessors3D.hh:242-292
-template <int Q, typename S>
-__device__ void outer_corner_cell(const PostArgs<Q>& a, uint32_t c)
+// One launch per post-processor kind over the cell list: 8 cells per workgroup.
+template <int Q, typename S, bool CORNER>
+__global__ __launch_bounds__(64) void outer_post_kernel(PostArgs<Q> a)
 {
   using V = typename StorageTraits<S>::V;
-  int x, y, z;
-  core_coords(a.div_z, a.div_y, a.ny, a.nz, c, x, y, z);
-  const int id = a.ids[c];
-  const double* w = a.walls + 4 * id;
-  const LocalInfo li = unpack_local_info(a.info[id]);
-  const int nrm[3] = {li.a, li.b, li.c};
-  V r1[3], r2[3], tmp[3];
-  for (int d = 0; d < 3; ++d) {
-    int o[3] = {0, 0, 0};
-    o[d] = -nrm[d];
-    cell_rho_u<Q, S, V>(a, x + o[0], y + o[1], z + o[2], r1[d], tmp);
-    o[d] = -2 * nrm[d];
-    cell_rho_u<Q, S, V>(a, x + o[0], y + o[1], z + o[2], r2[d], tmp);
-  }
-  const V rho = V(4) / V(9) * (r1[2] + r1[1] + r1[0]) - V(1) / V(9) * (r2[2] + r2[1] + r2[0]);
-  const V u0[3] = {V(w[1]), V(w[2]), V(w[3])};
-  V dA[3][3], u1[3], u2[3], r;
-  for (int d = 0; d < 3; ++d) {
-    int o[3] = {0, 0, 0};
-    o[d] = -nrm[d];
-    cell_rho_u<Q, S, V>(a, x + o[0], y + o[1], z + o[2], r, u1);
-    o[d] = -2 * nrm[d];
-    cell_rho_u<Q, S, V>(a, x + o[0], y + o[1], z + o[2], r, u2);
-    for (int b = 0; b < 3; ++b) dA[d][b] = V(-nrm[d]) * boundary_gradient(u0[b], u1[b], u2[b]);
-  }
-  store_reconstruction<Q, S, V>(a, x, y, z, w, rho, dA);
-}
-
-template <int Q, typename S>
-__global__ __launch_bounds__(64) void outer_edge_kernel(PostArgs<Q> a)
-{
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < a.count) outer_edge_cell<Q, S>(a, a.cells[k]);
-}
-
-template <int Q, typename S>
-__global__ __launch_bounds__(64) void outer_corner_kernel(PostArgs<Q> a)
-{
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < a.count) outer_corner_cell<Q, S>(a, a.cells[k]);
+  __shared__ V nb[64 / kPostLanes][kPostSlots][4];
+  const int slot = int(threadIdx.x % kPostLanes), j = int(threadIdx.x / kPostLanes);
+  const uint32_t k = blockIdx.x * (64 / kPostLanes) + uint32_t(j);
+  const bool valid = k < a.count;
+  if (valid && slot < kPostSlots) post_gather<Q, S, CORNER>(a, a.cells[k], slot, nb[j][slot]);
+  __syncthreads();
+  if (valid && slot == 0) post_finish<Q, S, CORNER>(a, a.cells[k], nb[j]);
 }
 
 }  // namespace lbmh

@@ -118,13 +118,15 @@ int launch_post_processors(lbmh_lattice* L, hipStream_t stream)
   if (!L->h_edges.empty()) {
     a.cells = L->d_edges;
     a.count = uint32_t(L->h_edges.size());
-    hipLaunchKernelGGL((outer_edge_kernel<Q, S>), dim3(grid_for(a.count, 64)), dim3(64), 0, stream, a);
+    hipLaunchKernelGGL((outer_post_kernel<Q, S, false>), dim3(grid_for(a.count, 64 / kPostLanes)), dim3(64), 0,
+                       stream, a);
     LBMH_HIP(hipGetLastError());
   }
   if (!L->h_corners.empty()) {
     a.cells = L->d_corners;
     a.count = uint32_t(L->h_corners.size());
-    hipLaunchKernelGGL((outer_corner_kernel<Q, S>), dim3(grid_for(a.count, 64)), dim3(64), 0, stream, a);
+    hipLaunchKernelGGL((outer_post_kernel<Q, S, true>), dim3(grid_for(a.count, 64 / kPostLanes)), dim3(64), 0,
+                       stream, a);
     LBMH_HIP(hipGetLastError());
   }
   return LBMH_OK;

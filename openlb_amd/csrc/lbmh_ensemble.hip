@@ -247,13 +247,14 @@ int launch_post(lbmh_ensemble* E, const EnsembleEntry<Q>* table, S* arena, const
                 hipStream_t stream, int only)
 {
   if (E->n_edges) {
-    hipLaunchKernelGGL((ensemble_post_kernel<Q, S, false>), dim3(grid_for(E->n_edges, 64)), dim3(64), 0, stream, table,
+    hipLaunchKernelGGL((ensemble_post_kernel<Q, S, false>), dim3(grid_for(E->n_edges, 64 / kPostLanes)), dim3(64), 0,
+                       stream, table,
                        E->d_edges, E->n_edges, arena, E->d_ids, E->d_walls, E->d_info, limits, only);
     LBMH_HIP(hipGetLastError());
   }
   if (E->n_corners) {
-    hipLaunchKernelGGL((ensemble_post_kernel<Q, S, true>), dim3(grid_for(E->n_corners, 64)), dim3(64), 0, stream,
-                       table, E->d_corners, E->n_corners, arena, E->d_ids, E->d_walls, E->d_info, limits, only);
+    hipLaunchKernelGGL((ensemble_post_kernel<Q, S, true>), dim3(grid_for(E->n_corners, 64 / kPostLanes)), dim3(64), 0,
+                       stream, table, E->d_corners, E->n_corners, arena, E->d_ids, E->d_walls, E->d_info, limits, only);
     LBMH_HIP(hipGetLastError());
   }
   return LBMH_OK;
