@@ -309,12 +309,13 @@ __device__ __forceinline__ void collide_local(V (&f)[Q], const LocalInfo& li, co
 // Regularized flat velocity / pressure boundary with a small register
 // footprint, for the whole-lattice kernel: the same operations in the same
 // order as collide_local, with the non-equilibrium parts folded into the six
-// stress accumulators instead of kept per population.
-template <int Q, typename V>
-__device__ __forceinline__ void collide_flat(V (&f)[Q], const LocalInfo& li, const double* w, V omega)
+// stress accumulators instead of kept per population.  The face axis DIR is a
+// compile-time constant, so the on-wall set is known at compile time and only
+// the normal set depends on the run-time orientation.
+template <int Q, typename V, int DIR>
+__device__ __forceinline__ void collide_flat_axis(V (&f)[Q], int orient, bool pressure, const double* w, V omega)
 {
-  const int dir = flat_dir(li), orient = info_component(li, dir);
-  const bool pressure = li.kind == kFlatPressure;
+  using L = Lattice<Q>;
   V u[3], rho;
   auto momenta = [&]() {
     u[0] = V(w[1]);
@@ -323,25 +324,24 @@ __device__ __forceinline__ void collide_flat(V (&f)[Q], const LocalInfo& li, con
     V on = V(0), nrm = V(0);
 #pragma unroll
     for (int i = 0; i < Q; ++i)
-      if (cdir<Q>(i, dir) == 0) on += f[i];
+      if (L::c[i][DIR] == 0) on += f[i];
 #pragma unroll
     for (int i = 0; i < Q; ++i)
-      if (cdir<Q>(i, dir) == orient) nrm += f[i];
+      if (L::c[i][DIR] == orient) nrm += f[i];
     if (pressure) {
       rho = V(w[0]);
-      u[dir] = V(orient) * ((V(2) * nrm + on + V(1)) / rho - V(1));
+      u[DIR] = V(orient) * ((V(2) * nrm + on + V(1)) / rho - V(1));
     } else {
-      rho = (V(2) * nrm + on + V(1)) / (V(1) + V(orient) * u[dir]);
+      rho = (V(2) * nrm + on + V(1)) / (V(1) + V(orient) * u[DIR]);
     }
   };
   momenta();
   V usq = norm_sqr(u);
   V pi[6] = {V(0), V(0), V(0), V(0), V(0), V(0)};
-  using L = Lattice<Q>;
   // BoundaryHelpers::computeStress: on-wall populations, then the normal ones
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
-    if (cdir<Q>(i, dir) != 0) continue;
+    if (L::c[i][DIR] != 0) continue;
     const V fneq = i == 0 ? V(0) : f[i] - feq_shifted<Q, V>(i, rho, u, usq);
     int k = 0;
 #pragma unroll
@@ -351,7 +351,7 @@ __device__ __forceinline__ void collide_flat(V (&f)[Q], const LocalInfo& li, con
   }
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
-    if (cdir<Q>(i, dir) != orient) continue;
+    if (L::c[i][DIR] == 0 || L::c[i][DIR] != orient) continue;
     const V fneq = f[i] - feq_shifted<Q, V>(i, rho, u, usq);
     int k = 0;
 #pragma unroll
@@ -369,6 +369,16 @@ __device__ __forceinline__ void collide_flat(V (&f)[Q], const LocalInfo& li, con
     f[i] *= V(1) - omega;
     f[i] += omega * e;
   }
+}
+
+template <int Q, typename V>
+__device__ __forceinline__ void collide_flat(V (&f)[Q], const LocalInfo& li, const double* w, V omega)
+{
+  const int dir = flat_dir(li), orient = info_component(li, dir);
+  const bool pressure = li.kind == kFlatPressure;
+  if (dir == 0) collide_flat_axis<Q, V, 0>(f, orient, pressure, w, omega);
+  else if (dir == 1) collide_flat_axis<Q, V, 1>(f, orient, pressure, w, omega);
+  else collide_flat_axis<Q, V, 2>(f, orient, pressure, w, omega);
 }
 
 // ---- post-processors (PostStream stage) ----------------------------------
