@@ -159,6 +159,15 @@ int lbmh_reduce_moments(lbmh_lattice* lattice, double out[4]);
  * lbmh_set_dynamics_ids drops all local boundaries. */
 #define LBMH_LOCAL_VELOCITY 0
 #define LBMH_LOCAL_PRESSURE 1
+/* boundary::InterpolatedVelocity / InterpolatedPressure
+ * (src/boundary/interpolatedVelocity3D.h:34-90, interpolatedPressure3D.h:37-90):
+ * flat faces get BGK with Dirichlet momenta and the finite-difference
+ * PlaneFdBoundaryProcessor3D (boundaryPostProcessors3D.hh:38-92), which runs
+ * before the edge processors; edges, corners and inner types are those of
+ * LocalVelocity.  A velocity face cell without a normal is an error (OpenLB
+ * throws); InterpolatedPressure leaves non-flat cells without dynamics. */
+#define LBMH_INTERP_VELOCITY 2
+#define LBMH_INTERP_PRESSURE 3
 int lbmh_set_local_boundary(lbmh_lattice* lattice, int id, int kind, double rho, const double u[3]);
 
 /* Boundary type and discrete normal of every core cell of an id map that is

@@ -130,10 +130,10 @@ __global__ __launch_bounds__(64) void ensemble_local_kernel(const EnsembleEntry<
   collide_stream_cell<Q, S, KIND, true, kLocalCollide>(a, lc.y);
 }
 
-// Edge (CORNER = false) or corner post-processors of the members that stepped
+// Plane, edge or corner post-processors (KIND) of the members that stepped
 // last (or of member `only` when only >= 0: initialize()).  Eight lanes per
 // (member, cell) entry, as outer_post_kernel.
-template <int Q, typename S, bool CORNER>
+template <int Q, typename S, int KIND>
 __global__ __launch_bounds__(64) void ensemble_post_kernel(const EnsembleEntry<Q>* __restrict__ table,
                                                            const uint2* __restrict__ cells, uint32_t count,
                                                            S* __restrict__ arena, const uint8_t* __restrict__ ids,
@@ -177,9 +177,9 @@ __global__ __launch_bounds__(64) void ensemble_post_kernel(const EnsembleEntry<Q
       a.omega = e.bp.omega;
     }
   }
-  if (valid && slot < kPostSlots) post_gather<Q, S, CORNER>(a, lc.y, slot, nb[j][slot]);
+  if (valid && slot < kPostSlots) post_gather<Q, S, KIND>(a, lc.y, slot, nb[j][slot]);
   __syncthreads();
-  if (valid && slot == 0) post_finish<Q, S, CORNER>(a, lc.y, nb[j]);
+  if (valid && slot == 0) post_finish<Q, S, KIND>(a, lc.y, nb[j]);
 }
 
 // Streaming of every lattice that stepped: rotate its columns, count the step.

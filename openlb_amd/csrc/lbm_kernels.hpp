@@ -136,7 +136,8 @@ __device__ __forceinline__ void collide_stream_cell(const StepArgs<Q, typename S
       // flat faces stay in this pass; edges, corners and the rest go to the list pass
       if (id >= 3) {
         const uint32_t k = a.info[id] & 0xffu;
-        if (k != kMovingWall && k != kFlatVelocity && k != kFlatPressure) return;
+        if (k != kMovingWall && k != kFlatVelocity && k != kFlatPressure && k != kFdVelocity && k != kFdPressure)
+          return;
       }
     }
   }
@@ -168,7 +169,10 @@ __device__ __forceinline__ void collide_stream_cell(const StepArgs<Q, typename S
       }
       if constexpr (LOCAL == kLocalFlat) {
         const LocalInfo li = unpack_local_info(a.info[id]);
-        if (li.kind != kMovingWall) {
+        if (li.kind == kFdVelocity || li.kind == kFdPressure) {
+          wall = false;
+          collide_dirichlet<Q, V>(f, li, w, a.bp.omega);
+        } else if (li.kind != kMovingWall) {
           wall = false;
           collide_flat<Q, V>(f, li, w, a.bp.omega);
         }

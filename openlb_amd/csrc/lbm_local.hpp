@@ -9,6 +9,9 @@
 //   outer corner     BGK with FixedVelocityBoundaryTuple + OuterVelocityCornerProcessor3D
 //   inner edge       CombinedRLBdynamics<BGK, InnerEdgeVelocityTuple3D<plane,n1,n2>>
 //   inner corner     CombinedRLBdynamics<BGK, InnerCornerVelocityTuple3D<nx,ny,nz>>
+// and boundary::InterpolatedVelocity / InterpolatedPressure (interpolatedVelocity3D.h:
+// 34-90, interpolatedPressure3D.h:37-90) give flat faces plain BGK with Dirichlet
+// momenta plus a PlaneFdBoundaryProcessor3D (boundaryPostProcessors3D.hh:38-92).
 // Here each such cell carries a dynamics id >= 3 whose entry holds the kind,
 // the normal (or plane and normals for edges) and the defined rho / u.  The
 // collisions run inside the fused step kernel; the edge and corner
@@ -35,8 +38,36 @@ enum LocalKind : uint32_t {
   kOuterCorner = 4,
   kInnerEdge = 5,
   kInnerCorner = 6,
-  kDeadWall = 7        // wall cell without a matching boundary type: no dynamics
+  kDeadWall = 7,       // wall cell without a matching boundary type: no dynamics
+  kFdVelocity = 8,     // InterpolatedVelocity face: BGK with BasicDirichletVelocityBoundaryTuple
+  kFdPressure = 9      // InterpolatedPressure face: BGK with BasicDirichletPressureBoundaryTuple
 };
+
+constexpr uint32_t kInvalidKind = 0xffu;
+
+// Dynamics kind a wall cell gets from its boundary type (0 flat, 1 external
+// corner, 2 internal corner, 3 external edge, 4 internal edge) under boundary
+// condition `bc` (0 LocalVelocity, 1 LocalPressure, 2 InterpolatedVelocity,
+// 3 InterpolatedPressure); `flat` means a flat type with a non-zero normal.
+// kInvalidKind where OpenLB throws (localPressure3D.h:46-58 for non-flat
+// cells; a flat cell without normal under the interpolated boundaries,
+// setBoundary3D.h:199-223).
+inline uint32_t local_kind(int bc, int type, bool flat)
+{
+  static const uint32_t kOthers[5] = {0, kOuterCorner, kInnerCorner, kOuterEdge, kInnerEdge};
+  switch (bc) {
+  case 1: return flat ? kFlatPressure : kInvalidKind;
+  case 3:
+    if (type == 0) return flat ? kFdPressure : kInvalidKind;
+    return kDeadWall;  // interpolatedPressure3D.h: no dynamics off flat faces
+  case 2:
+    if (type == 0) return flat ? kFdVelocity : kInvalidKind;
+    return kOthers[type];
+  default:
+    if (type == 0) return flat ? kFlatVelocity : kDeadWall;
+    return kOthers[type];
+  }
+}
 
 // Per-id packed description: kind in bits 0-7, then three signed 4-bit
 // fields (normal components, or plane / n1 / n2 for edges) stored + 8.
@@ -198,12 +229,14 @@ __device__ __forceinline__ void local_rho_u(const V (&f)[Q], const LocalInfo& li
   u[1] = V(w[2]);
   u[2] = V(w[3]);
   switch (li.kind) {
-  case kFlatVelocity: {
+  case kFlatVelocity:
+  case kFdVelocity: {
     const int dir = flat_dir(li);
     rho = bm_rho<Q, V>(f, dir, info_component(li, dir), u);
     break;
   }
-  case kFlatPressure: {
+  case kFlatPressure:
+  case kFdPressure: {
     const int dir = flat_dir(li), orient = info_component(li, dir);
     const V r = V(w[0]);
     V on = V(0), nrm = V(0);
@@ -281,7 +314,9 @@ __device__ __forceinline__ void collide_local(V (&f)[Q], const LocalInfo& li, co
                                               V& usq_out)
 {
   V rho, u[3];
-  if (li.kind != kOuterEdge && li.kind != kOuterCorner) {
+  const bool plain = li.kind == kOuterEdge || li.kind == kOuterCorner || li.kind == kFdVelocity ||
+                     li.kind == kFdPressure;
+  if (!plain) {
     V pi[6];
     local_rho_u<Q, V>(f, li, w, rho, u);
     if (li.kind == kFlatVelocity || li.kind == kFlatPressure) {
@@ -363,6 +398,21 @@ __device__ __forceinline__ void collide_flat_axis(V (&f)[Q], int orient, bool pr
   for (int i = 0; i < Q; ++i) f[i] = feq_shifted<Q, V>(i, rho, u, usq) + pi_to_fneq<Q, V>(i, pi);
   momenta();
   usq = norm_sqr(u);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const V e = feq_shifted<Q, V>(i, rho, u, usq);
+    f[i] *= V(1) - omega;
+    f[i] += omega * e;
+  }
+}
+
+// BGK with the Dirichlet momenta of an interpolated boundary face
+template <int Q, typename V>
+__device__ __forceinline__ void collide_dirichlet(V (&f)[Q], const LocalInfo& li, const double* w, V omega)
+{
+  V rho, u[3];
+  local_rho_u<Q, V>(f, li, w, rho, u);
+  const V usq = norm_sqr(u);
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     const V e = feq_shifted<Q, V>(i, rho, u, usq);
@@ -472,9 +522,10 @@ __device__ __forceinline__ V boundary_gradient(V u0, V u1, V u2)
 
 // Regularized reconstruction from rho, the cell's fixed u and the velocity
 // gradient dA[a][b] = d_a u_b          boundaryPostProcessors3D.hh:190-211, 270-290
+// The populations go to the lattice, or to `scratch` (Q values) when given.
 template <int Q, typename S, typename V>
-__device__ void store_reconstruction(const PostArgs<Q>& a, int x, int y, int z, const double* w, V rho,
-                                     const V (&dA)[3][3])
+__device__ void store_regularized(const PostArgs<Q>& a, int x, int y, int z, const V (&u)[3], V rho,
+                                  const V (&dA)[3][3], V* scratch = nullptr)
 {
   const V s = -rho / V(3) / V(a.omega);
   V pi[6];
@@ -484,15 +535,24 @@ __device__ void store_reconstruction(const PostArgs<Q>& a, int x, int y, int z, 
   pi[1] = (dA[0][1] + dA[1][0]) * s;
   pi[2] = (dA[0][2] + dA[2][0]) * s;
   pi[4] = (dA[1][2] + dA[2][1]) * s;
-  const V u[3] = {V(w[1]), V(w[2]), V(w[3])};
   const V usq = norm_sqr(u);
   S* pop = static_cast<S*>(a.pop);
   const uint32_t n = uint32_t(x + 1) * a.sx + uint32_t(y + 1) * a.sy + uint32_t(z + 1);
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     const V fi = feq_shifted<Q, V>(i, rho, u, usq) + pi_to_fneq<Q, V>(i, pi);
-    store_pop<S, V>(column_at(pop + uint64_t(i) * a.stride, wrap_index(n, a.shift[i], a.n_padded)), fi);
+    if (scratch) scratch[i] = fi;
+    else store_pop<S, V>(column_at(pop + uint64_t(i) * a.stride, wrap_index(n, a.shift[i], a.n_padded)), fi);
   }
+}
+
+// ... at the cell's fixed velocity w[1..3]
+template <int Q, typename S, typename V>
+__device__ void store_reconstruction(const PostArgs<Q>& a, int x, int y, int z, const double* w, V rho,
+                                     const V (&dA)[3][3])
+{
+  const V u[3] = {V(w[1]), V(w[2]), V(w[3])};
+  store_regularized<Q, S, V>(a, x, y, z, u, rho, dA);
 }
 
 __device__ __forceinline__ void core_coords(const FastDiv& div_z, const FastDiv& div_y, int ny, int nz, uint32_t c,
@@ -515,11 +575,24 @@ __device__ __forceinline__ void core_coords(const FastDiv& div_z, const FastDiv&
 //     slots: -n_d e_d and -2 n_d e_d for d = x, y, z
 constexpr int kPostLanes = 8;
 constexpr int kPostSlots = 6;
+//   plane (dir, orient)     PlaneFdBoundaryProcessor3D::apply, boundaryPostProcessors3D.hh:38-92
+//     slots, axis d = 0..2: -orient e_d and -2 orient e_d along the normal, +e_d and -e_d along the face
+enum PostKind : int { kPostEdge = 0, kPostCorner = 1, kPostPlane = 2 };
 
-__device__ __forceinline__ void post_offset(const LocalInfo& li, bool corner, int slot, int (&o)[3])
+__device__ __forceinline__ void post_offset(const LocalInfo& li, int kind, int slot, int (&o)[3])
 {
   o[0] = o[1] = o[2] = 0;
-  if (corner) {
+  if (kind == kPostPlane) {
+    const int d = slot >> 1, dir = flat_dir(li);
+    if (d == dir) {
+      const int orient = info_component(li, dir);
+      o[d] = (slot & 1) ? -2 * orient : -orient;
+    } else {
+      o[d] = (slot & 1) ? -1 : 1;
+    }
+    return;
+  }
+  if (kind == kPostCorner) {
     const int d = slot >> 1, n = info_component(li, d);
     o[d] = (slot & 1) ? -2 * n : -n;
     return;
@@ -536,7 +609,7 @@ __device__ __forceinline__ void post_offset(const LocalInfo& li, bool corner, in
 }
 
 // rho and u of neighbour `slot` of post-processed cell c
-template <int Q, typename S, bool CORNER>
+template <int Q, typename S, int KIND>
 __device__ void post_gather(const PostArgs<Q>& a, uint32_t c, int slot, typename StorageTraits<S>::V (&out)[4])
 {
   using V = typename StorageTraits<S>::V;
@@ -544,7 +617,7 @@ __device__ void post_gather(const PostArgs<Q>& a, uint32_t c, int slot, typename
   core_coords(a.div_z, a.div_y, a.ny, a.nz, c, x, y, z);
   const LocalInfo li = unpack_local_info(a.info[a.ids[c]]);
   int o[3];
-  post_offset(li, CORNER, slot, o);
+  post_offset(li, KIND, slot, o);
   V rho, u[3];
   cell_rho_u<Q, S, V>(a, x + o[0], y + o[1], z + o[2], rho, u);
   out[0] = rho;
@@ -555,8 +628,9 @@ __device__ void post_gather(const PostArgs<Q>& a, uint32_t c, int slot, typename
 
 // Density and velocity gradient from the six neighbour evaluations, then the
 // regularized reconstruction of cell c
-template <int Q, typename S, bool CORNER>
-__device__ void post_finish(const PostArgs<Q>& a, uint32_t c, const typename StorageTraits<S>::V (*nb)[4])
+template <int Q, typename S, int KIND>
+__device__ void post_finish(const PostArgs<Q>& a, uint32_t c, const typename StorageTraits<S>::V (*nb)[4],
+                            typename StorageTraits<S>::V* scratch = nullptr)
 {
   using V = typename StorageTraits<S>::V;
   int x, y, z;
@@ -564,9 +638,21 @@ __device__ void post_finish(const PostArgs<Q>& a, uint32_t c, const typename Sto
   const int id = a.ids[c];
   const double* w = a.walls + 4 * id;
   const LocalInfo li = unpack_local_info(a.info[id]);
-  const V u0[3] = {V(w[1]), V(w[2]), V(w[3])};
+  V u0[3] = {V(w[1]), V(w[2]), V(w[3])};
   V rho, dA[3][3];
-  if constexpr (CORNER) {
+  if constexpr (KIND == kPostPlane) {
+    // rho and u from the cell's own Dirichlet momenta
+    V f[Q];
+    load_core_cell<Q, S, V>(a, x, y, z, f);
+    local_rho_u<Q, V>(f, li, w, rho, u0);
+    const int dir = flat_dir(li), orient = info_component(li, dir);
+    for (int d = 0; d < 3; ++d)
+      for (int b = 0; b < 3; ++b)
+        dA[d][b] = d == dir ? V(-orient) * boundary_gradient(u0[b], nb[2 * d][1 + b], nb[2 * d + 1][1 + b])
+                            : (nb[2 * d][1 + b] - nb[2 * d + 1][1 + b]) / V(2);
+    store_regularized<Q, S, V>(a, x, y, z, u0, rho, dA, scratch);
+    return;
+  } else if constexpr (KIND == kPostCorner) {
     rho = V(4) / V(9) * (nb[4][0] + nb[2][0] + nb[0][0]) - V(1) / V(9) * (nb[5][0] + nb[3][0] + nb[1][0]);
     for (int d = 0; d < 3; ++d) {
       const int n = info_component(li, d);
@@ -586,7 +672,40 @@ __device__ void post_finish(const PostArgs<Q>& a, uint32_t c, const typename Sto
 }
 
 // One launch per post-processor kind over the cell list: 8 cells per workgroup.
-template <int Q, typename S, bool CORNER>
+// Pressure faces read the normal velocity of their face neighbours from those
+// neighbours' populations, which the same pass rewrites: their plane pass
+// computes every cell into `scratch` first, and plane_store_kernel writes the
+// populations afterwards.
+template <int Q, typename S>
+__global__ __launch_bounds__(64) void plane_scratch_kernel(PostArgs<Q> a, typename StorageTraits<S>::V* scratch)
+{
+  using V = typename StorageTraits<S>::V;
+  __shared__ V nb[64 / kPostLanes][kPostSlots][4];
+  const int slot = int(threadIdx.x % kPostLanes), j = int(threadIdx.x / kPostLanes);
+  const uint32_t k = blockIdx.x * (64 / kPostLanes) + uint32_t(j);
+  const bool valid = k < a.count;
+  if (valid && slot < kPostSlots) post_gather<Q, S, kPostPlane>(a, a.cells[k], slot, nb[j][slot]);
+  __syncthreads();
+  if (valid && slot == 0) post_finish<Q, S, kPostPlane>(a, a.cells[k], nb[j], scratch + uint64_t(k) * Q);
+}
+
+template <int Q, typename S>
+__global__ __launch_bounds__(64) void plane_store_kernel(PostArgs<Q> a, const typename StorageTraits<S>::V* scratch)
+{
+  using V = typename StorageTraits<S>::V;
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.count) return;
+  int x, y, z;
+  core_coords(a.div_z, a.div_y, a.ny, a.nz, a.cells[k], x, y, z);
+  S* pop = static_cast<S*>(a.pop);
+  const uint32_t n = uint32_t(x + 1) * a.sx + uint32_t(y + 1) * a.sy + uint32_t(z + 1);
+#pragma unroll
+  for (int i = 0; i < Q; ++i)
+    store_pop<S, V>(column_at(pop + uint64_t(i) * a.stride, wrap_index(n, a.shift[i], a.n_padded)),
+                    scratch[uint64_t(k) * Q + i]);
+}
+
+template <int Q, typename S, int KIND>
 __global__ __launch_bounds__(64) void outer_post_kernel(PostArgs<Q> a)
 {
   using V = typename StorageTraits<S>::V;
@@ -594,9 +713,9 @@ __global__ __launch_bounds__(64) void outer_post_kernel(PostArgs<Q> a)
   const int slot = int(threadIdx.x % kPostLanes), j = int(threadIdx.x / kPostLanes);
   const uint32_t k = blockIdx.x * (64 / kPostLanes) + uint32_t(j);
   const bool valid = k < a.count;
-  if (valid && slot < kPostSlots) post_gather<Q, S, CORNER>(a, a.cells[k], slot, nb[j][slot]);
+  if (valid && slot < kPostSlots) post_gather<Q, S, KIND>(a, a.cells[k], slot, nb[j][slot]);
   __syncthreads();
-  if (valid && slot == 0) post_finish<Q, S, CORNER>(a, a.cells[k], nb[j]);
+  if (valid && slot == 0) post_finish<Q, S, KIND>(a, a.cells[k], nb[j]);
 }
 
 }  // namespace lbmh

@@ -54,10 +54,13 @@ struct lbmh_lattice {
   uint32_t info[kMaxWalls] = {};     // packed LocalInfo per id
   uint32_t* d_info = nullptr;
   bool has_local = false;
-  std::vector<uint32_t> h_edges, h_corners, h_local;
+  std::vector<uint32_t> h_edges, h_corners, h_local, h_planes;
   uint32_t* d_edges = nullptr;
   uint32_t* d_corners = nullptr;
-  uint32_t* d_local = nullptr;     // every local-boundary cell (second pass of the step)
+  uint32_t* d_local = nullptr;     // local-boundary cells of the list pass (edges, corners, inner types)
+  uint32_t* d_planes = nullptr;    // interpolated-boundary faces (plane finite-difference processor)
+  bool pressure_planes = false;    // some plane is a pressure face: compute into d_scratch first
+  void* d_scratch = nullptr;
 };
 
 namespace {
@@ -97,7 +100,7 @@ BulkParams<V> bulk_params(const lbmh_lattice* L)
 template <int Q, typename S>
 int launch_post_processors(lbmh_lattice* L, hipStream_t stream)
 {
-  if (L->h_edges.empty() && L->h_corners.empty()) return LBMH_OK;
+  if (L->h_edges.empty() && L->h_corners.empty() && L->h_planes.empty()) return LBMH_OK;
   PostArgs<Q> a;
   a.pop = L->d_pop;
   a.stride = L->stride;
@@ -115,17 +118,33 @@ int launch_post_processors(lbmh_lattice* L, hipStream_t stream)
   a.div_z = FastDiv(uint32_t(L->n[2]));
   a.div_y = FastDiv(uint32_t(L->n[1]));
   a.omega = L->params.omega;
+  if (!L->h_planes.empty()) {
+    using V = typename StorageTraits<S>::V;
+    a.cells = L->d_planes;
+    a.count = uint32_t(L->h_planes.size());
+    if (L->pressure_planes) {
+      hipLaunchKernelGGL((plane_scratch_kernel<Q, S>), dim3(grid_for(a.count, 64 / kPostLanes)), dim3(64), 0, stream,
+                         a, static_cast<V*>(L->d_scratch));
+      LBMH_HIP(hipGetLastError());
+      hipLaunchKernelGGL((plane_store_kernel<Q, S>), dim3(grid_for(a.count, 64)), dim3(64), 0, stream, a,
+                         static_cast<const V*>(L->d_scratch));
+    } else {
+      hipLaunchKernelGGL((outer_post_kernel<Q, S, kPostPlane>), dim3(grid_for(a.count, 64 / kPostLanes)), dim3(64), 0,
+                         stream, a);
+    }
+    LBMH_HIP(hipGetLastError());
+  }
   if (!L->h_edges.empty()) {
     a.cells = L->d_edges;
     a.count = uint32_t(L->h_edges.size());
-    hipLaunchKernelGGL((outer_post_kernel<Q, S, false>), dim3(grid_for(a.count, 64 / kPostLanes)), dim3(64), 0,
+    hipLaunchKernelGGL((outer_post_kernel<Q, S, kPostEdge>), dim3(grid_for(a.count, 64 / kPostLanes)), dim3(64), 0,
                        stream, a);
     LBMH_HIP(hipGetLastError());
   }
   if (!L->h_corners.empty()) {
     a.cells = L->d_corners;
     a.count = uint32_t(L->h_corners.size());
-    hipLaunchKernelGGL((outer_post_kernel<Q, S, true>), dim3(grid_for(a.count, 64 / kPostLanes)), dim3(64), 0,
+    hipLaunchKernelGGL((outer_post_kernel<Q, S, kPostCorner>), dim3(grid_for(a.count, 64 / kPostLanes)), dim3(64), 0,
                        stream, a);
     LBMH_HIP(hipGetLastError());
   }
@@ -333,6 +352,8 @@ int lbmh_lattice_destroy(lbmh_lattice* L)
   if (L->d_edges) (void)hipFree(L->d_edges);
   if (L->d_corners) (void)hipFree(L->d_corners);
   if (L->d_local) (void)hipFree(L->d_local);
+  if (L->d_planes) (void)hipFree(L->d_planes);
+  if (L->d_scratch) (void)hipFree(L->d_scratch);
   delete L;
   return LBMH_OK;
 }
@@ -356,6 +377,8 @@ int lbmh_set_dynamics_ids(lbmh_lattice* L, const uint8_t* ids)
   L->h_edges.clear();
   L->h_corners.clear();
   L->h_local.clear();
+  L->h_planes.clear();
+  L->pressure_planes = false;
   for (int w = 0; w < kMaxWalls; ++w) {
     L->info[w] = pack_local_info(kMovingWall, 0, 0, 0);
     L->id_defined[w] = false;
@@ -411,11 +434,12 @@ int lbmh_classify_boundary(const int extent[3], const int periodic[3], const uin
 int lbmh_set_local_boundary(lbmh_lattice* L, int id, int kind, double rho, const double u[3])
 {
   if (!L || !u) return fail(LBMH_E_INVALID, "null argument");
-  if (kind != LBMH_LOCAL_VELOCITY && kind != LBMH_LOCAL_PRESSURE) return fail(LBMH_E_INVALID, "unknown local boundary kind");
+  if (kind < LBMH_LOCAL_VELOCITY || kind > LBMH_INTERP_PRESSURE) return fail(LBMH_E_INVALID, "unknown local boundary kind");
   if (id < 2 || id >= kMaxWalls) return fail(LBMH_E_INVALID, "boundary id must be in [2, 255]");
   if (L->kind != LBMH_BGK) return fail(LBMH_E_INVALID, "local boundaries need BGK bulk dynamics");
   if (L->h_ids.size() != L->ncore) return fail(LBMH_E_INVALID, "set the dynamics ids first");
-  if (kind == LBMH_LOCAL_PRESSURE && !(rho > 0)) return fail(LBMH_E_INVALID, "pressure boundary density must be positive");
+  if ((kind == LBMH_LOCAL_PRESSURE || kind == LBMH_INTERP_PRESSURE) && !(rho > 0))
+    return fail(LBMH_E_INVALID, "pressure boundary density must be positive");
   DeviceGuard guard(L->device);
   const int n[3] = {L->n[0], L->n[1], L->n[2]};
   const int per[3] = {L->periodic[0], L->periodic[1], L->periodic[2]};
@@ -426,7 +450,7 @@ int lbmh_set_local_boundary(lbmh_lattice* L, int id, int kind, double rho, const
   used[0] = used[1] = used[2] = true;
   std::map<std::tuple<int, int, int, int>, int> entry;  // (kind, a, b, c) -> id
   std::vector<uint8_t> next = L->h_ids;
-  std::vector<uint32_t> edges, corners, cells;
+  std::vector<uint32_t> edges, corners, cells, planes;
   int free_id = kMaxWalls - 1;
   for (int x = 0; x < n[0]; ++x)
     for (int y = 0; y < n[1]; ++y)
@@ -436,16 +460,10 @@ int lbmh_set_local_boundary(lbmh_lattice* L, int id, int kind, double rho, const
         int type, nrm[3];
         classify_boundary_cell(n, per, L->h_ids.data(), x, y, z, type, nrm);
         const bool flat = type == kTypeFlat && (nrm[0] || nrm[1] || nrm[2]);
-        uint32_t k;
-        if (kind == LBMH_LOCAL_PRESSURE) {
-          // LocalPressure handles flat walls only (localPressure3D.h:46-58)
-          if (!flat) return fail(LBMH_E_INVALID, "local pressure boundary on a non-flat wall cell");
-          k = kFlatPressure;
-        } else if (type == kTypeFlat) {
-          k = flat ? kFlatVelocity : kDeadWall;
-        } else {
-          static const uint32_t map[5] = {0, kOuterCorner, kInnerCorner, kOuterEdge, kInnerEdge};
-          k = map[type];
+        const uint32_t k = local_kind(kind, type, flat);
+        if (k == kInvalidKind) {
+          return fail(LBMH_E_INVALID, kind == LBMH_LOCAL_PRESSURE ? "local pressure boundary on a non-flat wall cell"
+                                                                 : "interpolated boundary on a wall cell without normal");
         }
         int a = nrm[0], b = nrm[1], cc = nrm[2];
         if (k == kOuterEdge || k == kInnerEdge) {
@@ -470,7 +488,9 @@ int lbmh_set_local_boundary(lbmh_lattice* L, int id, int kind, double rho, const
         }
         next[c] = uint8_t(it->second);
         // flat faces are collided by the whole-lattice pass, the rest by the list pass
-        if (k != kFlatVelocity && k != kFlatPressure) cells.push_back(c);
+        if (k != kFlatVelocity && k != kFlatPressure && k != kFdVelocity && k != kFdPressure) cells.push_back(c);
+        if (k == kFdVelocity || k == kFdPressure) planes.push_back(c);
+        if (k == kFdPressure) L->pressure_planes = true;
         if (k == kOuterEdge) edges.push_back(c);
         if (k == kOuterCorner) corners.push_back(c);
       }
@@ -479,6 +499,7 @@ int lbmh_set_local_boundary(lbmh_lattice* L, int id, int kind, double rho, const
   L->h_edges.insert(L->h_edges.end(), edges.begin(), edges.end());
   L->h_corners.insert(L->h_corners.end(), corners.begin(), corners.end());
   L->h_local.insert(L->h_local.end(), cells.begin(), cells.end());
+  L->h_planes.insert(L->h_planes.end(), planes.begin(), planes.end());
   LBMH_HIP(hipMemcpy(L->d_ids, L->h_ids.data(), L->ncore, hipMemcpyHostToDevice));
   LBMH_HIP(hipMemcpy(L->d_walls, L->walls, sizeof(L->walls), hipMemcpyHostToDevice));
   LBMH_HIP(hipMemcpy(L->d_info, L->info, sizeof(L->info), hipMemcpyHostToDevice));
@@ -495,6 +516,15 @@ int lbmh_set_local_boundary(lbmh_lattice* L, int id, int kind, double rho, const
   int rc = upload_list(L->h_edges, L->d_edges);
   if (rc == LBMH_OK) rc = upload_list(L->h_corners, L->d_corners);
   if (rc == LBMH_OK) rc = upload_list(L->h_local, L->d_local);
+  if (rc == LBMH_OK) rc = upload_list(L->h_planes, L->d_planes);
+  if (rc == LBMH_OK && L->pressure_planes) {
+    if (L->d_scratch) {
+      LBMH_HIP(hipFree(L->d_scratch));
+      L->d_scratch = nullptr;
+    }
+    const size_t elem = L->storage == LBMH_F64 ? sizeof(double) : sizeof(float);
+    LBMH_HIP(hipMalloc(&L->d_scratch, elem * L->q * L->h_planes.size()));
+  }
   if (rc != LBMH_OK) return rc;
   L->has_local = true;
   L->uniform_bulk = false;

@@ -88,7 +88,7 @@ struct Member {
   std::vector<uint8_t> h_ids;
   uint32_t info[kEnsembleWalls] = {};
   bool id_defined[kEnsembleWalls] = {};
-  std::vector<uint32_t> local, edges, corners;
+  std::vector<uint32_t> local, edges, corners, planes;
 };
 
 }  // namespace
@@ -117,8 +117,9 @@ struct lbmh_ensemble {
   uint2* d_local = nullptr;
   uint2* d_edges = nullptr;
   uint2* d_corners = nullptr;
+  uint2* d_planes = nullptr;
   size_t boundary_capacity = 0;
-  uint32_t n_local = 0, n_edges = 0, n_corners = 0;
+  uint32_t n_local = 0, n_edges = 0, n_corners = 0, n_planes = 0;
   bool has_local = false;            // some member ever had local boundaries
 };
 
@@ -195,22 +196,23 @@ int check_member(const lbmh_ensemble* E, int m)
 // Concatenate the local-boundary lists of the active members into the device lists.
 int rebuild_lists(lbmh_ensemble* E)
 {
-  std::vector<uint2> all[3];
+  std::vector<uint2> all[4];
   for (int m = 0; m < E->max_members; ++m) {
     const Member& mem = E->members[m];
     if (!mem.active) continue;
-    const std::vector<uint32_t>* src[3] = {&mem.local, &mem.edges, &mem.corners};
-    for (int k = 0; k < 3; ++k)
+    const std::vector<uint32_t>* src[4] = {&mem.local, &mem.edges, &mem.corners, &mem.planes};
+    for (int k = 0; k < 4; ++k)
       for (uint32_t c : *src[k]) all[k].push_back(make_uint2(uint32_t(m), c));
   }
-  for (int k = 0; k < 3; ++k)
+  for (int k = 0; k < 4; ++k)
     if (all[k].size() > E->boundary_capacity) return fail(LBMH_E_MEMORY, "boundary list capacity exceeded");
-  uint2* dst[3] = {E->d_local, E->d_edges, E->d_corners};
-  for (int k = 0; k < 3; ++k)
+  uint2* dst[4] = {E->d_local, E->d_edges, E->d_corners, E->d_planes};
+  for (int k = 0; k < 4; ++k)
     if (!all[k].empty()) LBMH_HIP(hipMemcpy(dst[k], all[k].data(), all[k].size() * sizeof(uint2), hipMemcpyHostToDevice));
   E->n_local = uint32_t(all[0].size());
   E->n_edges = uint32_t(all[1].size());
   E->n_corners = uint32_t(all[2].size());
+  E->n_planes = uint32_t(all[3].size());
   E->version += 1;
   return LBMH_OK;
 }
@@ -246,14 +248,19 @@ template <int Q, typename S>
 int launch_post(lbmh_ensemble* E, const EnsembleEntry<Q>* table, S* arena, const EnsembleLimits& limits,
                 hipStream_t stream, int only)
 {
+  if (E->n_planes) {
+    hipLaunchKernelGGL((ensemble_post_kernel<Q, S, kPostPlane>), dim3(grid_for(E->n_planes, 64 / kPostLanes)), dim3(64),
+                       0, stream, table, E->d_planes, E->n_planes, arena, E->d_ids, E->d_walls, E->d_info, limits, only);
+    LBMH_HIP(hipGetLastError());
+  }
   if (E->n_edges) {
-    hipLaunchKernelGGL((ensemble_post_kernel<Q, S, false>), dim3(grid_for(E->n_edges, 64 / kPostLanes)), dim3(64), 0,
+    hipLaunchKernelGGL((ensemble_post_kernel<Q, S, kPostEdge>), dim3(grid_for(E->n_edges, 64 / kPostLanes)), dim3(64), 0,
                        stream, table,
                        E->d_edges, E->n_edges, arena, E->d_ids, E->d_walls, E->d_info, limits, only);
     LBMH_HIP(hipGetLastError());
   }
   if (E->n_corners) {
-    hipLaunchKernelGGL((ensemble_post_kernel<Q, S, true>), dim3(grid_for(E->n_corners, 64 / kPostLanes)), dim3(64), 0,
+    hipLaunchKernelGGL((ensemble_post_kernel<Q, S, kPostCorner>), dim3(grid_for(E->n_corners, 64 / kPostLanes)), dim3(64), 0,
                        stream, table, E->d_corners, E->n_corners, arena, E->d_ids, E->d_walls, E->d_info, limits, only);
     LBMH_HIP(hipGetLastError());
   }
@@ -346,7 +353,7 @@ int lbmh_ensemble_create(const lbmh_ensemble_desc* d, lbmh_ensemble** out)
   E->boundary_capacity = d->boundary_capacity ? d->boundary_capacity : d->cell_capacity / 4 + 64;
   if (e == hipSuccess) e = hipMalloc((void**)&E->d_info, sizeof(uint32_t) * kEnsembleWalls * E->max_members);
   if (e == hipSuccess) e = hipMemset(E->d_info, 0, sizeof(uint32_t) * kEnsembleWalls * E->max_members);
-  for (uint2** list : {&E->d_local, &E->d_edges, &E->d_corners})
+  for (uint2** list : {&E->d_local, &E->d_edges, &E->d_corners, &E->d_planes})
     if (e == hipSuccess) e = hipMalloc((void**)list, sizeof(uint2) * E->boundary_capacity);
   if (e == hipSuccess) e = hipMemcpy(E->d_table, E->h_table.data(), E->h_table.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&E->capture_stream, hipStreamNonBlocking);
@@ -375,6 +382,7 @@ int lbmh_ensemble_destroy(lbmh_ensemble* E)
   if (E->d_local) (void)hipFree(E->d_local);
   if (E->d_edges) (void)hipFree(E->d_edges);
   if (E->d_corners) (void)hipFree(E->d_corners);
+  if (E->d_planes) (void)hipFree(E->d_planes);
   delete E;
   return LBMH_OK;
 }
@@ -533,7 +541,8 @@ int lbmh_ensemble_set_local_boundary(lbmh_ensemble* E, int m, int id, int kind, 
   int rc = check_member(E, m);
   if (rc != LBMH_OK) return rc;
   if (!u) return fail(LBMH_E_INVALID, "null velocity");
-  if (kind != LBMH_LOCAL_VELOCITY && kind != LBMH_LOCAL_PRESSURE) return fail(LBMH_E_INVALID, "unknown local boundary kind");
+  if (kind < LBMH_LOCAL_VELOCITY || kind > LBMH_INTERP_VELOCITY)
+    return fail(LBMH_E_INVALID, "ensembles support LocalVelocity, LocalPressure and InterpolatedVelocity boundaries");
   if (id < 2 || id >= kEnsembleWalls) return fail(LBMH_E_INVALID, "ensemble boundary id must be in [2, 63]");
   if (E->kind != LBMH_BGK) return fail(LBMH_E_INVALID, "local boundaries need BGK bulk dynamics");
   if (kind == LBMH_LOCAL_PRESSURE && !(rho > 0)) return fail(LBMH_E_INVALID, "pressure boundary density must be positive");
@@ -545,7 +554,7 @@ int lbmh_ensemble_set_local_boundary(lbmh_ensemble* E, int m, int id, int kind, 
   used[0] = used[1] = used[2] = true;
   std::map<std::tuple<int, int, int, int>, int> entry_ids;
   std::vector<uint8_t> next = mem.h_ids;
-  std::vector<uint32_t> cells, edges, corners;
+  std::vector<uint32_t> cells, edges, corners, planes;
   uint32_t info[kEnsembleWalls];
   double walls[kEnsembleWalls][4];
   std::memcpy(info, mem.info, sizeof(info));
@@ -558,15 +567,10 @@ int lbmh_ensemble_set_local_boundary(lbmh_ensemble* E, int m, int id, int kind, 
         int type, nrm[3];
         classify_boundary_cell(n, mem.periodic, mem.h_ids.data(), x, y, z, type, nrm);
         const bool flat = type == kTypeFlat && (nrm[0] || nrm[1] || nrm[2]);
-        uint32_t k;
-        if (kind == LBMH_LOCAL_PRESSURE) {
-          if (!flat) return fail(LBMH_E_INVALID, "local pressure boundary on a non-flat wall cell");
-          k = kFlatPressure;
-        } else if (type == kTypeFlat) {
-          k = flat ? kFlatVelocity : kDeadWall;
-        } else {
-          static const uint32_t map[5] = {0, kOuterCorner, kInnerCorner, kOuterEdge, kInnerEdge};
-          k = map[type];
+        const uint32_t k = local_kind(kind, type, flat);
+        if (k == kInvalidKind) {
+          return fail(LBMH_E_INVALID, kind == LBMH_LOCAL_PRESSURE ? "local pressure boundary on a non-flat wall cell"
+                                                                 : "interpolated boundary on a wall cell without normal");
         }
         int a = nrm[0], b = nrm[1], cc = nrm[2];
         if (k == kOuterEdge || k == kInnerEdge) {
@@ -587,7 +591,8 @@ int lbmh_ensemble_set_local_boundary(lbmh_ensemble* E, int m, int id, int kind, 
           it = entry_ids.emplace(key, free_id).first;
         }
         next[c] = uint8_t(it->second);
-        if (k != kFlatVelocity && k != kFlatPressure) cells.push_back(c);
+        if (k != kFlatVelocity && k != kFlatPressure && k != kFdVelocity) cells.push_back(c);
+        if (k == kFdVelocity) planes.push_back(c);
         if (k == kOuterEdge) edges.push_back(c);
         if (k == kOuterCorner) corners.push_back(c);
       }
@@ -605,6 +610,7 @@ int lbmh_ensemble_set_local_boundary(lbmh_ensemble* E, int m, int id, int kind, 
   mem.local.insert(mem.local.end(), cells.begin(), cells.end());
   mem.edges.insert(mem.edges.end(), edges.begin(), edges.end());
   mem.corners.insert(mem.corners.end(), corners.begin(), corners.end());
+  mem.planes.insert(mem.planes.end(), planes.begin(), planes.end());
   LBMH_HIP(hipMemcpy(E->d_info + size_t(m) * kEnsembleWalls, mem.info, sizeof(mem.info), hipMemcpyHostToDevice));
   LBMH_HIP(hipMemcpy(E->d_ids + mem.ids_off, mem.h_ids.data(), mem.ncore, hipMemcpyHostToDevice));
   E->has_local = true;
@@ -615,7 +621,7 @@ int lbmh_ensemble_initialize(lbmh_ensemble* E, int m)
 {
   int rc = check_member(E, m);
   if (rc != LBMH_OK) return rc;
-  if (!E->n_edges && !E->n_corners) return LBMH_OK;
+  if (!E->n_edges && !E->n_corners && !E->n_planes) return LBMH_OK;
   DeviceGuard guard(E->device);
   rc = with_types(E, [&](auto qc, auto s) -> int {
     constexpr int Q = decltype(qc)::value;

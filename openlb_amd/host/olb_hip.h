@@ -11,7 +11,7 @@
 //   setProcessingContext(Evaluation | Simulation)        superLattice.h:154-164
 //   get(x, y, z)[iPop] on the host mirror                superLattice.h:169-176
 //   statisticsOff()                                      superLattice.h:397-403
-//   boundary::set<LocalVelocity|LocalPressure>(...)      boundary/setBoundary.h:83-101
+//   boundary::set<Local|InterpolatedVelocity/Pressure>   boundary/setBoundary.h:83-101
 //   getStatistics()                                      superLattice.h:378
 //
 // Dynamics are tags carrying the C ABI's bulk kind; parameters are tags of
@@ -64,6 +64,8 @@ template <typename T, typename D> struct NoDynamics { static constexpr DynamicsR
 namespace boundary {
 template <typename T, typename D, typename MIXIN> struct LocalVelocity { static constexpr int kind = LBMH_LOCAL_VELOCITY; };
 template <typename T, typename D, typename MIXIN> struct LocalPressure { static constexpr int kind = LBMH_LOCAL_PRESSURE; };
+template <typename T, typename D, typename MIXIN> struct InterpolatedVelocity { static constexpr int kind = LBMH_INTERP_VELOCITY; };
+template <typename T, typename D, typename MIXIN> struct InterpolatedPressure { static constexpr int kind = LBMH_INTERP_PRESSURE; };
 }  // namespace boundary
 
 enum class ProcessingContext { Evaluation, Simulation };

@@ -92,11 +92,12 @@ class BlockLattice:
 
     def set_local_boundary(self, material: int, kind: str = "velocity", rho: float = 1.0,
                            u=(0.0, 0.0, 0.0)):
-        """boundary::set<LocalVelocity | LocalPressure>(lattice, geometry, material)
+        """boundary::set<LocalVelocity | LocalPressure | InterpolatedVelocity |
+        InterpolatedPressure>(lattice, geometry, material)
         followed by defineRhoU on the material (src/boundary/setBoundary.h:83-101):
         classify the cells carrying id ``material`` and give each the regularized
         boundary dynamics of its type."""
-        kinds = {"velocity": 0, "pressure": 1}
+        kinds = {"velocity": 0, "pressure": 1, "interp_velocity": 2, "interp_pressure": 3}
         uu = np.ascontiguousarray(u, dtype=np.float64)
         abi.check(abi.load().lbmh_set_local_boundary(self._h, int(material), kinds[kind], float(rho),
                                                      _ptr(uu)))

@@ -152,21 +152,23 @@ def cavity_benchmark(size=100, q=19):
                 walls={3: (1.0, (0.01, 0.0, 0.0))})
 
 
-def local_velocity_cavity(shape=(32, 32, 32), tau=0.6, lid_u=0.05, q=19):
+def local_velocity_cavity(shape=(32, 32, 32), tau=0.6, lid_u=0.05, q=19, boundary="velocity"):
     """Lid-driven cavity with OpenLB's default boundary of the benchmark harness:
     LocalVelocity (regularized) boundaries on the whole box, at rest on the walls
     (material 2) and moving along x on the lid (material 3).  The lid is the top
     face (y = ny-1) without its edges and corners, the cells OpenLB's
     ``rename(2, 3, 1, lid)`` selects (cavity3d.cpp:39-57, 68-76, 113-131;
-    blockGeometry.hh:537-565)."""
+    blockGeometry.hh:537-565).  boundary="interp_velocity" uses OpenLB's
+    InterpolatedVelocity instead (src/boundary/interpolatedVelocity3D.h)."""
     ids = np.full(shape, 2, dtype=np.uint8)
     ids[1:-1, 1:-1, 1:-1] = 1
     ids[1:-1, -1, 1:-1] = 3
     rho_u = np.zeros((4,) + tuple(shape))
     rho_u[0] = 1.0
-    return Case("local_velocity_cavity", q, "bgk", tuple(shape), (0, 0, 0), ids, rho_u, 1.0 / tau,
+    name = "local_velocity_cavity" if boundary == "velocity" else "interpolated_velocity_cavity"
+    return Case(name, q, "bgk", tuple(shape), (0, 0, 0), ids, rho_u, 1.0 / tau,
                 walls={2: (1.0, (0.0, 0.0, 0.0)), 3: (1.0, (lid_u, 0.0, 0.0))},
-                local={2: "velocity", 3: "velocity"})
+                local={2: boundary, 3: boundary})
 
 
 def cavity_benchmark_local(size=100, q=19):
@@ -178,11 +180,13 @@ def cavity_benchmark_local(size=100, q=19):
     return c
 
 
-def velocity_channel(shape=(24, 12, 10), u_in=0.03, rho_out=1.0, tau=0.6, q=19):
+def velocity_channel(shape=(24, 12, 10), u_in=0.03, rho_out=1.0, tau=0.6, q=19, inlet="velocity",
+                     outlet="pressure"):
     """Duct with a LocalVelocity inlet (x = 0, material 3, uniform u_in along x),
     a LocalPressure outlet (x = nx-1, material 4, density rho_out) and BounceBack
     walls (material 2) on the y and z faces, including the inlet's and outlet's
-    rims.  Starts at rest."""
+    rims.  Starts at rest.  inlet="interp_velocity" / outlet="interp_pressure"
+    select OpenLB's InterpolatedVelocity / InterpolatedPressure instead."""
     ids = np.full(shape, 2, dtype=np.uint8)
     ids[:, 1:-1, 1:-1] = 1
     ids[0, 1:-1, 1:-1] = 3
@@ -191,7 +195,7 @@ def velocity_channel(shape=(24, 12, 10), u_in=0.03, rho_out=1.0, tau=0.6, q=19):
     rho_u[0] = 1.0
     return Case("velocity_channel", q, "bgk", tuple(shape), (0, 0, 0), ids, rho_u, 1.0 / tau,
                 walls={3: (1.0, (u_in, 0.0, 0.0)), 4: (rho_out, (0.0, 0.0, 0.0))},
-                local={3: "velocity", 4: "pressure"})
+                local={3: inlet, 4: outlet})
 
 
 def step_cavity(shape=(14, 12, 16), block=(6, 5, 7), tau=0.6, lid_u=0.05, q=19):

@@ -24,8 +24,8 @@ struct olo_lattice {
   uint8_t* lkind;
   int8_t* lnorm;
   double* lval;
-  int64_t n_edge, n_corner;
-  int64_t *edge_cells, *corner_cells;
+  int64_t n_edge, n_corner, n_plane;
+  int64_t *edge_cells, *corner_cells, *plane_cells;
 };
 
 /* per-cell local boundary kinds */
@@ -37,7 +37,9 @@ enum {
   OLK_OUTER_CORNER = 4,   /* BGK + FixedVelocityBoundaryTuple, OuterVelocityCornerProcessor3D */
   OLK_INNER_EDGE = 5,     /* CombinedRLB + InnerEdgeVelocityTuple3D          momenta/aliases.h:204-209 */
   OLK_INNER_CORNER = 6,   /* CombinedRLB + InnerCornerVelocityTuple3D        momenta/aliases.h:212-217 */
-  OLK_DEAD = 7            /* boundary-material cell without a matching type: no dynamics */
+  OLK_DEAD = 7,           /* boundary-material cell without a matching type: no dynamics */
+  OLK_FD_VELOCITY = 8,    /* BGK + BasicDirichletVelocityBoundaryTuple, PlaneFdBoundaryProcessor3D */
+  OLK_FD_PRESSURE = 9     /* BGK + BasicDirichletPressureBoundaryTuple, PlaneFdBoundaryProcessor3D */
 };
 
 static char g_err[256];
@@ -299,7 +301,7 @@ static int append_cell(int64_t** list, int64_t* count, int64_t cell)
 int olo_define_local(olo_lattice* L, const uint8_t* materials, int material, int kind,
                      double rho, double ux, double uy, double uz)
 {
-  if (kind != OLO_LOCAL_VELOCITY && kind != OLO_LOCAL_PRESSURE) { set_err("olo_define_local: kind"); return -1; }
+  if (kind < OLO_LOCAL_VELOCITY || kind > OLO_INTERP_PRESSURE) { set_err("olo_define_local: kind"); return -1; }
   if (L->kind != OLO_BGK) { set_err("olo_define_local: local boundaries need BGK bulk dynamics"); return -1; }
   const int p = L->pad;
   if (!L->lkind) {
@@ -320,8 +322,16 @@ int olo_define_local(olo_lattice* L, const uint8_t* materials, int material, int
         if (kind == OLO_LOCAL_PRESSURE) {
           if (!flat) { set_err("olo_define_local: LocalPressure needs flat boundaries"); return -1; }
           k = OLK_FLAT_PRESSURE;
+        } else if (kind == OLO_INTERP_PRESSURE) {
+          /* interpolatedPressure3D.h:46-90: flat faces only, other types keep no dynamics */
+          if (type == 0 && !flat) { set_err("olo_define_local: InterpolatedPressure on a cell without normal"); return -1; }
+          k = flat ? OLK_FD_PRESSURE : OLK_DEAD;
         } else if (type == 0) {
-          k = flat ? OLK_FLAT_VELOCITY : OLK_DEAD;
+          if (kind == OLO_INTERP_VELOCITY && !flat) {
+            set_err("olo_define_local: InterpolatedVelocity on a cell without normal");
+            return -1;
+          }
+          k = flat ? (kind == OLO_INTERP_VELOCITY ? OLK_FD_VELOCITY : OLK_FLAT_VELOCITY) : OLK_DEAD;
         } else {
           static const int map[5] = {0, OLK_OUTER_CORNER, OLK_INNER_CORNER, OLK_OUTER_EDGE, OLK_INNER_EDGE};
           k = map[type];
@@ -342,6 +352,8 @@ int olo_define_local(olo_lattice* L, const uint8_t* materials, int material, int
         L->lval[4 * cell + 3] = uz;
         if (k == OLK_OUTER_EDGE && append_cell(&L->edge_cells, &L->n_edge, cell) != 0) return -1;
         if (k == OLK_OUTER_CORNER && append_cell(&L->corner_cells, &L->n_corner, cell) != 0) return -1;
+        if ((k == OLK_FD_VELOCITY || k == OLK_FD_PRESSURE) && append_cell(&L->plane_cells, &L->n_plane, cell) != 0)
+          return -1;
       }
   return 0;
 }
@@ -437,6 +449,7 @@ void olo_destroy(olo_lattice* L)
   free(L->lval);
   free(L->edge_cells);
   free(L->corner_cells);
+  free(L->plane_cells);
   free(L);
 }
 

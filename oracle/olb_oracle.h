@@ -26,6 +26,7 @@
  *                                       src/dynamics/momenta/elements.h:54-70,325-470,528-640,1622-1765,
  *                                       src/boundary/helper.h:31-80, src/dynamics/lbm.h:225-247,311-329
  *   outer edge / corner post-processors src/boundary/boundaryPostProcessors3D.hh:164-292,
+ *   plane finite-difference processor   src/boundary/boundaryPostProcessors3D.hh:38-92,
  *                                       src/utilities/finiteDifference3D.h:73-215
  *   lattice statistics                  src/core/latticeStatistics.h:49-71, .hh:75-90
  *
@@ -83,7 +84,11 @@ int olo_equilibrium(int q, int kind, int storage, int64_t n, const double* rho_u
  * assigns to that type.  rho/u are the values defineRhoU gives the material:
  * the wall velocity (LocalVelocity) or density and tangential velocity
  * (LocalPressure). */
-enum { OLO_LOCAL_VELOCITY = 0, OLO_LOCAL_PRESSURE = 1 };
+enum { OLO_LOCAL_VELOCITY = 0, OLO_LOCAL_PRESSURE = 1,
+       /* boundary::InterpolatedVelocity / InterpolatedPressure: BGK with
+        * Dirichlet momenta on flat faces plus PlaneFdBoundaryProcessor3D
+        * (interpolatedVelocity3D.h:34-90, interpolatedPressure3D.h:37-90) */
+       OLO_INTERP_VELOCITY = 2, OLO_INTERP_PRESSURE = 3 };
 int olo_define_local(olo_lattice* L, const uint8_t* materials, int material, int kind,
                      double rho, double ux, double uy, double uz);
 /* OpenLB's computeBoundaryTypeAndNormal for every core cell that is neither

@@ -272,12 +272,14 @@ static void FN(local_rho_u)(const olo_lattice* L, const V* f, int64_t n, V* rho,
   const double* lv = &L->lval[4 * n];
   for (int d = 0; d < 3; ++d) u[d] = (V)lv[1 + d];
   switch (k) {
-  case OLK_FLAT_VELOCITY: {
+  case OLK_FLAT_VELOCITY:
+  case OLK_FD_VELOCITY: {
     const int dir = ln[0] ? 0 : (ln[1] ? 1 : 2);
     *rho = FN(bm_rho)(L, f, dir, ln[dir], u);
     break;
   }
-  case OLK_FLAT_PRESSURE: {
+  case OLK_FLAT_PRESSURE:
+  case OLK_FD_PRESSURE: {
     /* FixedDensity + FixedPressureMomentum::computeU      elements.h:589-613 */
     const int dir = ln[0] ? 0 : (ln[1] ? 1 : 2), orient = ln[dir];
     const V r = (V)lv[0];
@@ -318,7 +320,7 @@ static V FN(collide_local)(const olo_lattice* L, V* f, int64_t n, V* usq_out)
   const V omega = (V)L->omega;
   V rho, u[3];
   if (k == OLK_DEAD || k == OLK_NONE) return (V)-1;
-  if (k != OLK_OUTER_EDGE && k != OLK_OUTER_CORNER) {
+  if (k != OLK_OUTER_EDGE && k != OLK_OUTER_CORNER && k != OLK_FD_VELOCITY && k != OLK_FD_PRESSURE) {
     V pi[6];
     FN(local_rho_u)(L, f, n, &rho, u);
     if (k == OLK_FLAT_VELOCITY || k == OLK_FLAT_PRESSURE) {
@@ -520,10 +522,57 @@ static void FN(outer_corner)(olo_lattice* L, int64_t n)
   FN(fd_reconstruct)(L, n, rho, dA);
 }
 
-/* PostStream post-processors by priority: edges (0), then corners (1)
- *                                                   blockLattice.hh:485-490 */
+/* PlaneFdBoundaryProcessor3D<dir,orient>::apply            boundaryPostProcessors3D.hh:38-92
+ * rho and u from the cell's Dirichlet momenta, the velocity gradient by a
+ * one-sided second-order difference along the normal and central differences
+ * along the face (finiteDifference3D.h:73-215), then the regularized
+ * reconstruction.  The result goes to `out` (q values) instead of the lattice
+ * so that the caller decides when neighbours see it. */
+static void FN(plane_fd)(const olo_lattice* L, int64_t n, V* out)
+{
+  const int8_t* ln = &L->lnorm[3 * n];
+  const int dir = ln[0] ? 0 : (ln[1] ? 1 : 2), orient = ln[dir];
+  V f[27], rho, u[3];
+  FN(load_cell)(L, n, f);
+  FN(local_rho_u)(L, f, n, &rho, u);
+  V dA[3][3], u1[3], u2[3];
+  for (int d = 0; d < 3; ++d) {
+    int o[3] = {0, 0, 0};
+    if (d == dir) {
+      o[d] = -orient;     FN(neighbor_u)(L, n, o, u1);
+      o[d] = -2 * orient; FN(neighbor_u)(L, n, o, u2);
+      for (int b = 0; b < 3; ++b) dA[d][b] = (V)(-orient) * FN(bgrad)(u[b], u1[b], u2[b]);
+    } else {
+      o[d] = 1;  FN(neighbor_u)(L, n, o, u1);
+      o[d] = -1; FN(neighbor_u)(L, n, o, u2);
+      for (int b = 0; b < 3; ++b) dA[d][b] = (u1[b] - u2[b]) / (V)2;
+    }
+  }
+  const V omega = (V)L->omega;
+  const V s = -rho / (V)3 / omega;
+  V pi[6];
+  pi[0] = (V)2 * dA[0][0] * s;
+  pi[3] = (V)2 * dA[1][1] * s;
+  pi[5] = (V)2 * dA[2][2] * s;
+  pi[1] = (dA[0][1] + dA[1][0]) * s;
+  pi[2] = (dA[0][2] + dA[2][0]) * s;
+  pi[4] = (dA[1][2] + dA[2][1]) * s;
+  const V usq = FN(usqr)(u);
+  for (int i = 0; i < L->q; ++i) out[i] = FN(feq)(L, i, rho, u, usq) + FN(pi_to_fneq)(L, i, pi);
+}
+
+/* The plane processors run one cell after the other in cell order, as the CPU
+ * platform applies them.  Their order against each other and against the
+ * edge processors (all priority 0) changes the result only by rounding: the
+ * regularized reconstruction keeps the partial population sums the Dirichlet
+ * momenta of the neighbours read. */
 static void FN(post_process)(olo_lattice* L)
 {
+  V out[27];
+  for (int64_t k = 0; k < L->n_plane; ++k) {
+    FN(plane_fd)(L, L->plane_cells[k], out);
+    FN(store_cell)(L, L->plane_cells[k], out);
+  }
   for (int64_t k = 0; k < L->n_edge; ++k) FN(outer_edge)(L, L->edge_cells[k]);
   for (int64_t k = 0; k < L->n_corner; ++k) FN(outer_corner)(L, L->corner_cells[k]);
 }

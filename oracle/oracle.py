@@ -17,7 +17,7 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 STORAGE = {"f64": 0, "f32": 1, "f16": 2}
 KINDS = {"bgk": 0, "forced_bgk": 1, "trt": 2, "forced_trt": 3, "smagorinsky": 4}
-LOCAL = {"velocity": 0, "pressure": 1}
+LOCAL = {"velocity": 0, "pressure": 1, "interp_velocity": 2, "interp_pressure": 3}
 
 _lib = None
 

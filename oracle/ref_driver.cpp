@@ -8,7 +8,7 @@
 // Input: a case directory holding
 //   case.txt  key/value lines (q, dyn, nx, ny, nz, px, py, pz, omega, magic,
 //             smago, fx, fy, fz, steps, wall <id> <rho> <ux> <uy> <uz>,
-//             local <id> velocity|pressure)
+//             local <id> velocity|pressure|interp_velocity|interp_pressure)
 //   ids.bin   uint8  [nx][ny][nz]  per-cell dynamics id
 //             (0 none, 1 bulk, 2 BounceBack, 3.. BounceBackVelocity walls)
 //   rhou.bin  double [4][nx][ny][nz] initial rho, ux, uy, uz
@@ -24,8 +24,9 @@
 //
 // A material listed as `local <id> velocity` (`pressure`) gets OpenLB's
 // LocalVelocity (LocalPressure) boundary with the bulk dynamics mixed in, as
-// cavity3d.cpp:72-73 sets it; its `wall` line gives the velocity (density)
-// that defineRhoU assigns.
+// cavity3d.cpp:72-73 sets it; `interp_velocity` (`interp_pressure`) gets
+// InterpolatedVelocity (InterpolatedPressure).  Its `wall` line gives the
+// velocity (density) that defineRhoU assigns.
 //
 // The setup follows the reference's example programs:
 //   geometry/materials  examples/laminar/cavity3dBenchmark/cavity3d.cpp:39-57
@@ -99,7 +100,8 @@ static CaseSpec readCase(const std::string& dir)
     else if (key == "steps") { long s; while (ls >> s) c.steps.push_back(s); }
     else if (key == "local") {
       int id; std::string kind; ls >> id >> kind;
-      if (kind != "velocity" && kind != "pressure") throw std::runtime_error("local kind " + kind);
+      if (kind != "velocity" && kind != "pressure" && kind != "interp_velocity" && kind != "interp_pressure")
+        throw std::runtime_error("local kind " + kind);
       c.local[id] = kind;
     }
     else if (key == "wall") {
@@ -214,8 +216,12 @@ static void runCase(const CaseSpec& c, const std::string& dir)
     if constexpr (std::is_same_v<BULK, BGKdynamics<T,DESCRIPTOR>>) {
       if (kind == "velocity") {
         boundary::set<boundary::LocalVelocity<T,DESCRIPTOR,BULK>>(lattice, geometry, id);
-      } else {
+      } else if (kind == "pressure") {
         boundary::set<boundary::LocalPressure<T,DESCRIPTOR,BULK>>(lattice, geometry, id);
+      } else if (kind == "interp_velocity") {
+        boundary::set<boundary::InterpolatedVelocity<T,DESCRIPTOR,BULK>>(lattice, geometry, id);
+      } else {
+        boundary::set<boundary::InterpolatedPressure<T,DESCRIPTOR,BULK>>(lattice, geometry, id);
       }
     } else {
       throw std::runtime_error("local boundaries need dyn bgk");

@@ -77,6 +77,15 @@ def cases():
     out.append(("channel19_velocity_pressure_f64", c, [0, 200], "f64"))
     c = W.step_cavity((14, 12, 16), block=(6, 5, 7), tau=0.6, lid_u=0.05)
     out.append(("step19_local_velocity_f64", c, [0, 100], "f64"))
+    # InterpolatedVelocity / InterpolatedPressure (finite-difference plane
+    # post-processors), the boundaries most of OpenLB's examples use
+    c = W.local_velocity_cavity((12, 10, 14), tau=0.6, lid_u=0.08, boundary="interp_velocity")
+    out.append(("cavity19_interp_velocity_f64", c, [0, 1, 200], "f64"))
+    c = W.local_velocity_cavity((10, 8, 9), tau=0.7, lid_u=0.1, q=27, boundary="interp_velocity")
+    out.append(("cavity27_interp_velocity_f64", c, [0, 100], "f64"))
+    c = W.velocity_channel((24, 12, 10), u_in=0.03, rho_out=1.0, tau=0.6, inlet="interp_velocity",
+                           outlet="interp_pressure")
+    out.append(("channel19_interp_velocity_pressure_f64", c, [0, 200], "f64"))
     return out
 
 

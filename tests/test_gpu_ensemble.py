@@ -195,7 +195,8 @@ def test_local_boundary_members_match_standalone(gpu_available, storage):
     cases = [W.local_velocity_cavity((9, 8, 11), tau=0.6, lid_u=0.07),
              W.lid_driven_cavity((7, 9, 6), tau=0.7, lid_u=0.05),
              W.velocity_channel((16, 8, 7), u_in=0.03, tau=0.65),
-             W.local_velocity_cavity((6, 7, 5), tau=0.8, lid_u=0.1)]
+             W.local_velocity_cavity((6, 7, 5), tau=0.8, lid_u=0.1),
+             W.local_velocity_cavity((8, 6, 7), tau=0.7, lid_u=0.06, boundary="interp_velocity")]
     with Ensemble(19, storage, "bgk", max_members=8, population_bytes=64 << 20, cell_capacity=1 << 20) as ens:
         members, f0s = [], []
         for c in cases:

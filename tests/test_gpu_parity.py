@@ -304,10 +304,14 @@ def test_gpu_initialize_matches_reference(gpu_available, name):
 
 @pytest.mark.parametrize("storage", ["f64", "f32"])
 def test_gpu_local_boundaries_match_oracle_larger(gpu_available, storage):
-    """A 40x36x44 cavity with LocalVelocity walls and lid, 300 steps, all cells,
-    against the oracle; and a duct with a velocity inlet and pressure outlet."""
+    """Larger cases, 300 steps, all cells, against the oracle: a 40x36x44 cavity with
+    LocalVelocity walls and lid, a duct with a LocalVelocity inlet and LocalPressure
+    outlet, and the same two with InterpolatedVelocity / InterpolatedPressure."""
     for case in (W.local_velocity_cavity((40, 36, 44), tau=0.56, lid_u=0.1),
-                 W.velocity_channel((60, 18, 22), u_in=0.04, rho_out=1.0, tau=0.58)):
+                 W.velocity_channel((60, 18, 22), u_in=0.04, rho_out=1.0, tau=0.58),
+                 W.local_velocity_cavity((30, 34, 28), tau=0.58, lid_u=0.1, boundary="interp_velocity"),
+                 W.velocity_channel((48, 20, 18), u_in=0.04, rho_out=1.0, tau=0.6, inlet="interp_velocity",
+                                    outlet="interp_pressure")):
         ora = O.OracleLattice(case, storage)
         ora.upload(O.initial_populations(case, storage))
         ora.initialize()
