@@ -118,6 +118,28 @@ int lbmh_ini_equilibrium(lbmh_lattice* lattice, const double* rho_u);
  * (src/core/blockLattice.hh:317-339). */
 int lbmh_step(lbmh_lattice* lattice, int64_t n_steps, void* stream);
 
+/* The two halves of lbmh_step for blocks that exchange overlap cells with
+ * other blocks: lbmh_collide launches the collision (and the periodic
+ * self-overlap refresh) without streaming; lbmh_stream shifts every column
+ * and runs the post-processors.  SuperLattice::collideAndStream collides all
+ * blocks, communicates, then streams all blocks (src/core/superLattice.hh:
+ * 601-663); one lbmh_step equals lbmh_collide followed by lbmh_stream. */
+int lbmh_collide(lbmh_lattice* lattice, void* stream);
+int lbmh_stream(lbmh_lattice* lattice, void* stream);
+
+/* Overlap refresh between two blocks on the same device.  After both blocks
+ * collided and before `dst` streams, lbmh_halo_copy copies all q populations
+ * of core cell src_cells[k] of `src` into overlap cell dst_cells[k] of `dst`
+ * (padded cell ids, the lists BlockCommunicationNeighborhood builds,
+ * src/communication/blockCommunicationNeighborhood.hh:89-107; the copy of
+ * ConcreteBlockCommunicator, src/core/blockLattice.hh:752-787, 918-924).
+ * Both blocks must share q and storage type; src may equal dst. */
+typedef struct lbmh_halo lbmh_halo;
+int lbmh_halo_create(lbmh_lattice* dst, const lbmh_lattice* src, const uint32_t* dst_cells,
+                     const uint32_t* src_cells, size_t n, lbmh_halo** out);
+int lbmh_halo_destroy(lbmh_halo* halo);
+int lbmh_halo_copy(lbmh_halo* halo, void* stream);
+
 /* Number of completed time steps and the PS shift of every column, in the
  * convention of CyclicColumn::_shift, normalised to [0, n_padded_cells)
  * (src/core/platform/cpu/sisd/column.h:193-204). */

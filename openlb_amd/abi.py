@@ -24,6 +24,7 @@ EXPORTS = (
     "lbmh_populations_download", "lbmh_ini_equilibrium", "lbmh_step",
     "lbmh_get_step_count", "lbmh_get_shifts", "lbmh_moments", "lbmh_reduce_moments", "lbmh_statistics",
     "lbmh_set_local_boundary", "lbmh_classify_boundary", "lbmh_initialize",
+    "lbmh_collide", "lbmh_stream", "lbmh_halo_create", "lbmh_halo_destroy", "lbmh_halo_copy",
     "lbmh_synchronize", "lbmh_lattice_footprint", "lbmh_abi_version", "lbmh_device_count",
     "lbmh_device_arch", "lbmh_last_error", "lbmh_set_launch_config", "lbmh_get_launch_config",
     "lbmh_autotune",
@@ -106,6 +107,11 @@ def load() -> ctypes.CDLL:
     L.lbmh_set_local_boundary.argtypes = [vp, i32, i32, ctypes.c_double, vp]
     L.lbmh_classify_boundary.argtypes = [vp, vp, vp, vp]
     L.lbmh_initialize.argtypes = [vp]
+    L.lbmh_collide.argtypes = [vp, vp]
+    L.lbmh_stream.argtypes = [vp, vp]
+    L.lbmh_halo_create.argtypes = [vp, vp, vp, vp, ctypes.c_size_t, ctypes.POINTER(vp)]
+    L.lbmh_halo_destroy.argtypes = [vp]
+    L.lbmh_halo_copy.argtypes = [vp, vp]
     L.lbmh_synchronize.argtypes = [vp]
     L.lbmh_lattice_footprint.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
     L.lbmh_device_count.argtypes = [ctypes.POINTER(i32)]

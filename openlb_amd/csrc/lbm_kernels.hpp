@@ -258,6 +258,24 @@ __global__ void gather_populations(H* dst, const S* pop, CoreGeom g, Shifts<Q> s
   }
 }
 
+// Overlap refresh between two blocks of one device (HomogeneousCopyTask::copy,
+// src/core/blockLattice.hh:752-787): all Q stored values of source core cell
+// p.y go to destination overlap cell p.x, each through its own column shift.
+// blockIdx.y is the population, so a wave copies consecutive plan entries of
+// one column.
+template <int Q, typename S>
+__global__ __launch_bounds__(256) void halo_copy_kernel(S* dst, uint64_t dst_stride, uint32_t dst_n, Shifts<Q> dsh,
+                                                        const S* src, uint64_t src_stride, uint32_t src_n,
+                                                        Shifts<Q> ssh, const uint2* pairs, uint32_t count)
+{
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = blockIdx.y;
+  if (k >= count || i >= uint32_t(Q)) return;
+  const uint2 p = pairs[k];
+  dst[uint64_t(i) * dst_stride + wrap_index(p.x, dsh.s[i], dst_n)] =
+      src[uint64_t(i) * src_stride + wrap_index(p.y, ssh.s[i], src_n)];
+}
+
 // Equilibrium from per-cell (rho, u), iniEquilibrium semantics.
 template <int Q, typename S>
 __global__ void ini_equilibrium_kernel(const double* rho_u, const uint8_t* ids, S* pop, CoreGeom g,

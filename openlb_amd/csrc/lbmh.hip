@@ -63,6 +63,13 @@ struct lbmh_lattice {
   void* d_scratch = nullptr;
 };
 
+struct lbmh_halo {
+  lbmh_lattice* dst = nullptr;
+  const lbmh_lattice* src = nullptr;
+  uint2* d_pairs = nullptr;   // (destination overlap cell, source core cell), padded ids
+  uint32_t count = 0;
+};
+
 namespace {
 
 CoreGeom core_geom(const lbmh_lattice* L)
@@ -151,8 +158,13 @@ int launch_post_processors(lbmh_lattice* L, hipStream_t stream)
   return LBMH_OK;
 }
 
+// Phases of a time step: the collision launches, and the stream (shift of
+// every column, post-processors).  A block that exchanges overlap cells with
+// other blocks runs them apart, with the copies in between.
+enum StepPhase : int { kPhaseCollide = 1, kPhaseStream = 2, kPhaseBoth = 3 };
+
 template <int Q, typename S, int KIND>
-int launch_kind(lbmh_lattice* L, hipStream_t stream)
+int launch_kind(lbmh_lattice* L, hipStream_t stream, int phase)
 {
   using V = typename StorageTraits<S>::V;
   StepArgs<Q, V> a;
@@ -177,6 +189,7 @@ int launch_kind(lbmh_lattice* L, hipStream_t stream)
   for (int i = 0; i < Q; ++i) a.shift[i] = uint32_t(L->shift[i]);
   // the dynamic LDS is never touched: it only limits resident workgroups per CU
   const unsigned lds = unsigned(L->launch.lds_bytes);
+  if (phase & kPhaseCollide) {
   bool launched = false;
   if constexpr (KIND == kBGK) {
     if (L->has_local) {
@@ -197,6 +210,8 @@ int launch_kind(lbmh_lattice* L, hipStream_t stream)
     hipLaunchKernelGGL((collide_stream_kernel<Q, S, KIND, true>), dim3(grid), dim3(kBlock), lds, stream, a);
   }
   LBMH_HIP(hipGetLastError());
+  }
+  if (!(phase & kPhaseStream)) return LBMH_OK;
   // stream: shift_i -= c_i . projection (blockLattice.hh:317-339)
   const int64_t N = L->n_padded;
   for (int i = 0; i < Q; ++i) {
@@ -210,25 +225,25 @@ int launch_kind(lbmh_lattice* L, hipStream_t stream)
 }
 
 template <int Q, typename S>
-int launch_step(lbmh_lattice* L, hipStream_t stream)
+int launch_step(lbmh_lattice* L, hipStream_t stream, int phase)
 {
   switch (L->kind) {
-  case LBMH_BGK: return launch_kind<Q, S, kBGK>(L, stream);
-  case LBMH_FORCED_BGK: return launch_kind<Q, S, kForcedBGK>(L, stream);
-  case LBMH_TRT: return launch_kind<Q, S, kTRT>(L, stream);
-  case LBMH_FORCED_TRT: return launch_kind<Q, S, kForcedTRT>(L, stream);
-  case LBMH_SMAGORINSKY: return launch_kind<Q, S, kSmagorinsky>(L, stream);
+  case LBMH_BGK: return launch_kind<Q, S, kBGK>(L, stream, phase);
+  case LBMH_FORCED_BGK: return launch_kind<Q, S, kForcedBGK>(L, stream, phase);
+  case LBMH_TRT: return launch_kind<Q, S, kTRT>(L, stream, phase);
+  case LBMH_FORCED_TRT: return launch_kind<Q, S, kForcedTRT>(L, stream, phase);
+  case LBMH_SMAGORINSKY: return launch_kind<Q, S, kSmagorinsky>(L, stream, phase);
   default: return fail(LBMH_E_INVALID, "unknown bulk kind");
   }
 }
 
 template <int Q>
-int step_q(lbmh_lattice* L, hipStream_t stream)
+int step_q(lbmh_lattice* L, hipStream_t stream, int phase = kPhaseBoth)
 {
   switch (L->storage) {
-  case LBMH_F64: return launch_step<Q, double>(L, stream);
-  case LBMH_F32: return launch_step<Q, float>(L, stream);
-  default: return launch_step<Q, _Float16>(L, stream);
+  case LBMH_F64: return launch_step<Q, double>(L, stream, phase);
+  case LBMH_F32: return launch_step<Q, float>(L, stream, phase);
+  default: return launch_step<Q, _Float16>(L, stream, phase);
   }
 }
 
@@ -643,6 +658,84 @@ int lbmh_step(lbmh_lattice* L, int64_t n_steps, void* stream)
     if (rc != LBMH_OK) return rc;
   }
   return LBMH_OK;
+}
+
+int lbmh_collide(lbmh_lattice* L, void* stream)
+{
+  if (!L) return fail(LBMH_E_INVALID, "null lattice");
+  DeviceGuard guard(L->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return L->q == 19 ? step_q<19>(L, s, kPhaseCollide) : step_q<27>(L, s, kPhaseCollide);
+}
+
+int lbmh_stream(lbmh_lattice* L, void* stream)
+{
+  if (!L) return fail(LBMH_E_INVALID, "null lattice");
+  DeviceGuard guard(L->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return L->q == 19 ? step_q<19>(L, s, kPhaseStream) : step_q<27>(L, s, kPhaseStream);
+}
+
+int lbmh_halo_create(lbmh_lattice* dst, const lbmh_lattice* src, const uint32_t* dst_cells,
+                     const uint32_t* src_cells, size_t n, lbmh_halo** out)
+{
+  if (!dst || !src || !out || (n && (!dst_cells || !src_cells))) return fail(LBMH_E_INVALID, "null argument");
+  *out = nullptr;
+  if (dst->q != src->q || dst->storage != src->storage) return fail(LBMH_E_INVALID, "blocks differ in q or storage");
+  if (dst->device != src->device) return fail(LBMH_E_INVALID, "blocks live on different devices");
+  if (n > 0xffffffffu) return fail(LBMH_E_INVALID, "too many overlap cells");
+  std::vector<uint2> pairs(n);
+  for (size_t k = 0; k < n; ++k) {
+    if (dst_cells[k] >= dst->n_padded || src_cells[k] >= src->n_padded)
+      return fail(LBMH_E_INVALID, "cell id outside its padded block");
+    pairs[k] = make_uint2(dst_cells[k], src_cells[k]);
+  }
+  auto* H = new (std::nothrow) lbmh_halo();
+  if (!H) return fail(LBMH_E_MEMORY, "host allocation failed");
+  H->dst = dst;
+  H->src = src;
+  H->count = uint32_t(n);
+  DeviceGuard guard(dst->device);
+  if (n) {
+    hipError_t e = hipMalloc((void**)&H->d_pairs, sizeof(uint2) * n);
+    if (e == hipSuccess) e = hipMemcpy(H->d_pairs, pairs.data(), sizeof(uint2) * n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      const int rc = hip_fail(e, "lbmh_halo_create");
+      lbmh_halo_destroy(H);
+      return rc;
+    }
+  }
+  *out = H;
+  return LBMH_OK;
+}
+
+int lbmh_halo_destroy(lbmh_halo* H)
+{
+  if (!H) return LBMH_OK;
+  DeviceGuard guard(H->dst->device);
+  if (H->d_pairs) (void)hipFree(H->d_pairs);
+  delete H;
+  return LBMH_OK;
+}
+
+int lbmh_halo_copy(lbmh_halo* H, void* stream)
+{
+  if (!H) return fail(LBMH_E_INVALID, "null halo");
+  if (!H->count) return LBMH_OK;
+  const lbmh_lattice* D = H->dst;
+  const lbmh_lattice* S_ = H->src;
+  DeviceGuard guard(D->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return with_types(D, [&](auto qc, auto st) -> int {
+    constexpr int Q = decltype(qc)::value;
+    using S = decltype(st);
+    hipLaunchKernelGGL((halo_copy_kernel<Q, S>), dim3(grid_for(H->count, 256), Q), dim3(256), 0, s,
+                       static_cast<S*>(D->d_pop), D->stride, D->n_padded, shifts_of<Q>(D),
+                       static_cast<const S*>(S_->d_pop), S_->stride, S_->n_padded, shifts_of<Q>(S_), H->d_pairs,
+                       H->count);
+    LBMH_HIP(hipGetLastError());
+    return LBMH_OK;
+  });
 }
 
 int lbmh_get_step_count(const lbmh_lattice* L, int64_t* steps)

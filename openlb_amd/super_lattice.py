@@ -1,0 +1,186 @@
+"""Several blocks of one lattice on one GPU, with the overlap exchange between
+them: the single-device part of OpenLB's SuperLattice.
+
+OpenLB splits a domain into cuboids (``CuboidDecomposition``) and gives each
+block an overlap layer.  Each time step collides every block, copies the
+overlap cells each block reads from its neighbours (and, across periodic
+faces, from the periodic image), then streams every block
+(``src/core/superLattice.hh:601-663``; copy lists from
+``src/communication/blockCommunicationNeighborhood.hh:89-107``, periodic images
+resolved through ``src/geometry/cuboidDecomposition.hh:155-177``).
+
+Here every block is a ``BlockLattice`` with its own PS columns.  A block that
+spans a whole periodic axis keeps the fused periodic refresh of its step
+kernel on that axis; every other overlap cell is filled by an
+``lbmh_halo_copy`` from the block that owns the cell.  The arithmetic per cell
+is the same as in a single block, so the result is bit-identical to the
+undivided lattice.  Over several GPUs the same lists become the send/receive
+buffers of the exchange.
+"""
+from __future__ import annotations
+
+import ctypes
+import itertools
+
+import numpy as np
+
+from . import abi
+from .lattice import BlockLattice
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def split_axis(n: int, parts: int) -> list[tuple[int, int]]:
+    """[begin, end) ranges of `parts` nearly equal pieces of 0..n-1."""
+    if parts < 1 or parts > n:
+        raise ValueError(f"cannot split {n} cells into {parts} blocks")
+    edges = [n * k // parts for k in range(parts + 1)]
+    return [(edges[k], edges[k + 1]) for k in range(parts)]
+
+
+class SuperLattice:
+    """A case split into blocks[0] x blocks[1] x blocks[2] cuboids on one device."""
+
+    def __init__(self, case, storage: str = "f32", blocks=(2, 1, 1), device: int = 0):
+        if getattr(case, "local", None):
+            # their classification and post-processors read neighbours across block interfaces
+            raise ValueError("local boundaries are supported on single blocks only")
+        self.case = case
+        self.storage = storage
+        self.shape = tuple(case.shape)
+        self.q = case.q
+        self.ranges = [split_axis(n, b) for n, b in zip(self.shape, blocks)]
+        self.blocks: dict[tuple[int, int, int], BlockLattice] = {}
+        self.origin: dict[tuple[int, int, int], tuple[int, int, int]] = {}
+        self.halos: list = []
+        # a block keeps its fused periodic refresh on an axis only where it
+        # spans the whole periodic axis (its own periodic image)
+        self_periodic = tuple(int(case.periodic[d] and len(self.ranges[d]) == 1) for d in range(3))
+        for key in itertools.product(*(range(len(r)) for r in self.ranges)):
+            lo = tuple(self.ranges[d][key[d]][0] for d in range(3))
+            hi = tuple(self.ranges[d][key[d]][1] for d in range(3))
+            sub = _sub_case(case, lo, hi, self_periodic)
+            self.blocks[key] = BlockLattice.from_case(sub, storage, device)
+            self.origin[key] = lo
+        self._build_halos(self_periodic)
+
+    # -- decomposition ---------------------------------------------------------
+
+    def _owner(self, g):
+        """Block key and local coordinates of global cell g (already wrapped)."""
+        key, local = [], []
+        for d in range(3):
+            for k, (a, b) in enumerate(self.ranges[d]):
+                if a <= g[d] < b:
+                    key.append(k)
+                    local.append(g[d] - a)
+                    break
+        return tuple(key), tuple(local)
+
+    def _build_halos(self, self_periodic):
+        lib = abi.load()
+        for key, lat in self.blocks.items():
+            lo = self.origin[key]
+            n = lat.shape
+            sy, sz = n[1] + 2, n[2] + 2
+            pairs: dict[tuple, tuple[list, list]] = {}
+            for x in range(-1, n[0] + 1):
+                for y in range(-1, n[1] + 1):
+                    for z in range(-1, n[2] + 1):
+                        r = (x, y, z)
+                        out = [d for d in range(3) if r[d] < 0 or r[d] >= n[d]]
+                        if not out:
+                            continue
+                        # the step kernel's periodic refresh covers overlap
+                        # cells whose every out-of-range axis is self-periodic
+                        if all(self_periodic[d] for d in out):
+                            continue
+                        g = [lo[d] + r[d] for d in range(3)]
+                        ok = True
+                        for d in range(3):
+                            if g[d] < 0 or g[d] >= self.shape[d]:
+                                if not self.case.periodic[d]:
+                                    ok = False
+                                    break
+                                g[d] %= self.shape[d]
+                        if not ok:
+                            continue
+                        src_key, s = self._owner(g)
+                        sn = self.blocks[src_key].shape
+                        dst_cell = ((x + 1) * sy + (y + 1)) * sz + (z + 1)
+                        src_cell = ((s[0] + 1) * (sn[1] + 2) + (s[1] + 1)) * (sn[2] + 2) + (s[2] + 1)
+                        lists = pairs.setdefault(src_key, ([], []))
+                        lists[0].append(dst_cell)
+                        lists[1].append(src_cell)
+            for src_key, (dst_cells, src_cells) in pairs.items():
+                d = np.asarray(dst_cells, dtype=np.uint32)
+                s = np.asarray(src_cells, dtype=np.uint32)
+                h = ctypes.c_void_p()
+                abi.check(lib.lbmh_halo_create(lat.handle, self.blocks[src_key].handle, _ptr(d), _ptr(s),
+                                               len(d), ctypes.byref(h)))
+                self.halos.append(h)
+
+    # -- time step -------------------------------------------------------------
+
+    def collide_and_stream(self, steps: int = 1):
+        lib = abi.load()
+        for _ in range(int(steps)):
+            for lat in self.blocks.values():
+                abi.check(lib.lbmh_collide(lat.handle, None))
+            for h in self.halos:
+                abi.check(lib.lbmh_halo_copy(h, None))
+            for lat in self.blocks.values():
+                abi.check(lib.lbmh_stream(lat.handle, None))
+
+    # -- data ------------------------------------------------------------------
+
+    def _scatter(self, full: np.ndarray, fn):
+        for key, lat in self.blocks.items():
+            lo = self.origin[key]
+            sl = tuple(slice(lo[d], lo[d] + lat.shape[d]) for d in range(3))
+            fn(lat, np.ascontiguousarray(full[(slice(None),) + sl]))
+
+    def upload_populations(self, f: np.ndarray, dtype: str = "f64"):
+        self._scatter(np.asarray(f), lambda lat, part: lat.upload_populations(part, dtype))
+
+    def ini_equilibrium(self, rho_u: np.ndarray):
+        self._scatter(np.asarray(rho_u), lambda lat, part: lat.ini_equilibrium(part))
+
+    def download_populations(self, dtype: str = "f64") -> np.ndarray:
+        out = None
+        for key, lat in self.blocks.items():
+            part = lat.download_populations(dtype)
+            if out is None:
+                out = np.empty((self.q,) + self.shape, dtype=part.dtype)
+            lo = self.origin[key]
+            out[(slice(None),) + tuple(slice(lo[d], lo[d] + lat.shape[d]) for d in range(3))] = part
+        return out
+
+    def synchronize(self):
+        next(iter(self.blocks.values())).synchronize()
+
+    def close(self):
+        lib = abi.load()
+        for h in self.halos:
+            lib.lbmh_halo_destroy(h)
+        self.halos = []
+        for lat in self.blocks.values():
+            lat.close()
+        self.blocks = {}
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def _sub_case(case, lo, hi, periodic):
+    import dataclasses
+    sl = tuple(slice(lo[d], hi[d]) for d in range(3))
+    ids = np.ascontiguousarray(case.ids[sl])
+    rho_u = None if case.rho_u is None else np.ascontiguousarray(case.rho_u[(slice(None),) + sl])
+    return dataclasses.replace(case, shape=tuple(hi[d] - lo[d] for d in range(3)), periodic=tuple(periodic),
+                               ids=ids, rho_u=rho_u)
