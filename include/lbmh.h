@@ -146,6 +146,27 @@ int lbmh_halo_copy(lbmh_halo* halo, void* stream);
 int lbmh_get_step_count(const lbmh_lattice* lattice, int64_t* steps);
 int lbmh_get_shifts(const lbmh_lattice* lattice, int64_t* shifts);
 
+/* OpenLB checkpoint of the block, as raw bytes in OpenLB's value type `real`
+ * (LBMH_F64 for a double lattice, LBMH_F32 for float).  The bytes are those
+ * Serializable::save(std::uint8_t*) writes for a ConcreteBlockLattice
+ * (src/core/serializer.hh:106-110, io/serializerIO.hh:96-106): POPULATION as
+ * q CyclicColumns {int64 shift, uint64 count, T[count]} over the padded block
+ * (src/core/platform/cpu/sisd/column.hh:66-89, gpu/cuda/column.hh:298-321),
+ * then the other fields OpenLB allocates for the block's dynamics (FORCE,
+ * VELOCITY, FixedDensity::RHO, FixedPressureMomentum / FixedVelocityMomentumGeneric
+ * VELOCITY) as Columns {uint64 count, T[count]}, in OpenLB's field order
+ * (src/core/data.hh:52-62).  Replaces getSerializableSize / save / load of
+ * ConcreteBlockLattice (src/core/blockLattice.hh:597-626); one block's bytes
+ * are also a one-block SuperLattice's (src/core/superLattice.hh:806-842).
+ * lbmh_checkpoint_load takes the populations of every padded cell and the
+ * shifts, after checking the sizes and, on the core cells, every field value
+ * the block's dynamics read against the lattice's own set-up; on any
+ * mismatch it changes nothing and returns LBMH_E_INVALID.  Both synchronise
+ * the device. */
+int lbmh_checkpoint_size(const lbmh_lattice* lattice, int real, size_t* bytes);
+int lbmh_checkpoint_save(lbmh_lattice* lattice, int real, void* buffer, size_t bytes);
+int lbmh_checkpoint_load(lbmh_lattice* lattice, int real, const void* buffer, size_t bytes);
+
 /* Per-cell moments in FP64, host layout [4][nx][ny][nz]: rho and u as the
  * cell's momenta define them (bulk: lbm<D>::computeRhoU, src/dynamics/lbm.h:
  * 300-307, forced: u += F/2).  Replaces SuperLatticePhysVelocity3D-style host
@@ -305,6 +326,15 @@ int lbmh_ensemble_ini_equilibrium(lbmh_ensemble* ensemble, int member, const dou
 int lbmh_ensemble_upload(lbmh_ensemble* ensemble, int member, const void* host, int host_dtype);
 int lbmh_ensemble_download(lbmh_ensemble* ensemble, int member, void* host, int host_dtype);
 int lbmh_ensemble_moments(lbmh_ensemble* ensemble, int member, double* rho_u);
+/* OpenLB checkpoint of one member, the same bytes lbmh_checkpoint_save
+ * writes for a lattice of the member's set-up (ConcreteBlockLattice
+ * serialization, src/core/blockLattice.hh:597-626).  A load replaces the
+ * member's populations and shifts; its step counters are left as they are.
+ * With remove and add, this preempts a member to host memory and resumes it
+ * later in any free slot. */
+int lbmh_ensemble_checkpoint_size(lbmh_ensemble* ensemble, int member, int real, size_t* bytes);
+int lbmh_ensemble_checkpoint_save(lbmh_ensemble* ensemble, int member, int real, void* buffer, size_t bytes);
+int lbmh_ensemble_checkpoint_load(lbmh_ensemble* ensemble, int member, int real, const void* buffer, size_t bytes);
 
 /* Advance every member that still has steps by up to n_steps steps: one
  * launch of the fused step kernel over all members' cells and one launch of

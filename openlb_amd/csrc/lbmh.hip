@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "boundary_classify.hpp"
+#include "checkpoint.hpp"
 #include "lbm_kernels.hpp"
 #include "lbmh.h"
 #include "lbmh_common.hpp"
@@ -749,6 +750,69 @@ int lbmh_get_shifts(const lbmh_lattice* L, int64_t* shifts)
 {
   if (!L || !shifts) return fail(LBMH_E_INVALID, "null argument");
   for (int i = 0; i < L->q; ++i) shifts[i] = L->shift[i];
+  return LBMH_OK;
+}
+
+namespace {
+
+CheckpointBlock checkpoint_block(const lbmh_lattice* L, std::vector<uint8_t>& no_ids)
+{
+  CheckpointBlock b;
+  b.q = L->q;
+  for (int d = 0; d < 3; ++d) {
+    b.n[d] = L->n[d];
+    b.periodic[d] = L->periodic[d];
+    b.force[d] = L->params.force[d];
+  }
+  b.storage = L->storage;
+  b.kind = L->kind;
+  b.n_padded = L->n_padded;
+  b.stride = L->stride;
+  b.d_pop = L->d_pop;
+  for (int i = 0; i < L->q; ++i) {
+    b.shift[i] = L->shift[i];
+    b.dist[i] = L->dist[i];
+  }
+  if (L->h_ids.size() == L->ncore) {
+    b.ids = L->h_ids.data();
+  } else {
+    no_ids.assign(L->ncore, 0);  // ids never set: every cell has id 0
+    b.ids = no_ids.data();
+  }
+  b.walls = &L->walls[0][0];
+  b.info = L->info;
+  b.n_ids = kMaxWalls;
+  return b;
+}
+
+}  // namespace
+
+int lbmh_checkpoint_size(const lbmh_lattice* L, int real, size_t* bytes)
+{
+  if (!L || !bytes) return fail(LBMH_E_INVALID, "null argument");
+  if (real != LBMH_F64 && real != LBMH_F32) return fail(LBMH_E_INVALID, "checkpoint value type must be F64 or F32");
+  std::vector<uint8_t> no_ids;
+  *bytes = checkpoint_bytes(checkpoint_block(L, no_ids), real);
+  return LBMH_OK;
+}
+
+int lbmh_checkpoint_save(lbmh_lattice* L, int real, void* buffer, size_t bytes)
+{
+  if (!L) return fail(LBMH_E_INVALID, "null lattice");
+  DeviceGuard guard(L->device);
+  std::vector<uint8_t> no_ids;
+  return checkpoint_save(checkpoint_block(L, no_ids), real, buffer, bytes);
+}
+
+int lbmh_checkpoint_load(lbmh_lattice* L, int real, const void* buffer, size_t bytes)
+{
+  if (!L) return fail(LBMH_E_INVALID, "null lattice");
+  DeviceGuard guard(L->device);
+  std::vector<uint8_t> no_ids;
+  int64_t shift[27];
+  const int rc = checkpoint_load(checkpoint_block(L, no_ids), real, buffer, bytes, shift);
+  if (rc != LBMH_OK) return rc;
+  for (int i = 0; i < L->q; ++i) L->shift[i] = shift[i];
   return LBMH_OK;
 }
 

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "boundary_classify.hpp"
+#include "checkpoint.hpp"
 #include "lbm_ensemble.hpp"
 #include "lbmh.h"
 #include "lbmh_common.hpp"
@@ -780,6 +781,100 @@ int lbmh_ensemble_download(lbmh_ensemble* E, int m, void* host, int dtype)
   if (rc != LBMH_OK) return rc;
   LBMH_HIP(hipMemcpy(host, staging.p, bytes, hipMemcpyDeviceToHost));
   return LBMH_OK;
+}
+
+namespace {
+
+// The member as one block for the checkpoint code; its wall table comes from
+// the device into `walls`.  Call after pull_table.
+int member_block(lbmh_ensemble* E, int m, CheckpointBlock& b, std::vector<double>& walls)
+{
+  const Member& mem = E->members[m];
+  walls.resize(size_t(4) * kEnsembleWalls);
+  LBMH_HIP(hipMemcpy(walls.data(), E->d_walls + size_t(m) * 4 * kEnsembleWalls, walls.size() * sizeof(double),
+                     hipMemcpyDeviceToHost));
+  b.q = E->q;
+  b.storage = E->storage;
+  b.kind = E->kind;
+  for (int d = 0; d < 3; ++d) {
+    b.n[d] = mem.n[d];
+    b.periodic[d] = mem.periodic[d];
+    b.force[d] = mem.params.force[d];
+  }
+  b.ids = mem.h_ids.data();
+  b.walls = walls.data();
+  b.info = mem.info;
+  b.n_ids = kEnsembleWalls;
+  return with_q(E, [&](auto qc) -> int {
+    constexpr int Q = decltype(qc)::value;
+    const EnsembleEntry<Q>& en = entry<Q>(E, m);
+    b.n_padded = en.n_padded;
+    b.stride = en.stride;
+    b.d_pop = static_cast<char*>(E->d_pop) + en.pop_offset * storage_size(E->storage);
+    for (int i = 0; i < Q; ++i) {
+      b.shift[i] = en.shift[i];
+      b.dist[i] = neighbor_distance<Q>(i, int64_t(en.sx), int64_t(en.sy));
+    }
+    return LBMH_OK;
+  });
+}
+
+}  // namespace
+
+int lbmh_ensemble_checkpoint_size(lbmh_ensemble* E, int m, int real, size_t* bytes)
+{
+  int rc = check_member(E, m);
+  if (rc != LBMH_OK) return rc;
+  if (!bytes) return fail(LBMH_E_INVALID, "null argument");
+  if (real != LBMH_F64 && real != LBMH_F32) return fail(LBMH_E_INVALID, "checkpoint value type must be F64 or F32");
+  DeviceGuard guard(E->device);
+  rc = pull_table(E);
+  if (rc != LBMH_OK) return rc;
+  CheckpointBlock b;
+  std::vector<double> walls;
+  rc = member_block(E, m, b, walls);
+  if (rc != LBMH_OK) return rc;
+  *bytes = checkpoint_bytes(b, real);
+  return LBMH_OK;
+}
+
+int lbmh_ensemble_checkpoint_save(lbmh_ensemble* E, int m, int real, void* buffer, size_t bytes)
+{
+  int rc = check_member(E, m);
+  if (rc != LBMH_OK) return rc;
+  DeviceGuard guard(E->device);
+  rc = pull_table(E);
+  if (rc != LBMH_OK) return rc;
+  CheckpointBlock b;
+  std::vector<double> walls;
+  rc = member_block(E, m, b, walls);
+  if (rc != LBMH_OK) return rc;
+  return checkpoint_save(b, real, buffer, bytes);
+}
+
+int lbmh_ensemble_checkpoint_load(lbmh_ensemble* E, int m, int real, const void* buffer, size_t bytes)
+{
+  int rc = check_member(E, m);
+  if (rc != LBMH_OK) return rc;
+  DeviceGuard guard(E->device);
+  rc = pull_table(E);
+  if (rc != LBMH_OK) return rc;
+  CheckpointBlock b;
+  std::vector<double> walls;
+  rc = member_block(E, m, b, walls);
+  if (rc != LBMH_OK) return rc;
+  int64_t shift[27];
+  rc = checkpoint_load(b, real, buffer, bytes, shift);
+  if (rc != LBMH_OK) return rc;
+  // only the member's shifts change in the table
+  return with_q(E, [&](auto qc) -> int {
+    constexpr int Q = decltype(qc)::value;
+    EnsembleEntry<Q>& en = entry<Q>(E, m);
+    for (int i = 0; i < Q; ++i) en.shift[i] = uint32_t(shift[i]);
+    char* dst = static_cast<char*>(E->d_table) + size_t(m) * E->entry_bytes + offsetof(EnsembleEntry<Q>, shift);
+    LBMH_HIP(hipMemcpy(dst, en.shift, sizeof(en.shift), hipMemcpyHostToDevice));
+    return LBMH_OK;
+  });
 }
 
 int lbmh_ensemble_moments(lbmh_ensemble* E, int m, double* rho_u)

@@ -139,6 +139,21 @@ class Ensemble:
         abi.check(abi.load().lbmh_ensemble_download(self._h, int(member), _ptr(out), code))
         return out
 
+    def checkpoint(self, member: int, real: str = "f64") -> bytes:
+        """OpenLB's serialized bytes of one member, as BlockLattice.checkpoint."""
+        code = {"f64": abi.F64, "f32": abi.F32}[real]
+        n = ctypes.c_size_t(0)
+        L = abi.load()
+        abi.check(L.lbmh_ensemble_checkpoint_size(self._h, int(member), code, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        abi.check(L.lbmh_ensemble_checkpoint_save(self._h, int(member), code, buf, n.value))
+        return buf.raw
+
+    def restore(self, member: int, data: bytes, real: str = "f64"):
+        code = {"f64": abi.F64, "f32": abi.F32}[real]
+        buf = ctypes.create_string_buffer(bytes(data), len(data))
+        abi.check(abi.load().lbmh_ensemble_checkpoint_load(self._h, int(member), code, buf, len(data)))
+
     def moments(self, member: int) -> np.ndarray:
         out = np.empty((4,) + self.shapes[member])
         abi.check(abi.load().lbmh_ensemble_moments(self._h, int(member), _ptr(out)))

@@ -11,7 +11,7 @@ import ctypes
 
 import numpy as np
 
-from . import abi
+from . import abi, checkpoint
 
 
 def _ptr(a: np.ndarray):
@@ -153,6 +153,36 @@ class BlockLattice:
         out = np.zeros(self.q, dtype=np.int64)
         abi.check(abi.load().lbmh_get_shifts(self._h, _ptr(out)))
         return out
+
+    # -- checkpoints (Serializable::save / load, src/core/serializer.h:145-189) --
+
+    def checkpoint(self, real: str = "f64") -> bytes:
+        """OpenLB's serialized bytes of this block for value type `real` ("f64"
+        for a double lattice, "f32" for float): Serializable::save(uint8_t*)."""
+        code = {"f64": abi.F64, "f32": abi.F32}[real]
+        n = ctypes.c_size_t(0)
+        L = abi.load()
+        abi.check(L.lbmh_checkpoint_size(self._h, code, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        abi.check(L.lbmh_checkpoint_save(self._h, code, buf, n.value))
+        return buf.raw
+
+    def restore(self, data: bytes, real: str = "f64"):
+        """Replace the populations and shifts by a checkpoint's (Serializable::
+        load(const uint8_t*) followed by postLoad)."""
+        code = {"f64": abi.F64, "f32": abi.F32}[real]
+        buf = ctypes.create_string_buffer(bytes(data), len(data))
+        abi.check(abi.load().lbmh_checkpoint_load(self._h, code, buf, len(data)))
+
+    def save(self, file_name: str, real: str = "f64", enforce_uint: bool = False) -> str:
+        """Write `<file_name>_rank0_size1.dat` as OpenLB's save(fileName) does; returns the path."""
+        path = checkpoint.parallel_file_name(file_name)
+        checkpoint.write_file(path, self.checkpoint(real), enforce_uint)
+        return path
+
+    def load(self, file_name: str, real: str = "f64", enforce_uint: bool = False):
+        """Read `<file_name>_rank0_size1.dat`, as OpenLB's load(fileName) does."""
+        self.restore(checkpoint.read_file(checkpoint.parallel_file_name(file_name), enforce_uint), real)
 
     def moments(self) -> np.ndarray:
         out = np.empty((4,) + self.shape)

@@ -135,6 +135,20 @@ def lid_driven_cavity(shape=(32, 32, 32), tau=0.6, lid_u=0.05, q=19, kind="bgk")
                 1.0 / tau, walls={3: (1.0, (lid_u, 0.0, 0.0))})
 
 
+def couette_channel(shape=(16, 12, 10), tau=0.7, wall_u=0.05, force=1e-5, q=19, kind="forced_bgk"):
+    """Plane Couette-Poiseuille channel, periodic in x and z: BounceBack at y = 0,
+    a BounceBackVelocity wall (id 3) moving along x at y = ny-1, and a uniform
+    body force along x on the fluid."""
+    ids = np.ones(shape, dtype=np.uint8)
+    ids[:, 0, :] = 2
+    ids[:, -1, :] = 3
+    rho_u = np.zeros((4,) + tuple(shape))
+    rho_u[0] = 1.0
+    forced = kind in ("forced_bgk", "forced_trt")
+    return Case("couette_channel", q, kind, tuple(shape), (1, 0, 1), ids, rho_u, 1.0 / tau,
+                force=(force if forced else 0.0, 0.0, 0.0), walls={3: (1.0, (wall_u, 0.0, 0.0))})
+
+
 def cavity_benchmark(size=100, q=19):
     """The cavity of OpenLB's benchmark harness in its bounce-back-lid variant
     (examples/laminar/cavity3dBenchmark/cavity3d.cpp:39-57, 85-87, 154-250):

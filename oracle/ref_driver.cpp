@@ -16,6 +16,14 @@
 //   pops_<step>.bin  double [q][nx][ny][nz] shifted populations of the core
 //                    cells after <step> collideAndStream() calls (0 = after
 //                    initialize()).
+//   checkpoint_<step>_rank0_size1.dat  (only with OLB_REF_SAVE set): OpenLB's
+//                    own serialization of the lattice after <step> steps
+//                    (Serializable::save, serializer.hh:81-97, base64 text of
+//                    the blocks of superLattice.hh:806-834).
+//   With OLB_REF_LOAD=<path> (a checkpoint written by either side, without
+//   its "_rank0_size1.dat" suffix) the state after initialize() is replaced by
+//   that checkpoint (Serializable::load, then postLoad) and <step> counts the
+//   steps taken after the load.
 //   normals.bin      int8 [4][nx][ny][nz] (only with OLB_REF_NORMALS set):
 //                    type and discrete normal that OpenLB's
 //                    computeBoundaryTypeAndNormal gives every core cell that
@@ -291,12 +299,19 @@ static void runCase(const CaseSpec& c, const std::string& dir)
   // averages are OpenLB's placeholders (latticeStatistics.hh:64-72).
   const bool stats = std::getenv("OLB_REF_STATS") != nullptr;
   if (stats) lattice.statisticsOn();
+  const bool save = std::getenv("OLB_REF_SAVE") != nullptr;
+  if (const char* restart = std::getenv("OLB_REF_LOAD")) {
+    if (!lattice.template load<false>(restart)) throw std::runtime_error(std::string("cannot load ") + restart);
+  }
   long done = 0;
   for (long target : c.steps) {
     for (; done < target; ++done) {
       lattice.collideAndStream();
     }
     dumpPopulations<DESCRIPTOR>(c, lattice, dir, target);
+    if (save) {
+      lattice.template save<false>(dir + "/checkpoint_" + std::to_string(target));
+    }
     if (stats) {
       auto& st = lattice.getStatistics();
       const std::string name = dir + "/stats_" + std::to_string(target) + ".txt";

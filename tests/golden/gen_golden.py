@@ -13,8 +13,9 @@ after ``SuperLattice::initialize()``, and snapshot k is the state after k calls 
 ``collideAndStream()``. A ``manifest.json`` records the parameters and a SHA-256
 of every array, ``statistics.json`` OpenLB's lattice statistics of each
 fixture's last snapshot, and ``normals.npz`` OpenLB's boundary classification
-of a few geometries (``--statistics-only`` / ``--normals-only`` write just
-those files).
+of a few geometries, and ``checkpoints/`` OpenLB's own checkpoint files of a
+few lattices (``--statistics-only`` / ``--normals-only`` /
+``--checkpoints-only`` write just those files).
 """
 from __future__ import annotations
 
@@ -213,6 +214,55 @@ def main():
         json.dump(manifest, fh, indent=1, sort_keys=True)
 
 
+def checkpoint_cases():
+    """(fixture name, Case, checkpoint step k, end step m, reference precision)."""
+    return [
+        ("tgv19_bgk_f64", W.taylor_green((8, 10, 6), u0=0.05, tau=0.6), 5, 25, "f64"),
+        ("tgv19_bgk_f32", W.taylor_green((8, 10, 6), u0=0.05, tau=0.6), 5, 25, "f32"),
+        ("couette19_forced_bgk_f64", W.couette_channel((8, 7, 6), tau=0.7, wall_u=0.05, force=1e-5), 5, 25, "f64"),
+        ("channel27_smagorinsky_f32", W.turbulent_channel((8, 7, 6), tau=0.51, smago=0.14, u_max=0.05,
+                                                          noise=0.2, seed=5, q=27), 5, 25, "f32"),
+        ("channel19_velocity_pressure_f64", W.velocity_channel((12, 8, 6), u_in=0.03, rho_out=1.0, tau=0.6),
+         5, 25, "f64"),
+        ("channel19_interp_velocity_pressure_f64",
+         W.velocity_channel((12, 8, 6), u_in=0.03, rho_out=1.0, tau=0.6, inlet="interp_velocity",
+                            outlet="interp_pressure"), 5, 25, "f64"),
+        ("cavity27_local_velocity_f64", W.local_velocity_cavity((8, 7, 6), tau=0.7, lid_u=0.1, q=27), 5, 25, "f64"),
+    ]
+
+
+def write_checkpoints():
+    """checkpoints/<name>.npz: OpenLB's own checkpoint file of the lattice after
+    k steps (Serializable::save, the .dat text as bytes), and its populations
+    after k and m steps.  The k-step file is what a checkpoint of the library
+    must reproduce; the m-step populations are what both sides reach when
+    they resume from it."""
+    out_dir = os.path.join(HERE, "checkpoints")
+    os.makedirs(out_dir, exist_ok=True)
+    manifest = {}
+    for name, case, k, m, precision in checkpoint_cases():
+        exe = os.path.join(ROOT, "oracle", "_ref", f"olb_ref_{precision}")
+        with tempfile.TemporaryDirectory() as d:
+            write_case_dir(d, case, [k, m])
+            subprocess.run([exe, d], check=True, cwd=d, stdout=subprocess.DEVNULL,
+                           env=dict(os.environ, OLB_REF_SAVE="1"))
+            with open(os.path.join(d, f"checkpoint_{k}_rank0_size1.dat"), "rb") as fh:
+                dat = np.frombuffer(fh.read(), dtype=np.uint8)
+            pops = {s: np.fromfile(os.path.join(d, f"pops_{s}.bin")).reshape((case.q,) + tuple(case.shape))
+                    for s in (k, m)}
+        meta = meta_of(case, [k, m], precision)
+        meta["checkpoint_step"] = k
+        arrays = {"ids": case.ids.astype(np.uint8), "rho_u": case.rho_u, "dat": dat,
+                  f"f_{k}": pops[k], f"f_{m}": pops[m]}
+        np.savez_compressed(os.path.join(out_dir, name + ".npz"), meta=np.array(json.dumps(meta)), **arrays)
+        meta["sha256"] = {a: hashlib.sha256(np.ascontiguousarray(v).tobytes()).hexdigest()
+                          for a, v in arrays.items()}
+        manifest[name] = meta
+        print(f"checkpoint {name}: shape {case.shape} k {k} m {m} ({len(dat)} chars)", flush=True)
+    with open(os.path.join(out_dir, "manifest.json"), "w") as fh:
+        json.dump(manifest, fh, indent=1, sort_keys=True)
+
+
 def write_statistics():
     """statistics.json: OpenLB's lattice statistics of each fixture's last
     snapshot.  OpenLB accumulates the statistics of step k+1 during its
@@ -237,8 +287,12 @@ def write_statistics():
 
 
 if __name__ == "__main__":
+    if "--checkpoints-only" in sys.argv:
+        write_checkpoints()
+        sys.exit(0)
     if "--statistics-only" not in sys.argv and "--normals-only" not in sys.argv:
         main()
     if "--normals-only" not in sys.argv:
         write_statistics()
     write_normals()
+    write_checkpoints()

@@ -73,3 +73,25 @@ def statistics_error(got, ref: dict) -> float:
     assert int(got[0]) == ref["cells"], (got[0], ref["cells"])
     keys = ("average_rho", "average_energy", "max_u")
     return max(abs(float(g) - ref[k]) / max(abs(ref[k]), 1e-300) for g, k in zip(got[1:], keys))
+
+
+CHECKPOINTS = os.path.join(GOLDEN, "checkpoints")
+
+
+def checkpoint_names():
+    return [os.path.basename(p)[:-4] for p in sorted(glob.glob(os.path.join(CHECKPOINTS, "*.npz")))]
+
+
+def load_checkpoint(name: str):
+    """(Case, meta, arrays, .dat text) of one checkpoint fixture: OpenLB's own
+    checkpoint file after meta["checkpoint_step"] steps and its populations
+    after every step of meta["steps"]."""
+    z = np.load(os.path.join(CHECKPOINTS, name + ".npz"), allow_pickle=False)
+    meta = json.loads(str(z["meta"]))
+    arrays = {k: z[k] for k in z.files if k not in ("meta", "dat")}
+    walls = {int(k): (v[0], tuple(v[1])) for k, v in meta["walls"].items()}
+    local = {int(k): v for k, v in meta.get("local", {}).items()}
+    case = W.Case(name, meta["q"], meta["kind"], tuple(meta["shape"]), tuple(meta["periodic"]),
+                  arrays["ids"], arrays["rho_u"], meta["omega"], meta["magic"], meta["smago"],
+                  tuple(meta["force"]), walls, local)
+    return case, meta, arrays, z["dat"].tobytes().decode("ascii")
