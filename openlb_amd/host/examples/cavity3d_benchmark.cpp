@@ -14,7 +14,9 @@
 //   * prints "size, steps, cuboids, ranks, threads, MLUPs" like the original.
 // Extra flags: --fp64 (double storage), --dump <file> (populations after the
 // run, double [q][nx][ny][nz]), --tau (override), --stats (lattice statistics
-// after the run).
+// after the run), --load <name> (resume from OpenLB checkpoint
+// <name>_rank0_size1.dat right after initialize(), as lattice.load(name)),
+// --save <name> (write that checkpoint after the run, as lattice.save(name)).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -27,7 +29,8 @@
 using namespace olb::hip;
 
 template <typename T>
-int run(int size, long steps, double tau, const char* dump, bool stats, bool bounceBack)
+int run(int size, long steps, double tau, const char* dump, bool stats, bool bounceBack, const char* load,
+        const char* save)
 {
   using DESCRIPTOR = descriptors::D3Q19;
   const int n = size + 1;
@@ -56,6 +59,7 @@ int run(int size, long steps, double tau, const char* dump, bool stats, bool bou
   lattice.iniEquilibrium([](int, int, int) { return true; }, [](int, int, int) { return 1.0; },
                          [](int, int, int) { return std::array<double, 3>{0.0, 0.0, 0.0}; });
   lattice.initialize();
+  if (load && !lattice.template load<false>(load)) throw std::runtime_error(std::string("cannot read checkpoint ") + load);
 
   for (int i = 0; i < 10; ++i) lattice.collideAndStream();
   lattice.synchronize();
@@ -66,6 +70,7 @@ int run(int size, long steps, double tau, const char* dump, bool stats, bool bou
   const double mlups = double(lattice.ncore()) * double(steps) / seconds / 1e6;
 
   lattice.setProcessingContext(ProcessingContext::Evaluation);
+  if (save && !lattice.template save<false>(save)) throw std::runtime_error(std::string("cannot write checkpoint ") + save);
   std::cout << size << ", " << steps << ", " << 1 << ", " << 1 << ", " << 1 << ", " << mlups << std::endl;
   if (stats) {
     const auto st = lattice.getStatistics();
@@ -91,6 +96,8 @@ int main(int argc, char** argv)
   const char* dump = nullptr;
   bool stats = false;
   bool bounceBack = false;
+  const char* load = nullptr;
+  const char* save = nullptr;
   double tau = -1;
   for (int i = 1; i < argc; ++i) {
     if (!std::strcmp(argv[i], "--size") && i + 1 < argc) size = std::atoi(argv[++i]);
@@ -100,15 +107,18 @@ int main(int argc, char** argv)
     else if (!std::strcmp(argv[i], "--tau") && i + 1 < argc) tau = std::atof(argv[++i]);
     else if (!std::strcmp(argv[i], "--stats")) stats = true;
     else if (!std::strcmp(argv[i], "--bounce-back")) bounceBack = true;
+    else if (!std::strcmp(argv[i], "--load") && i + 1 < argc) load = argv[++i];
+    else if (!std::strcmp(argv[i], "--save") && i + 1 < argc) save = argv[++i];
     else if (!std::strcmp(argv[i], "--no-results")) continue;
     else {
-      std::fprintf(stderr, "usage: %s [--size N] [--steps K] [--fp64] [--tau T] [--dump FILE] [--stats] [--bounce-back]\n", argv[0]);
+      std::fprintf(stderr, "usage: %s [--size N] [--steps K] [--fp64] [--tau T] [--dump FILE] [--stats] [--bounce-back] [--load NAME] [--save NAME]\n", argv[0]);
       return 2;
     }
   }
   if (tau <= 0) tau = 0.5 + 3.0 * 1e-5 * size;  // nu = nu_phys * dt / dx^2 with u_lattice 0.01
   try {
-    return fp64 ? run<double>(size, steps, tau, dump, stats, bounceBack) : run<float>(size, steps, tau, dump, stats, bounceBack);
+    return fp64 ? run<double>(size, steps, tau, dump, stats, bounceBack, load, save)
+                : run<float>(size, steps, tau, dump, stats, bounceBack, load, save);
   } catch (const std::exception& e) {
     std::fprintf(stderr, "error: %s\n", e.what());
     return 1;

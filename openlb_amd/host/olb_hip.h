@@ -13,6 +13,7 @@
 //   statisticsOff()                                      superLattice.h:397-403
 //   boundary::set<Local|InterpolatedVelocity/Pressure>   boundary/setBoundary.h:83-101
 //   getStatistics()                                      superLattice.h:378
+//   getSerializableSize / save / load (buffer or file)   core/serializer.h:145-189
 //
 // Dynamics are tags carrying the C ABI's bulk kind; parameters are tags of
 // the same names as OpenLB's fields.  HIP errors surface as std::runtime_error
@@ -21,8 +22,11 @@
 #pragma once
 
 #include <array>
+#include <cctype>
 #include <cstdint>
+#include <fstream>
 #include <functional>
+#include <iterator>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -89,6 +93,55 @@ template <typename T> constexpr int storage_of()
   else if constexpr (std::is_same_v<T, float>) return LBMH_F32;
   else return LBMH_F16;
 }
+
+// base64 with the standard alphabet and '=' padding, as OpenLB's
+// Base64Encoder / Base64Decoder write and read it (src/io/base64.hh:45-194)
+inline std::string base64_encode(const std::uint8_t* p, std::size_t n)
+{
+  static const char enc[] = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/";
+  std::string out;
+  out.reserve((n + 2) / 3 * 4);
+  for (std::size_t i = 0; i < n; i += 3) {
+    const std::uint32_t b0 = p[i], b1 = i + 1 < n ? p[i + 1] : 0u, b2 = i + 2 < n ? p[i + 2] : 0u;
+    const std::uint32_t v = (b0 << 16) | (b1 << 8) | b2;
+    out += enc[(v >> 18) & 63];
+    out += enc[(v >> 12) & 63];
+    out += i + 1 < n ? enc[(v >> 6) & 63] : '=';
+    out += i + 2 < n ? enc[v & 63] : '=';
+  }
+  return out;
+}
+
+// Decodes `bytes` bytes from the base64 text at `pos`, skipping white space;
+// advances `pos` past the groups it used.
+inline bool base64_decode(const std::string& text, std::size_t& pos, std::uint8_t* out, std::size_t bytes)
+{
+  auto value = [](char c) -> int {
+    if (c >= 'A' && c <= 'Z') return c - 'A';
+    if (c >= 'a' && c <= 'z') return c - 'a' + 26;
+    if (c >= '0' && c <= '9') return c - '0' + 52;
+    if (c == '+') return 62;
+    if (c == '/') return 63;
+    if (c == '=') return 0;
+    return -1;
+  };
+  std::size_t done = 0;
+  while (done < bytes) {
+    int v[4];
+    for (int k = 0; k < 4; ++k) {
+      while (pos < text.size() && std::isspace(static_cast<unsigned char>(text[pos]))) ++pos;
+      if (pos >= text.size() || (v[k] = value(text[pos])) < 0) return false;
+      ++pos;
+    }
+    const std::uint32_t w = (std::uint32_t(v[0]) << 18) | (std::uint32_t(v[1]) << 12) | (std::uint32_t(v[2]) << 6) | v[3];
+    const std::uint8_t b[3] = {std::uint8_t(w >> 16), std::uint8_t(w >> 8), std::uint8_t(w)};
+    for (int k = 0; k < 3 && done < bytes; ++k) out[done++] = b[k];
+  }
+  return true;
+}
+
+// createParallelFileName(name) + ".dat" of a one-rank run (src/io/fileName.hh:51-64)
+inline std::string parallel_file_name(const std::string& name) { return name + "_rank0_size1.dat"; }
 
 template <typename DYN> struct kind_of { static constexpr int value = -1; };
 template <template <typename, typename> class DYN, typename T, typename D>
@@ -325,7 +378,102 @@ class SuperLattice {
 
   lbmh_lattice* handle() const { return _handle; }
 
+  /// Serializable interface (src/core/serializer.h:145-189) in OpenLB's value
+  /// type T: the bytes Serializable::save(std::uint8_t*) writes for an OpenLB
+  /// block of the same set-up (FP16 storage converts to float).  A load checks
+  /// the field values the dynamics read against this lattice's set-up and
+  /// throws std::invalid_argument on a mismatch.
+  std::size_t getSerializableSize()
+  {
+    if (!_handle) initialize();
+    std::size_t n = 0;
+    detail::check(lbmh_checkpoint_size(_handle, real(), &n));
+    return n;
+  }
+
+  bool save(std::uint8_t* buffer)
+  {
+    const std::size_t n = getSerializableSize();
+    detail::check(lbmh_checkpoint_save(_handle, real(), buffer, n));
+    return true;
+  }
+
+  bool load(const std::uint8_t* buffer)
+  {
+    const std::size_t n = getSerializableSize();
+    detail::check(lbmh_checkpoint_load(_handle, real(), buffer, n));
+    _mirrorValid = false;
+    return true;
+  }
+
+  /// The file form: <logOutDir><fileName>_rank0_size1.dat holding the base64
+  /// of the byte count (size_t, or unsigned with enforceUint) and of the
+  /// bytes (src/core/serializer.hh:81-97, io/serializerIO.hh:40-94).  The
+  /// log output directory is OpenLB's default, "./tmp/", unless set with
+  /// setLogOutDir; includeLogOutputDir = false writes next to the program.
+  template <bool includeLogOutputDir = true>
+  bool save(std::string fileName = "", bool enforceUint = false)
+  {
+    std::vector<std::uint8_t> raw(getSerializableSize());
+    save(raw.data());
+    std::string text;
+    if (enforceUint) {
+      if (raw.size() > 0xffffffffu) return false;
+      const std::uint32_t size = std::uint32_t(raw.size());
+      text = detail::base64_encode(reinterpret_cast<const std::uint8_t*>(&size), sizeof(size));
+    } else {
+      const std::uint64_t size = raw.size();
+      text = detail::base64_encode(reinterpret_cast<const std::uint8_t*>(&size), sizeof(size));
+    }
+    text += detail::base64_encode(raw.data(), raw.size());
+    std::ofstream out(fullFileName<includeLogOutputDir>(fileName));
+    if (!out) return false;
+    out << text;
+    return bool(out);
+  }
+
+  template <bool includeLogOutputDir = true>
+  bool load(std::string fileName = "", bool enforceUint = false)
+  {
+    std::ifstream in(fullFileName<includeLogOutputDir>(fileName));
+    if (!in) return false;
+    const std::string text((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    std::size_t pos = 0;
+    std::uint64_t size = 0;
+    if (enforceUint) {
+      std::uint32_t s32 = 0;
+      if (!detail::base64_decode(text, pos, reinterpret_cast<std::uint8_t*>(&s32), sizeof(s32))) return false;
+      size = s32;
+    } else if (!detail::base64_decode(text, pos, reinterpret_cast<std::uint8_t*>(&size), sizeof(size))) {
+      return false;
+    }
+    if (size != getSerializableSize()) {
+      throw std::invalid_argument("checkpoint " + fileName + " holds " + std::to_string(size) +
+                                  " bytes, this lattice " + std::to_string(getSerializableSize()));
+    }
+    std::vector<std::uint8_t> raw(size);
+    if (!detail::base64_decode(text, pos, raw.data(), raw.size())) return false;
+    return load(raw.data());
+  }
+
+  static void setLogOutDir(std::string dir) { logOutDir() = std::move(dir); }
+
  private:
+  static std::string& logOutDir()
+  {
+    static std::string dir = "./tmp/";
+    return dir;
+  }
+
+  template <bool includeLogOutputDir>
+  static std::string fullFileName(std::string fileName)
+  {
+    if (fileName.empty()) fileName = "Serializable";
+    return (includeLogOutputDir ? logOutDir() : std::string()) + detail::parallel_file_name(fileName);
+  }
+
+  int real() const { return _storage == LBMH_F64 ? LBMH_F64 : LBMH_F32; }
+
   struct Wall { int id; double rho; double u[3]; };
   struct Local { int id; int kind; };
 

@@ -10,6 +10,7 @@ cell.  The product's .dat codec must read and write OpenLB's text exactly.
 Where the reference binary is built, OpenLB itself must resume from a
 checkpoint that carries only the core cells.
 """
+import json
 import os
 import subprocess
 import sys
@@ -115,3 +116,50 @@ def test_openlb_resumes_from_core_only_checkpoint(name):
     role, _ = K.cell_roles(case)
     flow = np.isin(role, ["bulk", "velocity", "pressure"])
     np.testing.assert_array_equal(got[:, flow], arrays[f"f_{m}"][:, flow])
+
+
+LIBRARY = os.path.join(G.CHECKPOINTS, "library")
+
+
+def _library_names():
+    return sorted(p[:-4] for p in os.listdir(LIBRARY) if p.endswith(".npz")) if os.path.isdir(LIBRARY) else []
+
+
+def test_library_checkpoints_cover_every_fixture():
+    assert _library_names() == G.checkpoint_names()
+
+
+@pytest.mark.parametrize("name", _library_names())
+def test_openlb_resumes_from_library_checkpoint(name):
+    """OpenLB loads the checkpoint the GPU library wrote mid-run
+    (tests/golden/gen_library_checkpoints.py: OpenLB's state at step k, then
+    library steps to k') and continues to step m; it must reach its own
+    uninterrupted populations within the time-step bounds of the parity tests
+    (FP64 1e-12, FP32 2e-5 of max |f_i|, cells that carry the flow)."""
+    case, meta, arrays, _ = G.load_checkpoint(name)
+    k, m = meta["steps"]
+    real = meta["precision"]
+    z = np.load(os.path.join(LIBRARY, name + ".npz"), allow_pickle=False)
+    info = json.loads(str(z["meta"]))
+    start = info["step"]
+    assert info["precision"] == real and k < start < m
+    raw = CK.decode_dat(z["dat"].tobytes().decode("ascii"))
+    # the library's file has OpenLB's layout for that step
+    shifts, _, fields = K.unpack(case, raw, real)
+    np.testing.assert_array_equal(shifts, K.rotate_shifts(case, start))
+    for f in K.fields(case):
+        np.testing.assert_array_equal(fields[f[0]], K.field_values(case, f).astype(fields[f[0]].dtype))
+    exe = _reference(real)
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import gen_golden
+    with tempfile.TemporaryDirectory() as d:
+        gen_golden.write_case_dir(d, case, [m - start])
+        CK.write_file(os.path.join(d, CK.parallel_file_name("library")), raw)
+        subprocess.run([exe, d], check=True, cwd=d, stdout=subprocess.DEVNULL,
+                       env=dict(os.environ, OLB_REF_LOAD=os.path.join(d, "library")))
+        got = np.fromfile(os.path.join(d, f"pops_{m - start}.bin")).reshape((case.q,) + tuple(case.shape))
+    role, _ = K.cell_roles(case)
+    flow = np.isin(role, ["bulk", "velocity", "pressure"])
+    err = G.rel_error(got, arrays[f"f_{m}"], case.q, flow)
+    print(f"OpenLB resumed from the library's step-{start} checkpoint of {name}: rel {err:.2e}", flush=True)
+    assert err <= {"f64": 1e-12, "f32": 2e-5}[real], err

@@ -58,3 +58,36 @@ def test_cpp_layer_reports_errors_as_exceptions(gpu_available):
                          text=True, timeout=60)
     assert out.returncode == 1
     assert "omega must lie in (0, 2)" in out.stderr
+
+
+@pytest.mark.parametrize("variant", ["bounce_back", "local_velocity"])
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_cpp_save_and_load_resume_the_run(gpu_available, storage, variant, tmp_path):
+    """lattice.save(name) / lattice.load(name) of the C++ layer: OpenLB's file
+    <name>_rank0_size1.dat, whose contents have OpenLB's layout, and a run that
+    resumes from it ends where the uninterrupted run ends, bit for bit."""
+    from oracle import checkpoint as K
+    from openlb_amd import checkpoint as CK
+
+    size = 16
+    base = [EXE, "--size", str(size)] + (["--fp64"] if storage == "f64" else []) + \
+        (["--bounce-back"] if variant == "bounce_back" else [])
+
+    def run(*extra):
+        out = subprocess.run(base + list(extra), capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, out.stdout + out.stderr
+
+    name = str(tmp_path / "cavity")
+    run("--steps", "7", "--save", name, "--dump", str(tmp_path / "a.bin"))
+    run("--steps", "5", "--load", name, "--dump", str(tmp_path / "b.bin"))
+    run("--steps", "22", "--dump", str(tmp_path / "c.bin"))
+    b = np.fromfile(tmp_path / "b.bin")
+    c = np.fromfile(tmp_path / "c.bin")
+    np.testing.assert_array_equal(b, c)
+    case = W.cavity_benchmark(size) if variant == "bounce_back" else W.cavity_benchmark_local(size)
+    real = storage
+    raw = CK.read_file(name + "_rank0_size1.dat")
+    shifts, pops, _ = K.unpack(case, raw, real)
+    np.testing.assert_array_equal(shifts, K.rotate_shifts(case, 17))
+    a = np.fromfile(tmp_path / "a.bin").reshape((19,) + case.shape)
+    np.testing.assert_array_equal(K.core_of(case, pops).astype(np.float64), a)
