@@ -173,6 +173,32 @@ int lbmh_checkpoint_load(lbmh_lattice* lattice, int real, const void* buffer, si
  * evaluation after setProcessingContext(Evaluation). */
 int lbmh_moments(lbmh_lattice* lattice, double* rho_u);
 
+/* Moment fields for output, evaluated on the device and written to `host` as
+ * float in the point order of a VTK ImageData piece: x fastest, then y, then
+ * z, over [-overlap, n + overlap) per axis (overlap 0 or 1), all components of
+ * a point together -- the buffer OpenLB's SuperVTMwriter3D::dataArray fills
+ * before encoding it (src/io/superVtmWriter3D.hh:437-458).  Each point is what
+ * OpenLB's functor evaluates on that cell through the cell's own momenta:
+ *   LBMH_FIELD_DENSITY        SuperLatticeDensity3D        (1 component)
+ *   LBMH_FIELD_VELOCITY       SuperLatticeVelocity3D       (3)
+ *   LBMH_FIELD_PHYS_VELOCITY  SuperLatticePhysVelocity3D   (3), u * conversion[0]
+ *   LBMH_FIELD_PHYS_PRESSURE  SuperLatticePhysPressure3D   (1),
+ *                             (rho - 1)/3 * conversion[1] + conversion[2]
+ * (src/functors/lattice/latticeDensity3D.hh, latticeVelocity3D.hh,
+ * latticePhysVelocity3D.hh:84-96, latticePhysPressure3D.hh:71-78), with
+ * conversion = {UnitConverter velocity factor, pressure factor,
+ * charPhysPressure}, NULL for the lattice-unit fields.  Overlap points on a
+ * periodic axis are their periodic image (the writer communicates first);
+ * beyond a non-periodic face they have no dynamics.  `floats` must be the
+ * number of points times the components.  Only these floats cross PCIe, not
+ * the populations. */
+#define LBMH_FIELD_DENSITY 0
+#define LBMH_FIELD_VELOCITY 1
+#define LBMH_FIELD_PHYS_VELOCITY 2
+#define LBMH_FIELD_PHYS_PRESSURE 3
+int lbmh_field_export(lbmh_lattice* lattice, int field, int overlap, const double conversion[3], float* host,
+                      size_t floats);
+
 /* FP64 sums over the cells with dynamics id 1: out = {sum rho, sum j_x,
  * sum j_y, sum j_z} (j = rho u without forcing shift).  Replaces
  * SuperLatticeFieldReductionO / reductionFunctionDevice
@@ -332,6 +358,9 @@ int lbmh_ensemble_moments(lbmh_ensemble* ensemble, int member, double* rho_u);
  * member's populations and shifts; its step counters are left as they are.
  * With remove and add, this preempts a member to host memory and resumes it
  * later in any free slot. */
+/* lbmh_field_export for one member. */
+int lbmh_ensemble_field_export(lbmh_ensemble* ensemble, int member, int field, int overlap,
+                               const double conversion[3], float* host, size_t floats);
 int lbmh_ensemble_checkpoint_size(lbmh_ensemble* ensemble, int member, int real, size_t* bytes);
 int lbmh_ensemble_checkpoint_save(lbmh_ensemble* ensemble, int member, int real, void* buffer, size_t bytes);
 int lbmh_ensemble_checkpoint_load(lbmh_ensemble* ensemble, int member, int real, const void* buffer, size_t bytes);

@@ -23,7 +23,7 @@ EXPORTS = (
     "lbmh_set_bulk_params", "lbmh_set_wall", "lbmh_populations_upload",
     "lbmh_populations_download", "lbmh_ini_equilibrium", "lbmh_step",
     "lbmh_get_step_count", "lbmh_get_shifts", "lbmh_checkpoint_size", "lbmh_checkpoint_save",
-    "lbmh_checkpoint_load", "lbmh_moments", "lbmh_reduce_moments", "lbmh_statistics",
+    "lbmh_checkpoint_load", "lbmh_field_export", "lbmh_moments", "lbmh_reduce_moments", "lbmh_statistics",
     "lbmh_set_local_boundary", "lbmh_classify_boundary", "lbmh_initialize",
     "lbmh_collide", "lbmh_stream", "lbmh_halo_create", "lbmh_halo_destroy", "lbmh_halo_copy",
     "lbmh_synchronize", "lbmh_lattice_footprint", "lbmh_abi_version", "lbmh_device_count",
@@ -33,7 +33,7 @@ EXPORTS = (
     "lbmh_ensemble_remove", "lbmh_ensemble_set_wall", "lbmh_ensemble_set_local_boundary",
     "lbmh_ensemble_initialize", "lbmh_ensemble_set_steps", "lbmh_ensemble_status",
     "lbmh_ensemble_ini_equilibrium", "lbmh_ensemble_upload", "lbmh_ensemble_download",
-    "lbmh_ensemble_moments", "lbmh_ensemble_checkpoint_size", "lbmh_ensemble_checkpoint_save",
+    "lbmh_ensemble_moments", "lbmh_ensemble_field_export", "lbmh_ensemble_checkpoint_size", "lbmh_ensemble_checkpoint_save",
     "lbmh_ensemble_checkpoint_load", "lbmh_ensemble_step", "lbmh_ensemble_step_graph",
     "lbmh_ensemble_set_launch_config", "lbmh_ensemble_usage", "lbmh_ensemble_status_all",
 )
@@ -103,6 +103,8 @@ def load() -> ctypes.CDLL:
     L.lbmh_step.argtypes = [vp, i64, vp]
     L.lbmh_get_step_count.argtypes = [vp, ctypes.POINTER(i64)]
     L.lbmh_get_shifts.argtypes = [vp, vp]
+    L.lbmh_field_export.argtypes = [vp, i32, i32, vp, vp, ctypes.c_size_t]
+    L.lbmh_ensemble_field_export.argtypes = [vp, i32, i32, i32, vp, vp, ctypes.c_size_t]
     L.lbmh_checkpoint_size.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_size_t)]
     L.lbmh_checkpoint_save.argtypes = [vp, i32, vp, ctypes.c_size_t]
     L.lbmh_checkpoint_load.argtypes = [vp, i32, vp, ctypes.c_size_t]

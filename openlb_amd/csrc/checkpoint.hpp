@@ -23,8 +23,9 @@
 // A save writes them from the lattice's own parameters with the values
 // OpenLB's set-up gives them: defineRhoU's values on the core cells, the
 // value of the momenta's initialize() on overlap cells that are periodic
-// images of cells with that dynamics (RHO = 1, velocities 0), zero
-// elsewhere.  A load takes the populations of every cell, including the
+// images of cells with that dynamics (RHO = 1 for BounceBack and
+// BounceBackVelocity, which defineDynamics puts on the overlap too; the
+// boundary setters skip the outermost layer), zero elsewhere.  A load takes the populations of every cell, including the
 // overlap, and checks on the core cells each value that the block's dynamics
 // read against the lattice's parameters.
 #pragma once
@@ -232,9 +233,10 @@ void fill_field_column(const CheckpointBlock& b, const CheckpointField& f, int c
           bool read;
           v = field_value(b, f, b.ids[(size_t(x - 1) * b.n[1] + (y - 1)) * b.n[2] + (z - 1)], comp, read);
         } else if (f.which == 2) {
+          // defineDynamics covers the overlap, boundary::set does not (setBoundary.h:87-88)
           const int id = image_id(b, x, y, z);
           const int r = id < 0 ? kRoleNone : checkpoint_role(b, id);
-          if (r == kRoleBounceBack || r == kRoleMovingWall || r == kRolePressure) v = 1.0;
+          if (r == kRoleBounceBack || r == kRoleMovingWall) v = 1.0;
         }
         out[k] = T(v);
       }

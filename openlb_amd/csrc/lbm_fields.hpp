@@ -1,0 +1,93 @@
+// Moment fields of a block, evaluated on the device in the point order of a
+// VTK ImageData piece, for the output writers (OpenLB's SuperVTMwriter3D
+// buffer, src/io/superVtmWriter3D.hh:437-458: z outermost, x fastest, over
+// [-overlap, extent + overlap) per axis, every component of a point together,
+// converted to float).
+//
+// Each point reports what OpenLB's functor reads from the cell there:
+//   density        cell.computeRho()                 functors/lattice/latticeDensity3D.hh
+//   velocity       cell.computeU(u)                  latticeVelocity3D.hh
+//   physVelocity   converter.getPhysVelocity(u)      latticePhysVelocity3D.hh:84-96
+//   physPressure   converter.getPhysPressure((rho - 1) / invCs2)   latticePhysPressure3D.hh:71-78
+// through the cell's own momenta (cell_rho_u: bulk populations, fixed wall
+// values, the regularized boundaries' density and velocity); bulk cells of a
+// forced lattice report u + F/2 (forcing::Guo's momenta).
+//
+// Overlap points are what OpenLB's overlap cells hold after the writer's
+// communicate() (superVtmWriter3D.hh:71-75): on a periodic axis the image's
+// populations and FORCE, the descriptor's fields, which is the image's value
+// for a bulk cell; every other cell there reports rho 1, u 0 -- the wall
+// dynamics' own fields are not communicated and keep initialize()'s values
+// (FixedDensity 1, velocity 0), the boundary setters leave the outermost
+// layer without dynamics (setBoundary.h:87-88), and beyond a non-periodic
+// face there are no dynamics.
+#pragma once
+
+#include "lbm_local.hpp"
+
+namespace lbmh {
+
+enum ExportField : int { kFieldDensity = 0, kFieldVelocity = 1, kFieldPhysVelocity = 2, kFieldPhysPressure = 3 };
+
+inline int field_components(int field) { return (field == kFieldVelocity || field == kFieldPhysVelocity) ? 3 : 1; }
+
+struct FieldConversion {
+  double velocity;         // UnitConverter::getConversionFactorVelocity
+  double pressure;         // UnitConverter::getConversionFactorPressure
+  double pressure_offset;  // charPhysPressure
+};
+
+template <int Q, typename S>
+__global__ __launch_bounds__(256) void field_export_kernel(PostArgs<Q> a, int overlap, int field, FieldConversion conv,
+                                                           int forced, double hfx, double hfy, double hfz,
+                                                           float* __restrict__ out, uint32_t points)
+{
+  using V = typename StorageTraits<S>::V;
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= points) return;
+  const uint32_t ex = uint32_t(a.nx + 2 * overlap), ey = uint32_t(a.ny + 2 * overlap);
+  const uint32_t t = p / ex;
+  const int x = int(p - t * ex) - overlap, y = int(t % ey) - overlap, z = int(t / ey) - overlap;
+  int r[3] = {x, y, z};
+  const int n[3] = {a.nx, a.ny, a.nz};
+  bool inside = true, image = false;
+  for (int d = 0; d < 3; ++d) {
+    if (r[d] >= 0 && r[d] < n[d]) continue;
+    image = true;
+    if (!((a.periodic >> d) & 1u)) inside = false;
+    r[d] = (r[d] + n[d]) % n[d];
+  }
+  const int id = inside ? a.ids[(uint32_t(r[0]) * uint32_t(a.ny) + uint32_t(r[1])) * uint32_t(a.nz) + uint32_t(r[2])] : 0;
+  V rho = V(1), u[3] = {V(0), V(0), V(0)};
+  if (inside && (!image || id == 1)) cell_rho_u<Q, S, V>(a, r[0], r[1], r[2], rho, u);
+  if (forced && inside && id == 1) {
+    u[0] += V(hfx);
+    u[1] += V(hfy);
+    u[2] += V(hfz);
+  }
+  switch (field) {
+  case kFieldDensity:
+    out[p] = float(rho);
+    break;
+  case kFieldVelocity:
+    out[3 * uint64_t(p)] = float(u[0]);
+    out[3 * uint64_t(p) + 1] = float(u[1]);
+    out[3 * uint64_t(p) + 2] = float(u[2]);
+    break;
+  case kFieldPhysVelocity:
+    for (int d = 0; d < 3; ++d) out[3 * uint64_t(p) + d] = float(V(conv.velocity) * u[d]);
+    break;
+  default: {
+    // (rho - 1.0) / invCs2 is evaluated in double and stored in the value
+    // type; the conversion a*b+c is one fused multiply-add, as the compiled
+    // OpenLB functor evaluates it
+    const V lattice_pressure = V((double(rho) - 1.0) / 3.0);
+    if constexpr (sizeof(V) == 4)
+      out[p] = __builtin_fmaf(float(conv.pressure), lattice_pressure, float(conv.pressure_offset));
+    else
+      out[p] = float(__builtin_fma(conv.pressure, lattice_pressure, conv.pressure_offset));
+  }
+  }
+}
+
+}  // namespace lbmh

@@ -17,6 +17,7 @@
 
 #include "boundary_classify.hpp"
 #include "checkpoint.hpp"
+#include "lbm_fields.hpp"
 #include "lbm_kernels.hpp"
 #include "lbmh.h"
 #include "lbmh_common.hpp"
@@ -103,12 +104,11 @@ BulkParams<V> bulk_params(const lbmh_lattice* L)
   return derive_bulk_params<V>(L->q, L->params);
 }
 
-// PostStream stage of the local boundaries: outer edges (priority 0), then
-// outer corners (priority 1), on the populations after the shift.
-template <int Q, typename S>
-int launch_post_processors(lbmh_lattice* L, hipStream_t stream)
+// Whole-block arguments of the per-cell momenta evaluation (post-processors,
+// field export); `cells` and `count` are left for the caller.
+template <int Q>
+PostArgs<Q> post_args(const lbmh_lattice* L)
 {
-  if (L->h_edges.empty() && L->h_corners.empty() && L->h_planes.empty()) return LBMH_OK;
   PostArgs<Q> a;
   a.pop = L->d_pop;
   a.stride = L->stride;
@@ -117,6 +117,8 @@ int launch_post_processors(lbmh_lattice* L, hipStream_t stream)
   a.ids = L->d_ids;
   a.walls = L->d_walls;
   a.info = L->d_info;
+  a.cells = nullptr;
+  a.count = 0;
   a.nx = L->n[0];
   a.ny = L->n[1];
   a.nz = L->n[2];
@@ -126,6 +128,16 @@ int launch_post_processors(lbmh_lattice* L, hipStream_t stream)
   a.div_z = FastDiv(uint32_t(L->n[2]));
   a.div_y = FastDiv(uint32_t(L->n[1]));
   a.omega = L->params.omega;
+  return a;
+}
+
+// PostStream stage of the local boundaries: outer edges (priority 0), then
+// outer corners (priority 1), on the populations after the shift.
+template <int Q, typename S>
+int launch_post_processors(lbmh_lattice* L, hipStream_t stream)
+{
+  if (L->h_edges.empty() && L->h_corners.empty() && L->h_planes.empty()) return LBMH_OK;
+  PostArgs<Q> a = post_args<Q>(L);
   if (!L->h_planes.empty()) {
     using V = typename StorageTraits<S>::V;
     a.cells = L->d_planes;
@@ -813,6 +825,40 @@ int lbmh_checkpoint_load(lbmh_lattice* L, int real, const void* buffer, size_t b
   const int rc = checkpoint_load(checkpoint_block(L, no_ids), real, buffer, bytes, shift);
   if (rc != LBMH_OK) return rc;
   for (int i = 0; i < L->q; ++i) L->shift[i] = shift[i];
+  return LBMH_OK;
+}
+
+int lbmh_field_export(lbmh_lattice* L, int field, int overlap, const double conversion[3], float* host, size_t floats)
+{
+  if (!L || !host) return fail(LBMH_E_INVALID, "null argument");
+  if (field < LBMH_FIELD_DENSITY || field > LBMH_FIELD_PHYS_PRESSURE) return fail(LBMH_E_INVALID, "unknown field");
+  if (overlap < 0 || overlap > 1) return fail(LBMH_E_INVALID, "overlap must be 0 or 1 (the block's padding)");
+  if ((field == LBMH_FIELD_PHYS_VELOCITY || field == LBMH_FIELD_PHYS_PRESSURE) && !conversion)
+    return fail(LBMH_E_INVALID, "physical fields need the unit conversion factors");
+  const uint64_t points = uint64_t(L->n[0] + 2 * overlap) * (L->n[1] + 2 * overlap) * (L->n[2] + 2 * overlap);
+  const size_t want = size_t(points) * field_components(field);
+  if (floats != want)
+    return fail(LBMH_E_INVALID, "field buffer holds " + std::to_string(floats) + " floats, the export " +
+                                    std::to_string(want));
+  if (points > 0xffffffffull) return fail(LBMH_E_INVALID, "block too large for one export");
+  DeviceGuard guard(L->device);
+  DeviceBuffer out;
+  LBMH_HIP(hipMalloc(&out.p, want * sizeof(float)));
+  FieldConversion conv{0.0, 0.0, 0.0};
+  if (conversion) conv = FieldConversion{conversion[0], conversion[1], conversion[2]};
+  const int forced = (L->kind == LBMH_FORCED_BGK || L->kind == LBMH_FORCED_TRT) ? 1 : 0;
+  const double* F = L->params.force;
+  const int rc = with_types(L, [&](auto qc, auto s) -> int {
+    constexpr int Q = decltype(qc)::value;
+    using S = decltype(s);
+    hipLaunchKernelGGL((field_export_kernel<Q, S>), dim3(grid_for(uint32_t(points), 256)), dim3(256), 0, 0,
+                       post_args<Q>(L), overlap, field, conv, forced, F[0] * 0.5, F[1] * 0.5, F[2] * 0.5,
+                       static_cast<float*>(out.p), uint32_t(points));
+    LBMH_HIP(hipGetLastError());
+    return LBMH_OK;
+  });
+  if (rc != LBMH_OK) return rc;
+  LBMH_HIP(hipMemcpy(host, out.p, want * sizeof(float), hipMemcpyDeviceToHost));
   return LBMH_OK;
 }
 

@@ -17,6 +17,7 @@
 
 #include "boundary_classify.hpp"
 #include "checkpoint.hpp"
+#include "lbm_fields.hpp"
 #include "lbm_ensemble.hpp"
 #include "lbmh.h"
 #include "lbmh_common.hpp"
@@ -875,6 +876,65 @@ int lbmh_ensemble_checkpoint_load(lbmh_ensemble* E, int m, int real, const void*
     LBMH_HIP(hipMemcpy(dst, en.shift, sizeof(en.shift), hipMemcpyHostToDevice));
     return LBMH_OK;
   });
+}
+
+int lbmh_ensemble_field_export(lbmh_ensemble* E, int m, int field, int overlap, const double conversion[3],
+                               float* host, size_t floats)
+{
+  int rc = check_member(E, m);
+  if (rc != LBMH_OK) return rc;
+  if (!host) return fail(LBMH_E_INVALID, "null argument");
+  if (field < LBMH_FIELD_DENSITY || field > LBMH_FIELD_PHYS_PRESSURE) return fail(LBMH_E_INVALID, "unknown field");
+  if (overlap < 0 || overlap > 1) return fail(LBMH_E_INVALID, "overlap must be 0 or 1 (the block's padding)");
+  if ((field == LBMH_FIELD_PHYS_VELOCITY || field == LBMH_FIELD_PHYS_PRESSURE) && !conversion)
+    return fail(LBMH_E_INVALID, "physical fields need the unit conversion factors");
+  const Member& mem = E->members[m];
+  const uint64_t points = uint64_t(mem.n[0] + 2 * overlap) * (mem.n[1] + 2 * overlap) * (mem.n[2] + 2 * overlap);
+  const size_t want = size_t(points) * field_components(field);
+  if (floats != want)
+    return fail(LBMH_E_INVALID, "field buffer holds " + std::to_string(floats) + " floats, the export " +
+                                    std::to_string(want));
+  DeviceGuard guard(E->device);
+  rc = pull_table(E);
+  if (rc != LBMH_OK) return rc;
+  DeviceBuffer out;
+  LBMH_HIP(hipMalloc(&out.p, want * sizeof(float)));
+  FieldConversion conv{0.0, 0.0, 0.0};
+  if (conversion) conv = FieldConversion{conversion[0], conversion[1], conversion[2]};
+  const int forced = (E->kind == LBMH_FORCED_BGK || E->kind == LBMH_FORCED_TRT) ? 1 : 0;
+  const double* F = mem.params.force;
+  rc = with_types(E, [&](auto qc, auto s) -> int {
+    constexpr int Q = decltype(qc)::value;
+    using S = decltype(s);
+    const EnsembleEntry<Q>& en = entry<Q>(E, m);
+    PostArgs<Q> a;
+    a.pop = static_cast<S*>(E->d_pop) + en.pop_offset;
+    a.stride = en.stride;
+    a.n_padded = en.n_padded;
+    for (int i = 0; i < Q; ++i) a.shift[i] = en.shift[i];
+    a.ids = E->d_ids + en.ids_offset;
+    a.walls = E->d_walls + size_t(m) * kEnsembleWalls * 4;
+    a.info = E->d_info + size_t(m) * kEnsembleWalls;
+    a.cells = nullptr;
+    a.count = 0;
+    a.nx = int(en.nx);
+    a.ny = int(en.ny);
+    a.nz = int(en.nz);
+    a.sx = en.sx;
+    a.sy = en.sy;
+    a.periodic = en.periodic;
+    a.div_z = en.div_z;
+    a.div_y = en.div_y;
+    a.omega = en.bp.omega;
+    hipLaunchKernelGGL((field_export_kernel<Q, S>), dim3(grid_for(uint32_t(points), 256)), dim3(256), 0, 0, a,
+                       overlap, field, conv, forced, F[0] * 0.5, F[1] * 0.5, F[2] * 0.5,
+                       static_cast<float*>(out.p), uint32_t(points));
+    LBMH_HIP(hipGetLastError());
+    return LBMH_OK;
+  });
+  if (rc != LBMH_OK) return rc;
+  LBMH_HIP(hipMemcpy(host, out.p, want * sizeof(float), hipMemcpyDeviceToHost));
+  return LBMH_OK;
 }
 
 int lbmh_ensemble_moments(lbmh_ensemble* E, int m, double* rho_u)

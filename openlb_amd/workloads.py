@@ -195,19 +195,21 @@ def cavity_benchmark_local(size=100, q=19):
 
 
 def velocity_channel(shape=(24, 12, 10), u_in=0.03, rho_out=1.0, tau=0.6, q=19, inlet="velocity",
-                     outlet="pressure"):
+                     outlet="pressure", periodic_z=False):
     """Duct with a LocalVelocity inlet (x = 0, material 3, uniform u_in along x),
     a LocalPressure outlet (x = nx-1, material 4, density rho_out) and BounceBack
     walls (material 2) on the y and z faces, including the inlet's and outlet's
     rims.  Starts at rest.  inlet="interp_velocity" / outlet="interp_pressure"
-    select OpenLB's InterpolatedVelocity / InterpolatedPressure instead."""
+    select OpenLB's InterpolatedVelocity / InterpolatedPressure instead.  With
+    periodic_z the z faces are periodic instead of walls (a plane channel)."""
     ids = np.full(shape, 2, dtype=np.uint8)
-    ids[:, 1:-1, 1:-1] = 1
-    ids[0, 1:-1, 1:-1] = 3
-    ids[-1, 1:-1, 1:-1] = 4
+    zs = slice(None) if periodic_z else slice(1, -1)
+    ids[:, 1:-1, zs] = 1
+    ids[0, 1:-1, zs] = 3
+    ids[-1, 1:-1, zs] = 4
     rho_u = np.zeros((4,) + tuple(shape))
     rho_u[0] = 1.0
-    return Case("velocity_channel", q, "bgk", tuple(shape), (0, 0, 0), ids, rho_u, 1.0 / tau,
+    return Case("velocity_channel", q, "bgk", tuple(shape), (0, 0, int(periodic_z)), ids, rho_u, 1.0 / tau,
                 walls={3: (1.0, (u_in, 0.0, 0.0)), 4: (rho_out, (0.0, 0.0, 0.0))},
                 local={3: inlet, 4: outlet})
 

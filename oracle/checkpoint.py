@@ -167,8 +167,9 @@ def _image_roles(case, role):
 
 def field_values(case, field):
     """float64 [components][padded cells] of one field, as OpenLB's set-up
-    leaves it: defineRhoU's values on the core, initialize()'s on overlap cells
-    that are periodic images of cells with the dynamics, zero elsewhere."""
+    leaves it: defineRhoU's values on the core, initialize()'s (RHO = 1) on
+    overlap cells that are periodic images of BounceBack / BounceBackVelocity
+    cells, zero elsewhere."""
     key, comps, which, fo = field
     role, face = cell_roles(case)
     P = padded_shape(case)
@@ -206,7 +207,9 @@ def field_values(case, field):
         img = _image_roles(case, role)
         pad = np.ones(P, dtype=bool)
         pad[core] = False
-        mask = pad & np.isin(img, ["bounce_back", "moving_wall", "pressure"])
+        # defineDynamics covers the overlap; boundary::set skips cells of
+        # neighbourhood radius 0, the outermost layer (setBoundary.h:87-88)
+        mask = pad & np.isin(img, ["bounce_back", "moving_wall"])
         out[0][mask] = 1.0
     return out.reshape(comps, -1)
 

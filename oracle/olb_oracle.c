@@ -543,6 +543,17 @@ int olo_get_shifts(const olo_lattice* L, int64_t* shifts)
   return 0;
 }
 
+int olo_field_export(const olo_lattice* L, int field, int overlap, const double* conv, float* out)
+{
+  if (field < 0 || field > 3 || overlap < 0 || overlap > L->pad) { set_err("olo_field_export: field or overlap"); return -1; }
+  switch (L->storage) {
+  case OLO_F64: field_export_f64(L, field, overlap, conv, out); break;
+  case OLO_F32: field_export_f32(L, field, overlap, conv, out); break;
+  default: field_export_f16(L, field, overlap, conv, out); break;
+  }
+  return 0;
+}
+
 int olo_moments(const olo_lattice* L, double* rho_u)
 {
   switch (L->storage) {

@@ -73,6 +73,10 @@ int olo_step(olo_lattice* L, int64_t steps, int threads);
 int olo_get_shifts(const olo_lattice* L, int64_t* shifts);
 /* rho, ux, uy, uz of every core cell as the bulk momenta define them, [4][nx][ny][nz] */
 int olo_moments(const olo_lattice* L, double* rho_u);
+/* VTK point buffer of one functor (0 density, 1 velocity, 2 physVelocity,
+ * 3 physPressure; conv = {velocity factor, pressure factor, charPhysPressure})
+ * over [-overlap, n + overlap) per axis, x fastest, as float. */
+int olo_field_export(const olo_lattice* L, int field, int overlap, const double* conv, float* out);
 /* shifted equilibrium populations f_i - t_i for given rho/u, iniEquilibrium
  * semantics (forced kinds inverse-shift u by F/2), layout [q][n] */
 int olo_equilibrium(int q, int kind, int storage, int64_t n, const double* rho_u,

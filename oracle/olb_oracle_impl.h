@@ -433,6 +433,57 @@ static void FN(cell_rho_u)(const olo_lattice* L, int64_t n, V* rho, V* u)
   }
 }
 
+/* The float buffer SuperVTMwriter3D::dataArray fills (io/superVtmWriter3D.hh:
+ * 437-458): points z-outer, x-fastest over [-overlap, n + overlap), each the
+ * functor's value on that cell -- density / velocity / physVelocity /
+ * physPressure (functors/lattice/latticeDensity3D.hh, latticeVelocity3D.hh,
+ * latticePhysVelocity3D.hh:84-96, latticePhysPressure3D.hh:71-78).  A point
+ * on a periodic axis holds its image's populations and FORCE after the
+ * writer's communicate() (superVtmWriter3D.hh:71-75), so an image of a bulk
+ * cell reports the bulk cell's moments; any other overlap cell reports rho 1,
+ * u 0: wall dynamics keep their initialize() fields there (RHO 1, velocity
+ * 0; those fields are not communicated), boundary::set leaves the outermost
+ * layer without dynamics (setBoundary.h:87-88), and beyond a non-periodic
+ * face there are none. */
+static void FN(field_export)(const olo_lattice* L, int field, int overlap, const double* conv, float* out)
+{
+  const int p = L->pad;
+  int64_t k = 0;
+  for (int z = -overlap; z < L->n[2] + overlap; ++z)
+    for (int y = -overlap; y < L->n[1] + overlap; ++y)
+      for (int x = -overlap; x < L->n[0] + overlap; ++x, ++k) {
+        int r[3] = {x, y, z};
+        int inside = 1, image = 0;
+        for (int d = 0; d < 3; ++d) {
+          if (r[d] >= 0 && r[d] < L->n[d]) continue;
+          image = 1;
+          if (!L->periodic[d]) inside = 0;
+          else r[d] = (r[d] % L->n[d] + L->n[d]) % L->n[d];
+        }
+        V rho = (V)1, u[3] = {0, 0, 0};
+        if (inside) {
+          const int64_t n = (int64_t)(r[0] + p) * L->proj[0] + (int64_t)(r[1] + p) * L->proj[1] + r[2] + p;
+          if (!image || L->ids[n] == 1) FN(cell_rho_u)(L, n, &rho, u);
+        }
+        const int dim = (field == 1 || field == 2) ? 3 : 1;
+        float* o = out + k * dim;
+        if (field == 0) {
+          o[0] = (float)rho;
+        } else if (field == 1) {
+          for (int d = 0; d < 3; ++d) o[d] = (float)u[d];
+        } else if (field == 2) {
+          for (int d = 0; d < 3; ++d) o[d] = (float)((V)conv[0] * u[d]);
+        } else {
+          /* getPhysPressure's a*b+c is contracted to one fused multiply-add
+           * in the compiled reference (its FP32 output matches the fused
+           * form on every point, the separately rounded one not) */
+          const V lp = (V)(((double)rho - 1.0) / 3.0);
+          const V cp = (V)conv[1], p0 = (V)conv[2];
+          o[0] = (float)(sizeof(V) == 4 ? (V)fmaf((float)cp, (float)lp, (float)p0) : (V)fma(cp, lp, p0));
+        }
+      }
+}
+
 /* fd::boundaryGradient / centralGradient                  finiteDifference.h:33-44 */
 static V FN(bgrad)(V u0, V u1, V u2) { return (-(V)3 * u0 + (V)4 * u1 - (V)1 * u2) / (V)2; }
 

@@ -48,6 +48,7 @@ def lib():
         L.olo_step.argtypes = [vp, i64, i32]
         L.olo_get_shifts.argtypes = [vp, vp]
         L.olo_moments.argtypes = [vp, vp]
+        L.olo_field_export.argtypes = [vp, i32, i32, vp, vp]
         L.olo_equilibrium.argtypes = [i32, i32, i32, i64, vp, dbl, dbl, dbl, vp]
         L.olo_round_to_storage.argtypes = [i32, i64, vp]
         L.olo_define_local.argtypes = [vp, vp, i32, i32, dbl, dbl, dbl, dbl]
@@ -120,6 +121,17 @@ class OracleLattice:
     def moments(self) -> np.ndarray:
         out = np.empty((4,) + self.shape)
         lib().olo_moments(self._h, _ptr(out))
+        return out
+
+    def field_export(self, field: int, overlap: int = 1, conversion=None) -> np.ndarray:
+        """float32 VTK point buffer of one functor (0 density, 1 velocity,
+        2 physVelocity, 3 physPressure), as SuperVTMwriter3D fills it."""
+        dim = 3 if field in (1, 2) else 1
+        points = int(np.prod([n + 2 * overlap for n in self.shape]))
+        out = np.empty(points * dim, dtype=np.float32)
+        conv = np.zeros(3) if conversion is None else np.asarray(conversion, dtype=np.float64)
+        if lib().olo_field_export(self._h, int(field), int(overlap), _ptr(conv), _ptr(out)) != 0:
+            raise RuntimeError("olo_field_export failed")
         return out
 
     def initialize(self):

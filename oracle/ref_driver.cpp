@@ -24,6 +24,13 @@
 //   its "_rank0_size1.dat" suffix) the state after initialize() is replaced by
 //   that checkpoint (Serializable::load, then postLoad) and <step> counts the
 //   steps taken after the load.
+//   vtkData/fields.pvd, vtkData/data/fields_iT<step>.vtm and
+//   vtkData/data/fields_iT<step>iC00000.vti  (only with OLB_REF_VTK set): after
+//                    the last step, SuperVTMwriter3D<T> "fields" with the
+//                    functors density, velocity, physVelocity and physPressure
+//                    (io/superVtmWriter3D.hh; functors/lattice/), the physical
+//                    ones through UnitConverter(dx 0.01, dt 1e-4, L 0.1, U 1,
+//                    nu 1e-5, rho 1000, p0 100).
 //   normals.bin      int8 [4][nx][ny][nz] (only with OLB_REF_NORMALS set):
 //                    type and discrete normal that OpenLB's
 //                    computeBoundaryTypeAndNormal gives every core cell that
@@ -300,6 +307,7 @@ static void runCase(const CaseSpec& c, const std::string& dir)
   const bool stats = std::getenv("OLB_REF_STATS") != nullptr;
   if (stats) lattice.statisticsOn();
   const bool save = std::getenv("OLB_REF_SAVE") != nullptr;
+  const bool vtk = std::getenv("OLB_REF_VTK") != nullptr;
   if (const char* restart = std::getenv("OLB_REF_LOAD")) {
     if (!lattice.template load<false>(restart)) throw std::runtime_error(std::string("cannot load ") + restart);
   }
@@ -321,6 +329,21 @@ static void runCase(const CaseSpec& c, const std::string& dir)
                    double(st.getAverageEnergy()), double(st.getMaxU()));
       std::fclose(f);
     }
+  }
+  if (vtk) {
+    singleton::directories().setOutputDir(dir + "/");
+    const UnitConverter<T,DESCRIPTOR> converter(T(0.01), T(1e-4), T(0.1), T(1), T(1e-5), T(1000), T(100));
+    SuperVTMwriter3D<T> writer("fields");
+    SuperLatticeDensity3D<T,DESCRIPTOR> density(lattice);
+    SuperLatticeVelocity3D<T,DESCRIPTOR> velocity(lattice);
+    SuperLatticePhysVelocity3D<T,DESCRIPTOR> physVelocity(lattice, converter);
+    SuperLatticePhysPressure3D<T,DESCRIPTOR> physPressure(lattice, converter);
+    writer.addFunctor(density);
+    writer.addFunctor(velocity);
+    writer.addFunctor(physVelocity);
+    writer.addFunctor(physPressure);
+    writer.createMasterFile();
+    writer.write(int(done));
   }
 }
 

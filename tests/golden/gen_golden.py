@@ -228,13 +228,16 @@ def checkpoint_cases():
          W.velocity_channel((12, 8, 6), u_in=0.03, rho_out=1.0, tau=0.6, inlet="interp_velocity",
                             outlet="interp_pressure"), 5, 25, "f64"),
         ("cavity27_local_velocity_f64", W.local_velocity_cavity((8, 7, 6), tau=0.7, lid_u=0.1, q=27), 5, 25, "f64"),
+        ("channel19_velocity_pressure_zperiodic_f64",
+         W.velocity_channel((10, 7, 6), u_in=0.03, rho_out=1.0, tau=0.6, periodic_z=True), 5, 25, "f64"),
     ]
 
 
 def write_checkpoints():
     """checkpoints/<name>.npz: OpenLB's own checkpoint file of the lattice after
-    k steps (Serializable::save, the .dat text as bytes), and its populations
-    after k and m steps.  The k-step file is what a checkpoint of the library
+    k steps (Serializable::save, the .dat text as bytes), its populations
+    after k and m steps, and the files its SuperVTMwriter3D writes for the
+    k-step state (pvd, vtm, vti; density, velocity, physVelocity, physPressure).  The k-step file is what a checkpoint of the library
     must reproduce; the m-step populations are what both sides reach when
     they resume from it."""
     out_dir = os.path.join(HERE, "checkpoints")
@@ -250,10 +253,20 @@ def write_checkpoints():
                 dat = np.frombuffer(fh.read(), dtype=np.uint8)
             pops = {s: np.fromfile(os.path.join(d, f"pops_{s}.bin")).reshape((case.q,) + tuple(case.shape))
                     for s in (k, m)}
+        # OpenLB's VTK output of the same k-step state (SuperVTMwriter3D "fields")
+        with tempfile.TemporaryDirectory() as d:
+            write_case_dir(d, case, [k])
+            subprocess.run([exe, d], check=True, cwd=d, stdout=subprocess.DEVNULL,
+                           env=dict(os.environ, OLB_REF_VTK="1"))
+            vtk = {}
+            for key, rel in (("pvd", "vtkData/fields.pvd"), ("vtm", f"vtkData/data/fields_iT{k:07d}.vtm"),
+                             ("vti", f"vtkData/data/fields_iT{k:07d}iC00000.vti")):
+                with open(os.path.join(d, rel), "rb") as fh:
+                    vtk[key] = np.frombuffer(fh.read(), dtype=np.uint8)
         meta = meta_of(case, [k, m], precision)
         meta["checkpoint_step"] = k
         arrays = {"ids": case.ids.astype(np.uint8), "rho_u": case.rho_u, "dat": dat,
-                  f"f_{k}": pops[k], f"f_{m}": pops[m]}
+                  f"f_{k}": pops[k], f"f_{m}": pops[m], **vtk}
         np.savez_compressed(os.path.join(out_dir, name + ".npz"), meta=np.array(json.dumps(meta)), **arrays)
         meta["sha256"] = {a: hashlib.sha256(np.ascontiguousarray(v).tobytes()).hexdigest()
                           for a, v in arrays.items()}

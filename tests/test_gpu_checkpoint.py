@@ -7,7 +7,10 @@ for byte.
 
 * The library resumes from OpenLB's checkpoint after k steps and reaches
   OpenLB's populations after m steps, within the time-step bounds of the
-  parity tests (FP64 1e-12, FP32 2e-5 of max |f_i|, cells that carry the flow).
+  parity tests (FP64 1e-12, FP32 2e-5 of max |f_i|, populations of the cells
+  that carry the flow, except those that stream in from beyond a
+  non-periodic face: the overlap leftovers there depend on how each side
+  fills its overlap and are discarded by every dynamics).
 * A checkpoint the library loads and saves again is the same bytes.
 * A checkpoint the library writes after further steps has OpenLB's shifts
   for that step count, the library's own core populations in rotated order,
@@ -35,9 +38,12 @@ pytestmark = pytest.mark.gpu
 TOL = {"f64": 1e-12, "f32": 2e-5}
 
 
-def flow_cells(case):
+def flow_populations(case):
+    """The populations of the cells that carry the flow (bulk and boundary
+    cells), without the slots that stream in from beyond a non-periodic face."""
     role, _ = K.cell_roles(case)
-    return np.isin(role, ["bulk", "velocity", "pressure"])
+    flow = np.isin(role, ["bulk", "velocity", "pressure"])
+    return flow[None] & ~G.streamed_from_outside(case)
 
 
 @pytest.mark.parametrize("name", G.checkpoint_names())
@@ -53,7 +59,7 @@ def test_resume_from_openlb_checkpoint(gpu_available, name):
         np.testing.assert_array_equal(lat.download_populations("f64"), arrays[f"f_{k}"])
         lat.collide_and_stream(m - k)
         got = lat.download_populations("f64")
-    err = G.rel_error(got, arrays[f"f_{m}"], case.q, flow_cells(case))
+    err = G.rel_error(got, arrays[f"f_{m}"], case.q, flow_populations(case))
     assert err <= TOL[real], err
 
 

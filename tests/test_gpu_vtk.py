@@ -1,0 +1,138 @@
+"""GPU tests of the device field export (lbmh_field_export,
+lbmh_ensemble_field_export) and the VTK writer on top of it.
+
+The checkers are OpenLB's own VTK files of the checkpoint fixtures' k-step
+states and the oracle's functor restatement (olo_field_export), which
+test_vtk_cpu.py pins to those files bit for bit.  The library restores
+OpenLB's k-step checkpoint, so both sides evaluate the same populations; the
+float arrays must agree within one float rounding of the value or 1e-7 of the
+field's largest magnitude (the device may sum the populations in another
+order before the cast to float; FP64 lattices agree exactly in practice).
+"""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import golden_cases as G
+from oracle import oracle as O
+from openlb_amd import abi
+from openlb_amd import checkpoint as CK
+from openlb_amd import vtk as V
+from openlb_amd import workloads as W
+from openlb_amd.ensemble import Ensemble
+from openlb_amd.lattice import BlockLattice
+
+pytestmark = pytest.mark.gpu
+
+FUNCTORS = ["density", "velocity", "physVelocity", "physPressure"]
+
+
+def openlb_arrays(name):
+    files = G.load_vtk(name)
+    with tempfile.NamedTemporaryFile("w", suffix=".vti", delete=False) as fh:
+        fh.write(files["vti"])
+        path = fh.name
+    try:
+        return V.read_vti_arrays(path)
+    finally:
+        os.unlink(path)
+
+
+def converter(real):
+    return V.UnitConverter(0.01, 1e-4, 0.1, 1.0, 1e-5, 1000.0, 100.0, real)
+
+
+def assert_close(got, ref, what):
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    tol = np.maximum(np.abs(ref) * 2.0 ** -23, 1e-7 * np.abs(ref).max())
+    bad = np.abs(got - ref) > tol
+    assert not bad.any(), (what, int(bad.sum()), float(np.abs(got - ref).max()))
+    return int((got != ref).sum())
+
+
+def functors(lat, real, member=None):
+    conv = converter(real)
+    return [V.SuperLatticeDensity3D(lat, member=member), V.SuperLatticeVelocity3D(lat, member=member),
+            V.SuperLatticePhysVelocity3D(lat, conv, member=member),
+            V.SuperLatticePhysPressure3D(lat, conv, member=member)]
+
+
+@pytest.mark.parametrize("name", G.checkpoint_names())
+def test_device_export_matches_openlb_vtk(gpu_available, name, tmp_path):
+    case, meta, _, text = G.load_checkpoint(name)
+    k, real = meta["checkpoint_step"], meta["precision"]
+    ref = openlb_arrays(name)
+    files = G.load_vtk(name)
+    with BlockLattice.from_case(case, real) as lat:
+        lat.restore(CK.decode_dat(text), real)
+        fs = functors(lat, real)
+        diffs = {key: assert_close(f.evaluate(1), ref[key], key) for key, f in zip(FUNCTORS, fs)}
+        w = V.SuperVTMwriter3D("fields", out_dir=str(tmp_path / "vtkData"))
+        for f in fs:
+            w.add_functor(f)
+        w.create_master_file()
+        w.write(k)
+    print(f"{name}: points differing from OpenLB's floats {diffs}", flush=True)
+    out = tmp_path / "vtkData"
+    assert (out / "fields.pvd").read_text() == files["pvd"]
+    assert (out / "data" / f"fields_iT{k:07d}.vtm").read_text() == files["vtm"]
+    vti = (out / "data" / f"fields_iT{k:07d}iC00000.vti").read_text()
+    head = [ln for ln in vti.splitlines() if ln.startswith("<")]
+    assert head == [ln for ln in files["vti"].splitlines() if ln.startswith("<")]
+    got = V.read_vti_arrays(str(out / "data" / f"fields_iT{k:07d}iC00000.vti"))
+    for key in FUNCTORS:
+        assert_close(got[key], ref[key], key)
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16"])
+def test_device_export_matches_oracle(gpu_available, storage):
+    """A forced Couette channel (periodic overlap, moving wall) and a local-velocity cavity, mid-run."""
+    for case in (W.couette_channel((9, 7, 6), tau=0.7, wall_u=0.05, force=1e-5),
+                 W.local_velocity_cavity((9, 8, 7), tau=0.7, lid_u=0.06)):
+        f0 = O.initial_populations(case, storage)
+        with BlockLattice.from_case(case, storage) as lat:
+            lat.upload_populations(f0, "f64")
+            lat.initialize()
+            lat.collide_and_stream(11)
+            state = lat.download_populations("f64")
+            real = "f64" if storage == "f64" else "f32"
+            conv = converter(real).factors()
+            ora = O.OracleLattice(case, storage)
+            ora.upload(state)
+            for field, f in enumerate(functors(lat, real)):
+                for overlap in (0, 1):
+                    assert_close(f.evaluate(overlap), ora.field_export(field, overlap, conv), (storage, field, overlap))
+
+
+def test_ensemble_member_export_equals_lattice_export(gpu_available):
+    case = W.couette_channel((9, 7, 6), tau=0.7, wall_u=0.05, force=0.0, kind="bgk")
+    f0 = O.initial_populations(case, "f32")
+    with Ensemble(19, "f32", "bgk", max_members=4, population_bytes=8 << 20, cell_capacity=1 << 16) as ens:
+        other = ens.add(W.lid_driven_cavity((6, 5, 7), tau=0.6, lid_u=0.05))
+        m = ens.add(case)
+        ens.upload(m, f0)
+        ens.upload(other, O.initial_populations(W.lid_driven_cavity((6, 5, 7), tau=0.6, lid_u=0.05), "f32"))
+        ens.step(9)
+        with BlockLattice.from_case(case, "f32") as lat:
+            lat.upload_populations(f0, "f64")
+            lat.collide_and_stream(9)
+            for a, b in zip(functors(ens, "f32", member=m), functors(lat, "f32")):
+                np.testing.assert_array_equal(a.evaluate(1), b.evaluate(1))
+
+
+def test_export_argument_errors(gpu_available):
+    case = W.taylor_green((6, 5, 4))
+    with BlockLattice.from_case(case, "f32") as lat:
+        out = np.empty(8 * 7 * 6, dtype=np.float32)
+        L = abi.load()
+        ptr = out.ctypes.data_as(abi.ctypes.c_void_p)
+        with pytest.raises(abi.LbmhError, match="floats"):
+            abi.check(L.lbmh_field_export(lat.handle, 1, 1, None, ptr, out.size))
+        with pytest.raises(abi.LbmhError, match="overlap"):
+            abi.check(L.lbmh_field_export(lat.handle, 0, 2, None, ptr, out.size))
+        with pytest.raises(abi.LbmhError, match="conversion"):
+            abi.check(L.lbmh_field_export(lat.handle, 3, 1, None, ptr, out.size))
+        abi.check(L.lbmh_field_export(lat.handle, 0, 1, None, ptr, out.size))
