@@ -295,6 +295,26 @@ def serving_bench(count=400, rate=200.0, budget_gb=200.0, tick_steps=64):
     return out
 
 
+def export_bench(lat):
+    """Result output without the populations: the VTK buffers of density and
+    velocity (lbmh_field_export over the block plus one overlap layer,
+    evaluated on the device) against a download of all populations."""
+    from openlb_amd import vtk
+    fs = [vtk.SuperLatticeDensity3D(lat), vtk.SuperLatticeVelocity3D(lat)]
+    for f in fs:
+        f.evaluate(1)
+    lat.synchronize()
+    t0 = time.perf_counter()
+    nbytes = sum(f.evaluate(1).nbytes for f in fs)
+    t_export = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pops = lat.download_populations(lat.storage)
+    t_pop = time.perf_counter() - t0
+    return {"fields": "density+velocity (float32, VTK point order, overlap 1)",
+            "field_bytes": int(nbytes), "field_export_s": round(t_export, 4),
+            "population_bytes": int(pops.nbytes), "population_download_s": round(t_pop, 4)}
+
+
 def measured_traffic(kernel_name, case, launch):
     """HBM bytes per launch of this kernel, workload and launch shape from the
     committed counter runs (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE scaled
@@ -447,6 +467,8 @@ def main():
     kernel_ms = [ev.elapsed_ms(marks[2 * k], marks[2 * k + 1]) for k in range(args.steps)]
     ev.destroy(marks)
     moments = lat.reduce_moments()
+    export = export_bench(lat)
+    log(f"export: density+velocity {export['field_export_s']} s vs populations {export['population_download_s']} s")
     lat.close()
     if not np.isfinite(moments).all():
         raise SystemExit("non-finite populations after the timed steps")
@@ -529,6 +551,7 @@ def main():
         "ensemble": ensemble,
         "cavity_benchmark": cavity,
         "serving": served,
+        "result_export": export,
     }
     print(json.dumps(out), flush=True)
 

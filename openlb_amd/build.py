@@ -55,7 +55,7 @@ def build_examples(verbose: bool = False):
         out = os.path.join(LIB_DIR, name)
         _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}", f"-I{HOST_DIR}",
               src, "-o", out + ".tmp", f"-L{LIB_DIR}", "-llbmh", "-Wl,-rpath,$ORIGIN",
-              "-L/opt/rocm/lib", "-Wl,-rpath-link,/opt/rocm/lib"], verbose)
+              "-L/opt/rocm/lib", "-Wl,-rpath-link,/opt/rocm/lib", "-lz"], verbose)
         os.replace(out + ".tmp", out)
 
 

@@ -16,7 +16,10 @@
 // run, double [q][nx][ny][nz]), --tau (override), --stats (lattice statistics
 // after the run), --load <name> (resume from OpenLB checkpoint
 // <name>_rank0_size1.dat right after initialize(), as lattice.load(name)),
-// --save <name> (write that checkpoint after the run, as lattice.save(name)).
+// --save <name> (write that checkpoint after the run, as lattice.save(name)),
+// --vtk <dir> (after the run, SuperVTMwriter3D "cavity3d" into <dir> with
+// physVelocity and physPressure, as cavity3d.cpp:90-111 writes them, through
+// UnitConverter(1/size, 0.01/size, 1, 1, 1e-3, 1)).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -30,7 +33,7 @@ using namespace olb::hip;
 
 template <typename T>
 int run(int size, long steps, double tau, const char* dump, bool stats, bool bounceBack, const char* load,
-        const char* save)
+        const char* save, const char* vtk)
 {
   using DESCRIPTOR = descriptors::D3Q19;
   const int n = size + 1;
@@ -71,6 +74,24 @@ int run(int size, long steps, double tau, const char* dump, bool stats, bool bou
 
   lattice.setProcessingContext(ProcessingContext::Evaluation);
   if (save && !lattice.template save<false>(save)) throw std::runtime_error(std::string("cannot write checkpoint ") + save);
+  if (vtk) {
+    // UnitConverterFromResolutionAndLatticeVelocity(N, 0.01, 1, 1, 1e-3, 1): dx = 1/N, dt = 0.01 dx
+    const T dx = T(1) / T(size), dt = T(0.01) * dx;
+    const UnitConverter<T, DESCRIPTOR> converter(dx, dt, T(1), T(1), T(1e-3), T(1));
+    SuperVTMwriter3D<T>::setVtkOutDir(std::string(vtk) + "/");
+    SuperVTMwriter3D<T> writer("cavity3d");
+    writer.setDeltaR(converter.getPhysDeltaX());
+    SuperLatticePhysVelocity3D<T, DESCRIPTOR> velocity(lattice, converter);
+    SuperLatticePhysPressure3D<T, DESCRIPTOR> pressure(lattice, converter);
+    writer.addFunctor(velocity);
+    writer.addFunctor(pressure);
+    writer.createMasterFile();
+    writer.write(int(10 + steps));
+    std::printf("vtk %.17g %.17g %.17g %.17g\n", double(converter.getConversionFactorVelocity()),
+                double(converter.getConversionFactorPressure()), double(converter.getCharPhysPressure()),
+                double(converter.getPhysDeltaX()));
+    std::fflush(stdout);
+  }
   std::cout << size << ", " << steps << ", " << 1 << ", " << 1 << ", " << 1 << ", " << mlups << std::endl;
   if (stats) {
     const auto st = lattice.getStatistics();
@@ -98,6 +119,7 @@ int main(int argc, char** argv)
   bool bounceBack = false;
   const char* load = nullptr;
   const char* save = nullptr;
+  const char* vtk = nullptr;
   double tau = -1;
   for (int i = 1; i < argc; ++i) {
     if (!std::strcmp(argv[i], "--size") && i + 1 < argc) size = std::atoi(argv[++i]);
@@ -109,16 +131,17 @@ int main(int argc, char** argv)
     else if (!std::strcmp(argv[i], "--bounce-back")) bounceBack = true;
     else if (!std::strcmp(argv[i], "--load") && i + 1 < argc) load = argv[++i];
     else if (!std::strcmp(argv[i], "--save") && i + 1 < argc) save = argv[++i];
+    else if (!std::strcmp(argv[i], "--vtk") && i + 1 < argc) vtk = argv[++i];
     else if (!std::strcmp(argv[i], "--no-results")) continue;
     else {
-      std::fprintf(stderr, "usage: %s [--size N] [--steps K] [--fp64] [--tau T] [--dump FILE] [--stats] [--bounce-back] [--load NAME] [--save NAME]\n", argv[0]);
+      std::fprintf(stderr, "usage: %s [--size N] [--steps K] [--fp64] [--tau T] [--dump FILE] [--stats] [--bounce-back] [--load NAME] [--save NAME] [--vtk DIR]\n", argv[0]);
       return 2;
     }
   }
   if (tau <= 0) tau = 0.5 + 3.0 * 1e-5 * size;  // nu = nu_phys * dt / dx^2 with u_lattice 0.01
   try {
-    return fp64 ? run<double>(size, steps, tau, dump, stats, bounceBack, load, save)
-                : run<float>(size, steps, tau, dump, stats, bounceBack, load, save);
+    return fp64 ? run<double>(size, steps, tau, dump, stats, bounceBack, load, save, vtk)
+                : run<float>(size, steps, tau, dump, stats, bounceBack, load, save, vtk);
   } catch (const std::exception& e) {
     std::fprintf(stderr, "error: %s\n", e.what());
     return 1;

@@ -14,6 +14,8 @@
 //   boundary::set<Local|InterpolatedVelocity/Pressure>   boundary/setBoundary.h:83-101
 //   getStatistics()                                      superLattice.h:378
 //   getSerializableSize / save / load (buffer or file)   core/serializer.h:145-189
+//   UnitConverter, SuperLatticeDensity3D / Velocity3D /
+//   PhysVelocity3D / PhysPressure3D, SuperVTMwriter3D    core/unitConverter.h, functors/lattice/, io/superVtmWriter3D.h
 //
 // Dynamics are tags carrying the C ABI's bulk kind; parameters are tags of
 // the same names as OpenLB's fields.  HIP errors surface as std::runtime_error
@@ -21,12 +23,17 @@
 // invalid requests as std::invalid_argument / std::logic_error.
 #pragma once
 
+#include <zlib.h>
+
 #include <array>
 #include <cctype>
+#include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <fstream>
 #include <functional>
 #include <iterator>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -528,6 +535,262 @@ void set(LATTICE& lattice, const Indicator& indicator)
   lattice.template setLocalBoundary<BC>(indicator);
 }
 }  // namespace boundary
+
+/// UnitConverter(physDeltaX, physDeltaT, charPhysLength, charPhysVelocity,
+/// physViscosity, physDensity, charPhysPressure)       src/core/unitConverter.h:152-173
+template <typename T, typename DESCRIPTOR>
+class UnitConverter {
+ public:
+  UnitConverter(T physDeltaX, T physDeltaT, T charPhysLength, T charPhysVelocity, T physViscosity, T physDensity,
+                T charPhysPressure = 0)
+      : _conversionLength(physDeltaX), _conversionTime(physDeltaT),
+        _conversionVelocity(_conversionLength / _conversionTime), _conversionDensity(physDensity),
+        _conversionMass(_conversionDensity * std::pow(_conversionLength, 3)),
+        _conversionViscosity(_conversionLength * _conversionLength / _conversionTime),
+        _conversionForce(_conversionMass * _conversionLength / (_conversionTime * _conversionTime)),
+        _conversionPressure(_conversionForce / std::pow(_conversionLength, 2)), _charPhysLength(charPhysLength),
+        _charPhysVelocity(charPhysVelocity), _physViscosity(physViscosity), _charPhysPressure(charPhysPressure),
+        _latticeRelaxationTime(_physViscosity / _conversionViscosity * T(3) + T(0.5))
+  {
+  }
+  T getConversionFactorLength() const { return _conversionLength; }
+  T getPhysDeltaX() const { return _conversionLength; }
+  T getConversionFactorVelocity() const { return _conversionVelocity; }
+  T getConversionFactorPressure() const { return _conversionPressure; }
+  T getCharPhysPressure() const { return _charPhysPressure; }
+  T getCharLatticeVelocity() const { return _charPhysVelocity / _conversionVelocity; }
+  T getLatticeRelaxationTime() const { return _latticeRelaxationTime; }
+  T getLatticeRelaxationFrequency() const { return T(1) / _latticeRelaxationTime; }
+  T getPhysVelocity(T latticeVelocity) const { return _conversionVelocity * latticeVelocity; }
+  T getPhysPressure(T latticePressure) const { return _conversionPressure * latticePressure + _charPhysPressure; }
+
+ private:
+  T _conversionLength, _conversionTime, _conversionVelocity, _conversionDensity, _conversionMass;
+  T _conversionViscosity, _conversionForce, _conversionPressure, _charPhysLength, _charPhysVelocity;
+  T _physViscosity, _charPhysPressure, _latticeRelaxationTime;
+};
+
+namespace detail {
+/// A lattice functor the VTK writer evaluates on the device (lbmh_field_export).
+template <typename T, typename DESCRIPTOR>
+class LatticeFieldF {
+ public:
+  LatticeFieldF(SuperLattice<T, DESCRIPTOR>& lattice, int field, std::string name, const double* conversion)
+      : _lattice(lattice), _field(field), _name(std::move(name))
+  {
+    if (conversion)
+      for (int k = 0; k < 3; ++k) _conversion[k] = conversion[k];
+    _hasConversion = conversion != nullptr;
+  }
+  virtual ~LatticeFieldF() = default;
+  std::string& getName() { return _name; }
+  int getTargetDim() const { return (_field == LBMH_FIELD_VELOCITY || _field == LBMH_FIELD_PHYS_VELOCITY) ? 3 : 1; }
+  const std::array<int, 3>& extent() const { return _lattice.extent(); }
+
+  /// The writer's float buffer over [-overlap, extent + overlap), x fastest.
+  std::vector<float> evaluate(int overlap)
+  {
+    if (!_lattice.handle()) _lattice.initialize();
+    const auto& n = _lattice.extent();
+    const std::size_t points = std::size_t(n[0] + 2 * overlap) * (n[1] + 2 * overlap) * (n[2] + 2 * overlap);
+    std::vector<float> out(points * std::size_t(getTargetDim()));
+    check(lbmh_field_export(_lattice.handle(), _field, overlap, _hasConversion ? _conversion : nullptr, out.data(),
+                            out.size()));
+    return out;
+  }
+
+ private:
+  SuperLattice<T, DESCRIPTOR>& _lattice;
+  int _field;
+  std::string _name;
+  double _conversion[3] = {0, 0, 0};
+  bool _hasConversion = false;
+};
+
+template <typename T, typename DESCRIPTOR>
+std::array<double, 3> conversion_of(const UnitConverter<T, DESCRIPTOR>& c)
+{
+  return {double(c.getConversionFactorVelocity()), double(c.getConversionFactorPressure()),
+          double(c.getCharPhysPressure())};
+}
+}  // namespace detail
+
+/// SuperLatticeDensity3D (src/functors/lattice/latticeDensity3D.hh)
+template <typename T, typename DESCRIPTOR>
+class SuperLatticeDensity3D : public detail::LatticeFieldF<T, DESCRIPTOR> {
+ public:
+  explicit SuperLatticeDensity3D(SuperLattice<T, DESCRIPTOR>& l)
+      : detail::LatticeFieldF<T, DESCRIPTOR>(l, LBMH_FIELD_DENSITY, "density", nullptr) {}
+};
+
+/// SuperLatticeVelocity3D (latticeVelocity3D.hh)
+template <typename T, typename DESCRIPTOR>
+class SuperLatticeVelocity3D : public detail::LatticeFieldF<T, DESCRIPTOR> {
+ public:
+  explicit SuperLatticeVelocity3D(SuperLattice<T, DESCRIPTOR>& l)
+      : detail::LatticeFieldF<T, DESCRIPTOR>(l, LBMH_FIELD_VELOCITY, "velocity", nullptr) {}
+};
+
+/// SuperLatticePhysVelocity3D (latticePhysVelocity3D.hh:84-96)
+template <typename T, typename DESCRIPTOR>
+class SuperLatticePhysVelocity3D : public detail::LatticeFieldF<T, DESCRIPTOR> {
+ public:
+  SuperLatticePhysVelocity3D(SuperLattice<T, DESCRIPTOR>& l, const UnitConverter<T, DESCRIPTOR>& c)
+      : detail::LatticeFieldF<T, DESCRIPTOR>(l, LBMH_FIELD_PHYS_VELOCITY, "physVelocity",
+                                             detail::conversion_of(c).data()) {}
+};
+
+/// SuperLatticePhysPressure3D (latticePhysPressure3D.hh:71-78)
+template <typename T, typename DESCRIPTOR>
+class SuperLatticePhysPressure3D : public detail::LatticeFieldF<T, DESCRIPTOR> {
+ public:
+  SuperLatticePhysPressure3D(SuperLattice<T, DESCRIPTOR>& l, const UnitConverter<T, DESCRIPTOR>& c)
+      : detail::LatticeFieldF<T, DESCRIPTOR>(l, LBMH_FIELD_PHYS_PRESSURE, "physPressure",
+                                             detail::conversion_of(c).data()) {}
+};
+
+/// SuperVTMwriter3D<T> (src/io/superVtmWriter3D.h:50-144) for a one-block
+/// lattice: <vtkOutDir><name>.pvd, data/<name>_iT<7>.vtm and
+/// data/<name>_iT<7>iC00000.vti with one Float32 DataArray per functor,
+/// base64 after a uint32 size prefix, zlib-compressed by default.  The
+/// piece's Origin and Spacing are the cuboid's (setOrigin / setDeltaR;
+/// 0 and 1 by default, a cuboid decomposition of spacing 1).
+template <typename T, typename OUT_T = float>
+class SuperVTMwriter3D {
+  static_assert(std::is_same_v<OUT_T, float>, "OpenLB writes its VTK buffers as float");
+
+ public:
+  SuperVTMwriter3D(const std::string& name, int overlap = 1, bool binary = true, bool compress = true)
+      : _name(name), _overlap(overlap), _binary(binary), _compress(compress)
+  {
+  }
+
+  /// Any of the lattice functors above (of either descriptor).
+  template <typename F>
+  void addFunctor(F& f)
+  {
+    _functors.push_back(Entry{[&f]() -> std::string& { return f.getName(); }, f.getTargetDim(),
+                              [&f]() { return f.extent(); }, [&f](int overlap) { return f.evaluate(overlap); }});
+  }
+  template <typename F>
+  void addFunctor(F& f, const std::string& name)
+  {
+    f.getName() = name;
+    addFunctor(f);
+  }
+  void clearAddedFunctors() { _functors.clear(); }
+  std::string getName() const { return _name; }
+  void setOrigin(std::array<T, 3> origin) { _origin = origin; }
+  void setDeltaR(T delta) { _delta = delta; }
+  /// singleton::directories().setVtkOutDir: the directory and its data/ must exist
+  static void setVtkOutDir(std::string dir) { vtkOutDir() = std::move(dir); }
+
+  void createMasterFile()
+  {
+    std::ofstream f(vtkOutDir() + _name + ".pvd", std::ios::trunc);
+    f << "<?xml version=\"1.0\"?>\n<VTKFile type=\"Collection\" version=\"0.1\" byte_order=\"LittleEndian\">\n"
+      << "<Collection>\n</Collection>\n</VTKFile>\n";
+    if (!f) throw std::runtime_error("cannot write " + vtkOutDir() + _name + ".pvd");
+  }
+
+  void write(int iT)
+  {
+    if (_functors.empty()) throw std::runtime_error("No functor to write");
+    const std::string step = fileName(iT, -1);
+    // the master file gets one DataSet line before its closing tags (25 characters)
+    const std::string pvd = vtkOutDir() + _name + ".pvd";
+    std::string text;
+    {
+      std::ifstream in(pvd);
+      if (!in) throw std::runtime_error("cannot read " + pvd + " (call createMasterFile first)");
+      text.assign(std::istreambuf_iterator<char>(in), std::istreambuf_iterator<char>());
+    }
+    text.resize(text.size() >= 25 ? text.size() - 25 : 0);
+    text += "<DataSet timestep=\"" + std::to_string(iT) + "\" group=\"\" part=\"\" file=\"data/" + step +
+            ".vtm\"/>\n</Collection>\n</VTKFile>\n";
+    std::ofstream(pvd, std::ios::trunc) << text;
+    std::ofstream vtm(vtkOutDir() + "data/" + step + ".vtm", std::ios::trunc);
+    vtm << "<?xml version=\"1.0\"?>\n<VTKFile type=\"vtkMultiBlockDataSet\" version=\"1.0\" "
+        << "byte_order=\"LittleEndian\">\n<vtkMultiBlockDataSet>\n"
+        << "<Block index=\"0\" >\n<DataSet index= \"0\" file=\"" << fileName(iT, 0) << ".vti\">\n</DataSet>\n</Block>\n"
+        << "</vtkMultiBlockDataSet>\n</VTKFile>\n";
+    if (!vtm) throw std::runtime_error("cannot write " + vtkOutDir() + "data/" + step + ".vtm");
+    writeVTI(vtkOutDir() + "data/" + fileName(iT, 0) + ".vti");
+  }
+
+ private:
+  struct Entry {
+    std::function<std::string&()> name;
+    int dim;
+    std::function<std::array<int, 3>()> extent;
+    std::function<std::vector<float>(int)> evaluate;
+  };
+
+  static std::string& vtkOutDir()
+  {
+    static std::string dir = "./tmp/vtkData/";
+    return dir;
+  }
+
+  std::string fileName(int iT, int iC) const
+  {
+    char buf[64];
+    if (iC < 0) std::snprintf(buf, sizeof(buf), "_iT%07d", iT);
+    else std::snprintf(buf, sizeof(buf), "_iT%07diC%05d", iT, iC);
+    return _name + buf;
+  }
+
+  void writeVTI(const std::string& path)
+  {
+    const std::array<int, 3> n = _functors.front().extent();
+    std::ostringstream out;
+    out << "<?xml version=\"1.0\"?>\n<VTKFile type=\"ImageData\" version=\"0.1\" ";
+    out << (_compress ? "byte_order=\"LittleEndian\" compressor=\"vtkZLibDataCompressor\">\n"
+                      : "byte_order=\"LittleEndian\">\n");
+    std::ostringstream ext;
+    ext << -_overlap << " " << n[0] + _overlap - 1 << " " << -_overlap << " " << n[1] + _overlap - 1 << " "
+        << -_overlap << " " << n[2] + _overlap - 1;
+    const double d = double(_delta);
+    out << "<ImageData WholeExtent=\"" << ext.str() << "\" Origin=\"" << double(_origin[0]) << " "
+        << double(_origin[1]) << " " << double(_origin[2]) << "\" Spacing=\"" << d << " " << d << " " << d << "\">\n";
+    out << "<Piece Extent=\"" << ext.str() << "\">\n<PointData>\n";
+    for (auto& f : _functors) {
+      std::string name = f.name();
+      for (char& c : name)
+        if (c == '<' || c == '>') c = '_';
+      out << "<DataArray type=\"Float32\" Name=\"" << name << "\" NumberOfComponents=\"" << f.dim << "\" ";
+      out << ((_compress || _binary) ? "format=\"binary\" encoding=\"base64\">\n" : "format=\"ascii\" >\n");
+      const std::vector<float> data = f.evaluate(_overlap);
+      const auto* bytes = reinterpret_cast<const std::uint8_t*>(data.data());
+      const uint32_t size = uint32_t(data.size() * sizeof(float));
+      if (_compress) {
+        uLongf packed = compressBound(size);
+        std::vector<std::uint8_t> z(packed);
+        if (compress2(z.data(), &packed, bytes, size, -1) != Z_OK) throw std::runtime_error("zlib compress2 failed");
+        const uint32_t head[4] = {1, size, size, uint32_t(packed)};
+        out << detail::base64_encode(reinterpret_cast<const std::uint8_t*>(head), sizeof(head))
+            << detail::base64_encode(z.data(), packed);
+      } else if (_binary) {
+        out << detail::base64_encode(reinterpret_cast<const std::uint8_t*>(&size), sizeof(size))
+            << detail::base64_encode(bytes, size);
+      } else {
+        for (float v : data) out << v << " ";
+      }
+      out << "\n</DataArray>\n";
+    }
+    out << "</PointData>\n</Piece>\n</ImageData>\n</VTKFile>\n";
+    std::ofstream file(path, std::ios::trunc);
+    file << out.str();
+    if (!file) throw std::runtime_error("cannot write " + path);
+  }
+
+  std::string _name;
+  int _overlap;
+  bool _binary, _compress;
+  std::array<T, 3> _origin{T(0), T(0), T(0)};
+  T _delta = T(1);
+  std::vector<Entry> _functors;
+};
 
 }  // namespace hip
 }  // namespace olb

@@ -91,3 +91,39 @@ def test_cpp_save_and_load_resume_the_run(gpu_available, storage, variant, tmp_p
     np.testing.assert_array_equal(shifts, K.rotate_shifts(case, 17))
     a = np.fromfile(tmp_path / "a.bin").reshape((19,) + case.shape)
     np.testing.assert_array_equal(K.core_of(case, pops).astype(np.float64), a)
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_cpp_vtk_writer_matches_python_writer(gpu_available, storage, tmp_path):
+    """SuperVTMwriter3D of the C++ layer writes the same three files as the
+    package's writer, which test_vtk_cpu.py pins to OpenLB's own output byte
+    for byte, for the same state and converter."""
+    from openlb_amd import vtk as V
+    from openlb_amd.lattice import BlockLattice
+
+    size, steps = 16, 20
+    cdir = tmp_path / "cpp"
+    (cdir / "data").mkdir(parents=True)
+    cmd = [EXE, "--size", str(size), "--steps", str(steps), "--vtk", str(cdir), "--dump", str(tmp_path / "p.bin")]
+    if storage == "f64":
+        cmd.append("--fp64")
+    out = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    vals = [float(v) for v in [ln for ln in out.stdout.splitlines() if ln.startswith("vtk ")][-1].split()[1:]]
+
+    class Conv:
+        def factors(self):
+            return np.array(vals[:3])
+
+    case = W.cavity_benchmark_local(size)
+    pops = np.fromfile(tmp_path / "p.bin").reshape((19,) + case.shape)
+    with BlockLattice.from_case(case, storage) as lat:
+        lat.upload_populations(pops, "f64")
+        w = V.SuperVTMwriter3D("cavity3d", out_dir=str(tmp_path / "py"))
+        w.add_functor(V.SuperLatticePhysVelocity3D(lat, Conv()))
+        w.add_functor(V.SuperLatticePhysPressure3D(lat, Conv()))
+        w.create_master_file()
+        w.write(10 + steps, delta=vals[3])
+    it = 10 + steps
+    for rel in ("cavity3d.pvd", f"data/cavity3d_iT{it:07d}.vtm", f"data/cavity3d_iT{it:07d}iC00000.vti"):
+        assert (cdir / rel).read_text() == (tmp_path / "py" / rel).read_text(), rel
