@@ -14,8 +14,9 @@ after ``SuperLattice::initialize()``, and snapshot k is the state after k calls 
 of every array, ``statistics.json`` OpenLB's lattice statistics of each
 fixture's last snapshot, and ``normals.npz`` OpenLB's boundary classification
 of a few geometries, and ``checkpoints/`` OpenLB's own checkpoint files of a
-few lattices (``--statistics-only`` / ``--normals-only`` /
-``--checkpoints-only`` write just those files).
+few lattices, and ``uq.json`` the sampling points and statistics of OpenLB's
+UQ module (``--statistics-only`` / ``--normals-only`` / ``--checkpoints-only``
+/ ``--uq-only`` write just those files).
 """
 from __future__ import annotations
 
@@ -276,6 +277,22 @@ def write_checkpoints():
         json.dump(manifest, fh, indent=1, sort_keys=True)
 
 
+def write_uq():
+    """uq.json: sampling points and mean / std of OpenLB's UQ module
+    (oracle/ref_uq.cpp) for Monte Carlo, GPC and quasi Monte Carlo set-ups."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "olb_ref_uq")
+    if not os.path.exists(exe):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "_ref/olb_ref_uq"], check=True)
+    out = subprocess.run([exe, "/root/reference/src/uq/new-joe-kuo-6.21201"], check=True,
+                         capture_output=True, text=True).stdout
+    data = json.loads(out)
+    data["_source"] = ("OpenLB 1.8 src/uq (UncertaintyQuantification<double>), oracle/ref_uq.cpp; "
+                       "response g(x) = sum_j (j+1) x_j^2 + sin(x_0)")
+    with open(os.path.join(HERE, "uq.json"), "w") as fh:
+        json.dump(data, fh, indent=1, sort_keys=True)
+    print(f"uq: {len(data) - 1} set-ups", flush=True)
+
+
 def write_statistics():
     """statistics.json: OpenLB's lattice statistics of each fixture's last
     snapshot.  OpenLB accumulates the statistics of step k+1 during its
@@ -303,9 +320,13 @@ if __name__ == "__main__":
     if "--checkpoints-only" in sys.argv:
         write_checkpoints()
         sys.exit(0)
+    if "--uq-only" in sys.argv:
+        write_uq()
+        sys.exit(0)
     if "--statistics-only" not in sys.argv and "--normals-only" not in sys.argv:
         main()
     if "--normals-only" not in sys.argv:
         write_statistics()
     write_normals()
     write_checkpoints()
+    write_uq()
