@@ -37,7 +37,8 @@ FLAGS = [
 
 
 HOST_DIR = os.path.join(PKG, "host")
-EXAMPLES = {"cavity3d_benchmark": os.path.join(HOST_DIR, "examples", "cavity3d_benchmark.cpp")}
+EXAMPLES = {"cavity3d_benchmark": os.path.join(HOST_DIR, "examples", "cavity3d_benchmark.cpp"),
+            "uq_cavity3d": os.path.join(HOST_DIR, "examples", "uq_cavity3d.cpp")}
 
 
 def _run(cmd, verbose):

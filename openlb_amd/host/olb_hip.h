@@ -14,6 +14,7 @@
 //   boundary::set<Local|InterpolatedVelocity/Pressure>   boundary/setBoundary.h:83-101
 //   getStatistics()                                      superLattice.h:378
 //   getSerializableSize / save / load (buffer or file)   core/serializer.h:145-189
+//   Ensemble: many set-ups advanced in one launch per step (no OpenLB counterpart)
 //   UnitConverter, SuperLatticeDensity3D / Velocity3D /
 //   PhysVelocity3D / PhysPressure3D, SuperVTMwriter3D    core/unitConverter.h, functors/lattice/, io/superVtmWriter3D.h
 //
@@ -176,7 +177,12 @@ class ConstCellView {
 // follows T (double / float); FP16 storage with FP32 arithmetic is selected
 // with T = float and storage = LBMH_F16.
 template <typename T, typename DESCRIPTOR>
+class Ensemble;
+
+template <typename T, typename DESCRIPTOR>
 class SuperLattice {
+  friend class Ensemble<T, DESCRIPTOR>;
+
  public:
   static constexpr int q = DESCRIPTOR::q;
 
@@ -536,6 +542,133 @@ void set(LATTICE& lattice, const Indicator& indicator)
 }
 }  // namespace boundary
 
+/// Many independent lattices advanced together, one launch per step for all
+/// of them (lbmh_ensemble_*).  OpenLB runs such sets (UQ samples, parameter
+/// sweeps) as one SuperLattice after the other; here every member is set up
+/// with the SuperLattice calls above -- defineDynamics, setParameter,
+/// boundary::set, defineWall, iniEquilibrium -- and handed to add() instead of
+/// initialize(), so no device lattice is created for it.  Members of one
+/// ensemble share the descriptor, the storage type and the bulk dynamics; each
+/// has its own extent, parameters, walls (ids < 64) and step budget.
+template <typename T, typename DESCRIPTOR>
+class Ensemble {
+ public:
+  Ensemble(int maxMembers, std::size_t populationBytes, std::size_t cellCapacity, int bulkKind = LBMH_BGK,
+           int device = 0, int storage = detail::storage_of<T>())
+  {
+    lbmh_ensemble_desc d{};
+    d.q = DESCRIPTOR::q;
+    d.storage = storage;
+    d.bulk_kind = bulkKind;
+    d.device = device;
+    d.max_members = maxMembers;
+    d.population_bytes = populationBytes;
+    d.cell_capacity = cellCapacity;
+    d.boundary_capacity = 0;
+    detail::check(lbmh_ensemble_create(&d, &_handle));
+    _extents.resize(std::size_t(maxMembers));
+  }
+  Ensemble(const Ensemble&) = delete;
+  Ensemble& operator=(const Ensemble&) = delete;
+  ~Ensemble() { lbmh_ensemble_destroy(_handle); }
+
+  /// Device bytes a member of this extent takes (populations, ids): the admission check.
+  std::pair<std::size_t, std::size_t> memberBytes(std::array<int, 3> extent) const
+  {
+    std::size_t pop = 0, ids = 0;
+    detail::check(lbmh_ensemble_member_bytes(_handle, extent.data(), &pop, &ids));
+    return {pop, ids};
+  }
+
+  /// Admit the set-up of `lattice` as a member that runs `steps` steps (< 0:
+  /// until removed); returns the member id.  The lattice itself stays
+  /// uninitialized and may be reused for the next member.
+  int add(const SuperLattice<T, DESCRIPTOR>& lattice, long steps = -1)
+  {
+    if (lattice._handle) throw std::logic_error("add() takes a lattice set-up that was not initialized");
+    int periodic[3];
+    for (int d = 0; d < 3; ++d) periodic[d] = lattice._periodic[d] ? 1 : 0;
+    int m = -1;
+    detail::check(lbmh_ensemble_add(_handle, lattice._extent.data(), periodic, lattice._ids.data(), &lattice._params,
+                                    int64_t(steps), &m));
+    for (const auto& w : lattice._walls)
+      if (!lattice.isLocal(w.id)) detail::check(lbmh_ensemble_set_wall(_handle, m, w.id, w.rho, w.u));
+    for (const auto& b : lattice._local) {
+      double rho = 1.0, u[3] = {0.0, 0.0, 0.0};
+      for (const auto& w : lattice._walls)
+        if (w.id == b.id) {
+          rho = w.rho;
+          for (int d = 0; d < 3; ++d) u[d] = w.u[d];
+        }
+      detail::check(lbmh_ensemble_set_local_boundary(_handle, m, b.id, b.kind, rho, u));
+    }
+    if (!lattice._rhoU.empty()) detail::check(lbmh_ensemble_ini_equilibrium(_handle, m, lattice._rhoU.data()));
+    if (!lattice._local.empty()) detail::check(lbmh_ensemble_initialize(_handle, m));
+    _extents[std::size_t(m)] = lattice._extent;
+    return m;
+  }
+
+  void remove(int member) { detail::check(lbmh_ensemble_remove(_handle, member)); }
+  void setSteps(int member, long steps) { detail::check(lbmh_ensemble_set_steps(_handle, member, int64_t(steps))); }
+
+  /// {steps done, steps left} of a member (synchronises the device).
+  std::pair<long, long> status(int member)
+  {
+    int64_t done = 0, left = 0;
+    detail::check(lbmh_ensemble_status(_handle, member, &done, &left));
+    return {long(done), long(left)};
+  }
+
+  /// n steps of every member that has steps left, asynchronously; with graph,
+  /// the step loop is replayed from a captured HIP graph.
+  void collideAndStream(long n = 1, bool graph = true, void* stream = nullptr)
+  {
+    detail::check(graph ? lbmh_ensemble_step_graph(_handle, int64_t(n), stream)
+                        : lbmh_ensemble_step(_handle, int64_t(n), stream));
+  }
+
+  /// FP64 rho and u of a member, [4][nx][ny][nz].
+  std::vector<double> moments(int member)
+  {
+    const auto& n = _extents.at(std::size_t(member));
+    std::vector<double> out(4 * std::size_t(n[0]) * n[1] * n[2]);
+    detail::check(lbmh_ensemble_moments(_handle, member, out.data()));
+    return out;
+  }
+
+  /// Shifted populations of a member in FP64, [q][nx][ny][nz].
+  std::vector<double> populations(int member)
+  {
+    const auto& n = _extents.at(std::size_t(member));
+    std::vector<double> out(std::size_t(DESCRIPTOR::q) * n[0] * n[1] * n[2]);
+    detail::check(lbmh_ensemble_download(_handle, member, out.data(), LBMH_F64));
+    return out;
+  }
+
+  /// One member's OpenLB checkpoint bytes (Serializable::save / load of its block).
+  std::vector<std::uint8_t> save(int member)
+  {
+    std::size_t n = 0;
+    detail::check(lbmh_ensemble_checkpoint_size(_handle, member, real(), &n));
+    std::vector<std::uint8_t> raw(n);
+    detail::check(lbmh_ensemble_checkpoint_save(_handle, member, real(), raw.data(), raw.size()));
+    return raw;
+  }
+  void load(int member, const std::vector<std::uint8_t>& raw)
+  {
+    detail::check(lbmh_ensemble_checkpoint_load(_handle, member, real(), raw.data(), raw.size()));
+  }
+
+  const std::array<int, 3>& extent(int member) const { return _extents.at(std::size_t(member)); }
+  lbmh_ensemble* handle() const { return _handle; }
+
+ private:
+  int real() const { return std::is_same_v<T, double> ? LBMH_F64 : LBMH_F32; }
+
+  lbmh_ensemble* _handle = nullptr;
+  std::vector<std::array<int, 3>> _extents;
+};
+
 /// UnitConverter(physDeltaX, physDeltaT, charPhysLength, charPhysVelocity,
 /// physViscosity, physDensity, charPhysPressure)       src/core/unitConverter.h:152-173
 template <typename T, typename DESCRIPTOR>
@@ -571,38 +704,59 @@ class UnitConverter {
 };
 
 namespace detail {
-/// A lattice functor the VTK writer evaluates on the device (lbmh_field_export).
+/// A lattice functor the VTK writer evaluates on the device: on a lattice
+/// (lbmh_field_export) or on one ensemble member (lbmh_ensemble_field_export).
 template <typename T, typename DESCRIPTOR>
 class LatticeFieldF {
  public:
   LatticeFieldF(SuperLattice<T, DESCRIPTOR>& lattice, int field, std::string name, const double* conversion)
-      : _lattice(lattice), _field(field), _name(std::move(name))
+      : _field(field), _name(std::move(name))
+  {
+    setConversion(conversion);
+    SuperLattice<T, DESCRIPTOR>* l = &lattice;
+    _extent = l->extent();
+    _export = [l](int f, int overlap, const double* conv, float* out, std::size_t n) {
+      if (!l->handle()) l->initialize();
+      check(lbmh_field_export(l->handle(), f, overlap, conv, out, n));
+    };
+  }
+  LatticeFieldF(Ensemble<T, DESCRIPTOR>& ensemble, int member, int field, std::string name, const double* conversion)
+      : _field(field), _name(std::move(name))
+  {
+    setConversion(conversion);
+    Ensemble<T, DESCRIPTOR>* e = &ensemble;
+    _extent = e->extent(member);
+    _export = [e, member](int f, int overlap, const double* conv, float* out, std::size_t n) {
+      check(lbmh_ensemble_field_export(e->handle(), member, f, overlap, conv, out, n));
+    };
+  }
+  virtual ~LatticeFieldF() = default;
+  std::string& getName() { return _name; }
+  int getTargetDim() const { return (_field == LBMH_FIELD_VELOCITY || _field == LBMH_FIELD_PHYS_VELOCITY) ? 3 : 1; }
+  const std::array<int, 3>& extent() const { return _extent; }
+
+  /// The writer's float buffer over [-overlap, extent + overlap), x fastest.
+  std::vector<float> evaluate(int overlap)
+  {
+    const auto& n = _extent;
+    const std::size_t points = std::size_t(n[0] + 2 * overlap) * (n[1] + 2 * overlap) * (n[2] + 2 * overlap);
+    std::vector<float> out(points * std::size_t(getTargetDim()));
+    _export(_field, overlap, _hasConversion ? _conversion : nullptr, out.data(), out.size());
+    return out;
+  }
+
+ private:
+  void setConversion(const double* conversion)
   {
     if (conversion)
       for (int k = 0; k < 3; ++k) _conversion[k] = conversion[k];
     _hasConversion = conversion != nullptr;
   }
-  virtual ~LatticeFieldF() = default;
-  std::string& getName() { return _name; }
-  int getTargetDim() const { return (_field == LBMH_FIELD_VELOCITY || _field == LBMH_FIELD_PHYS_VELOCITY) ? 3 : 1; }
-  const std::array<int, 3>& extent() const { return _lattice.extent(); }
 
-  /// The writer's float buffer over [-overlap, extent + overlap), x fastest.
-  std::vector<float> evaluate(int overlap)
-  {
-    if (!_lattice.handle()) _lattice.initialize();
-    const auto& n = _lattice.extent();
-    const std::size_t points = std::size_t(n[0] + 2 * overlap) * (n[1] + 2 * overlap) * (n[2] + 2 * overlap);
-    std::vector<float> out(points * std::size_t(getTargetDim()));
-    check(lbmh_field_export(_lattice.handle(), _field, overlap, _hasConversion ? _conversion : nullptr, out.data(),
-                            out.size()));
-    return out;
-  }
-
- private:
-  SuperLattice<T, DESCRIPTOR>& _lattice;
   int _field;
   std::string _name;
+  std::array<int, 3> _extent{};
+  std::function<void(int, int, const double*, float*, std::size_t)> _export;
   double _conversion[3] = {0, 0, 0};
   bool _hasConversion = false;
 };
@@ -621,6 +775,8 @@ class SuperLatticeDensity3D : public detail::LatticeFieldF<T, DESCRIPTOR> {
  public:
   explicit SuperLatticeDensity3D(SuperLattice<T, DESCRIPTOR>& l)
       : detail::LatticeFieldF<T, DESCRIPTOR>(l, LBMH_FIELD_DENSITY, "density", nullptr) {}
+  SuperLatticeDensity3D(Ensemble<T, DESCRIPTOR>& e, int member)
+      : detail::LatticeFieldF<T, DESCRIPTOR>(e, member, LBMH_FIELD_DENSITY, "density", nullptr) {}
 };
 
 /// SuperLatticeVelocity3D (latticeVelocity3D.hh)
@@ -629,6 +785,8 @@ class SuperLatticeVelocity3D : public detail::LatticeFieldF<T, DESCRIPTOR> {
  public:
   explicit SuperLatticeVelocity3D(SuperLattice<T, DESCRIPTOR>& l)
       : detail::LatticeFieldF<T, DESCRIPTOR>(l, LBMH_FIELD_VELOCITY, "velocity", nullptr) {}
+  SuperLatticeVelocity3D(Ensemble<T, DESCRIPTOR>& e, int member)
+      : detail::LatticeFieldF<T, DESCRIPTOR>(e, member, LBMH_FIELD_VELOCITY, "velocity", nullptr) {}
 };
 
 /// SuperLatticePhysVelocity3D (latticePhysVelocity3D.hh:84-96)
@@ -638,6 +796,9 @@ class SuperLatticePhysVelocity3D : public detail::LatticeFieldF<T, DESCRIPTOR> {
   SuperLatticePhysVelocity3D(SuperLattice<T, DESCRIPTOR>& l, const UnitConverter<T, DESCRIPTOR>& c)
       : detail::LatticeFieldF<T, DESCRIPTOR>(l, LBMH_FIELD_PHYS_VELOCITY, "physVelocity",
                                              detail::conversion_of(c).data()) {}
+  SuperLatticePhysVelocity3D(Ensemble<T, DESCRIPTOR>& e, int member, const UnitConverter<T, DESCRIPTOR>& c)
+      : detail::LatticeFieldF<T, DESCRIPTOR>(e, member, LBMH_FIELD_PHYS_VELOCITY, "physVelocity",
+                                             detail::conversion_of(c).data()) {}
 };
 
 /// SuperLatticePhysPressure3D (latticePhysPressure3D.hh:71-78)
@@ -646,6 +807,9 @@ class SuperLatticePhysPressure3D : public detail::LatticeFieldF<T, DESCRIPTOR> {
  public:
   SuperLatticePhysPressure3D(SuperLattice<T, DESCRIPTOR>& l, const UnitConverter<T, DESCRIPTOR>& c)
       : detail::LatticeFieldF<T, DESCRIPTOR>(l, LBMH_FIELD_PHYS_PRESSURE, "physPressure",
+                                             detail::conversion_of(c).data()) {}
+  SuperLatticePhysPressure3D(Ensemble<T, DESCRIPTOR>& e, int member, const UnitConverter<T, DESCRIPTOR>& c)
+      : detail::LatticeFieldF<T, DESCRIPTOR>(e, member, LBMH_FIELD_PHYS_PRESSURE, "physPressure",
                                              detail::conversion_of(c).data()) {}
 };
 
