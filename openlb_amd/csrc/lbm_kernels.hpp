@@ -75,6 +75,24 @@ __device__ __forceinline__ void push_periodic_halo(const StepArgs<Q, V>& a, S* p
                                                 int x, int y, int z)
 {
   using L = Lattice<Q>;
+  // Common case, a cell on a periodic z face away from the x and y faces
+  // (lanes run along z, so two waves of three hold one when nz = 192): the
+  // populations that cross are those with c_z pointing out of the face, and
+  // each goes to the overlap cell at the other end of the cell's own z row,
+  // padded index n + nz (z = 0) or n - nz (z = nz - 1).  The generic loop
+  // below gives the same slots at many times the instructions, which waves
+  // with a single such lane would otherwise all execute.
+  if ((a.periodic & 4u) && a.nz >= 2 && x > 0 && x + 1 < int(a.nx) && y > 0 && y + 1 < int(a.ny)) {
+    const uint32_t n = uint32_t(x + 1) * a.sx + uint32_t(y + 1) * a.sy + uint32_t(z + 1);
+    const int out = z == 0 ? -1 : 1;
+    const uint32_t hn = z == 0 ? n + a.nz : n - a.nz;
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      if (L::c[i][2] == out)
+        store_pop<S, V>(column_at(pop + uint64_t(i) * a.stride, wrap_index(hn, a.shift[i], a.n_padded)), f[i]);
+    }
+    return;
+  }
   const int n[3] = {int(a.nx), int(a.ny), int(a.nz)};
   const int r[3] = {x, y, z};
 #pragma unroll
