@@ -69,53 +69,48 @@ __device__ __forceinline__ uint32_t remap_xcd(uint32_t b, uint32_t nblocks, uint
 
 // Write populations that cross a periodic face into the halo cell they are
 // pulled from next step.  For velocity i of core cell r the target core cell
-// is m = wrap(r + c_i); if m != r + c_i the value must sit at m - c_i.
+// is m = wrap(r + c_i); if m != r + c_i the value must sit at m - c_i.  Per
+// axis d that slot differs from r by +n_d (r_d = 0, c_id = -1) or -n_d
+// (r_d = n_d - 1, c_id = +1) on a periodic axis and by 0 otherwise, so the
+// padded index of the slot is n + dn_i with dn_i a sum of per-lane terms
+// picked by the compile-time c_i; a velocity that leaves through a
+// non-periodic face is not pushed (the overlap there is never pulled).
 template <int Q, typename S, typename V>
 __device__ __forceinline__ void push_periodic_halo(const StepArgs<Q, V>& a, S* pop, const V (&f)[Q],
                                                 int x, int y, int z)
 {
   using L = Lattice<Q>;
-  // Common case, a cell on a periodic z face away from the x and y faces
-  // (lanes run along z, so two waves of three hold one when nz = 192): the
-  // populations that cross are those with c_z pointing out of the face, and
-  // each goes to the overlap cell at the other end of the cell's own z row,
-  // padded index n + nz (z = 0) or n - nz (z = nz - 1).  The generic loop
-  // below gives the same slots at many times the instructions, which waves
-  // with a single such lane would otherwise all execute.
-  if ((a.periodic & 4u) && a.nz >= 2 && x > 0 && x + 1 < int(a.nx) && y > 0 && y + 1 < int(a.ny)) {
-    const uint32_t n = uint32_t(x + 1) * a.sx + uint32_t(y + 1) * a.sy + uint32_t(z + 1);
-    const int out = z == 0 ? -1 : 1;
-    const uint32_t hn = z == 0 ? n + a.nz : n - a.nz;
-#pragma unroll
-    for (int i = 1; i < Q; ++i) {
-      if (L::c[i][2] == out)
-        store_pop<S, V>(column_at(pop + uint64_t(i) * a.stride, wrap_index(hn, a.shift[i], a.n_padded)), f[i]);
-    }
-    return;
-  }
-  const int n[3] = {int(a.nx), int(a.ny), int(a.nz)};
   const int r[3] = {x, y, z};
+  const int n[3] = {int(a.nx), int(a.ny), int(a.nz)};
+  const int stride[3] = {int(a.sx), int(a.sy), 1};
+  int lo[3], hi[3];         // padded-index offset of the slot when leaving through the low / high face
+  bool lo_out[3], hi_out[3]; // leaving through a non-periodic low / high face
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const bool per = (a.periodic >> d) & 1u;
+    const bool at_lo = r[d] == 0, at_hi = r[d] == n[d] - 1;
+    lo[d] = per && at_lo ? n[d] * stride[d] : 0;
+    hi[d] = per && at_hi ? -n[d] * stride[d] : 0;
+    lo_out[d] = !per && at_lo;
+    hi_out[d] = !per && at_hi;
+  }
+  const uint32_t base = uint32_t(x + 1) * a.sx + uint32_t(y + 1) * a.sy + uint32_t(z + 1);
 #pragma unroll
   for (int i = 1; i < Q; ++i) {
-    bool crossed = false, valid = true;
-    int h[3];
+    int dn = 0;
+    bool out = false;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const int o = r[d] + L::c[i][d];
-      h[d] = r[d];
-      if (o < 0 || o >= n[d]) {
-        if ((a.periodic >> d) & 1u) {
-          const int w = o < 0 ? n[d] - 1 : 0;
-          h[d] = w - L::c[i][d];
-          crossed = true;
-        } else {
-          valid = false;
-        }
+      if (L::c[i][d] < 0) {
+        dn += lo[d];
+        out = out || lo_out[d];
+      } else if (L::c[i][d] > 0) {
+        dn += hi[d];
+        out = out || hi_out[d];
       }
     }
-    if (valid && crossed) {
-      const uint32_t hn = uint32_t(h[0] + 1) * a.sx + uint32_t(h[1] + 1) * a.sy + uint32_t(h[2] + 1);
-      const uint32_t k = wrap_index(hn, a.shift[i], a.n_padded);
+    if (dn != 0 && !out) {
+      const uint32_t k = wrap_index(base + uint32_t(dn), a.shift[i], a.n_padded);
       store_pop<S, V>(column_at(pop + uint64_t(i) * a.stride, k), f[i]);
     }
   }
@@ -205,7 +200,13 @@ __device__ __forceinline__ void collide_stream_cell(const StepArgs<Q, typename S
       store_pop<S, V>(column_at(pop + uint64_t(i) * a.stride, wrap_index(n, a.shift[i], a.n_padded)), f[i]);
     }
   }
-  if (edge) push_periodic_halo<Q, S, V>(a, pop, f, int(x), int(y), int(z));
+  if (edge) {
+    // opaque coordinates: what the push derives from them cannot be hoisted
+    // above the collision, where it would hold registers through it
+    uint32_t hx = x, hy = y, hz = z;
+    asm volatile("" : "+v"(hx), "+v"(hy), "+v"(hz));
+    push_periodic_halo<Q, S, V>(a, pop, f, int(hx), int(hy), int(hz));
+  }
 }
 
 // One time step of a block, one lane per core cell.
