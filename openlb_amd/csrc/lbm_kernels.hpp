@@ -162,6 +162,8 @@ __device__ __forceinline__ void collide_stream_cell(const StepArgs<Q, typename S
   for (int i = 0; i < Q; ++i) {
     f[i] = load_pop<S, V>(column_at(pop + uint64_t(i) * a.stride, wrap_index(n, a.shift[i], a.n_padded)));
   }
+  uint32_t ns = n;
+  asm volatile("" : "+v"(ns));
   if (!IDS || id != 0) {
     if (!IDS || id == kIdBulk) {
       collide_bulk<Q, V, KIND>(f, a.bp);
@@ -197,7 +199,7 @@ __device__ __forceinline__ void collide_stream_cell(const StepArgs<Q, typename S
     }
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      store_pop<S, V>(column_at(pop + uint64_t(i) * a.stride, wrap_index(n, a.shift[i], a.n_padded)), f[i]);
+      store_pop<S, V>(column_at(pop + uint64_t(i) * a.stride, wrap_index(ns, a.shift[i], a.n_padded)), f[i]);
     }
   }
   if (edge) {
