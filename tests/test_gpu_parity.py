@@ -11,6 +11,8 @@ Tolerances are relative to max |f_i| of the unshifted populations. They are:
 Only cells with bulk dynamics are compared. OpenLB's own CPU and GPU platforms
 fill wall-cell populations that stream in from outside the domain differently.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -116,7 +118,25 @@ RAGGED = [
     ("channel_forced", lambda: W.porous_channel((33, 17, 19), seed=5, tau=0.55, force=1e-5,
                                                rmin=1, rmax=3)),
     ("channel27_smag", lambda: W.turbulent_channel((20, 12, 10), tau=0.505, noise=0.2, seed=6)),
+    ("y_periodic_walls", lambda: _walled(W.random_periodic((6, 9, 5), seed=7, tau=0.7), (0, 1, 0))),
+    ("xz_periodic27_thin", lambda: _walled(W.random_periodic((3, 6, 2), seed=8, q=27, tau=0.8), (1, 0, 1))),
 ]
+
+
+def _walled(case, periodic):
+    """The box with the given periodic axes and BounceBack walls on the faces of
+    the others: halo pushes of cells that lie on a periodic and a walled face."""
+    ids = case.ids.copy()
+    for d in range(3):
+        if not periodic[d]:
+            lo = [slice(None)] * 3
+            hi = [slice(None)] * 3
+            lo[d], hi[d] = 0, -1
+            ids[tuple(lo)] = 2
+            ids[tuple(hi)] = 2
+    rho_u = case.rho_u.copy()
+    rho_u[1:, ids == 2] = 0.0
+    return dataclasses.replace(case, periodic=tuple(periodic), ids=ids, rho_u=rho_u)
 
 
 def _with_holes(case):
