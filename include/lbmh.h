@@ -79,7 +79,11 @@ typedef struct lbmh_bulk_params {
 /* Create a block lattice.  Replaces the GPU_CUDA construction path of
  * ConcreteBlockLattice (src/core/blockLattice.hh:393-416), which allocates one
  * CyclicColumn per population (src/core/platform/gpu/cuda/column.hh:217-267).
- * Populations start at zero (f_i = t_i), every cell has id 0. */
+ * Populations start at zero (f_i = t_i), every cell has id 0.  Limits: fewer
+ * than 2^31 padded cells, and one population column ((nx+2)(ny+2)(nz+2) x
+ * bytes per population) below 2 GiB, i.e. up to 2^28 padded cells in FP64,
+ * 2^29 in FP32 and 2^30 in FP16 (the kernels use 32-bit byte offsets inside a
+ * column).  Ensemble members obey the same limits. */
 int lbmh_lattice_create(const lbmh_lattice_desc* desc, lbmh_lattice** out);
 int lbmh_lattice_destroy(lbmh_lattice* lattice);
 

@@ -317,6 +317,10 @@ int lbmh_lattice_create(const lbmh_lattice_desc* desc, lbmh_lattice** out)
   // 32-bit cell indices, with room for index + shift < 2^32 (CellID is uint32 in
   // OpenLB, src/core/blockStructure.h:36,80-82)
   if (npad >= (uint64_t(1) << 31)) return fail(LBMH_E_INVALID, "padded cell count must be below 2^31");
+  // kernels address a population column with 32-bit byte offsets and wrap
+  // them with an unsigned min, which needs a column below 2 GiB
+  if (npad * storage_size(desc->storage) >= (uint64_t(1) << 31))
+    return fail(LBMH_E_INVALID, "a population column must stay below 2 GiB (padded cells x bytes per population)");
   auto* L = new (std::nothrow) lbmh_lattice();
   if (!L) return fail(LBMH_E_MEMORY, "host allocation failed");
   L->device = desc->device;

@@ -412,6 +412,8 @@ int lbmh_ensemble_add(lbmh_ensemble* E, const int extent[3], const int periodic[
   const uint64_t sx = uint64_t(extent[0]) + 2, sy = uint64_t(extent[1]) + 2, sz = uint64_t(extent[2]) + 2;
   const uint64_t npad = sx * sy * sz;
   if (npad >= (uint64_t(1) << 31)) return fail(LBMH_E_INVALID, "padded cell count must be below 2^31");
+  if (npad * storage_size(E->storage) >= (uint64_t(1) << 31))
+    return fail(LBMH_E_INVALID, "a population column must stay below 2 GiB (padded cells x bytes per population)");
   const uint32_t ncore = uint32_t(uint64_t(extent[0]) * extent[1] * extent[2]);
   for (uint32_t c = 0; c < ncore; ++c)
     if (ids[c] >= kEnsembleWalls) return fail(LBMH_E_INVALID, "ensemble dynamics ids must be below 64");

@@ -56,6 +56,21 @@ def test_create_rejects_cell_index_overflow():
     assert "2^31" in lib.lbmh_last_error().decode()
 
 
+@pytest.mark.parametrize("storage,n", [("F64", 660), ("F32", 820), ("F16", 1030)])
+def test_create_rejects_columns_of_2_gib(storage, n):
+    """A population column must stay below 2 GiB: the kernels address it with
+    32-bit byte offsets (2^28 padded cells in FP64, 2^29 in FP32, 2^30 in FP16).
+    The check comes before any device call."""
+    lib = abi.load()
+    desc = abi.LatticeDesc()
+    desc.q, desc.nx, desc.ny, desc.nz = 19, n, n, n
+    desc.storage = getattr(abi, storage)
+    h = ctypes.c_void_p()
+    assert lib.lbmh_lattice_create(ctypes.byref(desc), ctypes.byref(h)) == abi.E_INVALID
+    assert not h.value
+    assert "2 GiB" in lib.lbmh_last_error().decode()
+
+
 def test_null_arguments_are_rejected():
     lib = abi.load()
     assert lib.lbmh_step(None, 1, None) == abi.E_INVALID
