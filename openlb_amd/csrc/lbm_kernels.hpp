@@ -162,6 +162,9 @@ __device__ __forceinline__ void collide_stream_cell(const StepArgs<Q, typename S
   for (int i = 0; i < Q; ++i) {
     f[i] = load_pop<S, V>(column_at(pop + uint64_t(i) * a.stride, wrap_index(n, a.shift[i], a.n_padded)));
   }
+  // The store addresses are recomputed after the collision from an opaque
+  // copy of n instead of held through it (Q fewer live registers: 30-55 VGPRs
+  // less per kernel, at 4 VALU per population).
   uint32_t ns = n;
   asm volatile("" : "+v"(ns));
   if (!IDS || id != 0) {
