@@ -165,6 +165,7 @@ __global__ __launch_bounds__(64) void ensemble_post_kernel(const EnsembleEntry<Q
       a.walls = walls + uint64_t(lc.x) * kEnsembleWalls * 4;
       a.info = info + uint64_t(lc.x) * kEnsembleWalls;
       a.cells = nullptr;
+      a.recs = nullptr;
       a.count = 0;
       a.nx = int(e.nx);
       a.ny = int(e.ny);
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(64) void ensemble_post_kernel(const EnsembleEntry<Q
   }
   if (valid && slot < kPostSlots) post_gather<Q, S, KIND>(a, lc.y, slot, nb[j][slot]);
   __syncthreads();
-  if (valid && slot == 0) post_finish<Q, S, KIND>(a, lc.y, nb[j]);
+  if (valid && slot == 0) post_finish<Q, S, KIND>(a, lc.y, int(a.ids[lc.y]), nb[j]);
 }
 
 // Streaming of every lattice that stepped: rotate its columns, count the step.

@@ -81,7 +81,7 @@ __host__ __device__ inline uint32_t pack_local_info(uint32_t kind, int a, int b,
   return kind | (uint32_t(a + 8) << 8) | (uint32_t(b + 8) << 12) | (uint32_t(c + 8) << 16);
 }
 
-__device__ __forceinline__ LocalInfo unpack_local_info(uint32_t w)
+__host__ __device__ inline LocalInfo unpack_local_info(uint32_t w)
 {
   LocalInfo r;
   r.kind = w & 0xffu;
@@ -206,12 +206,12 @@ __device__ __forceinline__ V pi_to_fneq(int i, const V (&pi)[6])
   return fneq;
 }
 
-__device__ __forceinline__ int flat_dir(const LocalInfo& li)
+__host__ __device__ inline int flat_dir(const LocalInfo& li)
 {
   return li.a ? 0 : (li.b ? 1 : 2);
 }
 
-__device__ __forceinline__ int info_component(const LocalInfo& li, int d)
+__host__ __device__ inline int info_component(const LocalInfo& li, int d)
 {
   return d == 0 ? li.a : (d == 1 ? li.b : li.c);
 }
@@ -443,6 +443,7 @@ struct PostArgs {
   const double* walls;      // [256][4]
   const uint32_t* info;     // [256] packed LocalInfo
   const uint32_t* cells;    // core indices of the cells to process
+  const struct PostCell* recs;  // the same cells with their neighbours resolved (lattices), or null
   uint32_t count;
   int nx, ny, nz;
   uint32_t sx, sy;
@@ -461,30 +462,25 @@ __device__ __forceinline__ void load_core_cell(const PostArgs<Q>& a, int x, int 
     f[i] = load_pop<S, V>(column_at(pop + uint64_t(i) * a.stride, wrap_index(n, a.shift[i], a.n_padded)));
 }
 
-// computeRho / computeU of the cell at core coordinates (x, y, z) through its
-// dynamics' momenta.  Periodic axes wrap; beyond a non-periodic face lies the
-// overlap, whose cells have no dynamics (OneDensity, ZeroMomentum).
+// computeRho / computeU of the cell at padded index n with dynamics id `id`
+// through its dynamics' momenta; n == kPostOutside is the overlap beyond a
+// non-periodic face (no dynamics: OneDensity, ZeroMomentum).
+constexpr uint32_t kPostOutside = 0xffffffffu;
+
 template <int Q, typename S, typename V>
-__device__ void cell_rho_u(const PostArgs<Q>& a, int x, int y, int z, V& rho, V (&u)[3])
+__device__ void rho_u_at(const PostArgs<Q>& a, uint32_t n, int id, V& rho, V (&u)[3])
 {
-  int r[3] = {x, y, z};
-  const int n[3] = {a.nx, a.ny, a.nz};
   rho = V(1);
   u[0] = u[1] = u[2] = V(0);
-  for (int d = 0; d < 3; ++d) {
-    if (r[d] < 0 || r[d] >= n[d]) {
-      if (!((a.periodic >> d) & 1u)) return;
-      r[d] = (r[d] % n[d] + n[d]) % n[d];
-    }
-  }
-  const uint32_t c = (uint32_t(r[0]) * uint32_t(a.ny) + uint32_t(r[1])) * uint32_t(a.nz) + uint32_t(r[2]);
-  const int id = a.ids[c];
-  if (id == 0 || id == 2) return;  // NoDynamics: OneDensity; BounceBack: FixedDensity 1, ZeroMomentum
+  if (n == kPostOutside || id == 0 || id == 2) return;  // NoDynamics: OneDensity; BounceBack: FixedDensity 1, ZeroMomentum
+  const S* pop = static_cast<const S*>(a.pop);
   if (id == 1) {
     // lbm<D>::computeRhoU                                      lbm.h:266-307
     using L = Lattice<Q>;
     V f[Q];
-    load_core_cell<Q, S, V>(a, r[0], r[1], r[2], f);
+#pragma unroll
+    for (int i = 0; i < Q; ++i)
+      f[i] = load_pop<S, V>(column_at(pop + uint64_t(i) * a.stride, wrap_index(n, a.shift[i], a.n_padded)));
     V s = V(0), j[3] = {V(0), V(0), V(0)};
 #pragma unroll
     for (int i = 0; i < Q; ++i) s += f[i];
@@ -509,8 +505,31 @@ __device__ void cell_rho_u(const PostArgs<Q>& a, int x, int y, int z, V& rho, V 
   }
   if (li.kind == kDeadWall) return;
   V f[Q];
-  load_core_cell<Q, S, V>(a, r[0], r[1], r[2], f);
+#pragma unroll
+  for (int i = 0; i < Q; ++i)
+    f[i] = load_pop<S, V>(column_at(pop + uint64_t(i) * a.stride, wrap_index(n, a.shift[i], a.n_padded)));
   local_rho_u<Q, V>(f, li, w, rho, u);
+}
+
+// computeRho / computeU of the cell at core coordinates (x, y, z) through its
+// dynamics' momenta.  Periodic axes wrap; beyond a non-periodic face lies the
+// overlap, whose cells have no dynamics (OneDensity, ZeroMomentum).
+template <int Q, typename S, typename V>
+__device__ void cell_rho_u(const PostArgs<Q>& a, int x, int y, int z, V& rho, V (&u)[3])
+{
+  int r[3] = {x, y, z};
+  const int n[3] = {a.nx, a.ny, a.nz};
+  rho = V(1);
+  u[0] = u[1] = u[2] = V(0);
+  for (int d = 0; d < 3; ++d) {
+    if (r[d] < 0 || r[d] >= n[d]) {
+      if (!((a.periodic >> d) & 1u)) return;
+      r[d] = (r[d] % n[d] + n[d]) % n[d];
+    }
+  }
+  const uint32_t c = (uint32_t(r[0]) * uint32_t(a.ny) + uint32_t(r[1])) * uint32_t(a.nz) + uint32_t(r[2]);
+  const uint32_t np = uint32_t(r[0] + 1) * a.sx + uint32_t(r[1] + 1) * a.sy + uint32_t(r[2] + 1);
+  rho_u_at<Q, S, V>(a, np, a.ids[c], rho, u);
 }
 
 // fd::boundaryGradient                                    finiteDifference.h:40-44
@@ -575,11 +594,23 @@ __device__ __forceinline__ void core_coords(const FastDiv& div_z, const FastDiv&
 //     slots: -n_d e_d and -2 n_d e_d for d = x, y, z
 constexpr int kPostLanes = 8;
 constexpr int kPostSlots = 6;
+
+// A post-processed cell of a lattice with its neighbours resolved on the host
+// when the boundary is set: its dynamics id, and the padded index and id of
+// each neighbour slot (periodic axes wrapped, kPostOutside beyond a
+// non-periodic face).  A list kernel then starts from one record instead of
+// the chain core index -> id -> table entry -> neighbour id -> ...
+struct PostCell {
+  uint32_t c;                   // core index
+  uint32_t id;
+  uint32_t nb[kPostSlots];
+  uint32_t nb_id[kPostSlots];
+};
 //   plane (dir, orient)     PlaneFdBoundaryProcessor3D::apply, boundaryPostProcessors3D.hh:38-92
 //     slots, axis d = 0..2: -orient e_d and -2 orient e_d along the normal, +e_d and -e_d along the face
 enum PostKind : int { kPostEdge = 0, kPostCorner = 1, kPostPlane = 2 };
 
-__device__ __forceinline__ void post_offset(const LocalInfo& li, int kind, int slot, int (&o)[3])
+__host__ __device__ inline void post_offset(const LocalInfo& li, int kind, int slot, int (&o)[3])
 {
   o[0] = o[1] = o[2] = 0;
   if (kind == kPostPlane) {
@@ -626,16 +657,29 @@ __device__ void post_gather(const PostArgs<Q>& a, uint32_t c, int slot, typename
   out[3] = u[2];
 }
 
+// rho and u of neighbour `slot` of a resolved post-processed cell
+template <int Q, typename S>
+__device__ void post_gather_rec(const PostArgs<Q>& a, const PostCell* r, int slot,
+                                typename StorageTraits<S>::V (&out)[4])
+{
+  using V = typename StorageTraits<S>::V;
+  V rho, u[3];
+  rho_u_at<Q, S, V>(a, r->nb[slot], int(r->nb_id[slot]), rho, u);
+  out[0] = rho;
+  out[1] = u[0];
+  out[2] = u[1];
+  out[3] = u[2];
+}
+
 // Density and velocity gradient from the six neighbour evaluations, then the
 // regularized reconstruction of cell c
 template <int Q, typename S, int KIND>
-__device__ void post_finish(const PostArgs<Q>& a, uint32_t c, const typename StorageTraits<S>::V (*nb)[4],
+__device__ void post_finish(const PostArgs<Q>& a, uint32_t c, int id, const typename StorageTraits<S>::V (*nb)[4],
                             typename StorageTraits<S>::V* scratch = nullptr)
 {
   using V = typename StorageTraits<S>::V;
   int x, y, z;
   core_coords(a.div_z, a.div_y, a.ny, a.nz, c, x, y, z);
-  const int id = a.ids[c];
   const double* w = a.walls + 4 * id;
   const LocalInfo li = unpack_local_info(a.info[id]);
   V u0[3] = {V(w[1]), V(w[2]), V(w[3])};
@@ -684,9 +728,10 @@ __global__ __launch_bounds__(64) void plane_scratch_kernel(PostArgs<Q> a, typena
   const int slot = int(threadIdx.x % kPostLanes), j = int(threadIdx.x / kPostLanes);
   const uint32_t k = blockIdx.x * (64 / kPostLanes) + uint32_t(j);
   const bool valid = k < a.count;
-  if (valid && slot < kPostSlots) post_gather<Q, S, kPostPlane>(a, a.cells[k], slot, nb[j][slot]);
+  if (valid && slot < kPostSlots) post_gather_rec<Q, S>(a, a.recs + k, slot, nb[j][slot]);
   __syncthreads();
-  if (valid && slot == 0) post_finish<Q, S, kPostPlane>(a, a.cells[k], nb[j], scratch + uint64_t(k) * Q);
+  if (valid && slot == 0)
+    post_finish<Q, S, kPostPlane>(a, a.recs[k].c, int(a.recs[k].id), nb[j], scratch + uint64_t(k) * Q);
 }
 
 template <int Q, typename S>
@@ -713,9 +758,9 @@ __global__ __launch_bounds__(64) void outer_post_kernel(PostArgs<Q> a)
   const int slot = int(threadIdx.x % kPostLanes), j = int(threadIdx.x / kPostLanes);
   const uint32_t k = blockIdx.x * (64 / kPostLanes) + uint32_t(j);
   const bool valid = k < a.count;
-  if (valid && slot < kPostSlots) post_gather<Q, S, KIND>(a, a.cells[k], slot, nb[j][slot]);
+  if (valid && slot < kPostSlots) post_gather_rec<Q, S>(a, a.recs + k, slot, nb[j][slot]);
   __syncthreads();
-  if (valid && slot == 0) post_finish<Q, S, KIND>(a, a.cells[k], nb[j]);
+  if (valid && slot == 0) post_finish<Q, S, KIND>(a, a.recs[k].c, int(a.recs[k].id), nb[j]);
 }
 
 }  // namespace lbmh

@@ -63,6 +63,10 @@ struct lbmh_lattice {
   uint32_t* d_planes = nullptr;    // interpolated-boundary faces (plane finite-difference processor)
   bool pressure_planes = false;    // some plane is a pressure face: compute into d_scratch first
   void* d_scratch = nullptr;
+  // the post-processed lists with their neighbours resolved (PostCell records)
+  PostCell* d_edge_recs = nullptr;
+  PostCell* d_corner_recs = nullptr;
+  PostCell* d_plane_recs = nullptr;
 };
 
 struct lbmh_halo {
@@ -118,6 +122,7 @@ PostArgs<Q> post_args(const lbmh_lattice* L)
   a.walls = L->d_walls;
   a.info = L->d_info;
   a.cells = nullptr;
+  a.recs = nullptr;
   a.count = 0;
   a.nx = L->n[0];
   a.ny = L->n[1];
@@ -131,6 +136,47 @@ PostArgs<Q> post_args(const lbmh_lattice* L)
   return a;
 }
 
+// The cells of a post-processor list with their neighbours resolved, as the
+// device's cell_rho_u would find them: offsets of the processor's slots,
+// periodic axes wrapped, kPostOutside beyond a non-periodic face.
+std::vector<PostCell> post_records(const lbmh_lattice* L, const std::vector<uint32_t>& cells, int kind)
+{
+  const int n[3] = {L->n[0], L->n[1], L->n[2]};
+  const uint32_t sx = L->size[1] * L->size[2], sy = L->size[2];
+  std::vector<PostCell> out(cells.size());
+  for (size_t k = 0; k < cells.size(); ++k) {
+    const uint32_t c = cells[k];
+    const int r[3] = {int(c / (uint32_t(n[1]) * uint32_t(n[2]))), int((c / uint32_t(n[2])) % uint32_t(n[1])),
+                      int(c % uint32_t(n[2]))};
+    PostCell& p = out[k];
+    p.c = c;
+    p.id = L->h_ids[c];
+    const LocalInfo li = unpack_local_info(L->info[p.id]);
+    for (int slot = 0; slot < kPostSlots; ++slot) {
+      int o[3];
+      post_offset(li, kind, slot, o);
+      int q[3];
+      bool outside = false;
+      for (int d = 0; d < 3; ++d) {
+        q[d] = r[d] + o[d];
+        if (q[d] < 0 || q[d] >= n[d]) {
+          if (!L->periodic[d]) outside = true;
+          else q[d] = (q[d] % n[d] + n[d]) % n[d];
+        }
+      }
+      if (outside) {
+        p.nb[slot] = kPostOutside;
+        p.nb_id[slot] = 0;
+      } else {
+        const uint32_t nc = (uint32_t(q[0]) * uint32_t(n[1]) + uint32_t(q[1])) * uint32_t(n[2]) + uint32_t(q[2]);
+        p.nb[slot] = uint32_t(q[0] + 1) * sx + uint32_t(q[1] + 1) * sy + uint32_t(q[2] + 1);
+        p.nb_id[slot] = L->h_ids[nc];
+      }
+    }
+  }
+  return out;
+}
+
 // PostStream stage of the local boundaries: outer edges (priority 0), then
 // outer corners (priority 1), on the populations after the shift.
 template <int Q, typename S>
@@ -141,6 +187,7 @@ int launch_post_processors(lbmh_lattice* L, hipStream_t stream)
   if (!L->h_planes.empty()) {
     using V = typename StorageTraits<S>::V;
     a.cells = L->d_planes;
+    a.recs = L->d_plane_recs;
     a.count = uint32_t(L->h_planes.size());
     if (L->pressure_planes) {
       hipLaunchKernelGGL((plane_scratch_kernel<Q, S>), dim3(grid_for(a.count, 64 / kPostLanes)), dim3(64), 0, stream,
@@ -156,6 +203,7 @@ int launch_post_processors(lbmh_lattice* L, hipStream_t stream)
   }
   if (!L->h_edges.empty()) {
     a.cells = L->d_edges;
+    a.recs = L->d_edge_recs;
     a.count = uint32_t(L->h_edges.size());
     hipLaunchKernelGGL((outer_post_kernel<Q, S, kPostEdge>), dim3(grid_for(a.count, 64 / kPostLanes)), dim3(64), 0,
                        stream, a);
@@ -163,6 +211,7 @@ int launch_post_processors(lbmh_lattice* L, hipStream_t stream)
   }
   if (!L->h_corners.empty()) {
     a.cells = L->d_corners;
+    a.recs = L->d_corner_recs;
     a.count = uint32_t(L->h_corners.size());
     hipLaunchKernelGGL((outer_post_kernel<Q, S, kPostCorner>), dim3(grid_for(a.count, 64 / kPostLanes)), dim3(64), 0,
                        stream, a);
@@ -386,6 +435,9 @@ int lbmh_lattice_destroy(lbmh_lattice* L)
   if (L->d_local) (void)hipFree(L->d_local);
   if (L->d_planes) (void)hipFree(L->d_planes);
   if (L->d_scratch) (void)hipFree(L->d_scratch);
+  if (L->d_edge_recs) (void)hipFree(L->d_edge_recs);
+  if (L->d_corner_recs) (void)hipFree(L->d_corner_recs);
+  if (L->d_plane_recs) (void)hipFree(L->d_plane_recs);
   delete L;
   return LBMH_OK;
 }
@@ -549,6 +601,22 @@ int lbmh_set_local_boundary(lbmh_lattice* L, int id, int kind, double rho, const
   if (rc == LBMH_OK) rc = upload_list(L->h_corners, L->d_corners);
   if (rc == LBMH_OK) rc = upload_list(L->h_local, L->d_local);
   if (rc == LBMH_OK) rc = upload_list(L->h_planes, L->d_planes);
+  // every list is resolved again: this boundary may have relabelled
+  // neighbours of the cells of an earlier one
+  auto upload_records = [&](const std::vector<uint32_t>& h, int kind, PostCell*& d) -> int {
+    if (d) {
+      LBMH_HIP(hipFree(d));
+      d = nullptr;
+    }
+    if (h.empty()) return LBMH_OK;
+    const std::vector<PostCell> recs = post_records(L, h, kind);
+    LBMH_HIP(hipMalloc((void**)&d, sizeof(PostCell) * recs.size()));
+    LBMH_HIP(hipMemcpy(d, recs.data(), sizeof(PostCell) * recs.size(), hipMemcpyHostToDevice));
+    return LBMH_OK;
+  };
+  if (rc == LBMH_OK) rc = upload_records(L->h_edges, kPostEdge, L->d_edge_recs);
+  if (rc == LBMH_OK) rc = upload_records(L->h_corners, kPostCorner, L->d_corner_recs);
+  if (rc == LBMH_OK) rc = upload_records(L->h_planes, kPostPlane, L->d_plane_recs);
   if (rc == LBMH_OK && L->pressure_planes) {
     if (L->d_scratch) {
       LBMH_HIP(hipFree(L->d_scratch));
