@@ -280,7 +280,10 @@ int lbmh_synchronize(lbmh_lattice* lattice);
  * 256-cell chunks inside a window of 8*G chunks, so a cache line two chunks
  * share is fetched into one L2 instead of two.  Defaults: lds_bytes 40960
  * (4 workgroups per CU) and xcd_group 4 for FP64/FP32 storage, lds_bytes 0
- * and xcd_group 8 for FP16 storage. */
+ * and xcd_group 8 for FP16 storage; FP64/FP32 lattices with local velocity /
+ * pressure boundaries switch to lds_bytes 0 (their whole-lattice pass also
+ * collides the flat faces, whose divergent waves need the occupancy).  A
+ * shape set here or chosen by lbmh_autotune is kept. */
 typedef struct lbmh_launch_config {
   int lds_bytes;    /* 0 .. 65536                                             */
   int xcd_group;    /* 0 (dispatch order) .. 64                               */

@@ -47,6 +47,7 @@ struct lbmh_lattice {
   uint32_t n_partials = 0;
   bool uniform_bulk = false;   // every core cell has id 1: skip the id read
   lbmh_launch_config launch{49152, 4};
+  bool launch_chosen = false;     // set by lbmh_set_launch_config / lbmh_autotune: keep it
   lbmh_bulk_params params{};
   double walls[kMaxWalls][4] = {};
   size_t footprint = 0;
@@ -68,6 +69,17 @@ struct lbmh_lattice {
   PostCell* d_corner_recs = nullptr;
   PostCell* d_plane_recs = nullptr;
 };
+
+// Library default launch shape.  FP64/FP32: a 40 KB LDS reservation caps
+// the resident workgroups at 4 per CU (fewer concurrent DRAM streams), XCD
+// groups of 4.  FP16: no cap, groups of 8.  With local boundaries the
+// whole-lattice pass collides the flat faces too, and its divergent waves
+// need the occupancy: no cap (201^3 cavity, FP32: 298 us against 324 us).
+static lbmh_launch_config default_launch(int storage, bool local)
+{
+  if (storage == LBMH_F16) return lbmh_launch_config{0, 8};
+  return local ? lbmh_launch_config{0, 4} : lbmh_launch_config{40960, 4};
+}
 
 struct lbmh_halo {
   lbmh_lattice* dst = nullptr;
@@ -396,7 +408,7 @@ int lbmh_lattice_create(const lbmh_lattice_desc* desc, lbmh_lattice** out)
   L->params.omega = 1.0;
   // measured best shapes on MI355X for the 256^3 D3Q19 step: FP16 columns
   // want full occupancy, FP32/FP64 three to four workgroups per CU
-  L->launch = desc->storage == LBMH_F16 ? lbmh_launch_config{0, 8} : lbmh_launch_config{40960, 4};
+  L->launch = default_launch(L->storage, false);
   for (int w = 0; w < kMaxWalls; ++w) L->walls[w][0] = 1.0;
 
   DeviceGuard guard(L->device);
@@ -458,6 +470,7 @@ int lbmh_set_dynamics_ids(lbmh_lattice* L, const uint8_t* ids)
   // a new id map drops the local boundaries defined on the previous one
   L->h_ids.assign(ids, ids + L->ncore);
   L->has_local = false;
+  if (!L->launch_chosen) L->launch = default_launch(L->storage, false);
   L->h_edges.clear();
   L->h_corners.clear();
   L->h_local.clear();
@@ -628,6 +641,7 @@ int lbmh_set_local_boundary(lbmh_lattice* L, int id, int kind, double rho, const
   if (rc != LBMH_OK) return rc;
   L->has_local = true;
   L->uniform_bulk = false;
+  if (!L->launch_chosen) L->launch = default_launch(L->storage, true);
   return LBMH_OK;
 }
 
@@ -1029,6 +1043,7 @@ int lbmh_set_launch_config(lbmh_lattice* L, const lbmh_launch_config* cfg)
   if (cfg->lds_bytes < 0 || cfg->lds_bytes > 65536) return fail(LBMH_E_INVALID, "lds_bytes must be in [0, 65536]");
   if (cfg->xcd_group < 0 || cfg->xcd_group > 64) return fail(LBMH_E_INVALID, "xcd_group must be in [0, 64]");
   L->launch = *cfg;
+  L->launch_chosen = true;
   return LBMH_OK;
 }
 
@@ -1099,6 +1114,7 @@ int lbmh_autotune(lbmh_lattice* L, int trial_steps, lbmh_launch_config* chosen)
     }
   }
   L->launch = best;
+  L->launch_chosen = true;
   if (chosen) *chosen = best;
   return LBMH_OK;
 }

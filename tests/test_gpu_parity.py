@@ -245,6 +245,26 @@ def test_full_size_porous_channel_conserves_mass(gpu_available):
     assert m1[1] > 0 and abs(m1[2]) < 1e-3 * m1[1] and abs(m1[3]) < 1e-3 * m1[1]
 
 
+def test_local_boundary_lattices_default_to_full_occupancy(gpu_available):
+    """A lattice with local velocity boundaries drops the LDS cap by default
+    (its whole-lattice pass also collides the flat faces); the shape changes
+    scheduling only, so a capped run gives the same populations."""
+    case = W.local_velocity_cavity((9, 8, 7), tau=0.7, lid_u=0.06)
+    f0 = O.initial_populations(case, "f32")
+    runs = []
+    for cap in (None, 40960):
+        with BlockLattice.from_case(case, "f32") as lat:
+            if cap is None:
+                assert lat.launch_config() == {"lds_bytes": 0, "xcd_group": 4}
+            else:
+                lat.set_launch_config(cap, 4)
+            lat.upload_populations(f0, "f64")
+            lat.initialize()
+            lat.collide_and_stream(13)
+            runs.append(lat.download_populations("f32"))
+    np.testing.assert_array_equal(runs[0], runs[1])
+
+
 def test_launch_configs_give_identical_results(gpu_available):
     """Launch shape is scheduling only: every configuration must give bit-identical
     populations. The autotuner must leave the lattice's own state untouched."""
