@@ -279,8 +279,9 @@ int lbmh_synchronize(lbmh_lattice* lattice);
  * population streams.  xcd_group G > 0 hands each XCD runs of G consecutive
  * 256-cell chunks inside a window of 8*G chunks, so a cache line two chunks
  * share is fetched into one L2 instead of two.  Defaults: lds_bytes 40960
- * (4 workgroups per CU) and xcd_group 4 for FP64/FP32 storage, lds_bytes 0
- * and xcd_group 8 for FP16 storage; FP64/FP32 lattices with local velocity /
+ * (4 workgroups per CU) and xcd_group 4 for FP32 storage, lds_bytes 49152
+ * and xcd_group 8 for FP64, lds_bytes 0 and xcd_group 8 for FP16 storage;
+ * FP64/FP32 lattices with local velocity /
  * pressure boundaries switch to lds_bytes 0 (their whole-lattice pass also
  * collides the flat faces, whose divergent waves need the occupancy).  A
  * shape set here or chosen by lbmh_autotune is kept. */

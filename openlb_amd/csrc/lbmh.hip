@@ -70,15 +70,19 @@ struct lbmh_lattice {
   PostCell* d_plane_recs = nullptr;
 };
 
-// Library default launch shape.  FP64/FP32: a 40 KB LDS reservation caps
-// the resident workgroups at 4 per CU (fewer concurrent DRAM streams), XCD
-// groups of 4.  FP16: no cap, groups of 8.  With local boundaries the
-// whole-lattice pass collides the flat faces too, and its divergent waves
-// need the occupancy: no cap (201^3 cavity, FP32: 298 us against 324 us).
+// Library default launch shape, from sweeps of (LDS reservation, XCD group)
+// on the porous channel, the turbulent channel and a Taylor-Green box.  FP32:
+// a 40 KB LDS reservation caps the resident workgroups at 4 per CU (fewer
+// concurrent DRAM streams), XCD groups of 4.  FP64: 48 KB (3 per CU), groups
+// of 8, 1-2% ahead of the FP32 shape.  FP16: no cap, groups of 8.  With
+// local boundaries the whole-lattice pass collides the flat faces too, and
+// its divergent waves need the occupancy: no cap (201^3 cavity, FP32: 298 us
+// against 324 us).
 static lbmh_launch_config default_launch(int storage, bool local)
 {
   if (storage == LBMH_F16) return lbmh_launch_config{0, 8};
-  return local ? lbmh_launch_config{0, 4} : lbmh_launch_config{40960, 4};
+  if (local) return lbmh_launch_config{0, 4};
+  return storage == LBMH_F64 ? lbmh_launch_config{49152, 8} : lbmh_launch_config{40960, 4};
 }
 
 struct lbmh_halo {
