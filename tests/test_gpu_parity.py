@@ -54,9 +54,7 @@ def test_gpu_matches_reference(gpu_available, name):
 @pytest.mark.parametrize("name", [n for n in G.names() if n.endswith("_f64")])
 def test_gpu_fp32_storage_vs_fp64_reference(gpu_available, name):
     case, meta, arrays = G.load(name)
-    steps = [min(meta["steps"][-1], 200)]
-    if steps[0] != meta["steps"][-1]:
-        pytest.skip("reference snapshot beyond 200 steps only")
+    steps = [meta["steps"][-1]]
     got = run_gpu(case, "f32", arrays["f_0"].astype(np.float64), steps)
     err = G.rel_error(got[steps[0]], arrays[f"f_{steps[0]}"], case.q, case.ids == 1)
     print(f"{name} f32-storage: rel {err:.3e}", flush=True)
@@ -68,8 +66,6 @@ def test_gpu_fp16_storage_velocity_error(gpu_available, name):
     """FP16 shifted storage, FP32 collision: the velocity against OpenLB's FP64 run."""
     case, meta, arrays = G.load(name)
     s = meta["steps"][-1]
-    if s > 200:
-        pytest.skip("reference snapshot beyond 200 steps only")
     with BlockLattice.from_case(case, "f16") as lat:
         lat.upload_populations(arrays["f_0"].astype(np.float64), "f64")
         lat.collide_and_stream(s)
