@@ -382,7 +382,9 @@ int lbmh_ensemble_step(lbmh_ensemble* ensemble, int64_t n_steps, void* stream);
 int lbmh_ensemble_step_graph(lbmh_ensemble* ensemble, int64_t n_steps, void* stream);
 
 /* Launch shape of the ensemble step kernel (see lbmh_launch_config); the
- * defaults are the single-lattice ones for the storage type. */
+ * defaults are the single-lattice ones for the storage type, except that an
+ * FP64/FP32 ensemble with local-boundary members reserves 24 KB of LDS (their
+ * flat faces are collided in the step kernel).  A shape set here is kept. */
 int lbmh_ensemble_set_launch_config(lbmh_ensemble* ensemble, const lbmh_launch_config* config);
 /* members, cells in the launch, arena bytes in use and capacity */
 int lbmh_ensemble_usage(const lbmh_ensemble* ensemble, int* members, size_t* cells, size_t* bytes_used,
