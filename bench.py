@@ -231,6 +231,34 @@ def turbulent_bench(shape=(384, 192, 192), steps=5000):
     return out
 
 
+def porous_variants(steps=50):
+    """Config 2's other sizes and precisions: the same seeded porous channel at
+    512x256x256 in FP32, and at 256^3 and 512x256x256 in FP64 (library default
+    launch shapes).  MLUPS over cells with dynamics; TB/s from the algorithmic
+    bytes 2*19*(bytes per population) per cell update."""
+    from openlb_amd import workloads as W
+    from openlb_amd.lattice import BlockLattice
+    out = []
+    for shape, storage in (((512, 256, 256), "f32"), ((256, 256, 256), "f64"), ((512, 256, 256), "f64")):
+        case = W.porous_channel(shape)
+        cells = case.n_active
+        with BlockLattice.from_case(case, storage) as lat:
+            lat.ini_equilibrium(case.rho_u)
+            lat.collide_and_stream(10)
+            lat.synchronize()
+            t0 = time.perf_counter()
+            lat.collide_and_stream(steps)
+            lat.synchronize()
+            dt = (time.perf_counter() - t0) / steps
+            launch = lat.launch_config()
+        b = STORE_BYTES[storage]
+        out.append({"shape": list(shape), "storage": storage, "steps": steps, "ms_per_step": round(dt * 1e3, 4),
+                    "mlups": round(cells / dt / 1e6, 1),
+                    "hbm_gbs": round(2 * 19 * b * cells / dt / 1e9, 1), "launch": launch})
+        log(f"porous channel {shape} {storage}: {cells / dt / 1e6:.0f} MLUPS")
+    return out
+
+
 def cavity_bench(size=200, steps=200, cpu_size=100, cpu_seconds=10.0, no_cpu=False):
     """OpenLB's own benchmark harness case (examples/laminar/cavity3dBenchmark,
     cavity3d.cpp:154-250) in its default configuration: a (size+1)^3 lid-driven
@@ -425,6 +453,8 @@ def main():
     ap.add_argument("--no-turbulent", dest="turbulent", action="store_false",
                     help="skip the D3Q27 Smagorinsky channel precision run (config 3)")
     ap.add_argument("--turbulent-steps", type=int, default=5000)
+    ap.add_argument("--no-variants", dest="variants", action="store_false",
+                    help="skip config 2's 512x256x256 FP32 and FP64 lines")
     ap.add_argument("--no-cavity", dest="cavity", action="store_false",
                     help="skip OpenLB's cavity benchmark case (LocalVelocity boundaries)")
     ap.add_argument("--cavity-size", type=int, default=200)
@@ -503,6 +533,11 @@ def main():
         log("turbulent channel (config 3) ...")
         turbulent = turbulent_bench(steps=args.turbulent_steps)
 
+    variants = None
+    if args.variants:
+        log("porous channel variants (config 2) ...")
+        variants = porous_variants()
+
     cavity = None
     if args.cavity:
         log("cavity benchmark (LocalVelocity) ...")
@@ -557,6 +592,7 @@ def main():
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },
         "cpu_baseline": cpu,
+        "porous_channel_variants": variants,
         "turbulent_channel": turbulent,
         "ensemble": ensemble,
         "cavity_benchmark": cavity,
