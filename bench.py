@@ -110,7 +110,8 @@ def ensemble_bench(count=512, steps=2000, seed=20261204):
     steps. The plain baseline gives every cavity its own lattice and launches its steps
     one after another, as the reference runs UQ samples one SuperLattice at a time. The
     ensemble advances all members with one launch per step and replays the step loop
-    from a HIP graph. Velocity error: the FP16- and FP32-stored ensembles against the FP64
+    from a HIP graph, with the launch shape the ensemble autotuner picks on the
+    members themselves (outside the timed region). Velocity error: the FP16- and FP32-stored ensembles against the FP64
     ensemble, relative to each cavity's lid velocity (the FP64 path is pinned to OpenLB at
     1e-12); the FP32 figure on the member with the largest FP16 error tells how sensitive
     that flow is to rounding at all."""
@@ -150,6 +151,7 @@ def ensemble_bench(count=512, steps=2000, seed=20261204):
             members = [ens.add(c, steps=steps) for c in cases]
             for m, c in zip(members, cases):
                 ens.ini_equilibrium(m, c.rho_u)
+            launch = ens.autotune(trial_steps=5)     # members restored afterwards
             ens.step(0, graph=True)
             t0 = time.perf_counter()
             ens.step(steps, graph=True)
@@ -158,7 +160,7 @@ def ensemble_bench(count=512, steps=2000, seed=20261204):
             if done0 != steps:
                 raise SystemExit(f"ensemble member ran {done0} steps, expected {steps}")
             out[f"ensemble_{storage}"] = {"seconds": round(dt, 3), "solves_per_s": round(count / dt, 2),
-                                          "mlups": round(lups / dt / 1e6, 1)}
+                                          "mlups": round(lups / dt / 1e6, 1), "launch": launch}
             log(f"ensemble {storage}: {dt:.2f} s")
             final[storage] = [ens.moments(m)[1:] for m in members]
 

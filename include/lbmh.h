@@ -386,6 +386,14 @@ int lbmh_ensemble_step_graph(lbmh_ensemble* ensemble, int64_t n_steps, void* str
  * FP64/FP32 ensemble with local-boundary members reserves 24 KB of LDS (their
  * flat faces are collided in the step kernel).  A shape set here is kept. */
 int lbmh_ensemble_set_launch_config(lbmh_ensemble* ensemble, const lbmh_launch_config* config);
+
+/* Time the candidate launch shapes (LDS reservation x XCD group) on the
+ * ensemble's own members, trial_steps timed steps each, keep the fastest and
+ * return it in *chosen (may be NULL).  No OpenLB counterpart.  The members'
+ * populations and step state are copied aside and restored after every
+ * candidate, so the ensemble ends as it began; this needs device memory for
+ * one copy of the populations in use. */
+int lbmh_ensemble_autotune(lbmh_ensemble* ensemble, int trial_steps, lbmh_launch_config* chosen);
 /* members, cells in the launch, arena bytes in use and capacity */
 int lbmh_ensemble_usage(const lbmh_ensemble* ensemble, int* members, size_t* cells, size_t* bytes_used,
                         size_t* bytes_capacity);

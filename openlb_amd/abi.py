@@ -35,7 +35,7 @@ EXPORTS = (
     "lbmh_ensemble_ini_equilibrium", "lbmh_ensemble_upload", "lbmh_ensemble_download",
     "lbmh_ensemble_moments", "lbmh_ensemble_field_export", "lbmh_ensemble_checkpoint_size", "lbmh_ensemble_checkpoint_save",
     "lbmh_ensemble_checkpoint_load", "lbmh_ensemble_step", "lbmh_ensemble_step_graph",
-    "lbmh_ensemble_set_launch_config", "lbmh_ensemble_usage", "lbmh_ensemble_status_all",
+    "lbmh_ensemble_set_launch_config", "lbmh_ensemble_autotune", "lbmh_ensemble_usage", "lbmh_ensemble_status_all",
 )
 
 
@@ -149,6 +149,7 @@ def load() -> ctypes.CDLL:
     L.lbmh_ensemble_step.argtypes = [vp, i64, vp]
     L.lbmh_ensemble_step_graph.argtypes = [vp, i64, vp]
     L.lbmh_ensemble_set_launch_config.argtypes = [vp, ctypes.POINTER(LaunchConfig)]
+    L.lbmh_ensemble_autotune.argtypes = [vp, i32, ctypes.POINTER(LaunchConfig)]
     L.lbmh_ensemble_usage.argtypes = [vp, ctypes.POINTER(i32), szp, szp, szp]
     L.lbmh_ensemble_status_all.argtypes = [vp, vp, vp]
     L.lbmh_last_error.restype = ctypes.c_char_p

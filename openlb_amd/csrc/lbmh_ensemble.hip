@@ -1030,6 +1030,88 @@ int lbmh_ensemble_set_launch_config(lbmh_ensemble* E, const lbmh_launch_config* 
   return LBMH_OK;
 }
 
+int lbmh_ensemble_autotune(lbmh_ensemble* E, int trial_steps, lbmh_launch_config* chosen)
+{
+  if (!E) return fail(LBMH_E_INVALID, "null ensemble");
+  if (trial_steps < 1) return fail(LBMH_E_INVALID, "trial_steps must be >= 1");
+  DeviceGuard guard(E->device);
+  // The trials run real steps: the members' populations and the device table
+  // (shifts, step counters) are copied aside first and put back after each
+  // candidate, so every candidate times the same members and the ensemble
+  // ends as it began.
+  uint64_t end = 0;
+  for (const Member& m : E->members)
+    if (m.active) end = std::max(end, m.pop_off + m.pop_bytes);
+  struct Scratch {
+    void* pop = nullptr;
+    void* table = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~Scratch()
+    {
+      if (pop) (void)hipFree(pop);
+      if (table) (void)hipFree(table);
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+    }
+  } sc;
+  const size_t table_bytes = E->h_table.size();
+  LBMH_HIP(hipDeviceSynchronize());
+  if (end) {
+    const hipError_t e = hipMalloc(&sc.pop, end);
+    if (e != hipSuccess) return fail(LBMH_E_MEMORY, "autotune needs device memory for a copy of the members' populations");
+    LBMH_HIP(hipMemcpy(sc.pop, E->d_pop, end, hipMemcpyDeviceToDevice));
+  }
+  LBMH_HIP(hipMalloc(&sc.table, table_bytes));
+  LBMH_HIP(hipMemcpy(sc.table, E->d_table, table_bytes, hipMemcpyDeviceToDevice));
+  LBMH_HIP(hipEventCreate(&sc.e0));
+  LBMH_HIP(hipEventCreate(&sc.e1));
+  auto restore = [&]() -> int {
+    LBMH_HIP(hipDeviceSynchronize());
+    if (end) LBMH_HIP(hipMemcpy(E->d_pop, sc.pop, end, hipMemcpyDeviceToDevice));
+    LBMH_HIP(hipMemcpy(E->d_table, sc.table, table_bytes, hipMemcpyDeviceToDevice));
+    return LBMH_OK;
+  };
+  static const int kLds[] = {0, 16384, 24576, 32768, 40960, 49152};
+  static const int kGroup[] = {4, 8};
+  const lbmh_launch_config before = E->launch;
+  const bool before_chosen = E->launch_chosen;
+  lbmh_launch_config best = before;
+  float best_ms = 0.f;
+  bool have = false;
+  int rc = LBMH_OK;
+  for (int group : kGroup) {
+    for (int lds : kLds) {
+      E->launch = lbmh_launch_config{lds, group};
+      E->launch_chosen = true;  // time exactly this shape
+      rc = launch_steps(E, 1, nullptr);
+      if (rc == LBMH_OK && hipEventRecord(sc.e0, nullptr) != hipSuccess) rc = fail(LBMH_E_DEVICE, "hipEventRecord");
+      if (rc == LBMH_OK) rc = launch_steps(E, trial_steps, nullptr);
+      if (rc == LBMH_OK && hipEventRecord(sc.e1, nullptr) != hipSuccess) rc = fail(LBMH_E_DEVICE, "hipEventRecord");
+      if (rc == LBMH_OK && hipEventSynchronize(sc.e1) != hipSuccess) rc = fail(LBMH_E_DEVICE, "hipEventSynchronize");
+      float ms = 0.f;
+      if (rc == LBMH_OK && hipEventElapsedTime(&ms, sc.e0, sc.e1) != hipSuccess) rc = fail(LBMH_E_DEVICE, "hipEventElapsedTime");
+      const int rr = restore();
+      if (rc == LBMH_OK) rc = rr;
+      if (rc != LBMH_OK) {
+        E->launch = before;
+        E->launch_chosen = before_chosen;
+        E->version += 1;
+        return rc;
+      }
+      if (!have || ms < best_ms) {
+        best_ms = ms;
+        best = E->launch;
+        have = true;
+      }
+    }
+  }
+  E->launch = best;
+  E->launch_chosen = true;
+  E->version += 1;  // a captured step loop is rebuilt with the chosen shape
+  if (chosen) *chosen = best;
+  return LBMH_OK;
+}
+
 int lbmh_ensemble_usage(const lbmh_ensemble* E, int* members, size_t* cells, size_t* used, size_t* capacity)
 {
   if (!E) return fail(LBMH_E_INVALID, "null ensemble");

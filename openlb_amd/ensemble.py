@@ -167,6 +167,13 @@ class Ensemble:
         cfg = abi.LaunchConfig(int(lds_bytes), int(xcd_group))
         abi.check(abi.load().lbmh_ensemble_set_launch_config(self._h, ctypes.byref(cfg)))
 
+    def autotune(self, trial_steps: int = 5) -> dict:
+        """Time the launch shapes on the members themselves and keep the fastest;
+        the members' populations and step state are restored afterwards."""
+        cfg = abi.LaunchConfig()
+        abi.check(abi.load().lbmh_ensemble_autotune(self._h, int(trial_steps), ctypes.byref(cfg)))
+        return {"lds_bytes": cfg.lds_bytes, "xcd_group": cfg.xcd_group}
+
     def usage(self) -> dict:
         m = ctypes.c_int()
         cells, used, cap = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()

@@ -214,3 +214,35 @@ def test_local_boundary_members_match_standalone(gpu_available, storage):
         for m, c, f0 in zip(members, cases, f0s):
             assert ens.status(m) == (41, -1)
             np.testing.assert_array_equal(ens.download(m), standalone(c, storage, f0, 41))
+
+
+def test_autotune_leaves_the_members_as_they_were(gpu_available):
+    """lbmh_ensemble_autotune runs real steps on the members: their populations
+    and step state must come back exactly, bounded budgets included, and the
+    ensemble then steps bit-identically to one that was never tuned."""
+    cases = small_members(seed=11, count=4) + [W.local_velocity_cavity((9, 8, 7), tau=0.7, lid_u=0.06)]
+    finals = []
+    for tune in (False, True):
+        with Ensemble(19, "f32", "bgk", max_members=8, population_bytes=64 << 20, cell_capacity=1 << 20) as ens:
+            members = []
+            for c in cases:
+                m = ens.add(c, steps=25)
+                ens.upload(m, O.initial_populations(c, "f32"))
+                if c.local:
+                    ens.initialize(m)
+                members.append(m)
+            ens.step(7)
+            if tune:
+                before = [ens.download(m) for m in members]
+                chosen = ens.autotune(trial_steps=2)
+                assert chosen["lds_bytes"] in (0, 16384, 24576, 32768, 40960, 49152)
+                assert chosen["xcd_group"] in (4, 8)
+                for m, f in zip(members, before):
+                    assert ens.status(m) == (7, 18)
+                    np.testing.assert_array_equal(ens.download(m), f)
+            ens.step(30, graph=True)
+            for m in members:
+                assert ens.status(m) == (25, 0)
+            finals.append([ens.download(m) for m in members])
+    for a, b in zip(*finals):
+        np.testing.assert_array_equal(a, b)
