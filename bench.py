@@ -192,11 +192,12 @@ def ensemble_bench(count=512, steps=2000, seed=20261204):
     return out
 
 
-def turbulent_bench(shape=(384, 192, 192), steps=5000):
+def turbulent_bench(shape=(384, 192, 192), steps=5000, no_cpu=False, cpu_seconds=10.0):
     """Config 3: D3Q27 Smagorinsky-BGK channel (C = 0.14, tau = 0.505). It is periodic in x
     and z, with BounceBack walls at y, a laminar profile and seeded +/-1% velocity noise.
     It runs `steps` steps in FP64, FP32 and FP16 storage. It reports throughput and the
-    max / L2 errors of rho and u of the FP32 and FP16 runs against the FP64 run. The FP64
+    max / L2 errors of rho and u of the FP32 and FP16 runs against the FP64 run, and
+    OpenLB's own CPU path on a 128^3 sample beside it. The FP64
     device path is pinned to OpenLB at 1e-12; D3Q27 Smagorinsky is the generic path, whose
     golden case matches at 2e-15."""
     from openlb_amd import workloads as W
@@ -230,6 +231,10 @@ def turbulent_bench(shape=(384, 192, 192), steps=5000):
             "u_max_over_umax": float(np.abs(d[1:]).max() / umax),
             "u_l2_relative": float(np.sqrt((d[1:] ** 2).sum() / (ref[1:] ** 2).sum()))}
     out["u_max_f64"] = umax
+    if not no_cpu:
+        # OpenLB's CPU_SIMD + OpenMP path on a 128^3 sample of the same channel (FP32)
+        out["cpu_baseline_f32"] = cpu_baseline(case, "f32", cpu_seconds)
+        log(f"turbulent channel cpu baseline {out['cpu_baseline_f32']['value']} MLUPS")
     return out
 
 
@@ -533,7 +538,7 @@ def main():
     turbulent = None
     if args.turbulent:
         log("turbulent channel (config 3) ...")
-        turbulent = turbulent_bench(steps=args.turbulent_steps)
+        turbulent = turbulent_bench(steps=args.turbulent_steps, no_cpu=args.no_cpu)
 
     variants = None
     if args.variants:
