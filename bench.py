@@ -104,7 +104,30 @@ def calibrate_copy(mib: int, repeats: int = 5):
                       "repeats": repeats, "copy_gbs": round(2 * nbytes / dt / 1e9, 1)}), flush=True)
 
 
-def ensemble_bench(count=512, steps=2000, seed=20261204):
+def ensemble_cpu_baseline(cases, steps):
+    """OpenLB's CPU_SIMD + OpenMP path, the way OpenLB runs UQ samples: one
+    lattice after the other.  Every 32nd cavity runs 200 steps; the aggregate
+    rate over that sample prices the whole ensemble's cell updates."""
+    from oracle import compare_cpu
+    if not os.path.exists(compare_cpu.REF_SIMD):
+        return None
+    threads = min(16, os.cpu_count() or 1)
+    sample = cases[::32]
+    lups = secs = 0.0
+    for c in sample:
+        r = compare_cpu.time_reference(c, 200, threads)
+        lups += r["cells"] * r["steps"]
+        secs += r["seconds"]
+    rate = lups / secs
+    total = sum(c.ncore for c in cases) * steps / rate
+    return {"value": round(len(cases) / total, 3), "unit": "solves/s", "mlups": round(rate / 1e6, 1),
+            "cores": threads, "kind": "reference",
+            "sample": f"{len(sample)} of the {len(cases)} cavities, 200 steps each, OpenLB 1.8 CPU_SIMD (AVX2) + "
+                      f"OpenMP, {threads} threads, one lattice after the other; solves/s = members / "
+                      f"(all members' cell updates at {steps} steps / aggregate rate)"}
+
+
+def ensemble_bench(count=512, steps=2000, seed=20261204, no_cpu=False):
     """Config 4: `count` independent D3Q19 BGK lid-driven cavities (edges log-uniform
     in [16, 96], tau in [0.52, 1.2], lid velocity in [0.02, 0.1]), each run for `steps`
     steps. The plain baseline gives every cavity its own lattice and launches its steps
@@ -189,6 +212,10 @@ def ensemble_bench(count=512, steps=2000, seed=20261204):
                                                      out["plain_f32"]["solves_per_s"], 2)
     out["speedup_f32_ensemble_vs_plain_f32"] = round(out["ensemble_f32"]["solves_per_s"] /
                                                      out["plain_f32"]["solves_per_s"], 2)
+    if not no_cpu:
+        out["cpu_baseline"] = ensemble_cpu_baseline(cases, steps)
+        if out["cpu_baseline"]:
+            log(f"ensemble cpu baseline {out['cpu_baseline']['value']} solves/s")
     return out
 
 
@@ -533,7 +560,7 @@ def main():
     ensemble = None
     if args.ensemble:
         log("ensemble (config 4) ...")
-        ensemble = ensemble_bench(args.ensemble_members, args.ensemble_steps)
+        ensemble = ensemble_bench(args.ensemble_members, args.ensemble_steps, no_cpu=args.no_cpu)
 
     turbulent = None
     if args.turbulent:
