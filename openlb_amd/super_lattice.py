@@ -33,15 +33,25 @@ def _ptr(a: np.ndarray):
 
 
 def split_axis(n: int, parts: int) -> list[tuple[int, int]]:
-    """[begin, end) ranges of `parts` nearly equal pieces of 0..n-1."""
+    """[begin, end) ranges of the `parts` pieces OpenLB's ``Cuboid::divide`` cuts
+    0..n-1 into: piece k has (n + parts - k - 1) // parts cells, so the pieces
+    differ by at most one cell and the larger ones come first
+    (``src/geometry/cuboid.hh:184-211``)."""
     if parts < 1 or parts > n:
         raise ValueError(f"cannot split {n} cells into {parts} blocks")
-    edges = [n * k // parts for k in range(parts + 1)]
-    return [(edges[k], edges[k + 1]) for k in range(parts)]
+    out, begin = [], 0
+    for k in range(parts):
+        size = (n + parts - k - 1) // parts
+        out.append((begin, begin + size))
+        begin += size
+    return out
 
 
 class SuperLattice:
-    """A case split into blocks[0] x blocks[1] x blocks[2] cuboids on one device."""
+    """A case split into blocks[0] x blocks[1] x blocks[2] cuboids on one device:
+    the cuboids, and their order (x outermost, z innermost), of OpenLB's
+    ``Cuboid::divide({blocks})`` (``src/geometry/cuboid.hh:184-211``), which
+    ``CuboidDecomposition::splitRegular`` uses."""
 
     def __init__(self, case, storage: str = "f32", blocks=(2, 1, 1), device: int = 0):
         if getattr(case, "local", None):

@@ -90,3 +90,22 @@ def test_library_classification_matches_reference(geometry):
     z = np.load(f"{G.GOLDEN}/normals.npz", allow_pickle=False)
     got = classify_boundary(z[f"{geometry}_ids"], tuple(z[f"{geometry}_periodic"]))
     np.testing.assert_array_equal(got, z[f"{geometry}_normals"])
+
+
+def test_split_axis_cuts_as_openlb_divides_a_cuboid():
+    """Cuboid::divide (src/geometry/cuboid.hh:184-211) gives piece k of n cells
+    in p parts (n + p - k - 1) / p cells: 10 in 3 -> 4, 3, 3; 7 in 2 -> 4, 3."""
+    from openlb_amd.super_lattice import split_axis
+
+    assert split_axis(10, 3) == [(0, 4), (4, 7), (7, 10)]
+    assert split_axis(7, 2) == [(0, 4), (4, 7)]
+    assert split_axis(64, 3) == [(0, 22), (22, 43), (43, 64)]
+    for n in (1, 7, 64, 257):
+        for parts in range(1, min(n, 6) + 1):
+            r = split_axis(n, parts)
+            sizes = [b - a for a, b in r]
+            assert r[0][0] == 0 and r[-1][1] == n
+            assert all(r[k][1] == r[k + 1][0] for k in range(parts - 1))
+            assert sizes == sorted(sizes, reverse=True) and max(sizes) - min(sizes) <= 1
+    with pytest.raises(ValueError):
+        split_axis(3, 4)

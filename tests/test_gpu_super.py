@@ -12,7 +12,7 @@ import pytest
 from oracle import oracle as O
 from openlb_amd import workloads as W
 from openlb_amd.lattice import BlockLattice
-from openlb_amd.super_lattice import SuperLattice, split_axis
+from openlb_amd.super_lattice import SuperLattice
 
 pytestmark = pytest.mark.gpu
 
@@ -44,11 +44,3 @@ def test_split_lattice_equals_single_block(gpu_available, label, make, storage, 
     # with the padded extent and never reach a fluid cell: compare bulk cells
     mask = case.ids == 1
     np.testing.assert_array_equal(got[:, mask], ref[:, mask])
-
-
-def test_split_axis_covers_every_cell():
-    for n in (1, 7, 64):
-        for parts in range(1, min(n, 5) + 1):
-            r = split_axis(n, parts)
-            assert r[0][0] == 0 and r[-1][1] == n
-            assert all(a < b for a, b in r) and all(r[k][1] == r[k + 1][0] for k in range(parts - 1))
