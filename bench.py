@@ -15,9 +15,10 @@ The output is one JSON line on stdout. ``value`` is MLUPS, i.e. updated cells
 steps. That region is bracketed by device synchronisations. ``roofline`` prices
 the kernel's algorithmic traffic, 2*Q*sizeof(stored population) bytes per cell
 update, against its average per-launch duration measured with HIP events on the
-launch stream. ``cpu_baseline`` is the oracle (the CPU restatement of OpenLB's time
-step, OpenMP over x-planes) timed on the host cores on a bounded sample of the same
-workload.
+launch stream. ``cpu_baseline`` is OpenLB's own CPU path (CPU_SIMD + OpenMP, built
+from the reference sources into oracle/_ref, kind "reference") timed on the host
+cores on a bounded sample of the same workload; where that binary is absent, the
+oracle (the CPU restatement of OpenLB's time step, kind "port") is timed instead.
 """
 from __future__ import annotations
 
