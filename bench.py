@@ -105,6 +105,73 @@ def calibrate_copy(mib: int, repeats: int = 5):
                       "repeats": repeats, "copy_gbs": round(2 * nbytes / dt / 1e9, 1)}), flush=True)
 
 
+def copy_ceiling(bytes_moved: int, repeats: int = 10):
+    """Plain copies moving the step kernel's algorithmic bytes per launch, half
+    read and half written, timed with HIP events on the null stream: the
+    runtime's device-to-device hipMemcpyAsync and a vectorised elementwise
+    kernel. A plain copy is the most favourable access pattern for the HBM, so
+    it prices what a kernel that must move these bytes can reach on this part,
+    beside the vendor peak."""
+    ev = HipEvents()
+    hip = ev.hip
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    hip.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                   ctypes.c_void_p]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    half = bytes_moved // 2
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), half) != 0:
+        raise RuntimeError("hipMalloc failed")
+    if hip.hipMalloc(ctypes.byref(b), half) != 0:
+        hip.hipFree(a)
+        raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(a, 1, half) != 0:
+            raise RuntimeError("hipMemset failed")
+        evs = ev.create(2 * repeats + 2)
+        times = []
+        for r in range(repeats + 1):  # the first copy is a warm-up
+            ev.record(evs[2 * r])
+            if hip.hipMemcpyAsync(b, a, half, 3, None) != 0:  # hipMemcpyDeviceToDevice
+                raise RuntimeError("hipMemcpyAsync failed")
+            ev.record(evs[2 * r + 1])
+        for r in range(1, repeats + 1):
+            times.append(ev.elapsed_ms(evs[2 * r], evs[2 * r + 1]))
+        ev.destroy(evs)
+    finally:
+        hip.hipDeviceSynchronize()
+        hip.hipFree(a)
+        hip.hipFree(b)
+    ms = float(np.mean(times))
+    memcpy = {"gbs": round(2 * half / (ms * 1e-3) / 1e9, 1), "avg_ms": round(ms, 5)}
+
+    # a vectorised elementwise kernel (torch's neg over float32, 16-byte loads
+    # and stores) moving the same bytes, on the same null stream
+    import torch
+    src = torch.ones(half // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    evs = ev.create(2 * repeats + 2)
+    for r in range(repeats + 1):
+        ev.record(evs[2 * r])
+        torch.neg(src, out=dst)
+        ev.record(evs[2 * r + 1])
+    times = [ev.elapsed_ms(evs[2 * r], evs[2 * r + 1]) for r in range(1, repeats + 1)]
+    ev.destroy(evs)
+    torch.cuda.synchronize()
+    if not bool((dst == -1).all()):
+        raise RuntimeError("elementwise copy gave wrong values")
+    del src, dst
+    torch.cuda.empty_cache()
+    ms = float(np.mean(times))
+    elementwise = {"gbs": round(2 * (half // 4) * 4 / (ms * 1e-3) / 1e9, 1), "avg_ms": round(ms, 5)}
+    return {"gbs": max(memcpy["gbs"], elementwise["gbs"]), "hipMemcpyAsync": memcpy,
+            "elementwise_f32": elementwise, "repeats": repeats,
+            "how": f"{half} B read and written (the step kernel's bytes per launch) by hipMemcpyAsync "
+                   f"device-to-device and by a vectorised float32 elementwise kernel, HIP events on the "
+                   f"null stream; gbs is the faster"}
+
+
 def ensemble_cpu_baseline(cases, steps):
     """OpenLB's CPU_SIMD + OpenMP path, the way OpenLB runs UQ samples: one
     lattice after the other.  Every 32nd cavity runs 200 steps; the aggregate
@@ -600,6 +667,9 @@ def main():
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     log(f"{mlups:.0f} MLUPS, kernel avg {avg_ms:.4f} ms (min {min(kernel_ms):.4f}), "
         f"{achieved:.0f} GB/s algorithmic")
+    copy = copy_ceiling(bytes_per_launch)
+    log(f"copies of the same bytes: hipMemcpyAsync {copy['hipMemcpyAsync']['gbs']} GB/s, "
+        f"elementwise {copy['elementwise_f32']['gbs']} GB/s")
 
     ensemble = None
     if args.ensemble:
@@ -668,6 +738,8 @@ def main():
             "kernel": kernel_name,
             "kernel_avg_ms": round(avg_ms, 5),
             "algorithmic_bytes_per_launch": bytes_per_launch,
+            "device_copy": copy,
+            "frac_of_device_copy": round(achieved / copy["gbs"], 4),
         },
         "cpu_baseline": cpu,
         "porous_channel_variants": variants,
