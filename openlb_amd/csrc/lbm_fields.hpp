@@ -37,17 +37,39 @@ struct FieldConversion {
   double pressure_offset;  // charPhysPressure
 };
 
+// Lanes run along z, the axis the population columns are contiguous in: a
+// workgroup of 256 lanes covers 32 z x 8 x points of one y row, so every
+// population column is read in whole 128-byte lines, and the eight waves'
+// float stores into the same lines of the x-fastest output merge in the L2.
+// (One lane per point in output order puts consecutive lanes one x plane
+// apart, i.e. one cache line per lane and population.)
+constexpr uint32_t kExportTileZ = 32, kExportTileX = 8;
+
+inline uint32_t field_export_blocks(int nx, int ny, int nz, int overlap)
+{
+  const uint32_t ex = uint32_t(nx + 2 * overlap), ey = uint32_t(ny + 2 * overlap), ez = uint32_t(nz + 2 * overlap);
+  return ((ex + kExportTileX - 1) / kExportTileX) * ey * ((ez + kExportTileZ - 1) / kExportTileZ);
+}
+
+// launched with field_export_blocks(...) workgroups of kExportTileZ * kExportTileX lanes
 template <int Q, typename S>
 __global__ __launch_bounds__(256) void field_export_kernel(PostArgs<Q> a, int overlap, int field, FieldConversion conv,
                                                            int forced, double hfx, double hfy, double hfz,
                                                            float* __restrict__ out, uint32_t points)
 {
+  static_assert(kExportTileZ * kExportTileX == 256, "one workgroup is one tile");
   using V = typename StorageTraits<S>::V;
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= points) return;
   const uint32_t ex = uint32_t(a.nx + 2 * overlap), ey = uint32_t(a.ny + 2 * overlap);
-  const uint32_t t = p / ex;
-  const int x = int(p - t * ex) - overlap, y = int(t % ey) - overlap, z = int(t / ey) - overlap;
+  const uint32_t ez = uint32_t(a.nz + 2 * overlap);
+  const uint32_t tiles_x = (ex + kExportTileX - 1) / kExportTileX;
+  const uint32_t rest = blockIdx.x / tiles_x;
+  const uint32_t px = (blockIdx.x - rest * tiles_x) * kExportTileX + threadIdx.x / kExportTileZ;
+  const uint32_t py = rest % ey;
+  const uint32_t pz = (rest / ey) * kExportTileZ + threadIdx.x % kExportTileZ;
+  if (px >= ex || py >= ey || pz >= ez) return;
+  const uint32_t p = (pz * ey + py) * ex + px;
+  if (p >= points) return;
+  const int x = int(px) - overlap, y = int(py) - overlap, z = int(pz) - overlap;
   int r[3] = {x, y, z};
   const int n[3] = {a.nx, a.ny, a.nz};
   bool inside = true, image = false;

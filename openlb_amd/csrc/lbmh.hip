@@ -941,8 +941,9 @@ int lbmh_field_export(lbmh_lattice* L, int field, int overlap, const double conv
   const int rc = with_types(L, [&](auto qc, auto s) -> int {
     constexpr int Q = decltype(qc)::value;
     using S = decltype(s);
-    hipLaunchKernelGGL((field_export_kernel<Q, S>), dim3(grid_for(uint32_t(points), 256)), dim3(256), 0, 0,
-                       post_args<Q>(L), overlap, field, conv, forced, F[0] * 0.5, F[1] * 0.5, F[2] * 0.5,
+    hipLaunchKernelGGL((field_export_kernel<Q, S>),
+                       dim3(field_export_blocks(L->n[0], L->n[1], L->n[2], overlap)),
+                       dim3(kExportTileZ * kExportTileX), 0, 0, post_args<Q>(L), overlap, field, conv, forced, F[0] * 0.5, F[1] * 0.5, F[2] * 0.5,
                        static_cast<float*>(out.p), uint32_t(points));
     LBMH_HIP(hipGetLastError());
     return LBMH_OK;

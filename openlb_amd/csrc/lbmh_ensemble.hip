@@ -905,6 +905,7 @@ int lbmh_ensemble_field_export(lbmh_ensemble* E, int m, int field, int overlap, 
   if (floats != want)
     return fail(LBMH_E_INVALID, "field buffer holds " + std::to_string(floats) + " floats, the export " +
                                     std::to_string(want));
+  if (points > 0xffffffffull) return fail(LBMH_E_INVALID, "member too large for one export");
   DeviceGuard guard(E->device);
   rc = pull_table(E);
   if (rc != LBMH_OK) return rc;
@@ -937,7 +938,8 @@ int lbmh_ensemble_field_export(lbmh_ensemble* E, int m, int field, int overlap, 
     a.div_z = en.div_z;
     a.div_y = en.div_y;
     a.omega = en.bp.omega;
-    hipLaunchKernelGGL((field_export_kernel<Q, S>), dim3(grid_for(uint32_t(points), 256)), dim3(256), 0, 0, a,
+    hipLaunchKernelGGL((field_export_kernel<Q, S>), dim3(field_export_blocks(a.nx, a.ny, a.nz, overlap)),
+                       dim3(kExportTileZ * kExportTileX), 0, 0, a,
                        overlap, field, conv, forced, F[0] * 0.5, F[1] * 0.5, F[2] * 0.5,
                        static_cast<float*>(out.p), uint32_t(points));
     LBMH_HIP(hipGetLastError());
