@@ -89,9 +89,12 @@ def test_device_export_matches_openlb_vtk(gpu_available, name, tmp_path):
 
 @pytest.mark.parametrize("storage", ["f64", "f32", "f16"])
 def test_device_export_matches_oracle(gpu_available, storage):
-    """A forced Couette channel (periodic overlap, moving wall) and a local-velocity cavity, mid-run."""
+    """A forced Couette channel (periodic overlap, moving wall) and a local-velocity cavity, mid-run;
+    the 19 x 6 x 70 channel spans several of the kernel's 32 (z) x 8 (x) point tiles, with
+    partial tiles at the high ends of both axes."""
     for case in (W.couette_channel((9, 7, 6), tau=0.7, wall_u=0.05, force=1e-5),
-                 W.local_velocity_cavity((9, 8, 7), tau=0.7, lid_u=0.06)):
+                 W.local_velocity_cavity((9, 8, 7), tau=0.7, lid_u=0.06),
+                 W.couette_channel((19, 6, 70), tau=0.7, wall_u=0.05, force=1e-5)):
         f0 = O.initial_populations(case, storage)
         with BlockLattice.from_case(case, storage) as lat:
             lat.upload_populations(f0, "f64")
