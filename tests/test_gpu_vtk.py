@@ -110,8 +110,9 @@ def test_device_export_matches_oracle(gpu_available, storage):
                     assert_close(f.evaluate(overlap), ora.field_export(field, overlap, conv), (storage, field, overlap))
 
 
-def test_ensemble_member_export_equals_lattice_export(gpu_available):
-    case = W.couette_channel((9, 7, 6), tau=0.7, wall_u=0.05, force=0.0, kind="bgk")
+@pytest.mark.parametrize("shape", [(9, 7, 6), (19, 6, 70)], ids=["one_tile", "several_tiles"])
+def test_ensemble_member_export_equals_lattice_export(gpu_available, shape):
+    case = W.couette_channel(shape, tau=0.7, wall_u=0.05, force=0.0, kind="bgk")
     f0 = O.initial_populations(case, "f32")
     with Ensemble(19, "f32", "bgk", max_members=4, population_bytes=8 << 20, cell_capacity=1 << 16) as ens:
         other = ens.add(W.lid_driven_cavity((6, 5, 7), tau=0.6, lid_u=0.05))
