@@ -209,6 +209,20 @@ int lbmh_field_export(lbmh_lattice* lattice, int field, int overlap, const doubl
  * (src/core/platform/gpu/cuda/fieldReduction.hh:152-170). */
 int lbmh_reduce_moments(lbmh_lattice* lattice, double out[4]);
 
+/* lbmh_reduce_moments for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in OpenLB.  The same four FP64 sums, bit for bit,
+ * written to out_device (4 doubles of device memory the caller owns),
+ * asynchronously on `stream` (NULL = null stream), ordered after the work
+ * already queued there; nothing is allocated, copied to the host or
+ * synchronised.  The per-workgroup partial sums live in a fixed workspace of
+ * the library on each device (1024 workgroups x 4 doubles, 32 KiB, held until
+ * the process ends); a reduction that would need more partials is refused
+ * with LBMH_E_INVALID.  Every call on one device shares that workspace:
+ * calls that overlap on one device (different streams, different host
+ * threads) race and must be serialised by the caller. */
+int lbmh_reduce_moments_noalloc(lbmh_lattice* lattice, double* out_device, void* stream);
+
 /* Local (regularized) boundaries, OpenLB's boundary::LocalVelocity and
  * boundary::LocalPressure with BGK bulk dynamics mixed in
  * (src/boundary/localVelocity3D.h:36-116, localPressure3D.h:36-66, set by

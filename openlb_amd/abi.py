@@ -23,7 +23,8 @@ EXPORTS = (
     "lbmh_set_bulk_params", "lbmh_set_wall", "lbmh_populations_upload",
     "lbmh_populations_download", "lbmh_ini_equilibrium", "lbmh_step",
     "lbmh_get_step_count", "lbmh_get_shifts", "lbmh_checkpoint_size", "lbmh_checkpoint_save",
-    "lbmh_checkpoint_load", "lbmh_field_export", "lbmh_moments", "lbmh_reduce_moments", "lbmh_statistics",
+    "lbmh_checkpoint_load", "lbmh_field_export", "lbmh_moments", "lbmh_reduce_moments",
+    "lbmh_reduce_moments_noalloc", "lbmh_statistics",
     "lbmh_set_local_boundary", "lbmh_classify_boundary", "lbmh_initialize",
     "lbmh_collide", "lbmh_stream", "lbmh_halo_create", "lbmh_halo_destroy", "lbmh_halo_copy",
     "lbmh_synchronize", "lbmh_lattice_footprint", "lbmh_abi_version", "lbmh_device_count",
@@ -113,6 +114,7 @@ def load() -> ctypes.CDLL:
     L.lbmh_ensemble_checkpoint_load.argtypes = [vp, i32, i32, vp, ctypes.c_size_t]
     L.lbmh_moments.argtypes = [vp, vp]
     L.lbmh_reduce_moments.argtypes = [vp, vp]
+    L.lbmh_reduce_moments_noalloc.argtypes = [vp, vp, vp]
     L.lbmh_statistics.argtypes = [vp, vp]
     L.lbmh_set_local_boundary.argtypes = [vp, i32, i32, ctypes.c_double, vp]
     L.lbmh_classify_boundary.argtypes = [vp, vp, vp, vp]

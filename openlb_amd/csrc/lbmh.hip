@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -25,6 +26,39 @@
 using namespace lbmh;
 
 thread_local std::string lbmh::g_error;
+
+// Workspace of lbmh_reduce_moments_noalloc: the per-workgroup partial sums of
+// the FP64 moment reduction, kScratchPartials workgroups x 4 doubles (32 KiB
+// per device, held until the process ends).  Host code never uses the name
+// `scratch` as an address: each device's copy is reached through the pointer
+// hipGetSymbolAddress returns for it (scratch_address).
+static constexpr uint32_t kScratchPartials = 1024;
+static __device__ __align__(256) double scratch[4 * kScratchPartials];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device `device`, looked up once
+// per device with that device current.
+int scratch_address(int device, double** out)
+{
+  if (device < 0 || device >= kMaxDevices) return fail(LBMH_E_INVALID, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return fail(LBMH_E_DEVICE, "hipGetSymbolAddress(scratch) returned null");
+    g_scratch_ptr[device] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return LBMH_OK;
+}
+
+}  // namespace
 
 struct lbmh_lattice {
   int device = 0;
@@ -999,6 +1033,35 @@ int lbmh_reduce_moments(lbmh_lattice* L, double out[4])
   hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(kBlock), 0, 0, result, L->d_partials, blocks);
   LBMH_HIP(hipGetLastError());
   LBMH_HIP(hipMemcpy(out, result, sizeof(double) * 4, hipMemcpyDeviceToHost));
+  return LBMH_OK;
+}
+
+int lbmh_reduce_moments_noalloc(lbmh_lattice* L, double* out_device, void* stream)
+{
+  if (!L || !out_device) return fail(LBMH_E_INVALID, "null argument");
+  DeviceGuard guard(L->device);
+  // the same workgroup count as lbmh_reduce_moments, so the sums agree bit for bit
+  const uint32_t blocks = std::min<uint32_t>(L->n_partials, grid_for(L->ncore, kBlock));
+  if (blocks > kScratchPartials)
+    return fail(LBMH_E_INVALID, "reduction needs " + std::to_string(blocks) + " partial sums, the workspace holds " +
+                                    std::to_string(kScratchPartials));
+  double* partials = nullptr;
+  const int rs = scratch_address(L->device, &partials);
+  if (rs != LBMH_OK) return rs;
+  const CoreGeom g = core_geom(L);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // every one of the `blocks` partials is written before it is read: nothing to clear
+  const int rc = with_types(L, [&](auto qc, auto st) -> int {
+    constexpr int Q = decltype(qc)::value;
+    using S = decltype(st);
+    hipLaunchKernelGGL((reduce_moments_kernel<Q, S>), dim3(blocks), dim3(kBlock), 0, s, partials,
+                       L->uniform_bulk ? nullptr : L->d_ids, static_cast<const S*>(L->d_pop), g, shifts_of<Q>(L));
+    LBMH_HIP(hipGetLastError());
+    return LBMH_OK;
+  });
+  if (rc != LBMH_OK) return rc;
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(kBlock), 0, s, out_device, partials, blocks);
+  LBMH_HIP(hipGetLastError());
   return LBMH_OK;
 }
 
