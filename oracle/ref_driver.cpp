@@ -8,7 +8,10 @@
 // Input: a case directory holding
 //   case.txt  key/value lines (q, dyn, nx, ny, nz, px, py, pz, omega, magic,
 //             smago, fx, fy, fz, steps, wall <id> <rho> <ux> <uy> <uz>,
-//             local <id> velocity|pressure|interp_velocity|interp_pressure)
+//             local <id> velocity|pressure|interp_velocity|interp_pressure,
+//             split <width>: CuboidDecomposition::splitRegular(0, width),
+//             i.e. max(1, n_d / width) cuboids per axis cut by Cuboid::divide,
+//             cuboidDecomposition.hh:755-763, cuboid.hh:184-211)
 //   ids.bin   uint8  [nx][ny][nz]  per-cell dynamics id
 //             (0 none, 1 bulk, 2 BounceBack, 3.. BounceBackVelocity walls)
 //   rhou.bin  double [4][nx][ny][nz] initial rho, ux, uy, uz
@@ -51,6 +54,7 @@
 #include "olb.h"
 
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -79,6 +83,7 @@ struct CaseSpec {
   std::vector<long> steps;
   std::vector<WallSpec> walls;
   std::map<int, std::string> local;
+  int split = 0;   // splitRegular width; 0 keeps one cuboid
   std::vector<unsigned char> ids;
   std::vector<double> rhou;
 };
@@ -113,6 +118,7 @@ static CaseSpec readCase(const std::string& dir)
     else if (key == "fy") ls >> c.force[1];
     else if (key == "fz") ls >> c.force[2];
     else if (key == "steps") { long s; while (ls >> s) c.steps.push_back(s); }
+    else if (key == "split") ls >> c.split;
     else if (key == "local") {
       int id; std::string kind; ls >> id >> kind;
       if (kind != "velocity" && kind != "pressure" && kind != "interp_velocity" && kind != "interp_pressure")
@@ -137,23 +143,40 @@ static CaseSpec readCase(const std::string& dir)
   return c;
 }
 
+// Calls f(local block index, local cell, global cell) for every core cell of
+// every block this process holds; cuboid origins are integers (spacing 1,
+// origin 0).
+template <typename F>
+static void forEachCoreCell(SuperGeometry<T,3>& geometry, F f)
+{
+  auto& load = geometry.getLoadBalancer();
+  auto& cuboids = geometry.getCuboidDecomposition();
+  for (int iC = 0; iC < load.size(); ++iC) {
+    const auto& cuboid = cuboids.get(load.glob(iC));
+    const auto origin = cuboid.getOrigin();
+    const auto extent = cuboid.getExtent();
+    const int o[3] = {int(std::lround(double(origin[0]))), int(std::lround(double(origin[1]))),
+                      int(std::lround(double(origin[2])))};
+    for (int x = 0; x < extent[0]; ++x)
+      for (int y = 0; y < extent[1]; ++y)
+        for (int z = 0; z < extent[2]; ++z)
+          f(iC, LatticeR<3>{x, y, z}, LatticeR<3>{o[0] + x, o[1] + y, o[2] + z});
+  }
+}
+
 template <typename DESCRIPTOR>
-static void dumpPopulations(const CaseSpec& c, SuperLattice<T,DESCRIPTOR>& lattice,
+static void dumpPopulations(const CaseSpec& c, SuperGeometry<T,3>& geometry, SuperLattice<T,DESCRIPTOR>& lattice,
                             const std::string& dir, long step)
 {
   const std::size_t nCore = std::size_t(c.n[0]) * c.n[1] * c.n[2];
   std::vector<double> out(std::size_t(DESCRIPTOR::q) * nCore);
-  auto& block = lattice.getBlock(0);
-  for (int x = 0; x < c.n[0]; ++x) {
-    for (int y = 0; y < c.n[1]; ++y) {
-      for (int z = 0; z < c.n[2]; ++z) {
-        auto cell = block.get(LatticeR<3>{x, y, z});
-        for (int i = 0; i < int(DESCRIPTOR::q); ++i) {
-          out[std::size_t(i) * nCore + coreIndex(c, x, y, z)] = double(cell[i]);
-        }
-      }
+  forEachCoreCell(geometry, [&](int iC, LatticeR<3> local, LatticeR<3> global) {
+    auto cell = lattice.getBlock(iC).get(local);
+    const std::size_t k = coreIndex(c, global[0], global[1], global[2]);
+    for (int i = 0; i < int(DESCRIPTOR::q); ++i) {
+      out[std::size_t(i) * nCore + k] = double(cell[i]);
     }
-  }
+  });
   const std::string name = dir + "/pops_" + std::to_string(step) + ".bin";
   std::FILE* f = std::fopen(name.c_str(), "wb");
   if (!f || std::fwrite(out.data(), sizeof(double), out.size(), f) != out.size()) {
@@ -191,18 +214,20 @@ static void dumpNormals(const CaseSpec& c, SuperGeometry<T,3>& geometry, const s
 template <typename DESCRIPTOR, typename BULK, bool FORCED>
 static void runCase(const CaseSpec& c, const std::string& dir)
 {
-  // One cuboid of exactly n[0] x n[1] x n[2] cells, spacing 1.
+  // One cuboid of exactly n[0] x n[1] x n[2] cells, spacing 1, split into
+  // several with splitRegular when the case asks for it.
   CuboidDecomposition3D<T> cuboids(Vector<T,3>{0, 0, 0}, T{1},
                                    Vector<int,3>{c.n[0], c.n[1], c.n[2]}, 1);
+  if (c.split > 0) {
+    cuboids.splitRegular(0, c.split);
+  }
   cuboids.setPeriodicity({bool(c.periodic[0]), bool(c.periodic[1]), bool(c.periodic[2])});
-  BlockLoadBalancer<T> load(0, 1, 1, 0);
+  BlockLoadBalancer<T> load(0, 1, cuboids.size(), 0);
   // overlap 1: the halo depth the time step actually refreshes (superLattice.hh:138-143)
   SuperGeometry<T,3> geometry(cuboids, load, 1);
-  auto& blockGeometry = geometry.getBlockGeometry(0);
-  for (int x = 0; x < c.n[0]; ++x)
-    for (int y = 0; y < c.n[1]; ++y)
-      for (int z = 0; z < c.n[2]; ++z)
-        blockGeometry.set(LatticeR<3>{x, y, z}, int(c.ids[coreIndex(c, x, y, z)]));
+  forEachCoreCell(geometry, [&](int iC, LatticeR<3> local, LatticeR<3> global) {
+    geometry.getBlockGeometry(iC).set(local, int(c.ids[coreIndex(c, global[0], global[1], global[2])]));
+  });
   // Materials were written into the block directly, which does not flag the
   // super geometry for communication; rename(0, 0) changes no material but
   // sets the flag (superGeometry.hh:323-333), so communicate() then fills the
@@ -212,6 +237,7 @@ static void runCase(const CaseSpec& c, const std::string& dir)
   geometry.communicate();
 
   if (std::getenv("OLB_REF_NORMALS")) {
+    if (cuboids.size() != 1) throw std::runtime_error("normals are dumped for one cuboid only");
     dumpNormals(c, geometry, dir);
   }
 
@@ -252,38 +278,31 @@ static void runCase(const CaseSpec& c, const std::string& dir)
 
   // Per-cell rho/u in the same order as the example programs: define the
   // moments, then initialise populations at equilibrium.
-  auto& block = lattice.getBlock(0);
-  for (int x = 0; x < c.n[0]; ++x) {
-    for (int y = 0; y < c.n[1]; ++y) {
-      for (int z = 0; z < c.n[2]; ++z) {
-        const std::size_t k = coreIndex(c, x, y, z);
-        const int id = c.ids[k];
-        if (id == 0) continue;
-        const std::size_t nCore = c.ids.size();
-        T rho = T(c.rhou[k]);
-        T u[3] = {T(c.rhou[nCore + k]), T(c.rhou[2 * nCore + k]), T(c.rhou[3 * nCore + k])};
-        auto cell = block.get(LatticeR<3>{x, y, z});
-        if constexpr (FORCED) {
-          if (id == 1) {
-            cell.template setField<descriptors::FORCE>(
-              Vector<T,3>{T(c.force[0]), T(c.force[1]), T(c.force[2])});
-          }
-        }
-        cell.defineRhoU(rho, u);
-        cell.iniEquilibrium(rho, u);
+  const std::size_t nCore = c.ids.size();
+  forEachCoreCell(geometry, [&](int iC, LatticeR<3> local, LatticeR<3> global) {
+    const std::size_t k = coreIndex(c, global[0], global[1], global[2]);
+    const int id = c.ids[k];
+    if (id == 0) return;
+    T rho = T(c.rhou[k]);
+    T u[3] = {T(c.rhou[nCore + k]), T(c.rhou[2 * nCore + k]), T(c.rhou[3 * nCore + k])};
+    auto cell = lattice.getBlock(iC).get(local);
+    if constexpr (FORCED) {
+      if (id == 1) {
+        cell.template setField<descriptors::FORCE>(
+          Vector<T,3>{T(c.force[0]), T(c.force[1]), T(c.force[2])});
       }
     }
-  }
+    cell.defineRhoU(rho, u);
+    cell.iniEquilibrium(rho, u);
+  });
   // Moving walls: fixed density and wall velocity (cavity3d.cpp:124-136).
   for (const auto& w : c.walls) {
-    for (int x = 0; x < c.n[0]; ++x)
-      for (int y = 0; y < c.n[1]; ++y)
-        for (int z = 0; z < c.n[2]; ++z) {
-          if (c.ids[coreIndex(c, x, y, z)] != w.id) continue;
-          auto cell = block.get(LatticeR<3>{x, y, z});
-          T u[3] = {T(w.u[0]), T(w.u[1]), T(w.u[2])};
-          cell.defineRhoU(T(w.rho), u);
-        }
+    forEachCoreCell(geometry, [&](int iC, LatticeR<3> local, LatticeR<3> global) {
+      if (c.ids[coreIndex(c, global[0], global[1], global[2])] != w.id) return;
+      auto cell = lattice.getBlock(iC).get(local);
+      T u[3] = {T(w.u[0]), T(w.u[1]), T(w.u[2])};
+      cell.defineRhoU(T(w.rho), u);
+    });
   }
   lattice.initialize();
 
@@ -316,7 +335,7 @@ static void runCase(const CaseSpec& c, const std::string& dir)
     for (; done < target; ++done) {
       lattice.collideAndStream();
     }
-    dumpPopulations<DESCRIPTOR>(c, lattice, dir, target);
+    dumpPopulations<DESCRIPTOR>(c, geometry, lattice, dir, target);
     if (save) {
       lattice.template save<false>(dir + "/checkpoint_" + std::to_string(target));
     }

@@ -121,7 +121,7 @@ def write_normals():
     np.savez_compressed(os.path.join(HERE, "normals.npz"), **arrays)
 
 
-def write_case_dir(d, case, steps):
+def write_case_dir(d, case, steps, split=0):
     case.ids.astype(np.uint8).tofile(os.path.join(d, "ids.bin"))
     np.ascontiguousarray(case.rho_u, dtype=np.float64).tofile(os.path.join(d, "rhou.bin"))
     lines = [f"q {case.q}", f"dyn {case.kind}"]
@@ -134,6 +134,8 @@ def write_case_dir(d, case, steps):
         lines.append(f"wall {wid} {rho!r} {u[0]!r} {u[1]!r} {u[2]!r}")
     for mid, kind in case.local.items():
         lines.append(f"local {mid} {kind}")
+    if split:
+        lines.append(f"split {split}")
     with open(os.path.join(d, "case.txt"), "w") as fh:
         fh.write("\n".join(lines) + "\n")
 
@@ -316,7 +318,92 @@ def write_statistics():
         json.dump(out, fh, indent=1, sort_keys=True)
 
 
+def multiblock_cases():
+    """(fixture name, Case, splitRegular width, checkpoint step k or None,
+    steps, reference precision, keep f_0).  The width gives max(1, n_d //
+    width) cuboids per axis, cut by Cuboid::divide; the shapes are chosen for
+    the divisions in the names.  With k, OpenLB's checkpoint and VTK files of
+    the k-step state are kept too.  Without f_0 a test starts from the
+    oracle's equilibrium initialisation (1-2 ulp from OpenLB's)."""
+    return [
+        ("tgv19_bgk_f64_split2x2x1", W.taylor_green((16, 16, 10), u0=0.05, tau=0.6), 8, 5, [0, 1, 5, 100], "f64",
+         True),
+        ("channel19_forced_bgk_f64_split3x1x2",
+         W.porous_channel((24, 10, 16), seed=3, tau=0.55, force=1e-5, rmin=1, rmax=2), 8, None, [0, 100], "f64",
+         True),
+        ("channel19_forced_bgk_f32_split3x1x2",
+         W.porous_channel((24, 10, 16), seed=3, tau=0.55, force=1e-5, rmin=1, rmax=2), 8, 5, [0, 5, 100], "f32",
+         True),
+        ("cavity19_bgk_f64_split2x2x2", W.lid_driven_cavity((16, 17, 18), tau=0.6, lid_u=0.08), 8, 5,
+         [0, 5, 100], "f64", True),
+        ("periodic27_trt_f64_split2x1x2", W.random_periodic((12, 6, 13), seed=21, q=27, kind="trt", tau=0.8,
+                                                            magic=0.25), 6, None, [0, 50], "f64", False),
+        ("channel27_smagorinsky_f64_split2x2x1",
+         W.turbulent_channel((12, 12, 7), tau=0.51, smago=0.14, u_max=0.05, noise=0.2, seed=8, q=27), 6, None,
+         [0, 60], "f64", False),
+    ]
+
+
+def division_of(case, width):
+    return [max(1, n // width) for n in case.shape]
+
+
+def write_multiblock():
+    """multiblock/<name>.npz: a case run by OpenLB on several cuboids
+    (CuboidDecomposition::splitRegular) on one process, with the overlap
+    exchange between them (SuperCommunicator, periodic images through the
+    cuboid decomposition): its populations after every listed step, its own
+    checkpoint file after k steps (every block, in load-balancer order) and
+    the files its SuperVTMwriter3D writes for that state (pvd, vtm, one vti
+    per cuboid)."""
+    out_dir = os.path.join(HERE, "multiblock")
+    os.makedirs(out_dir, exist_ok=True)
+    manifest = {}
+    for name, case, width, k, steps, precision, keep_f0 in multiblock_cases():
+        exe = os.path.join(ROOT, "oracle", "_ref", f"olb_ref_{precision}")
+        blocks = division_of(case, width)
+        arrays = {"ids": case.ids.astype(np.uint8), "rho_u": case.rho_u}
+        with tempfile.TemporaryDirectory() as d:
+            write_case_dir(d, case, steps, split=width)
+            env = dict(os.environ, OLB_REF_SAVE="1") if k is not None else None
+            subprocess.run([exe, d], check=True, cwd=d, stdout=subprocess.DEVNULL, env=env)
+            if k is not None:
+                with open(os.path.join(d, f"checkpoint_{k}_rank0_size1.dat"), "rb") as fh:
+                    arrays["dat"] = np.frombuffer(fh.read(), dtype=np.uint8)
+            pops = {s: np.fromfile(os.path.join(d, f"pops_{s}.bin")).reshape((case.q,) + tuple(case.shape))
+                    for s in steps}
+        if k is not None:
+            with tempfile.TemporaryDirectory() as d:
+                write_case_dir(d, case, [k], split=width)
+                subprocess.run([exe, d], check=True, cwd=d, stdout=subprocess.DEVNULL,
+                               env=dict(os.environ, OLB_REF_VTK="1"))
+                for key, rel in (("pvd", "vtkData/fields.pvd"), ("vtm", f"vtkData/data/fields_iT{k:07d}.vtm")):
+                    with open(os.path.join(d, rel), "rb") as fh:
+                        arrays[key] = np.frombuffer(fh.read(), dtype=np.uint8)
+                for iC in range(int(np.prod(blocks))):
+                    with open(os.path.join(d, f"vtkData/data/fields_iT{k:07d}iC{iC:05d}.vti"), "rb") as fh:
+                        arrays[f"vti_{iC}"] = np.frombuffer(fh.read(), dtype=np.uint8)
+        store = np.float64 if precision == "f64" else np.float32
+        kept = [s for s in steps if s != 0 or keep_f0]
+        for s in kept:
+            arrays[f"f_{s}"] = pops[s].astype(store)
+        meta = meta_of(case, kept, precision)
+        meta.update({"split_width": width, "blocks": blocks, "checkpoint_step": k,
+                     "start": "f_0" if keep_f0 else "oracle",
+                     "source": "OpenLB 1.8 CPU_SISD, oracle/ref_driver.cpp, CuboidDecomposition::splitRegular"})
+        np.savez_compressed(os.path.join(out_dir, name + ".npz"), meta=np.array(json.dumps(meta)), **arrays)
+        meta["sha256"] = {a: hashlib.sha256(np.ascontiguousarray(v).tobytes()).hexdigest()
+                          for a, v in arrays.items()}
+        manifest[name] = meta
+        print(f"multiblock {name}: shape {case.shape} blocks {blocks} steps {kept}", flush=True)
+    with open(os.path.join(out_dir, "manifest.json"), "w") as fh:
+        json.dump(manifest, fh, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
+    if "--multiblock-only" in sys.argv:
+        write_multiblock()
+        sys.exit(0)
     if "--checkpoints-only" in sys.argv:
         write_checkpoints()
         sys.exit(0)
@@ -329,4 +416,5 @@ if __name__ == "__main__":
         write_statistics()
     write_normals()
     write_checkpoints()
+    write_multiblock()
     write_uq()

@@ -129,3 +129,26 @@ def load_vtk(name: str) -> dict:
     """OpenLB's VTK files {"pvd", "vtm", "vti"} (text) of a checkpoint fixture's k-step state."""
     z = np.load(os.path.join(CHECKPOINTS, name + ".npz"), allow_pickle=False)
     return {k: z[k].tobytes().decode("ascii") for k in ("pvd", "vtm", "vti")}
+
+
+MULTIBLOCK = os.path.join(GOLDEN, "multiblock")
+
+
+def multiblock_names():
+    return [os.path.basename(p)[:-4] for p in sorted(glob.glob(os.path.join(MULTIBLOCK, "*.npz")))]
+
+
+def load_multiblock(name: str):
+    """(Case, meta, arrays) of a case OpenLB ran on several cuboids
+    (CuboidDecomposition::splitRegular, meta["blocks"] cuboids per axis).
+    meta["start"] is "f_0" when OpenLB's state after initialize() is kept,
+    "oracle" when a test starts from the oracle's equilibrium initialisation."""
+    z = np.load(os.path.join(MULTIBLOCK, name + ".npz"), allow_pickle=False)
+    meta = json.loads(str(z["meta"]))
+    arrays = {k: z[k] for k in z.files if k != "meta"}
+    walls = {int(k): (v[0], tuple(v[1])) for k, v in meta["walls"].items()}
+    local = {int(k): v for k, v in meta.get("local", {}).items()}
+    case = W.Case(name, meta["q"], meta["kind"], tuple(meta["shape"]), tuple(meta["periodic"]),
+                  arrays["ids"], arrays["rho_u"], meta["omega"], meta["magic"], meta["smago"],
+                  tuple(meta["force"]), walls, local)
+    return case, meta, arrays

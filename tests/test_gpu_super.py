@@ -4,11 +4,16 @@ C ABI lbmh_collide / lbmh_halo_copy / lbmh_stream).
 Splitting a domain into cuboids changes no arithmetic: every cell runs the same
 collision on the same values, and the overlap copies deliver exactly the values
 a single block's periodic refresh and streaming would read.  So a split lattice
-must be bit-identical to the undivided one, which test_gpu_parity.py pins to
-OpenLB."""
+must be bit-identical to the undivided one, must match the oracle's undivided
+run within the single-block bounds, and must reach the populations OpenLB
+itself computes on the same cuboids (tests/golden/multiblock/, OpenLB run with
+CuboidDecomposition::splitRegular: the overlap exchange of
+src/communication/superCommunicator.hh:189-218 with periodic images resolved
+through src/geometry/cuboidDecomposition.hh:155-177)."""
 import numpy as np
 import pytest
 
+import golden_cases as G
 from oracle import oracle as O
 from openlb_amd import workloads as W
 from openlb_amd.lattice import BlockLattice
@@ -44,3 +49,39 @@ def test_split_lattice_equals_single_block(gpu_available, label, make, storage, 
     # with the padded extent and never reach a fluid cell: compare bulk cells
     mask = case.ids == 1
     np.testing.assert_array_equal(got[:, mask], ref[:, mask])
+    if storage in ORACLE_TOL:
+        # the oracle's undivided run, at the bounds of test_gpu_matches_oracle_ragged
+        ora = O.OracleLattice(case, storage)
+        ora.upload(f0)
+        ora.step(steps, threads=8)
+        err = G.rel_error(got, ora.download(), case.q, mask)
+        print(f"{label} {storage} split {blocks} vs oracle: rel {err:.3e}", flush=True)
+        assert err <= ORACLE_TOL[storage]
+
+
+ORACLE_TOL = {"f64": 1e-13, "f32": 1e-5}
+TOL = {"f64": 1e-12, "f32": 2e-5}
+
+
+@pytest.mark.parametrize("name", G.multiblock_names())
+def test_split_lattice_matches_openlb_multicuboid(gpu_available, name):
+    """SuperLattice on OpenLB's own division reaches OpenLB's populations."""
+    case, meta, arrays = G.load_multiblock(name)
+    precision = meta["precision"]
+    f0 = arrays["f_0"].astype(np.float64) if meta["start"] == "f_0" else O.initial_populations(case, precision)
+    flow = (case.ids != 0)[None] & ~G.streamed_from_outside(case)
+    with SuperLattice(case, precision, tuple(meta["blocks"])) as sup:
+        assert len(sup.blocks) == int(np.prod(meta["blocks"]))
+        sup.upload_populations(f0, "f64")
+        done = 0
+        for s in [s for s in meta["steps"] if s > 0]:
+            sup.collide_and_stream(s - done)
+            done = s
+            got = sup.download_populations("f64")
+            ref = arrays[f"f_{s}"].astype(np.float64)
+            err = G.rel_error(got, ref, case.q, case.ids == 1)
+            err_flow = G.rel_error(got, ref, case.q, flow)
+            print(f"{name} blocks {meta['blocks']} step {s}: rel {err:.3e} (cells with dynamics {err_flow:.3e})",
+                  flush=True)
+            assert err <= TOL[precision]
+            assert err_flow <= TOL[precision]
