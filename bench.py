@@ -79,6 +79,24 @@ class HipEvents:
             self.hip.hipEventDestroy(e)
 
 
+def pattern_ceiling(lat, ev, repeats=20):
+    """Average launch time (ms) of lbmh_pattern_ceiling on this lattice: the step
+    kernel's memory access pattern without its arithmetic, on the null stream the
+    step kernel runs on. It leaves the lattice freely streamed."""
+    from openlb_amd import abi
+    lib = abi.load()
+    abi.check(lib.lbmh_pattern_ceiling(lat.handle, 2, None))      # warm-up
+    marks = ev.create(2 * repeats)
+    for k in range(repeats):
+        ev.record(marks[2 * k])
+        abi.check(lib.lbmh_pattern_ceiling(lat.handle, 1, None))
+        ev.record(marks[2 * k + 1])
+    lat.synchronize()
+    ms = [ev.elapsed_ms(marks[2 * k], marks[2 * k + 1]) for k in range(repeats)]
+    ev.destroy(marks)
+    return float(np.mean(ms))
+
+
 def calibrate_copy(mib: int, repeats: int = 5):
     """Device-to-device copies of a known size: the reference point that scales the
     FETCH_SIZE / WRITE_SIZE counters of the step kernel to HBM bytes."""
@@ -179,7 +197,7 @@ def ensemble_cpu_baseline(cases, steps):
     from oracle import compare_cpu
     if not os.path.exists(compare_cpu.REF_SIMD):
         return None
-    threads = min(16, os.cpu_count() or 1)
+    threads = compare_cpu.default_threads()
     sample = cases[::32]
     lups = secs = 0.0
     for c in sample:
@@ -188,11 +206,12 @@ def ensemble_cpu_baseline(cases, steps):
         secs += r["seconds"]
     rate = lups / secs
     total = sum(c.ncore for c in cases) * steps / rate
+    host = compare_cpu.host_info(threads)
     return {"value": round(len(cases) / total, 3), "unit": "solves/s", "mlups": round(rate / 1e6, 1),
-            "cores": threads, "kind": "reference",
-            "sample": f"{len(sample)} of the {len(cases)} cavities, 200 steps each, OpenLB 1.8 CPU_SIMD (AVX2) + "
-                      f"OpenMP, {threads} threads, one lattice after the other; solves/s = members / "
-                      f"(all members' cell updates at {steps} steps / aggregate rate)"}
+            "cores": threads, "kind": "reference", **host,
+            "sample": f"{len(sample)} of the {len(cases)} cavities, 200 steps each, OpenLB 1.8 CPU_SIMD "
+                      f"({host['isa']}) + OpenMP, {threads} threads, one lattice after the other; solves/s = "
+                      f"members / (all members' cell updates at {steps} steps / aggregate rate)"}
 
 
 def guarded_ensemble(cases, steps, ref, tau_guard=0.53):
@@ -437,14 +456,15 @@ def cavity_bench(size=200, steps=200, cpu_size=100, cpu_seconds=10.0, no_cpu=Fal
     if not no_cpu:
         from oracle import compare_cpu
         if os.path.exists(compare_cpu.REF_SIMD):
-            threads = min(16, os.cpu_count() or 1)
+            threads = compare_cpu.default_threads()
             sample = W.cavity_benchmark_local(cpu_size)
             probe = compare_cpu.time_reference(sample, 5, threads)
             n = max(5, int(cpu_seconds * probe["mlups"] * 1e6 / sample.n_active))
             res = compare_cpu.time_reference(sample, n, threads)
             out["cpu_reference"] = {"mlups": round(res["mlups"], 2), "cores": threads,
+                                    **compare_cpu.host_info(threads),
                                     "sample": f"cavity {cpu_size + 1}^3 local velocity f32, {n} steps, "
-                                              f"OpenLB 1.8 CPU_SIMD + OpenMP"}
+                                              f"OpenLB 1.8 CPU_SIMD ({res['isa']}) + OpenMP"}
     return out
 
 
@@ -474,6 +494,7 @@ def serving_bench(count=400, rate=200.0, budget_gb=200.0, tick_steps=64):
                 "arrival_window_s": round(reqs[-1].arrival, 3), "tick_steps": tick_steps,
                 "arena_bytes": budget, "hbm_budget_bytes": int(budget_gb * 1e9),
                 "offered_mlups": round(sum(r.lups for r in reqs) / reqs[-1].arrival / 1e6, 1),
+                "mean_mlup_per_request": round(sum(r.lups for r in reqs) / count / 1e6, 2),
                 "fp16_share": round(sum(r.storage == "f16" for r in reqs) / count, 3)})
     return out
 
@@ -543,7 +564,7 @@ def cpu_baseline(case, storage, seconds):
     workload. If that binary is absent or the storage is not FP32, the oracle is timed
     instead (kind "port"; about 8x slower than OpenLB's SIMD path on the same cores)."""
     from oracle import compare_cpu
-    threads = min(16, os.cpu_count() or 1)
+    threads = compare_cpu.default_threads()
     sample = cpu_sample(case)
     shape = sample.shape
     if storage == "f32" and os.path.exists(compare_cpu.REF_SIMD):
@@ -551,9 +572,10 @@ def cpu_baseline(case, storage, seconds):
         steps = max(10, int(seconds * probe["mlups"] * 1e6 / sample.n_active))
         res = compare_cpu.time_reference(sample, steps, threads)
         return {"value": round(res["mlups"], 2), "unit": "MLUPS", "cores": threads, "kind": "reference",
+                **compare_cpu.host_info(threads),
                 "sample": f"{sample.name} {shape[0]}x{shape[1]}x{shape[2]} {storage}, {steps} steps in "
-                          f"{res['seconds']:.1f} s, OpenLB 1.8 CPU_SIMD (AVX2) + OpenMP, {threads} threads, "
-                          f"oracle/_ref/olb_ref_simd_f32"}
+                          f"{res['seconds']:.1f} s, OpenLB 1.8 CPU_SIMD ({res['isa']}) + OpenMP, {threads} "
+                          f"threads, {os.path.relpath(compare_cpu.reference_binary()[0], ROOT)}"}
     return cpu_baseline_port(sample, storage, seconds, threads)
 
 
@@ -572,7 +594,9 @@ def cpu_baseline_port(sample, storage, seconds, threads):
         if dt >= seconds:
             break
     mlups = sample.n_active * steps / dt / 1e6
+    from oracle import compare_cpu
     return {"value": round(mlups, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+            **compare_cpu.host_info(threads),
             "sample": f"{sample.name} {shape[0]}x{shape[1]}x{shape[2]} {storage}, {steps} steps "
                       f"in {dt:.1f} s, oracle/liboracle.so (OpenMP, {threads} threads)"}
 
@@ -654,9 +678,10 @@ def main():
     moments = lat.reduce_moments()
     export = export_bench(lat)
     log(f"export: density+velocity {export['field_export_s']} s vs populations {export['population_download_s']} s")
-    lat.close()
     if not np.isfinite(moments).all():
         raise SystemExit("non-finite populations after the timed steps")
+    pattern_ms = pattern_ceiling(lat, ev, repeats=20)
+    lat.close()
 
     cells = case.n_active
     mlups = cells * args.steps / wall / 1e6
@@ -670,6 +695,11 @@ def main():
     copy = copy_ceiling(bytes_per_launch)
     log(f"copies of the same bytes: hipMemcpyAsync {copy['hipMemcpyAsync']['gbs']} GB/s, "
         f"elementwise {copy['elementwise_f32']['gbs']} GB/s")
+    pattern = {"gbs": round(bytes_per_launch / (pattern_ms * 1e-3) / 1e9, 1), "avg_ms": round(pattern_ms, 5),
+               "how": "lbmh_pattern_ceiling: the step kernel's PS in-place read-modify-write of the Q columns "
+                      "(same cells, shifts, shift evolution, launch shape and stream) with the arithmetic taken "
+                      "out, 20 launches timed with HIP events after the timed steps"}
+    log(f"pattern ceiling {pattern['gbs']} GB/s; kernel at {achieved / pattern['gbs']:.3f} of it")
 
     ensemble = None
     if args.ensemble:
@@ -696,6 +726,15 @@ def main():
         log("serving (config 5) ...")
         served = serving_bench(args.serving_requests, args.serving_rate)
         log(f"serving: p50 {served['p50_s']} s, p99 {served['p99_s']} s, {served['aggregate_mlups']} MLUPS")
+        # the same stream at a rate the device sustains: 60% of the capacity the
+        # saturated run measured, in requests per second of this stream's mean size
+        per_req = served["mean_mlup_per_request"]
+        rate = round(0.6 * served["aggregate_mlups"] / per_req, 2)
+        log(f"serving at a sustainable rate ({rate} req/s) ...")
+        steady = serving_bench(args.serving_requests, rate)
+        steady["rate_rule"] = "0.6 x the aggregate MLUPS of the saturated run / mean MLUP per request"
+        log(f"serving at {rate} req/s: p50 {steady['p50_s']} s, p99 {steady['p99_s']} s")
+        served = {"saturated": served, "sustainable": steady}
 
     cpu = None
     if not args.no_cpu:
@@ -740,6 +779,8 @@ def main():
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "device_copy": copy,
             "frac_of_device_copy": round(achieved / copy["gbs"], 4),
+            "pattern_ceiling": pattern,
+            "frac_of_pattern_ceiling": round(achieved / pattern["gbs"], 4),
         },
         "cpu_baseline": cpu,
         "porous_channel_variants": variants,

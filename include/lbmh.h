@@ -314,6 +314,17 @@ int lbmh_get_launch_config(const lbmh_lattice* lattice, lbmh_launch_config* conf
  * more copy of the lattice. */
 int lbmh_autotune(lbmh_lattice* lattice, int trial_steps, lbmh_launch_config* chosen);
 
+/* Measurement only, no OpenLB counterpart: n_steps passes of the time step's
+ * memory access pattern with its arithmetic taken out.  Each pass reads every
+ * core cell's Q populations at their shifted slots and writes the same values
+ * back in place, with the step kernel's cell mapping, addressing and launch
+ * shape, then advances the shifts as a stream would.  Its rate is the ceiling
+ * of the periodic-shift in-place read-modify-write on the device, the figure
+ * the step kernel is priced against beside the vendor HBM peak.  The lattice
+ * is left freely streamed (no collision, no overlap refresh): use it on a
+ * lattice whose state is no longer needed.  Asynchronous on `stream`. */
+int lbmh_pattern_ceiling(lbmh_lattice* lattice, int64_t n_steps, void* stream);
+
 /* ---- Ensembles: many independent lattices advanced by one launch per step ----
  *
  * No single OpenLB counterpart.  The reference runs uncertainty-quantification

@@ -29,7 +29,7 @@ EXPORTS = (
     "lbmh_collide", "lbmh_stream", "lbmh_halo_create", "lbmh_halo_destroy", "lbmh_halo_copy",
     "lbmh_synchronize", "lbmh_lattice_footprint", "lbmh_abi_version", "lbmh_device_count",
     "lbmh_device_arch", "lbmh_last_error", "lbmh_set_launch_config", "lbmh_get_launch_config",
-    "lbmh_autotune",
+    "lbmh_autotune", "lbmh_pattern_ceiling",
     "lbmh_ensemble_create", "lbmh_ensemble_destroy", "lbmh_ensemble_member_bytes", "lbmh_ensemble_add",
     "lbmh_ensemble_remove", "lbmh_ensemble_set_wall", "lbmh_ensemble_set_local_boundary",
     "lbmh_ensemble_initialize", "lbmh_ensemble_set_steps", "lbmh_ensemble_status",
@@ -131,6 +131,7 @@ def load() -> ctypes.CDLL:
     L.lbmh_set_launch_config.argtypes = [vp, ctypes.POINTER(LaunchConfig)]
     L.lbmh_get_launch_config.argtypes = [vp, ctypes.POINTER(LaunchConfig)]
     L.lbmh_autotune.argtypes = [vp, i32, ctypes.POINTER(LaunchConfig)]
+    L.lbmh_pattern_ceiling.argtypes = [vp, i64, vp]
     i3 = ctypes.c_int * 3
     szp = ctypes.POINTER(ctypes.c_size_t)
     L.lbmh_ensemble_create.argtypes = [ctypes.POINTER(EnsembleDesc), ctypes.POINTER(vp)]

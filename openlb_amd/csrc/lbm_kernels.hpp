@@ -223,6 +223,38 @@ __global__ __launch_bounds__(kBlock) void collide_stream_kernel(StepArgs<Q, type
   collide_stream_cell<Q, S, KIND, IDS, LOCAL>(a, b * kBlock + threadIdx.x);
 }
 
+// Measurement kernel, not part of a time step: the step kernel's access
+// pattern with its arithmetic taken out.  Each lane reads the Q populations of
+// its cell at their shifted slots and writes the same values back in place,
+// with the step kernel's cell mapping, addressing and launch shape.  The values
+// pass through an empty asm so the stores cannot be dropped as no-ops.  Its
+// rate is the ceiling of the PS in-place read-modify-write on this device,
+// against which the step kernel is priced beside the vendor peak.
+template <int Q, typename S>
+__global__ __launch_bounds__(kBlock) void pattern_ceiling_kernel(StepArgs<Q, float> a)
+{
+  uint32_t b = blockIdx.x;
+  if (a.xcd_group) b = remap_xcd(b, gridDim.x, a.xcd_group);
+  const uint32_t c = b * kBlock + threadIdx.x;
+  if (c >= a.ncore) return;
+  const uint32_t t = a.div_z.div(c);
+  const uint32_t z = c - t * a.nz;
+  const uint32_t x = a.div_y.div(t);
+  const uint32_t y = t - x * a.ny;
+  const uint32_t n = (x + 1) * a.sx + (y + 1) * a.sy + (z + 1);
+  S* pop = static_cast<S*>(a.pop);
+  S f[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) f[i] = *column_at(pop + uint64_t(i) * a.stride, wrap_index(n, a.shift[i], a.n_padded));
+  uint32_t ns = n;
+  asm volatile("" : "+v"(ns));
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    asm volatile("" : "+v"(f[i]));
+    *column_at(pop + uint64_t(i) * a.stride, wrap_index(ns, a.shift[i], a.n_padded)) = f[i];
+  }
+}
+
 // The same time step for the listed local-boundary cells (the cells the
 // whole-lattice pass skipped).  The two passes touch disjoint populations.
 template <int Q, typename S, int KIND>

@@ -275,10 +275,10 @@ int launch_post_processors(lbmh_lattice* L, hipStream_t stream)
 // other blocks runs them apart, with the copies in between.
 enum StepPhase : int { kPhaseCollide = 1, kPhaseStream = 2, kPhaseBoth = 3 };
 
-template <int Q, typename S, int KIND>
-int launch_kind(lbmh_lattice* L, hipStream_t stream, int phase)
+// Arguments of a whole-block pass with the block's current shifts and launch shape.
+template <int Q, typename V>
+StepArgs<Q, V> step_args(const lbmh_lattice* L)
 {
-  using V = typename StorageTraits<S>::V;
   StepArgs<Q, V> a;
   a.pop = L->d_pop;
   a.stride = L->stride;
@@ -297,8 +297,27 @@ int launch_kind(lbmh_lattice* L, hipStream_t stream, int phase)
   a.walls = L->d_walls;
   a.info = L->d_info;
   a.bp = bulk_params<V>(L);
-  const uint32_t grid = (L->ncore + kBlock - 1) / kBlock;
   for (int i = 0; i < Q; ++i) a.shift[i] = uint32_t(L->shift[i]);
+  return a;
+}
+
+// stream: shift_i -= c_i . projection (blockLattice.hh:317-339)
+void advance_shifts(lbmh_lattice* L)
+{
+  const int64_t N = L->n_padded;
+  for (int i = 0; i < L->q; ++i) {
+    int64_t s = (L->shift[i] - L->dist[i]) % N;
+    if (s < 0) s += N;
+    L->shift[i] = s;
+  }
+}
+
+template <int Q, typename S, int KIND>
+int launch_kind(lbmh_lattice* L, hipStream_t stream, int phase)
+{
+  using V = typename StorageTraits<S>::V;
+  const StepArgs<Q, V> a = step_args<Q, V>(L);
+  const uint32_t grid = (L->ncore + kBlock - 1) / kBlock;
   // the dynamic LDS is never touched: it only limits resident workgroups per CU
   const unsigned lds = unsigned(L->launch.lds_bytes);
   if (phase & kPhaseCollide) {
@@ -324,13 +343,7 @@ int launch_kind(lbmh_lattice* L, hipStream_t stream, int phase)
   LBMH_HIP(hipGetLastError());
   }
   if (!(phase & kPhaseStream)) return LBMH_OK;
-  // stream: shift_i -= c_i . projection (blockLattice.hh:317-339)
-  const int64_t N = L->n_padded;
-  for (int i = 0; i < Q; ++i) {
-    int64_t s = (L->shift[i] - L->dist[i]) % N;
-    if (s < 0) s += N;
-    L->shift[i] = s;
-  }
+  advance_shifts(L);
   L->steps += 1;
   if (L->has_local) return launch_post_processors<Q, S>(L, stream);
   return LBMH_OK;
@@ -793,6 +806,29 @@ int lbmh_step(lbmh_lattice* L, int64_t n_steps, void* stream)
   for (int64_t k = 0; k < n_steps; ++k) {
     const int rc = L->q == 19 ? step_q<19>(L, s) : step_q<27>(L, s);
     if (rc != LBMH_OK) return rc;
+  }
+  return LBMH_OK;
+}
+
+int lbmh_pattern_ceiling(lbmh_lattice* L, int64_t n_steps, void* stream)
+{
+  if (!L) return fail(LBMH_E_INVALID, "null lattice");
+  if (n_steps < 0) return fail(LBMH_E_INVALID, "negative step count");
+  DeviceGuard guard(L->device);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const uint32_t grid = (L->ncore + kBlock - 1) / kBlock;
+  for (int64_t k = 0; k < n_steps; ++k) {
+    const int rc = with_types(L, [&](auto qc, auto s) -> int {
+      constexpr int Q = decltype(qc)::value;
+      using S = decltype(s);
+      hipLaunchKernelGGL((pattern_ceiling_kernel<Q, S>), dim3(grid), dim3(kBlock), unsigned(L->launch.lds_bytes), st,
+                         step_args<Q, float>(L));
+      LBMH_HIP(hipGetLastError());
+      return LBMH_OK;
+    });
+    if (rc != LBMH_OK) return rc;
+    advance_shifts(L);
+    L->steps += 1;
   }
   return LBMH_OK;
 }

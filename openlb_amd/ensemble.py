@@ -86,16 +86,21 @@ class Ensemble:
                                                (ctypes.c_int * 3)(*case.periodic), _ptr(ids),
                                                ctypes.byref(p), int(steps), ctypes.byref(m)))
         local = getattr(case, "local", {}) or {}
-        for wid, (rho, u) in case.walls.items():
-            if wid in local:
-                continue
-            abi.check(abi.load().lbmh_ensemble_set_wall(self._h, m.value, int(wid), float(rho),
-                                                        (ctypes.c_double * 3)(*map(float, u))))
-        for mid, kind in local.items():
-            rho, u = case.walls.get(mid, (1.0, (0.0, 0.0, 0.0)))
-            abi.check(abi.load().lbmh_ensemble_set_local_boundary(
-                self._h, m.value, int(mid), {"velocity": 0, "pressure": 1, "interp_velocity": 2}[kind], float(rho),
-                (ctypes.c_double * 3)(*map(float, u))))
+        try:
+            for wid, (rho, u) in case.walls.items():
+                if wid in local:
+                    continue
+                abi.check(abi.load().lbmh_ensemble_set_wall(self._h, m.value, int(wid), float(rho),
+                                                            (ctypes.c_double * 3)(*map(float, u))))
+            for mid, kind in local.items():
+                rho, u = case.walls.get(mid, (1.0, (0.0, 0.0, 0.0)))
+                abi.check(abi.load().lbmh_ensemble_set_local_boundary(
+                    self._h, m.value, int(mid), {"velocity": 0, "pressure": 1, "interp_velocity": 2}[kind],
+                    float(rho), (ctypes.c_double * 3)(*map(float, u))))
+        except BaseException:
+            # a half-configured member must not keep its slot and arena space
+            abi.load().lbmh_ensemble_remove(self._h, m.value)
+            raise
         self.shapes[m.value] = tuple(case.shape)
         return m.value
 

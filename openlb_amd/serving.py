@@ -6,10 +6,13 @@ type) hold the running solves on the device.
 
 The loop repeats:
 
-1. Admit every request that has arrived, in arrival order, while its footprint
-   fits the ensemble's arenas and a slot is free. Admission is checked against the
-   HBM budget the arenas were sized to; a request that does not fit waits at the
-   head of the queue.
+1. Admit the requests that have arrived, in arrival order, whose footprint fits
+   the ensemble's arenas and for which a slot is free. Admission is checked against
+   the HBM budget the arenas were sized to. A request that does not fit yet stays
+   queued and later, smaller requests may pass it (skip-ahead admission), up to
+   ``max_bypass`` of them: after that nothing behind it is admitted until it fits,
+   so a large solve cannot starve. A request that could not fit even in an empty
+   arena is rejected at once with an error result instead of waiting forever.
 2. Advance every ensemble that has running solves by one tick of ``tick_steps`` steps
    (a HIP-graph replay). Each solve carries its own step budget, so it stops at
    exactly its requested step count even in mid-tick.
@@ -56,6 +59,7 @@ class Result:
     finished: float
     moments: np.ndarray | None = None
     populations: np.ndarray | None = None
+    error: str | None = None        # set when the request was rejected and never ran
 
     @property
     def latency(self) -> float:
@@ -102,18 +106,21 @@ def poisson_channel_stream(count=400, rate=200.0, seed=20261205, size=(24, 128),
 class SolveServer:
     def __init__(self, budget_bytes: int, storages=("f32", "f16"), max_members: int = 1024,
                  tick_steps: int = 64, keep_moments: bool = False, keep_populations: bool = False,
-                 device: int = 0):
+                 device: int = 0, max_bypass: int = 64):
         self.tick_steps = int(tick_steps)
+        self.max_bypass = int(max_bypass)
         self.keep_moments = keep_moments
         self.keep_populations = keep_populations
         share = budget_bytes // len(storages)
         self.ens = {}
+        self.capacity = {}
         for st in storages:
             # id arena: one byte per cell; a member's populations take at least
             # 2*19 bytes per cell, so cells <= pop bytes / 38
             cells = share // (19 * {"f32": 4, "f16": 2, "f64": 8}[st]) + (1 << 20)
             self.ens[st] = Ensemble(19, st, "bgk", max_members=max_members,
                                     population_bytes=share, cell_capacity=cells, device=device)
+            self.capacity[st] = (self.ens[st].usage()["bytes_capacity"], cells)
         self.max_active = 0
 
     def close(self):
@@ -126,37 +133,81 @@ class SolveServer:
     def __exit__(self, *exc):
         self.close()
 
+    def never_fits(self, req: Request) -> str | None:
+        """Why the request could not run even on an empty server, or None."""
+        if req.storage not in self.ens:
+            return f"storage {req.storage} is not served (served: {sorted(self.ens)})"
+        pop, ids = Ensemble.bytes_for(19, req.storage, req.case.shape)
+        pop_cap, cell_cap = self.capacity[req.storage]
+        if pop > pop_cap:
+            return f"needs {pop} population bytes, the {req.storage} arena holds {pop_cap}"
+        if ids > cell_cap:
+            return f"needs {ids} id bytes, the {req.storage} id arena holds {cell_cap}"
+        return None
+
     def _try_admit(self, req: Request):
+        ens = self.ens[req.storage]
         try:
-            m = self.ens[req.storage].add(req.case, steps=req.steps)
+            m = ens.add(req.case, steps=req.steps)
         except abi.LbmhError as exc:
             if exc.code == abi.E_MEMORY:
                 return None
             raise
-        self.ens[req.storage].ini_equilibrium(m, initial_moments(req.case))
-        self.ens[req.storage].initialize(m)
+        try:
+            ens.ini_equilibrium(m, initial_moments(req.case))
+            ens.initialize(m)
+        except BaseException:
+            ens.remove(m)
+            raise
         return m
 
     def run(self, requests, realtime: bool = True) -> list[Result]:
         """Serve the stream. With realtime=True a request becomes visible at its
-        arrival time on the wall clock. Otherwise every request is present at time zero."""
-        queue = collections.deque(sorted(requests, key=lambda r: r.arrival))
-        active = {}      # (storage, member) -> (request, admitted)
+        arrival time on the wall clock. Otherwise every request is present at time zero.
+        Rejected requests come back as results with ``error`` set and no steps run."""
+        queue = sorted(requests, key=lambda r: r.arrival)
+        head = 0                 # requests before `head` in `queue` have all arrived
+        waiting = []             # arrived, not yet admitted: [request, times passed over]
+        active = {}              # (storage, member) -> (request, admitted)
         results = []
         t0 = time.perf_counter()
-        while queue or active:
+
+        def reject(req, why, at):
+            results.append(Result(req.rid, req.storage, req.case.ncore, 0, req.arrival, at, at, error=why))
+
+        while head < len(queue) or waiting or active:
             now = time.perf_counter() - t0
-            while queue and (not realtime or queue[0].arrival <= now):
-                req = queue[0]
-                m = self._try_admit(req)
+            while head < len(queue) and (not realtime or queue[head].arrival <= now):
+                req = queue[head]
+                head += 1
+                why = self.never_fits(req)
+                if why is None:
+                    waiting.append([req, 0])
+                else:
+                    reject(req, why, time.perf_counter() - t0)
+            # skip-ahead admission in arrival order, bounded by max_bypass
+            kept = []
+            blocked = False
+            for entry in waiting:
+                req = entry[0]
+                m = None if blocked else self._try_admit(req)
                 if m is None:
-                    break
+                    if not active and not blocked:
+                        # nothing runs, so nothing will free up: it cannot ever fit
+                        reject(req, "does not fit in the empty server", time.perf_counter() - t0)
+                        continue
+                    kept.append(entry)
+                    blocked = blocked or entry[1] >= self.max_bypass
+                    continue
                 active[(req.storage, m)] = (req, time.perf_counter() - t0)
-                queue.popleft()
+                for older in kept:
+                    older[1] += 1
+                    blocked = blocked or older[1] >= self.max_bypass
+            waiting = kept
             self.max_active = max(self.max_active, len(active))
             if not active:
-                if queue:
-                    time.sleep(max(0.0, queue[0].arrival - (time.perf_counter() - t0)))
+                if head < len(queue) and not waiting:
+                    time.sleep(max(0.0, queue[head].arrival - (time.perf_counter() - t0)))
                 continue
             running = {st for st, _ in active}
             for st in running:
@@ -181,10 +232,12 @@ class SolveServer:
 
 
 def summarize(results: list[Result], makespan: float | None = None) -> dict:
+    rejected = sum(r.error is not None for r in results)
+    results = [r for r in results if r.error is None]
     lat = np.array([r.latency for r in results])
     lups = sum(r.cells * r.steps for r in results)
     span = makespan if makespan is not None else max(r.finished for r in results)
-    return {"requests": len(results), "p50_s": round(float(np.percentile(lat, 50)), 4),
+    return {"requests": len(results), "rejected": rejected, "p50_s": round(float(np.percentile(lat, 50)), 4),
             "p99_s": round(float(np.percentile(lat, 99)), 4), "max_s": round(float(lat.max()), 4),
             "makespan_s": round(span, 3), "aggregate_mlups": round(lups / span / 1e6, 1),
             "solves_per_s": round(len(results) / span, 2)}

@@ -57,3 +57,41 @@ def test_admission_waits_for_memory(gpu_available):
         assert res.moments[1][res.moments[0] > 0].mean() > 0
     summary = serving.summarize(results)
     assert summary["requests"] == 6 and summary["p99_s"] >= summary["p50_s"] > 0
+
+
+def test_oversize_request_is_rejected_not_queued(gpu_available):
+    """A solve larger than its storage's whole arena comes back as an error result at
+    once; the other requests still run."""
+    pop, _ = Ensemble.bytes_for(19, "f32", (16, 16, 16))
+    reqs = [serving.channel_request(0, 0.0, 40, 30, 0.05, "f32"),      # > the arena
+            serving.channel_request(1, 0.0, 16, 30, 0.05, "f32")]
+    with serving.SolveServer(budget_bytes=2 * pop + 256, storages=("f32",), max_members=4,
+                             tick_steps=16) as srv:
+        assert srv.never_fits(reqs[0]) is not None and srv.never_fits(reqs[1]) is None
+        results = srv.run(reqs, realtime=False)
+    by_id = {r.rid: r for r in results}
+    assert set(by_id) == {0, 1}
+    assert by_id[0].error and "population bytes" in by_id[0].error and by_id[0].steps == 0
+    assert by_id[1].error is None and by_id[1].steps == 30
+    summary = serving.summarize(results)
+    assert summary["requests"] == 1 and summary["rejected"] == 1
+
+
+def test_small_requests_pass_a_waiting_large_one(gpu_available):
+    """Skip-ahead admission: while a large solve waits for memory, later small solves
+    that fit are admitted; after max_bypass of them the large one is next."""
+    big_pop, _ = Ensemble.bytes_for(19, "f32", (24, 24, 24))
+    small_pop, _ = Ensemble.bytes_for(19, "f32", (8, 8, 8))
+    first = serving.channel_request(0, 0.0, 8, 200, 0.05, "f32")      # holds memory for a while
+    big = serving.channel_request(1, 0.0, 24, 20, 0.05, "f32")
+    smalls = [serving.channel_request(2 + k, 0.0, 8, 20, 0.05, "f32") for k in range(5)]
+    # the arena holds the large solve alone, or a few small ones, never both the large and the first
+    budget = big_pop + small_pop // 2
+    with serving.SolveServer(budget_bytes=budget, storages=("f32",), max_members=8, tick_steps=10,
+                             max_bypass=2) as srv:
+        results = srv.run([first, big] + smalls, realtime=False)
+    by_id = {r.rid: r for r in results}
+    assert all(r.error is None for r in results) and len(results) == 7
+    # two small solves were admitted while the large one waited, the others only after it
+    passed = [k for k in range(2, 7) if by_id[k].admitted < by_id[1].admitted]
+    assert passed == [2, 3], (passed, {k: (r.admitted, r.finished) for k, r in by_id.items()})
