@@ -109,6 +109,21 @@ int lbmh_set_wall(lbmh_lattice* lattice, int id, double rho, const double u[3]);
 int lbmh_populations_upload(lbmh_lattice* lattice, const void* host, int host_dtype);
 int lbmh_populations_download(lbmh_lattice* lattice, void* host, int host_dtype);
 
+/* Population column i of a host block in OpenLB's own storage, as a host
+ * CyclicColumn holds it: (nx+2p)(ny+2p)(nz+2p) values for overlap width
+ * `padding` p >= 1, element k the value of padded cell (k - shift) mod count
+ * (cell id (x+p)(ny+2p)(nz+2p) + (y+p)(nz+2p) + (z+p), src/core/
+ * blockStructure.h:65-78), shift in (-count, count), values in OpenLB's value
+ * type (LBMH_F64 or LBMH_F32, shifted f_i - t_i).  Upload sets the lattice's
+ * core cells and its one overlap layer from the host column; download writes
+ * those cells of the host column and leaves its other overlap layers and its
+ * shift as they are.  These are the host/device copies of a CyclicColumn's
+ * processing-context switch (src/core/platform/gpu/cuda/column.hh:157-187,
+ * 285-296), for blocks whose overlap is wider than the lattice's.  Both
+ * synchronise the device. */
+int lbmh_column_upload(lbmh_lattice* lattice, int i, const void* host, int64_t shift, int padding, int host_dtype);
+int lbmh_column_download(lbmh_lattice* lattice, int i, void* host, int64_t shift, int padding, int host_dtype);
+
 /* Equilibrium initialisation on the device from per-cell moments, host layout
  * [4][nx][ny][nz] = rho, ux, uy, uz.  Replaces SuperLattice::iniEquilibrium
  * (src/core/blockLattice.hh:202-220; dynamics/interface.h:111-121): forced
@@ -284,6 +299,10 @@ int lbmh_statistics(lbmh_lattice* lattice, double out[4]);
 /* Wait for all work queued on the lattice's device.  Replaces
  * gpu::cuda::device::synchronize (src/core/platform/gpu/cuda/device.hh:64-69). */
 int lbmh_synchronize(lbmh_lattice* lattice);
+/* The same for a device ordinal, without a lattice: the synchronisation
+ * SuperLattice performs before each post-processing stage on a GPU platform
+ * (src/core/superLattice.hh:666-671). */
+int lbmh_device_synchronize(int device);
 
 /* Launch shape of the time-step kernel.  No OpenLB counterpart: the CUDA
  * platform launches every kernel with 32-thread blocks (src/core/platform/gpu/

@@ -21,13 +21,13 @@ OK, E_INVALID, E_DEVICE, E_MEMORY, E_STATE = 0, -1, -2, -3, -4
 EXPORTS = (
     "lbmh_lattice_create", "lbmh_lattice_destroy", "lbmh_set_dynamics_ids",
     "lbmh_set_bulk_params", "lbmh_set_wall", "lbmh_populations_upload",
-    "lbmh_populations_download", "lbmh_ini_equilibrium", "lbmh_step",
+    "lbmh_populations_download", "lbmh_column_upload", "lbmh_column_download", "lbmh_ini_equilibrium", "lbmh_step",
     "lbmh_get_step_count", "lbmh_get_shifts", "lbmh_checkpoint_size", "lbmh_checkpoint_save",
     "lbmh_checkpoint_load", "lbmh_field_export", "lbmh_moments", "lbmh_reduce_moments",
     "lbmh_reduce_moments_noalloc", "lbmh_statistics",
     "lbmh_set_local_boundary", "lbmh_classify_boundary", "lbmh_initialize",
     "lbmh_collide", "lbmh_stream", "lbmh_halo_create", "lbmh_halo_destroy", "lbmh_halo_copy",
-    "lbmh_synchronize", "lbmh_lattice_footprint", "lbmh_abi_version", "lbmh_device_count",
+    "lbmh_synchronize", "lbmh_device_synchronize", "lbmh_lattice_footprint", "lbmh_abi_version", "lbmh_device_count",
     "lbmh_device_arch", "lbmh_last_error", "lbmh_set_launch_config", "lbmh_get_launch_config",
     "lbmh_autotune", "lbmh_pattern_ceiling",
     "lbmh_ensemble_create", "lbmh_ensemble_destroy", "lbmh_ensemble_member_bytes", "lbmh_ensemble_add",
@@ -100,6 +100,8 @@ def load() -> ctypes.CDLL:
     L.lbmh_set_wall.argtypes = [vp, i32, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
     L.lbmh_populations_upload.argtypes = [vp, vp, i32]
     L.lbmh_populations_download.argtypes = [vp, vp, i32]
+    L.lbmh_column_upload.argtypes = [vp, i32, vp, i64, i32, i32]
+    L.lbmh_column_download.argtypes = [vp, i32, vp, i64, i32, i32]
     L.lbmh_ini_equilibrium.argtypes = [vp, vp]
     L.lbmh_step.argtypes = [vp, i64, vp]
     L.lbmh_get_step_count.argtypes = [vp, ctypes.POINTER(i64)]
@@ -125,6 +127,7 @@ def load() -> ctypes.CDLL:
     L.lbmh_halo_destroy.argtypes = [vp]
     L.lbmh_halo_copy.argtypes = [vp, vp]
     L.lbmh_synchronize.argtypes = [vp]
+    L.lbmh_device_synchronize.argtypes = [i32]
     L.lbmh_lattice_footprint.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
     L.lbmh_device_count.argtypes = [ctypes.POINTER(i32)]
     L.lbmh_device_arch.argtypes = [i32, ctypes.c_char_p, ctypes.c_size_t]

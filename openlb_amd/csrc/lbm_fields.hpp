@@ -45,10 +45,18 @@ struct FieldConversion {
 // apart, i.e. one cache line per lane and population.)
 constexpr uint32_t kExportTileZ = 32, kExportTileX = 8;
 
-inline uint32_t field_export_blocks(int nx, int ny, int nz, int overlap)
+// Workgroups of the export grid.  The tiles pad x to a multiple of 8 and z to
+// a multiple of 32, so the grid can hold several times the points of a thin
+// block; the caller refuses a grid beyond HIP's 2^32 - 1 work-items per launch.
+inline uint64_t field_export_blocks(int nx, int ny, int nz, int overlap)
 {
-  const uint32_t ex = uint32_t(nx + 2 * overlap), ey = uint32_t(ny + 2 * overlap), ez = uint32_t(nz + 2 * overlap);
+  const uint64_t ex = uint64_t(nx + 2 * overlap), ey = uint64_t(ny + 2 * overlap), ez = uint64_t(nz + 2 * overlap);
   return ((ex + kExportTileX - 1) / kExportTileX) * ey * ((ez + kExportTileZ - 1) / kExportTileZ);
+}
+
+inline bool field_export_grid_fits(uint64_t blocks)
+{
+  return blocks * (kExportTileZ * kExportTileX) <= 0xffffffffull;
 }
 
 // launched with field_export_blocks(...) workgroups of kExportTileZ * kExportTileX lanes

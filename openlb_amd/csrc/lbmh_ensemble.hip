@@ -821,10 +821,11 @@ int member_block(lbmh_ensemble* E, int m, CheckpointBlock& b, std::vector<double
     constexpr int Q = decltype(qc)::value;
     const EnsembleEntry<Q>& en = entry<Q>(E, m);
     b.n_padded = en.n_padded;
+    b.ring = en.n_padded;
     b.stride = en.stride;
     b.d_pop = static_cast<char*>(E->d_pop) + en.pop_offset * storage_size(E->storage);
     for (int i = 0; i < Q; ++i) {
-      b.shift[i] = en.shift[i];
+      b.shift[i] = b.rshift[i] = en.shift[i];
       b.dist[i] = neighbor_distance<Q>(i, int64_t(en.sx), int64_t(en.sy));
     }
     return LBMH_OK;
@@ -905,7 +906,8 @@ int lbmh_ensemble_field_export(lbmh_ensemble* E, int m, int field, int overlap, 
   if (floats != want)
     return fail(LBMH_E_INVALID, "field buffer holds " + std::to_string(floats) + " floats, the export " +
                                     std::to_string(want));
-  if (points > 0xffffffffull) return fail(LBMH_E_INVALID, "member too large for one export");
+  if (points > 0xffffffffull || !field_export_grid_fits(field_export_blocks(mem.n[0], mem.n[1], mem.n[2], overlap)))
+    return fail(LBMH_E_INVALID, "member too large for one export (its 32z x 8x tiles exceed 2^32 work-items)");
   DeviceGuard guard(E->device);
   rc = pull_table(E);
   if (rc != LBMH_OK) return rc;
@@ -938,7 +940,7 @@ int lbmh_ensemble_field_export(lbmh_ensemble* E, int m, int field, int overlap, 
     a.div_z = en.div_z;
     a.div_y = en.div_y;
     a.omega = en.bp.omega;
-    hipLaunchKernelGGL((field_export_kernel<Q, S>), dim3(field_export_blocks(a.nx, a.ny, a.nz, overlap)),
+    hipLaunchKernelGGL((field_export_kernel<Q, S>), dim3(uint32_t(field_export_blocks(a.nx, a.ny, a.nz, overlap))),
                        dim3(kExportTileZ * kExportTileX), 0, 0, a,
                        overlap, field, conv, forced, F[0] * 0.5, F[1] * 0.5, F[2] * 0.5,
                        static_cast<float*>(out.p), uint32_t(points));

@@ -132,6 +132,20 @@ class BlockLattice:
         abi.check(abi.load().lbmh_populations_download(self._h, _ptr(out), code))
         return out
 
+    def upload_column(self, i: int, column: np.ndarray, shift: int, padding: int):
+        """Population column i from a host block in OpenLB's storage order:
+        (nx+2p)(ny+2p)(nz+2p) values, element k the padded cell (k - shift) mod count."""
+        column = np.ascontiguousarray(column)
+        dtype = {np.dtype(np.float64): abi.F64, np.dtype(np.float32): abi.F32}[column.dtype]
+        abi.check(abi.load().lbmh_column_upload(self._h, int(i), _ptr(column), int(shift), int(padding), dtype))
+
+    def download_column(self, i: int, column: np.ndarray, shift: int, padding: int):
+        """Write the core cells and the first overlap layer of column i into
+        `column` (OpenLB storage order at `shift`); other entries stay as they are."""
+        assert column.flags.c_contiguous
+        dtype = {np.dtype(np.float64): abi.F64, np.dtype(np.float32): abi.F32}[column.dtype]
+        abi.check(abi.load().lbmh_column_download(self._h, int(i), _ptr(column), int(shift), int(padding), dtype))
+
     def ini_equilibrium(self, rho_u: np.ndarray):
         a = np.ascontiguousarray(rho_u, dtype=np.float64)
         if a.shape != (4,) + self.shape:

@@ -1,0 +1,72 @@
+"""OpenLB's own cavity3dBenchmark program (examples/laminar/cavity3dBenchmark/
+cavity3d.cpp: D3Q19 BGK, LocalVelocity boundaries on the lid and the walls,
+Re 1000) built in the build container on OpenLB's CPU_SISD platform and on the
+GPU_HIP platform (integration/openlb), whose blocks run their time step through
+liblbmh; as written (value type float) and with `using T = double;`.  Each run
+writes OpenLB's VTK files; the physical velocity and pressure of the GPU_HIP
+runs are compared with the CPU runs'."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from openlb_amd import vtk
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BUILD = os.path.join(ROOT, "integration", "openlb", "_build")
+BINARIES = {k: os.path.join(BUILD, k) for k in ("cavity3d_gpu_hip", "cavity3d_cpu", "cavity3d_gpu_hip_f64",
+                                                 "cavity3d_cpu_f64")}
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not all(os.path.exists(b) for b in BINARIES.values()),
+                                 reason="cavity3d binaries not built (make -C integration/openlb)")]
+
+SIZE, STEPS = 24, 60
+FIELDS = ("physVelocity", "physPressure")
+
+
+@pytest.fixture(scope="module")
+def runs(tmp_path_factory):
+    out = {}
+    for name, binary in BINARIES.items():
+        cwd = tmp_path_factory.mktemp(name)
+        print(f"running {name} --size {SIZE} --steps {STEPS}", flush=True)
+        p = subprocess.run([binary, "--size", str(SIZE), "--steps", str(STEPS)], cwd=cwd, capture_output=True,
+                           text=True, timeout=240)
+        assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+        line = [ln for ln in p.stdout.splitlines() if ln.count(",") == 5][-1]
+        size, steps = (int(float(v)) for v in line.split(",")[:2])
+        assert (size, steps) == (SIZE, STEPS)
+        out[name] = {f: vtk.read_vti_arrays(os.path.join(cwd, "tmp", "vtkData", "data",
+                                                          f"{f}_iT0000000iC00000.vti"))[f] for f in FIELDS}
+        out[name]["mlups"] = float(line.split(",")[5])
+    return out
+
+
+def max_err(a, b):
+    return float(np.abs(a.astype(np.float64) - b.astype(np.float64)).max())
+
+
+def test_double_precision_run_matches_openlb(gpu_available, runs):
+    """T = double on both platforms: the GPU_HIP run equals OpenLB's CPU run to
+    FP64 rounding (the lattice's FP64 path is pinned to OpenLB at 1e-12)."""
+    for f in FIELDS:
+        a, b = runs["cavity3d_gpu_hip_f64"][f], runs["cavity3d_cpu_f64"][f]
+        scale = float(np.abs(b).max())
+        err = max_err(a, b)
+        print(f"f64 {f}: max |gpu_hip - cpu| = {err:.3e} of max {scale:.3e}", flush=True)
+        assert err <= 1e-6 * scale  # the writer stores float: 1e-6 of the field's range is a few float ulps
+
+
+def test_float_run_as_close_to_double_as_openlbs_own(gpu_available, runs):
+    """T = float, the program as written: at tau = 0.5007 rounding is amplified,
+    so two float implementations differ by about as much as each differs from the
+    double run.  The GPU_HIP float run must stay as close to OpenLB's double run
+    as OpenLB's own float run is (within a factor 2)."""
+    for f in FIELDS:
+        ref = runs["cavity3d_cpu_f64"][f]
+        gpu = max_err(runs["cavity3d_gpu_hip"][f], ref)
+        cpu = max_err(runs["cavity3d_cpu"][f], ref)
+        print(f"f32 {f}: |gpu_hip - cpu f64| = {gpu:.3e}, |cpu f32 - cpu f64| = {cpu:.3e}", flush=True)
+        assert gpu <= 2.0 * cpu + 1e-7

@@ -91,10 +91,12 @@ def test_device_export_matches_openlb_vtk(gpu_available, name, tmp_path):
 def test_device_export_matches_oracle(gpu_available, storage):
     """A forced Couette channel (periodic overlap, moving wall) and a local-velocity cavity, mid-run;
     the 19 x 6 x 70 channel spans several of the kernel's 32 (z) x 8 (x) point tiles, with
-    partial tiles at the high ends of both axes."""
+    partial tiles at the high ends of both axes; the one-cell-thick (nz = 1) periodic block
+    leaves most lanes of every 32-wide z tile idle."""
     for case in (W.couette_channel((9, 7, 6), tau=0.7, wall_u=0.05, force=1e-5),
                  W.local_velocity_cavity((9, 8, 7), tau=0.7, lid_u=0.06),
-                 W.couette_channel((19, 6, 70), tau=0.7, wall_u=0.05, force=1e-5)):
+                 W.couette_channel((19, 6, 70), tau=0.7, wall_u=0.05, force=1e-5),
+                 W.taylor_green((13, 10, 1), u0=0.04, tau=0.7)):
         f0 = O.initial_populations(case, storage)
         with BlockLattice.from_case(case, storage) as lat:
             lat.upload_populations(f0, "f64")
