@@ -97,6 +97,17 @@ int lbmh_set_dynamics_ids(lbmh_lattice* lattice, const uint8_t* ids);
  * ForcedBGKdynamics for a uniform force (src/dynamics/forcing.h:37-74). */
 int lbmh_set_bulk_params(lbmh_lattice* lattice, const lbmh_bulk_params* params);
 
+/* Per-cell force of the forced bulk kinds (LBMH_FORCED_BGK, LBMH_FORCED_TRT):
+ * host array [3][nx][ny][nz] of lattice-unit forces, replacing the uniform
+ * lbmh_bulk_params.force on every bulk cell; NULL returns to the uniform force.
+ * Replaces the per-cell descriptors::FORCE field that ForcedBGKdynamics and
+ * ForcedTRTdynamics read in their Guo forcing (src/dynamics/forcing.h:37-74,
+ * src/dynamics/momenta/elements.h:1480-1488) and that moments, statistics,
+ * equilibrium initialisation and field export read for the F/2 velocity shift.
+ * Non-finite values, and a lattice whose bulk kind is not forced, are
+ * LBMH_E_INVALID. */
+int lbmh_set_force_field(lbmh_lattice* lattice, const double* force);
+
 /* Moving wall id (3..255): fixed density rho and wall velocity u.  Replaces the
  * RHO / VELOCITY fields of BounceBackVelocity cells
  * (src/dynamics/momenta/elements.h:237-262, collision.h:323-348). */

@@ -64,6 +64,7 @@ private:
   std::vector<bool> _uploaded;     // columns uploaded since the last set-up
 
   lbmh_bulk_params _params{};
+  std::vector<double> _forceField;  // per-cell FORCE [3][core cells] on the device, empty for a uniform one
   std::array<double,4> _statistics{0.0, 1.0, 0.0, 0.0};
   bool _collided = false;
   bool _postDone = false;
@@ -110,6 +111,9 @@ private:
     desc.device = 0;
     check(lbmh_lattice_create(&desc, &_lattice), "lbmh_lattice_create");
     _kind = kind;
+    // the new lattice holds no parameters yet: all-ones bytes (NaNs) match no block's
+    std::memset(&_params, 0xff, sizeof(_params));
+    _forceField.clear();
     std::fill(_uploaded.begin(), _uploaded.end(), false);
     auto& population = _block.template getField<descriptors::POPULATION>();
     for (unsigned iPop = 0; iPop < q; ++iPop) {
@@ -292,8 +296,8 @@ private:
     }
   }
 
-  /// Bulk parameters; the forced kinds read the FORCE field, which the device
-  /// takes as one value for all bulk cells
+  /// Bulk parameters; the forced kinds read the FORCE field: one value for all
+  /// bulk cells goes into the parameters, a varying one to the device field
   void readParameters(const DynamicsEntry* bulk, const std::vector<const DynamicsEntry*>& role)
   {
     lbmh_bulk_params p{};
@@ -301,29 +305,38 @@ private:
     if (bulk && bulk->parameters) {
       bulk->parameters(p);
     }
+    std::vector<double> field;
     if (_kind == LBMH_FORCED_BGK || _kind == LBMH_FORCED_TRT) {
       if constexpr (DESCRIPTOR::template provides<descriptors::FORCE>()) {
         auto& force = _block.template getField<descriptors::FORCE>();
-        bool first = true;
         const int nx = _block.getNx(), ny = _block.getNy(), nz = _block.getNz();
+        const std::size_t ncore = std::size_t(nx) * ny * nz;
+        field.assign(3 * ncore, 0.0);
+        bool first = true;
+        bool uniform = true;
         for (int x = 0; x < nx; ++x) {
           for (int y = 0; y < ny; ++y) {
             for (int z = 0; z < nz; ++z) {
-              if (role[std::size_t(coreIndex(x, y, z))]->info.role != Role::Bulk) {
+              const std::size_t c = std::size_t(coreIndex(x, y, z));
+              if (role[c]->info.role != Role::Bulk) {
                 continue;
               }
               const CellID iCell = _block.getCellId(x, y, z);
               for (int d = 0; d < 3; ++d) {
                 const double f = double(force[d][iCell]);
+                field[d * ncore + c] = f;
                 if (first) {
                   p.force[d] = f;
                 } else if (f != p.force[d]) {
-                  throw std::runtime_error("GPU_HIP: the forced bulk dynamics take one FORCE value for the block");
+                  uniform = false;
                 }
               }
               first = false;
             }
           }
+        }
+        if (uniform) {
+          field.clear();
         }
       } else {
         throw std::runtime_error("GPU_HIP: forced dynamics on a descriptor without FORCE");
@@ -332,6 +345,10 @@ private:
     if (std::memcmp(&p, &_params, sizeof(p)) != 0) {
       check(lbmh_set_bulk_params(_lattice, &p), "lbmh_set_bulk_params");
       _params = p;
+    }
+    if (field != _forceField) {
+      check(lbmh_set_force_field(_lattice, field.empty() ? nullptr : field.data()), "lbmh_set_force_field");
+      _forceField = std::move(field);
     }
   }
 

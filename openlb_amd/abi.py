@@ -20,7 +20,7 @@ OK, E_INVALID, E_DEVICE, E_MEMORY, E_STATE = 0, -1, -2, -3, -4
 # every function include/lbmh.h declares
 EXPORTS = (
     "lbmh_lattice_create", "lbmh_lattice_destroy", "lbmh_set_dynamics_ids",
-    "lbmh_set_bulk_params", "lbmh_set_wall", "lbmh_populations_upload",
+    "lbmh_set_bulk_params", "lbmh_set_force_field", "lbmh_set_wall", "lbmh_populations_upload",
     "lbmh_populations_download", "lbmh_column_upload", "lbmh_column_download", "lbmh_ini_equilibrium", "lbmh_step",
     "lbmh_get_step_count", "lbmh_get_shifts", "lbmh_checkpoint_size", "lbmh_checkpoint_save",
     "lbmh_checkpoint_load", "lbmh_field_export", "lbmh_moments", "lbmh_reduce_moments",
@@ -97,6 +97,7 @@ def load() -> ctypes.CDLL:
     L.lbmh_lattice_destroy.argtypes = [vp]
     L.lbmh_set_dynamics_ids.argtypes = [vp, vp]
     L.lbmh_set_bulk_params.argtypes = [vp, ctypes.POINTER(BulkParams)]
+    L.lbmh_set_force_field.argtypes = [vp, vp]
     L.lbmh_set_wall.argtypes = [vp, i32, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
     L.lbmh_populations_upload.argtypes = [vp, vp, i32]
     L.lbmh_populations_download.argtypes = [vp, vp, i32]

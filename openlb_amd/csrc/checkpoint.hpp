@@ -82,6 +82,11 @@ struct CheckpointBlock {
   const uint32_t* info = nullptr; // packed LocalInfo per id
   int n_ids = 0;                  // entries of walls / info
   double force[3] = {0, 0, 0};
+  const double* force_field = nullptr;  // per-cell FORCE [3][ncore], or null (force)
+  // per padded cell: the role of the cell an overlap cell holds a copy of,
+  // where another block (or this one across a periodic face) fills it; -1
+  // where no halo does (then the block's own periodic images apply)
+  const int8_t* overlap_role = nullptr;
 };
 
 // Role of a dynamics id in OpenLB's field set.
@@ -130,6 +135,14 @@ inline std::vector<CheckpointField> checkpoint_fields(const CheckpointBlock& b)
   const size_t ncore = size_t(b.n[0]) * b.n[1] * b.n[2];
   bool seen[256] = {};
   for (size_t c = 0; c < ncore; ++c) seen[b.ids[c]] = true;
+  if (b.overlap_role) {
+    // OpenLB defines dynamics on the overlap too, so walls seen only there
+    // allocate their fields as well
+    for (uint32_t k = 0; k < b.n_padded; ++k) {
+      const int r = b.overlap_role[k];
+      if (r == kRoleBounceBack || r == kRoleMovingWall) role[r] = true;
+    }
+  }
   for (int id = 0; id < 256; ++id) {
     if (!seen[id]) continue;
     const int r = checkpoint_role(b, id);
@@ -200,9 +213,10 @@ void openlb_to_ring(const T* in, uint32_t n, int64_t s, T* ring, uint32_t m, int
   }
 }
 
-// Core-cell value of component `comp` of field `f`, and whether the block's
-// dynamics read it.
-inline double field_value(const CheckpointBlock& b, const CheckpointField& f, int id, int comp, bool& read)
+// Core-cell value of component `comp` of field `f` at core cell `cell` (flat
+// core index), and whether the block's dynamics read it.
+inline double field_value(const CheckpointBlock& b, const CheckpointField& f, int id, int comp, bool& read,
+                          size_t cell)
 {
   read = false;
   const int r = checkpoint_role(b, id);
@@ -211,7 +225,7 @@ inline double field_value(const CheckpointBlock& b, const CheckpointField& f, in
   case 0:
     if (r != kRoleBulk) return 0.0;
     read = true;
-    return b.force[comp];
+    return b.force_field ? b.force_field[size_t(comp) * b.n[0] * b.n[1] * b.n[2] + cell] : b.force[comp];
   case 1:
     if (r != kRoleMovingWall) return 0.0;
     read = true;
@@ -260,11 +274,17 @@ void fill_field_column(const CheckpointBlock& b, const CheckpointField& f, int c
         double v = 0.0;
         if (core) {
           bool read;
-          v = field_value(b, f, b.ids[(size_t(x - 1) * b.n[1] + (y - 1)) * b.n[2] + (z - 1)], comp, read);
+          const size_t cell = (size_t(x - 1) * b.n[1] + (y - 1)) * b.n[2] + (z - 1);
+          v = field_value(b, f, b.ids[cell], comp, read, cell);
         } else if (f.which == 2) {
           // defineDynamics covers the overlap, boundary::set does not (setBoundary.h:87-88)
-          const int id = image_id(b, x, y, z);
-          const int r = id < 0 ? kRoleNone : checkpoint_role(b, id);
+          int r;
+          if (b.overlap_role && b.overlap_role[k] >= 0) {
+            r = b.overlap_role[k];
+          } else {
+            const int id = image_id(b, x, y, z);
+            r = id < 0 ? kRoleNone : checkpoint_role(b, id);
+          }
           if (r == kRoleBounceBack || r == kRoleMovingWall) v = 1.0;
         }
         out[k] = T(v);
@@ -358,7 +378,8 @@ int checkpoint_load_typed(const CheckpointBlock& b, const uint8_t* in, int64_t* 
         for (int y = 0; y < b.n[1]; ++y)
           for (int z = 0; z < b.n[2]; ++z) {
             bool read;
-            const double v = field_value(b, f, b.ids[(size_t(x) * b.n[1] + y) * b.n[2] + z], d, read);
+            const size_t cell = (size_t(x) * b.n[1] + y) * b.n[2] + z;
+            const double v = field_value(b, f, b.ids[cell], d, read, cell);
             const size_t k = (size_t(x + 1) * P[1] + (y + 1)) * P[2] + (z + 1);
             if (read && !(column[k] == T(v))) {
               return fail(LBMH_E_INVALID, std::string("checkpoint field ") + kFieldNames[f.which] + " at cell (" +

@@ -67,6 +67,7 @@ __device__ __forceinline__ StepArgs<Q, typename StorageTraits<S>::V> member_args
   a.xcd_group = 0;
   a.walls = walls + uint64_t(lat) * kEnsembleWalls * 4;
   a.info = info ? info + uint64_t(lat) * kEnsembleWalls : nullptr;
+  a.force = nullptr;  // members take the uniform force
   a.bp.omega = V(e.bp.omega);
   a.bp.omega2 = V(e.bp.omega2);
   a.bp.guo = V(e.bp.guo);

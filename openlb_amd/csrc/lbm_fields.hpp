@@ -23,6 +23,7 @@
 // face there are no dynamics.
 #pragma once
 
+#include "lbm_kernels.hpp"
 #include "lbm_local.hpp"
 
 namespace lbmh {
@@ -63,6 +64,7 @@ inline bool field_export_grid_fits(uint64_t blocks)
 template <int Q, typename S>
 __global__ __launch_bounds__(256) void field_export_kernel(PostArgs<Q> a, int overlap, int field, FieldConversion conv,
                                                            int forced, double hfx, double hfy, double hfz,
+                                                           const typename StorageTraits<S>::V* force,
                                                            float* __restrict__ out, uint32_t points)
 {
   static_assert(kExportTileZ * kExportTileX == 256, "one workgroup is one tile");
@@ -87,13 +89,15 @@ __global__ __launch_bounds__(256) void field_export_kernel(PostArgs<Q> a, int ov
     if (!((a.periodic >> d) & 1u)) inside = false;
     r[d] = (r[d] + n[d]) % n[d];
   }
-  const int id = inside ? a.ids[(uint32_t(r[0]) * uint32_t(a.ny) + uint32_t(r[1])) * uint32_t(a.nz) + uint32_t(r[2])] : 0;
+  const uint32_t core = (uint32_t(r[0]) * uint32_t(a.ny) + uint32_t(r[1])) * uint32_t(a.nz) + uint32_t(r[2]);
+  const int id = inside ? a.ids[core] : 0;
   V rho = V(1), u[3] = {V(0), V(0), V(0)};
   if (inside && (!image || id == 1)) cell_rho_u<Q, S, V>(a, r[0], r[1], r[2], rho, u);
   if (forced && inside && id == 1) {
-    u[0] += V(hfx);
-    u[1] += V(hfy);
-    u[2] += V(hfz);
+    const uint32_t ncore = uint32_t(a.nx) * uint32_t(a.ny) * uint32_t(a.nz);
+    u[0] += half_force<V>(force, ncore, core, 0, hfx);
+    u[1] += half_force<V>(force, ncore, core, 1, hfy);
+    u[2] += half_force<V>(force, ncore, core, 2, hfz);
   }
   switch (field) {
   case kFieldDensity:

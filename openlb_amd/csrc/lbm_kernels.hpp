@@ -52,6 +52,7 @@ struct StepArgs {
   uint32_t xcd_group;       // runs of consecutive chunks per XCD (0: identity)
   const double* walls;      // [kMaxWalls][4] rho, ux, uy, uz
   const uint32_t* info;     // [kMaxWalls] packed LocalInfo (local-boundary lattices only)
+  const V* force;           // per-cell FORCE [3][ncore] of the forced kinds, or null (bp.force)
   BulkParams<V> bp;
 };
 
@@ -178,7 +179,20 @@ __device__ __forceinline__ void collide_stream_cell(const StepArgs<Q, typename S
   asm volatile("" : "+v"(ns));
   if (!IDS || id != 0) {
     if (!IDS || id == kIdBulk) {
-      collide_bulk<Q, V, KIND>(f, a.bp);
+      if constexpr (KIND == kForcedBGK || KIND == kForcedTRT) {
+        // ForcedBGK / ForcedTRT read the cell's own FORCE (momenta/elements.h:1480-1488)
+        BulkParams<V> p = a.bp;
+        if (a.force) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            p.force[d] = a.force[uint64_t(d) * a.ncore + c];
+            p.half_force[d] = p.force[d] * V(0.5);
+          }
+        }
+        collide_bulk<Q, V, KIND>(f, p);
+      } else {
+        collide_bulk<Q, V, KIND>(f, a.bp);
+      }
     } else if (id == 2) {
       revert<Q, V>(f);
     } else {
@@ -373,10 +387,19 @@ __global__ __launch_bounds__(256) void halo_copy_kernel(S* dst, uint64_t dst_str
       src[uint64_t(i) * src_stride + wrap_index(p.y, ssh.s[i], src_n)];
 }
 
+// Half the FORCE of core cell c, component d: the per-cell field [3][ncore]
+// where the lattice has one, else the uniform value (F/2 given in FP64).
+template <typename V>
+__device__ __forceinline__ V half_force(const V* force, uint32_t ncore, uint32_t c, int d, double uniform_half)
+{
+  return force ? force[uint64_t(d) * ncore + c] * V(0.5) : V(uniform_half);
+}
+
 // Equilibrium from per-cell (rho, u), iniEquilibrium semantics.
 template <int Q, typename S>
 __global__ void ini_equilibrium_kernel(const double* rho_u, const uint8_t* ids, S* pop, CoreGeom g,
-                                       Shifts<Q> sh, double hfx, double hfy, double hfz, int forced)
+                                       Shifts<Q> sh, double hfx, double hfy, double hfz, int forced,
+                                       const typename StorageTraits<S>::V* force)
 {
   using V = typename StorageTraits<S>::V;
   using L = Lattice<Q>;
@@ -388,9 +411,9 @@ __global__ void ini_equilibrium_kernel(const double* rho_u, const uint8_t* ids, 
   V u[3] = {V(rho_u[uint64_t(g.ncore) + c]), V(rho_u[2 * uint64_t(g.ncore) + c]),
             V(rho_u[3 * uint64_t(g.ncore) + c])};
   if (forced && id == 1) {
-    u[0] -= V(hfx);
-    u[1] -= V(hfy);
-    u[2] -= V(hfz);
+    u[0] -= half_force<V>(force, g.ncore, c, 0, hfx);
+    u[1] -= half_force<V>(force, g.ncore, c, 1, hfy);
+    u[2] -= half_force<V>(force, g.ncore, c, 2, hfz);
   }
   const V drho = rho - V(1);
   const V usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
@@ -412,7 +435,8 @@ __global__ void ini_equilibrium_kernel(const double* rho_u, const uint8_t* ids, 
 // Per-cell rho, u in FP64 ([4][ncore]); u includes F/2 on forced bulk cells.
 template <int Q, typename S>
 __global__ void moments_kernel(double* out, const uint8_t* ids, const S* pop, CoreGeom g, Shifts<Q> sh,
-                               double hfx, double hfy, double hfz, int forced)
+                               double hfx, double hfy, double hfz, int forced,
+                               const typename StorageTraits<S>::V* force)
 {
   using V = typename StorageTraits<S>::V;
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -427,9 +451,9 @@ __global__ void moments_kernel(double* out, const uint8_t* ids, const S* pop, Co
   double u[3] = {double(j[0] / rho), double(j[1] / rho), double(j[2] / rho)};
   const int id = ids ? ids[c] : 1;
   if (forced && id == 1) {
-    u[0] += double(V(hfx));
-    u[1] += double(V(hfy));
-    u[2] += double(V(hfz));
+    u[0] += double(half_force<V>(force, g.ncore, c, 0, hfx));
+    u[1] += double(half_force<V>(force, g.ncore, c, 1, hfy));
+    u[2] += double(half_force<V>(force, g.ncore, c, 2, hfz));
   }
   out[c] = double(rho);
   out[uint64_t(g.ncore) + c] = u[0];
@@ -496,7 +520,8 @@ template <int Q, typename S>
 __global__ __launch_bounds__(kBlock) void statistics_kernel(double* partial, const uint8_t* ids, const S* pop,
                                                             CoreGeom g, Shifts<Q> sh, const double* walls,
                                                             const uint32_t* info, double omega,
-                                                            double hfx, double hfy, double hfz, int forced)
+                                                            double hfx, double hfy, double hfz, int forced,
+                                                            const typename StorageTraits<S>::V* force)
 {
   using V = typename StorageTraits<S>::V;
   double acc[4] = {0, 0, 0, 0};
@@ -530,9 +555,9 @@ __global__ __launch_bounds__(kBlock) void statistics_kernel(double* partial, con
       const V r = V(1) + drho;
       V u[3] = {j[0] / r, j[1] / r, j[2] / r};
       if (forced) {
-        u[0] += V(hfx);
-        u[1] += V(hfy);
-        u[2] += V(hfz);
+        u[0] += half_force<V>(force, g.ncore, c, 0, hfx);
+        u[1] += half_force<V>(force, g.ncore, c, 1, hfy);
+        u[2] += half_force<V>(force, g.ncore, c, 2, hfz);
       }
       rho = double(r);
       usq = double(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);

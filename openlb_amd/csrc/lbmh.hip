@@ -7,6 +7,7 @@
 // codes with a per-thread message; nothing throws across the boundary.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -92,10 +93,17 @@ struct lbmh_lattice {
   size_t footprint = 0;
   // local (regularized) boundaries
   std::vector<uint8_t> h_ids;        // host copy of the dynamics ids
+  std::vector<lbmh_halo*> halos_in;  // halos that fill this block's overlap
+  std::vector<lbmh_halo*> halos_out; // halos that read this block
   bool id_defined[kMaxWalls] = {};   // ids given a wall or boundary entry
   uint32_t info[kMaxWalls] = {};     // packed LocalInfo per id
   uint32_t* d_info = nullptr;
   bool has_local = false;
+  // per-cell FORCE of the forced kinds: [3][ncore] on the device in the
+  // arithmetic type (double for FP64 storage, float otherwise) and in FP64 on
+  // the host (checkpoints); empty: the uniform params.force
+  void* d_force = nullptr;
+  std::vector<double> h_force;
   std::vector<uint32_t> h_edges, h_corners, h_local, h_planes;
   uint32_t* d_edges = nullptr;
   uint32_t* d_corners = nullptr;
@@ -125,9 +133,11 @@ static lbmh_launch_config default_launch(int storage, bool local)
 }
 
 struct lbmh_halo {
-  lbmh_lattice* dst = nullptr;
-  const lbmh_lattice* src = nullptr;
+  lbmh_lattice* dst = nullptr;  // null once the destination block is destroyed
+  const lbmh_lattice* src = nullptr;  // null once the source block is destroyed
+  int device = 0;
   uint2* d_pairs = nullptr;   // (destination overlap cell, source core cell), padded ids
+  std::vector<uint2> pairs;   // the same on the host: which cell each overlap cell copies
   uint32_t count = 0;
 };
 
@@ -155,6 +165,13 @@ Shifts<Q> shifts_of(const lbmh_lattice* L)
   Shifts<Q> s;
   for (int i = 0; i < Q; ++i) s.s[i] = uint32_t(L->rshift[i]);
   return s;
+}
+
+// The lattice's per-cell FORCE in the arithmetic type of storage S, or null.
+template <typename S>
+const typename StorageTraits<S>::V* force_field(const lbmh_lattice* L)
+{
+  return static_cast<const typename StorageTraits<S>::V*>(L->d_force);
 }
 
 template <typename V>
@@ -301,6 +318,7 @@ StepArgs<Q, V> step_args(const lbmh_lattice* L)
   a.xcd_group = uint32_t(L->launch.xcd_group);
   a.walls = L->d_walls;
   a.info = L->d_info;
+  a.force = static_cast<const V*>(L->d_force);
   a.bp = bulk_params<V>(L);
   for (int i = 0; i < Q; ++i) {
     a.shift[i] = uint32_t(L->rshift[i]);
@@ -600,7 +618,10 @@ int lbmh_lattice_destroy(lbmh_lattice* L)
 {
   if (!L) return LBMH_OK;
   DeviceGuard guard(L->device);
+  for (lbmh_halo* H : L->halos_in) H->dst = nullptr;
+  for (lbmh_halo* H : L->halos_out) H->src = nullptr;
   unmap_columns(L);
+  if (L->d_force) (void)hipFree(L->d_force);
   if (L->d_ids) (void)hipFree(L->d_ids);
   if (L->d_walls) (void)hipFree(L->d_walls);
   if (L->d_partials) (void)hipFree(L->d_partials);
@@ -652,6 +673,40 @@ int lbmh_set_bulk_params(lbmh_lattice* L, const lbmh_bulk_params* p)
   if (!L || !p) return fail(LBMH_E_INVALID, "null argument");
   if (!valid_bulk_params(L->kind, *p)) return LBMH_E_INVALID;
   L->params = *p;
+  return LBMH_OK;
+}
+
+int lbmh_set_force_field(lbmh_lattice* L, const double* force)
+{
+  if (!L) return fail(LBMH_E_INVALID, "null lattice");
+  if (L->kind != LBMH_FORCED_BGK && L->kind != LBMH_FORCED_TRT)
+    return fail(LBMH_E_INVALID, "a FORCE field needs a forced bulk kind (LBMH_FORCED_BGK / LBMH_FORCED_TRT)");
+  DeviceGuard guard(L->device);
+  LBMH_HIP(hipDeviceSynchronize());
+  const size_t n = size_t(3) * L->ncore;
+  const size_t es = L->storage == LBMH_F64 ? sizeof(double) : sizeof(float);
+  if (!force) {
+    if (L->d_force) {
+      LBMH_HIP(hipFree(L->d_force));
+      L->footprint -= n * es;
+    }
+    L->d_force = nullptr;
+    L->h_force.clear();
+    return LBMH_OK;
+  }
+  for (size_t k = 0; k < n; ++k)
+    if (!std::isfinite(force[k])) return fail(LBMH_E_INVALID, "FORCE field holds a non-finite value");
+  if (!L->d_force) {
+    LBMH_HIP(hipMalloc(&L->d_force, n * es));
+    L->footprint += n * es;
+  }
+  if (L->storage == LBMH_F64) {
+    LBMH_HIP(hipMemcpy(L->d_force, force, n * es, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> v(force, force + n);
+    LBMH_HIP(hipMemcpy(L->d_force, v.data(), n * es, hipMemcpyHostToDevice));
+  }
+  L->h_force.assign(force, force + n);
   return LBMH_OK;
 }
 
@@ -1019,7 +1074,7 @@ int lbmh_ini_equilibrium(lbmh_lattice* L, const double* rho_u)
     using S = decltype(s);
     hipLaunchKernelGGL((ini_equilibrium_kernel<Q, S>), dim3(grid), dim3(256), 0, 0, (const double*)staging.p,
                        L->uniform_bulk ? nullptr : L->d_ids, static_cast<S*>(L->d_pop), g, shifts_of<Q>(L),
-                       hf[0] * 0.5, hf[1] * 0.5, hf[2] * 0.5, forced);
+                       hf[0] * 0.5, hf[1] * 0.5, hf[2] * 0.5, forced, force_field<S>(L));
     LBMH_HIP(hipGetLastError());
     return LBMH_OK;
   });
@@ -1098,7 +1153,11 @@ int lbmh_halo_create(lbmh_lattice* dst, const lbmh_lattice* src, const uint32_t*
   if (!H) return fail(LBMH_E_MEMORY, "host allocation failed");
   H->dst = dst;
   H->src = src;
+  H->device = dst->device;
   H->count = uint32_t(n);
+  H->pairs = pairs;
+  dst->halos_in.push_back(H);
+  const_cast<lbmh_lattice*>(src)->halos_out.push_back(H);
   DeviceGuard guard(dst->device);
   if (n) {
     hipError_t e = hipMalloc((void**)&H->d_pairs, sizeof(uint2) * n);
@@ -1116,7 +1175,13 @@ int lbmh_halo_create(lbmh_lattice* dst, const lbmh_lattice* src, const uint32_t*
 int lbmh_halo_destroy(lbmh_halo* H)
 {
   if (!H) return LBMH_OK;
-  DeviceGuard guard(H->dst->device);
+  DeviceGuard guard(H->device);
+  if (H->dst) H->dst->halos_in.erase(std::remove(H->dst->halos_in.begin(), H->dst->halos_in.end(), H),
+                                     H->dst->halos_in.end());
+  if (H->src) {
+    auto& out = const_cast<lbmh_lattice*>(H->src)->halos_out;
+    out.erase(std::remove(out.begin(), out.end(), H), out.end());
+  }
   if (H->d_pairs) (void)hipFree(H->d_pairs);
   delete H;
   return LBMH_OK;
@@ -1125,6 +1190,7 @@ int lbmh_halo_destroy(lbmh_halo* H)
 int lbmh_halo_copy(lbmh_halo* H, void* stream)
 {
   if (!H) return fail(LBMH_E_INVALID, "null halo");
+  if (!H->dst || !H->src) return fail(LBMH_E_INVALID, "a block of this halo was destroyed");
   if (!H->count) return LBMH_OK;
   const lbmh_lattice* D = H->dst;
   const lbmh_lattice* S_ = H->src;
@@ -1158,7 +1224,16 @@ int lbmh_get_shifts(const lbmh_lattice* L, int64_t* shifts)
 
 namespace {
 
-CheckpointBlock checkpoint_block(const lbmh_lattice* L, std::vector<uint8_t>& no_ids)
+// Role (CheckpointRole) of dynamics id `id` of lattice S
+int role_in(const lbmh_lattice* S, int id)
+{
+  CheckpointBlock c;
+  c.info = S->info;
+  c.n_ids = kMaxWalls;
+  return checkpoint_role(c, id);
+}
+
+CheckpointBlock checkpoint_block(const lbmh_lattice* L, std::vector<uint8_t>& no_ids, std::vector<int8_t>& overlap)
 {
   CheckpointBlock b;
   b.q = L->q;
@@ -1186,7 +1261,23 @@ CheckpointBlock checkpoint_block(const lbmh_lattice* L, std::vector<uint8_t>& no
   }
   b.walls = &L->walls[0][0];
   b.info = L->info;
+  b.force_field = L->h_force.empty() ? nullptr : L->h_force.data();
   b.n_ids = kMaxWalls;
+  if (!L->halos_in.empty()) {
+    overlap.assign(L->n_padded, int8_t(-1));
+    for (const lbmh_halo* H : L->halos_in) {
+      const lbmh_lattice* S = H->src;
+      if (!S) continue;
+      for (const uint2& p : H->pairs) {
+        const uint32_t sy = uint32_t(S->n[1] + 2), sz = uint32_t(S->n[2] + 2);
+        const int x = int(p.y / (sy * sz)) - 1, y = int(p.y / sz % sy) - 1, z = int(p.y % sz) - 1;
+        const bool core = x >= 0 && x < S->n[0] && y >= 0 && y < S->n[1] && z >= 0 && z < S->n[2];
+        const size_t c = (size_t(x) * S->n[1] + y) * S->n[2] + z;
+        overlap[p.x] = int8_t(core && S->h_ids.size() == S->ncore ? role_in(S, S->h_ids[c]) : kRoleNone);
+      }
+    }
+    b.overlap_role = overlap.data();
+  }
   return b;
 }
 
@@ -1197,7 +1288,8 @@ int lbmh_checkpoint_size(const lbmh_lattice* L, int real, size_t* bytes)
   if (!L || !bytes) return fail(LBMH_E_INVALID, "null argument");
   if (real != LBMH_F64 && real != LBMH_F32) return fail(LBMH_E_INVALID, "checkpoint value type must be F64 or F32");
   std::vector<uint8_t> no_ids;
-  *bytes = checkpoint_bytes(checkpoint_block(L, no_ids), real);
+  std::vector<int8_t> overlap;
+  *bytes = checkpoint_bytes(checkpoint_block(L, no_ids, overlap), real);
   return LBMH_OK;
 }
 
@@ -1206,7 +1298,8 @@ int lbmh_checkpoint_save(lbmh_lattice* L, int real, void* buffer, size_t bytes)
   if (!L) return fail(LBMH_E_INVALID, "null lattice");
   DeviceGuard guard(L->device);
   std::vector<uint8_t> no_ids;
-  return checkpoint_save(checkpoint_block(L, no_ids), real, buffer, bytes);
+  std::vector<int8_t> overlap;
+  return checkpoint_save(checkpoint_block(L, no_ids, overlap), real, buffer, bytes);
 }
 
 int lbmh_checkpoint_load(lbmh_lattice* L, int real, const void* buffer, size_t bytes)
@@ -1214,8 +1307,9 @@ int lbmh_checkpoint_load(lbmh_lattice* L, int real, const void* buffer, size_t b
   if (!L) return fail(LBMH_E_INVALID, "null lattice");
   DeviceGuard guard(L->device);
   std::vector<uint8_t> no_ids;
+  std::vector<int8_t> overlap;
   int64_t shift[27];
-  const int rc = checkpoint_load(checkpoint_block(L, no_ids), real, buffer, bytes, shift);
+  const int rc = checkpoint_load(checkpoint_block(L, no_ids, overlap), real, buffer, bytes, shift);
   if (rc != LBMH_OK) return rc;
   for (int i = 0; i < L->q; ++i) L->shift[i] = L->rshift[i] = shift[i];
   return LBMH_OK;
@@ -1248,7 +1342,7 @@ int lbmh_field_export(lbmh_lattice* L, int field, int overlap, const double conv
     hipLaunchKernelGGL((field_export_kernel<Q, S>),
                        dim3(uint32_t(field_export_blocks(L->n[0], L->n[1], L->n[2], overlap))),
                        dim3(kExportTileZ * kExportTileX), 0, 0, post_args<Q>(L), overlap, field, conv, forced, F[0] * 0.5, F[1] * 0.5, F[2] * 0.5,
-                       static_cast<float*>(out.p), uint32_t(points));
+                       force_field<S>(L), static_cast<float*>(out.p), uint32_t(points));
     LBMH_HIP(hipGetLastError());
     return LBMH_OK;
   });
@@ -1274,7 +1368,7 @@ int lbmh_moments(lbmh_lattice* L, double* rho_u)
     using S = decltype(s);
     hipLaunchKernelGGL((moments_kernel<Q, S>), dim3(grid), dim3(256), 0, 0, (double*)out.p,
                        L->uniform_bulk ? nullptr : L->d_ids, static_cast<const S*>(L->d_pop), g, shifts_of<Q>(L),
-                       hf[0] * 0.5, hf[1] * 0.5, hf[2] * 0.5, forced);
+                       hf[0] * 0.5, hf[1] * 0.5, hf[2] * 0.5, forced, force_field<S>(L));
     LBMH_HIP(hipGetLastError());
     return LBMH_OK;
   });
@@ -1350,7 +1444,7 @@ int lbmh_statistics(lbmh_lattice* L, double out[4])
     hipLaunchKernelGGL((statistics_kernel<Q, S>), dim3(blocks), dim3(kBlock), 0, 0, L->d_partials,
                        L->uniform_bulk ? nullptr : L->d_ids, static_cast<const S*>(L->d_pop), g, shifts_of<Q>(L),
                        static_cast<const double*>(L->d_walls), L->has_local ? L->d_info : nullptr,
-                       L->params.omega, hf[0] * 0.5, hf[1] * 0.5, hf[2] * 0.5, forced);
+                       L->params.omega, hf[0] * 0.5, hf[1] * 0.5, hf[2] * 0.5, forced, force_field<S>(L));
     LBMH_HIP(hipGetLastError());
     return LBMH_OK;
   });
@@ -1425,6 +1519,10 @@ int lbmh_autotune(lbmh_lattice* L, int trial_steps, lbmh_launch_config* chosen)
   T->uniform_bulk = L->uniform_bulk;
   T->has_local = L->has_local;  // times the whole-lattice pass; the clone has no boundary lists
   T->params = L->params;
+  if (!L->h_force.empty()) {  // the forced step reads the FORCE field: time that too
+    rc = lbmh_set_force_field(T, L->h_force.data());
+    if (rc != LBMH_OK) return rc;
+  }
   LBMH_HIP(hipEventCreate(&cleanup.e0));
   LBMH_HIP(hipEventCreate(&cleanup.e1));
   static const int kLds[] = {0, 24576, 32768, 40960, 49152, 65536};

@@ -721,7 +721,8 @@ int lbmh_ensemble_ini_equilibrium(lbmh_ensemble* E, int m, const double* rho_u)
     S* pop = static_cast<S*>(E->d_pop) + en.pop_offset;
     hipLaunchKernelGGL((ini_equilibrium_kernel<Q, S>), dim3(grid_for(mem.ncore, 256)), dim3(256), 0, 0,
                        (const double*)staging.p, E->d_ids + mem.ids_off, pop, geom_of(en), shifts_of(en),
-                       mem.params.force[0] * 0.5, mem.params.force[1] * 0.5, mem.params.force[2] * 0.5, forced);
+                       mem.params.force[0] * 0.5, mem.params.force[1] * 0.5, mem.params.force[2] * 0.5, forced,
+                       nullptr);
     LBMH_HIP(hipGetLastError());
     return LBMH_OK;
   });
@@ -943,7 +944,7 @@ int lbmh_ensemble_field_export(lbmh_ensemble* E, int m, int field, int overlap, 
     hipLaunchKernelGGL((field_export_kernel<Q, S>), dim3(uint32_t(field_export_blocks(a.nx, a.ny, a.nz, overlap))),
                        dim3(kExportTileZ * kExportTileX), 0, 0, a,
                        overlap, field, conv, forced, F[0] * 0.5, F[1] * 0.5, F[2] * 0.5,
-                       static_cast<float*>(out.p), uint32_t(points));
+                       nullptr, static_cast<float*>(out.p), uint32_t(points));
     LBMH_HIP(hipGetLastError());
     return LBMH_OK;
   });
@@ -972,7 +973,7 @@ int lbmh_ensemble_moments(lbmh_ensemble* E, int m, double* rho_u)
     const S* pop = static_cast<const S*>(E->d_pop) + en.pop_offset;
     hipLaunchKernelGGL((moments_kernel<Q, S>), dim3(grid_for(mem.ncore, 256)), dim3(256), 0, 0, (double*)out.p,
                        E->d_ids + mem.ids_off, pop, geom_of(en), shifts_of(en), mem.params.force[0] * 0.5,
-                       mem.params.force[1] * 0.5, mem.params.force[2] * 0.5, forced);
+                       mem.params.force[1] * 0.5, mem.params.force[2] * 0.5, forced, nullptr);
     LBMH_HIP(hipGetLastError());
     return LBMH_OK;
   });

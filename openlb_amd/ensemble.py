@@ -76,6 +76,8 @@ class Ensemble:
     def add(self, case, steps: int = -1) -> int:
         """Add a workloads.Case as a member. It returns the member id. The walls come
         from case.walls, and the populations start at zero."""
+        if getattr(case, "force_field", None) is not None:
+            raise ValueError("ensemble members take a uniform FORCE; a per-cell FORCE field needs a BlockLattice")
         ids = np.ascontiguousarray(case.ids, dtype=np.uint8)
         p = abi.BulkParams()
         p.omega, p.magic, p.smago = case.omega, case.magic, case.smago
@@ -186,3 +188,142 @@ class Ensemble:
                                                  ctypes.byref(used), ctypes.byref(cap)))
         return {"members": m.value, "cells": cells.value, "bytes_used": used.value,
                 "bytes_capacity": cap.value}
+
+
+# FP16 population storage keeps the velocity error under 1e-3 of the driving
+# velocity for the cavity ensembles down to tau ~ 0.53; closer to BGK's
+# stability limit (tau -> 1/2) weakly damped non-hydrodynamic modes amplify the
+# FP16 rounding past that bound.  Members below this tau run in FP32 storage.
+FP16_TAU_GUARD = 0.53
+
+
+def guarded_storage(case, storage: str, tau_guard: float | None = FP16_TAU_GUARD) -> str:
+    """The storage a member asked to run in `storage` gets under the precision
+    guard: FP16 requests with tau < tau_guard run in FP32; None turns the guard off."""
+    if storage == "f16" and tau_guard is not None and 1.0 / case.omega < tau_guard:
+        return "f32"
+    return storage
+
+
+class GuardedEnsemble:
+    """An FP16 ensemble with the per-member precision guard: members whose tau is
+    below ``tau_guard`` go to an FP32 ensemble beside it (``guarded_storage``).
+    Member ids are the composite's own; every call goes to the member's
+    ensemble, and ``step`` advances both ensembles, back to back on the stream.
+    Each member runs the same per-cell code as it would in a plain ensemble of
+    its storage, so its result is bit-identical to that."""
+
+    def __init__(self, q: int = 19, kind: str = "bgk", max_members: int = 512,
+                 population_bytes: int = 1 << 30, cell_capacity: int = 1 << 24, device: int = 0,
+                 boundary_capacity: int = 0, tau_guard: float | None = FP16_TAU_GUARD,
+                 fp32_population_bytes: int | None = None, fp32_cell_capacity: int | None = None,
+                 fp32_max_members: int | None = None):
+        self.q, self.kind, self.tau_guard = q, kind, tau_guard
+        self.parts: dict[str, Ensemble] = {}
+        self.parts["f16"] = Ensemble(q, "f16", kind, max_members, population_bytes, cell_capacity, device,
+                                     boundary_capacity)
+        try:
+            self.parts["f32"] = Ensemble(
+                q, "f32", kind, fp32_max_members or max_members,
+                fp32_population_bytes if fp32_population_bytes is not None else max(population_bytes // 4, 1 << 20),
+                fp32_cell_capacity if fp32_cell_capacity is not None else max(cell_capacity // 4, 1 << 16),
+                device, boundary_capacity)
+        except BaseException:
+            self.parts["f16"].close()
+            raise
+        self._where: dict[int, tuple[str, int]] = {}
+        self._count = {"f16": 0, "f32": 0}
+        self._next = 0
+
+    @classmethod
+    def for_cases(cls, cases, q: int = 19, kind: str = "bgk", tau_guard: float | None = FP16_TAU_GUARD,
+                  device: int = 0, slack: int = 1 << 20):
+        """A guarded ensemble whose arenas hold exactly `cases` (plus `slack` bytes each)."""
+        need = {"f16": [0, 0, 0], "f32": [0, 0, 0]}
+        for c in cases:
+            st = guarded_storage(c, "f16", tau_guard)
+            pop, ids = Ensemble.bytes_for(q, st, c.shape)
+            need[st][0] += pop
+            need[st][1] += ids
+            need[st][2] += 1
+        return cls(q, kind, max(need["f16"][2], 1), need["f16"][0] + slack, need["f16"][1] + slack, device,
+                   tau_guard=tau_guard, fp32_population_bytes=need["f32"][0] + slack,
+                   fp32_cell_capacity=need["f32"][1] + slack, fp32_max_members=max(need["f32"][2], 1))
+
+    def close(self):
+        for e in self.parts.values():
+            e.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def storage_of(self, member: int) -> str:
+        return self._where[member][0]
+
+    def _part(self, member: int) -> tuple[Ensemble, int]:
+        st, m = self._where[member]
+        return self.parts[st], m
+
+    def add(self, case, steps: int = -1) -> int:
+        st = guarded_storage(case, "f16", self.tau_guard)
+        m = self.parts[st].add(case, steps)
+        gid = self._next
+        self._next += 1
+        self._where[gid] = (st, m)
+        self._count[st] += 1
+        return gid
+
+    def remove(self, member: int):
+        ens, m = self._part(member)
+        ens.remove(m)
+        self._count[self._where.pop(member)[0]] -= 1
+
+    def initialize(self, member: int):
+        ens, m = self._part(member)
+        ens.initialize(m)
+
+    def set_steps(self, member: int, steps: int):
+        ens, m = self._part(member)
+        ens.set_steps(m, steps)
+
+    def status(self, member: int) -> tuple[int, int]:
+        ens, m = self._part(member)
+        return ens.status(m)
+
+    def ini_equilibrium(self, member: int, rho_u: np.ndarray):
+        ens, m = self._part(member)
+        ens.ini_equilibrium(m, rho_u)
+
+    def upload(self, member: int, f: np.ndarray, dtype: str = "f64"):
+        ens, m = self._part(member)
+        ens.upload(m, f, dtype)
+
+    def download(self, member: int, dtype: str = "f64") -> np.ndarray:
+        ens, m = self._part(member)
+        return ens.download(m, dtype)
+
+    def moments(self, member: int) -> np.ndarray:
+        ens, m = self._part(member)
+        return ens.moments(m)
+
+    def checkpoint(self, member: int, real: str = "f64") -> bytes:
+        ens, m = self._part(member)
+        return ens.checkpoint(m, real)
+
+    def restore(self, member: int, data: bytes, real: str = "f64"):
+        ens, m = self._part(member)
+        ens.restore(m, data, real)
+
+    def step(self, steps: int = 1, graph: bool = False, stream: int | None = None):
+        for st, ens in self.parts.items():
+            if self._count[st]:
+                ens.step(steps, graph=graph, stream=stream)
+
+    def autotune(self, trial_steps: int = 5) -> dict:
+        return {st: ens.autotune(trial_steps) for st, ens in self.parts.items() if self._count[st]}
+
+    def usage(self) -> dict:
+        return {st: ens.usage() for st, ens in self.parts.items()}

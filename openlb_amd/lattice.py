@@ -49,6 +49,8 @@ class BlockLattice:
         lat = cls(case.q, case.shape, case.periodic, storage, case.kind, device)
         lat.define_dynamics(case.ids)
         lat.set_parameters(omega=case.omega, magic=case.magic, smago=case.smago, force=case.force)
+        if getattr(case, "force_field", None) is not None:
+            lat.set_force_field(case.force_field)
         local = getattr(case, "local", {}) or {}
         for wid, (rho, u) in case.walls.items():
             if wid not in local:
@@ -115,6 +117,18 @@ class BlockLattice:
             p.force[d] = float(force[d])
         abi.check(abi.load().lbmh_set_bulk_params(self._h, ctypes.byref(p)))
 
+    def set_force_field(self, force):
+        """Per-cell FORCE of the forced bulk kinds, shape (3, nx, ny, nz), or None
+        for the uniform force of set_parameters (the FORCE field ForcedBGKdynamics
+        reads, src/dynamics/forcing.h:37-74)."""
+        if force is None:
+            abi.check(abi.load().lbmh_set_force_field(self._h, None))
+            return
+        a = np.ascontiguousarray(force, dtype=np.float64)
+        if a.shape != (3,) + self.shape:
+            raise ValueError(f"force field shape {a.shape} != {(3,) + self.shape}")
+        abi.check(abi.load().lbmh_set_force_field(self._h, _ptr(a)))
+
     def set_wall(self, wall_id: int, rho: float, u):
         u3 = (ctypes.c_double * 3)(*map(float, u))
         abi.check(abi.load().lbmh_set_wall(self._h, int(wall_id), float(rho), u3))
@@ -180,6 +194,12 @@ class BlockLattice:
         buf = ctypes.create_string_buffer(n.value)
         abi.check(L.lbmh_checkpoint_save(self._h, code, buf, n.value))
         return buf.raw
+
+    def checkpoint_size(self, real: str = "f64") -> int:
+        """Bytes of this block's checkpoint (Serializable::getSerializableSize)."""
+        n = ctypes.c_size_t(0)
+        abi.check(abi.load().lbmh_checkpoint_size(self._h, {"f64": abi.F64, "f32": abi.F32}[real], ctypes.byref(n)))
+        return n.value
 
     def restore(self, data: bytes, real: str = "f64"):
         """Replace the populations and shifts by a checkpoint's (Serializable::

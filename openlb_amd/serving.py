@@ -2,7 +2,9 @@
 
 Requests are independent D3Q19 solves. Each has its own extent, step count,
 driving velocity and storage type (FP32 or FP16). Two ensembles (one per storage
-type) hold the running solves on the device.
+type) hold the running solves on the device. By default the precision guard runs
+an FP16 request whose tau is below ``ensemble.FP16_TAU_GUARD`` in the FP32
+ensemble, where FP16 rounding would exceed the velocity bound.
 
 The loop repeats:
 
@@ -32,7 +34,7 @@ import time
 import numpy as np
 
 from . import abi, workloads
-from .ensemble import Ensemble
+from .ensemble import FP16_TAU_GUARD, Ensemble, guarded_storage
 
 
 @dataclasses.dataclass
@@ -41,7 +43,7 @@ class Request:
     arrival: float          # seconds after the start of the stream
     case: workloads.Case
     steps: int
-    storage: str            # "f32" or "f16"
+    storage: str            # "f32" or "f16"; the precision guard may run an FP16 request in FP32
 
     @property
     def lups(self) -> int:
@@ -60,6 +62,7 @@ class Result:
     moments: np.ndarray | None = None
     populations: np.ndarray | None = None
     error: str | None = None        # set when the request was rejected and never ran
+    promoted: bool = False          # FP16 request run in FP32 storage by the precision guard
 
     @property
     def latency(self) -> float:
@@ -106,8 +109,12 @@ def poisson_channel_stream(count=400, rate=200.0, seed=20261205, size=(24, 128),
 class SolveServer:
     def __init__(self, budget_bytes: int, storages=("f32", "f16"), max_members: int = 1024,
                  tick_steps: int = 64, keep_moments: bool = False, keep_populations: bool = False,
-                 device: int = 0, max_bypass: int = 64):
+                 device: int = 0, max_bypass: int = 64, precision_guard: float | None = FP16_TAU_GUARD):
+        """``precision_guard``: FP16 requests with tau below it run in FP32 storage
+        (``ensemble.guarded_storage``) when the server runs an FP32 ensemble; None
+        runs every request in the storage it asks for."""
         self.tick_steps = int(tick_steps)
+        self.precision_guard = precision_guard
         self.max_bypass = int(max_bypass)
         self.keep_moments = keep_moments
         self.keep_populations = keep_populations
@@ -145,6 +152,15 @@ class SolveServer:
             return f"needs {ids} id bytes, the {req.storage} id arena holds {cell_cap}"
         return None
 
+    def guard(self, req: Request) -> Request:
+        """The request as it runs: an FP16 request below the precision guard's tau
+        goes to the FP32 ensemble."""
+        if req.storage == "f16" and "f32" in self.ens:
+            st = guarded_storage(req.case, "f16", self.precision_guard)
+            if st != req.storage:
+                return dataclasses.replace(req, storage=st)
+        return req
+
     def _try_admit(self, req: Request):
         ens = self.ens[req.storage]
         try:
@@ -169,6 +185,7 @@ class SolveServer:
         head = 0                 # requests before `head` in `queue` have all arrived
         waiting = []             # arrived, not yet admitted: [request, times passed over]
         active = {}              # (storage, member) -> (request, admitted)
+        promoted = set()         # rids the precision guard moved to FP32
         results = []
         t0 = time.perf_counter()
 
@@ -180,6 +197,10 @@ class SolveServer:
             while head < len(queue) and (not realtime or queue[head].arrival <= now):
                 req = queue[head]
                 head += 1
+                run_as = self.guard(req)
+                if run_as is not req:
+                    promoted.add(req.rid)
+                    req = run_as
                 why = self.never_fits(req)
                 if why is None:
                     waiting.append([req, 0])
@@ -220,7 +241,8 @@ class SolveServer:
                         continue
                     if done[m] != req.steps:
                         raise RuntimeError(f"request {req.rid} ran {done[m]} of {req.steps} steps")
-                    res = Result(req.rid, st, req.case.ncore, req.steps, req.arrival, adm, finished)
+                    res = Result(req.rid, st, req.case.ncore, req.steps, req.arrival, adm, finished,
+                                 promoted=req.rid in promoted)
                     if self.keep_moments:
                         res.moments = self.ens[st].moments(m)
                     if self.keep_populations:
@@ -240,4 +262,5 @@ def summarize(results: list[Result], makespan: float | None = None) -> dict:
     return {"requests": len(results), "rejected": rejected, "p50_s": round(float(np.percentile(lat, 50)), 4),
             "p99_s": round(float(np.percentile(lat, 99)), 4), "max_s": round(float(lat.max()), 4),
             "makespan_s": round(span, 3), "aggregate_mlups": round(lups / span / 1e6, 1),
-            "solves_per_s": round(len(results) / span, 2)}
+            "solves_per_s": round(len(results) / span, 2),
+            "promoted_to_f32": sum(r.promoted for r in results)}

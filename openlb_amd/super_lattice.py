@@ -24,7 +24,7 @@ import itertools
 
 import numpy as np
 
-from . import abi
+from . import abi, checkpoint
 from .lattice import BlockLattice
 
 
@@ -171,6 +171,77 @@ class SuperLattice:
     def synchronize(self):
         next(iter(self.blocks.values())).synchronize()
 
+    # -- checkpoints -----------------------------------------------------------
+
+    def checkpoint(self, real: str = "f64") -> bytes:
+        """OpenLB's SuperLattice serialization (Serializable::save(uint8_t*)):
+        every block's serialized bytes in load-balancer order, nothing between
+        them (src/core/superLattice.hh:816-832, serializer.hh:106-110); on one
+        process the load balancer holds the cuboids in their own order.  The
+        overlap fields are those OpenLB's set-up leaves, with the walls of the
+        neighbouring blocks seen through the halos."""
+        return b"".join(lat.checkpoint(real) for lat in self.blocks.values())
+
+    def restore(self, data: bytes, real: str = "f64"):
+        """Serializable::load(const uint8_t*) + postLoad over the blocks.  The
+        split must match the file's (the byte counts of every block); a block
+        that refuses its part leaves every block as it was."""
+        data = bytes(data)
+        sizes = [lat.checkpoint_size(real) for lat in self.blocks.values()]
+        if sum(sizes) != len(data):
+            raise ValueError(f"checkpoint holds {len(data)} bytes, the {len(sizes)} blocks take {sum(sizes)}")
+        before = [lat.checkpoint(real) for lat in self.blocks.values()]
+        done = []
+        off = 0
+        try:
+            for lat, n in zip(self.blocks.values(), sizes):
+                lat.restore(data[off:off + n], real)
+                done.append(lat)
+                off += n
+        except BaseException:
+            for lat, old in zip(done, before):
+                lat.restore(old, real)
+            raise
+
+    def save(self, file_name: str, real: str = "f64", enforce_uint: bool = False) -> str:
+        """Write `<file_name>_rank0_size1.dat` as OpenLB's SuperLattice::save(fileName) does."""
+        path = checkpoint.parallel_file_name(file_name)
+        checkpoint.write_file(path, self.checkpoint(real), enforce_uint)
+        return path
+
+    def load(self, file_name: str, real: str = "f64", enforce_uint: bool = False):
+        self.restore(checkpoint.read_file(checkpoint.parallel_file_name(file_name), enforce_uint), real)
+
+    # -- evaluation ------------------------------------------------------------
+
+    def cuboids(self):
+        """[(block key, origin in cells)] in OpenLB's cuboid order."""
+        return [(key, self.origin[key]) for key in self.blocks]
+
+    def field_export(self, key, field: int, overlap: int, conversion=None) -> np.ndarray:
+        """float32 points of one moment field (lbmh_field_export's fields) over
+        cuboid `key` and `overlap` cells around it, in VTK point order, as
+        OpenLB's SuperVTMwriter3D writes the cuboid's piece: its writer
+        communicates the overlap first, so an overlap cell that copies a bulk
+        cell of another cuboid reports that cell's moments; every other overlap
+        cell follows the block's own rule (periodic images, else rho = 1, u = 0)."""
+        lat = self.blocks[key]
+        out = _export(lat, field, overlap, conversion)
+        if overlap:
+            dim = out.size // int(np.prod([n + 2 * overlap for n in lat.shape]))
+            flat = out.reshape(-1, dim)
+            cache = {}
+            for src_key, (dst, src) in self._pieces()[key].items():
+                if src_key not in cache:
+                    cache[src_key] = _export(self.blocks[src_key], field, 0, conversion).reshape(-1, dim)
+                flat[dst] = cache[src_key][src]
+        return out
+
+    def _pieces(self):
+        if getattr(self, "_piece_map", None) is None:
+            self._piece_map = {key: piece_sources(self.case, self.ranges, key) for key in self.blocks}
+        return self._piece_map
+
     def close(self):
         lib = abi.load()
         for h in self.halos:
@@ -187,10 +258,62 @@ class SuperLattice:
         self.close()
 
 
+def _export(lat, field: int, overlap: int, conversion) -> np.ndarray:
+    dim = 3 if field in (1, 2) else 1
+    points = int(np.prod([n + 2 * overlap for n in lat.shape]))
+    out = np.empty(points * dim, dtype=np.float32)
+    conv = None if conversion is None else np.ascontiguousarray(conversion, dtype=np.float64)
+    abi.check(abi.load().lbmh_field_export(lat.handle, int(field), int(overlap),
+                                           None if conv is None else _ptr(conv), _ptr(out), out.size))
+    return out
+
+
+def piece_sources(case, ranges, key):
+    """The overlap points of cuboid `key`'s VTK piece (overlap 1) that hold a
+    copy of a bulk cell of a cuboid on another axis split, after OpenLB's
+    writer communicates the overlap (src/io/superVtmWriter3D.hh,
+    SuperLatticeF functors on communicated blocks): {source key: (point
+    indices in this piece, point indices in the source cuboid's core export)},
+    both in VTK point order (x fastest).  Overlap cells across a face the
+    block spans periodically by itself are its own images and not listed."""
+    shape = case.shape
+    lo = [ranges[d][key[d]][0] for d in range(3)]
+    n = [ranges[d][key[d]][1] - lo[d] for d in range(3)]
+    self_periodic = [bool(case.periodic[d] and len(ranges[d]) == 1) for d in range(3)]
+    ex, ey = n[0] + 2, n[1] + 2
+    out: dict = {}
+    for x in range(-1, n[0] + 1):
+        for y in range(-1, n[1] + 1):
+            for z in range(-1, n[2] + 1):
+                r = (x, y, z)
+                away = [d for d in range(3) if r[d] < 0 or r[d] >= n[d]]
+                if not away or all(self_periodic[d] for d in away):
+                    continue
+                g = [lo[d] + r[d] for d in range(3)]
+                ok = True
+                for d in range(3):
+                    if not 0 <= g[d] < shape[d]:
+                        if not case.periodic[d]:
+                            ok = False
+                            break
+                        g[d] %= shape[d]
+                if not ok or case.ids[tuple(g)] != 1:
+                    continue
+                src_key = tuple(next(k for k, (a, b) in enumerate(ranges[d]) if a <= g[d] < b) for d in range(3))
+                s = [g[d] - ranges[d][src_key[d]][0] for d in range(3)]
+                sn = [ranges[d][src_key[d]][1] - ranges[d][src_key[d]][0] for d in range(3)]
+                lists = out.setdefault(src_key, ([], []))
+                lists[0].append(((z + 1) * ey + (y + 1)) * ex + (x + 1))
+                lists[1].append((s[2] * sn[1] + s[1]) * sn[0] + s[0])
+    return {k: (np.asarray(a, dtype=np.int64), np.asarray(b, dtype=np.int64)) for k, (a, b) in out.items()}
+
+
 def _sub_case(case, lo, hi, periodic):
     import dataclasses
     sl = tuple(slice(lo[d], hi[d]) for d in range(3))
     ids = np.ascontiguousarray(case.ids[sl])
     rho_u = None if case.rho_u is None else np.ascontiguousarray(case.rho_u[(slice(None),) + sl])
+    ff = getattr(case, "force_field", None)
+    ff = None if ff is None else np.ascontiguousarray(ff[(slice(None),) + sl])
     return dataclasses.replace(case, shape=tuple(hi[d] - lo[d] for d in range(3)), periodic=tuple(periodic),
-                               ids=ids, rho_u=rho_u)
+                               ids=ids, rho_u=rho_u, force_field=ff)

@@ -61,9 +61,16 @@ class _Functor:
     default_name = "density"
 
     def __init__(self, lattice, converter: UnitConverter | None = None, member: int | None = None):
-        """`lattice` is a BlockLattice, or an Ensemble with `member`."""
+        """`lattice` is a BlockLattice, a SuperLattice, or an Ensemble with `member`."""
         self.lattice, self.converter, self.member = lattice, converter, member
         self.name = self.default_name
+
+    def pieces(self):
+        """[(origin in cells, functor of one cuboid)]: one per cuboid of a
+        SuperLattice, in OpenLB's cuboid order; a block is cuboid 0."""
+        if self.member is None and hasattr(getattr(self, "lattice", None), "cuboids"):
+            return [(lo, _Piece(self, key)) for key, lo in self.lattice.cuboids()]
+        return [((0, 0, 0), self)]
 
     def shape(self):
         if self.member is None:
@@ -72,6 +79,8 @@ class _Functor:
 
     def evaluate(self, overlap: int = 1) -> np.ndarray:
         """float32 [points * dim] in VTK point order over the extended extent."""
+        if self.member is None and hasattr(self.lattice, "cuboids"):
+            raise ValueError("a SuperLattice functor is evaluated per cuboid: use pieces()")
         n = self.shape()
         points = int(np.prod([v + 2 * overlap for v in n]))
         out = np.empty(points * self.dim, dtype=np.float32)
@@ -87,6 +96,25 @@ class _Functor:
             abi.check(L.lbmh_ensemble_field_export(self.lattice._h, int(self.member), self.field, int(overlap),
                                                    cptr, optr, out.size))
         return out
+
+
+class _Piece:
+    """A functor restricted to one cuboid of a SuperLattice."""
+
+    def __init__(self, functor: _Functor, key):
+        self.functor, self.key = functor, key
+        self.dim, self.field = functor.dim, functor.field
+
+    @property
+    def name(self):
+        return self.functor.name
+
+    def shape(self):
+        return tuple(self.functor.lattice.blocks[self.key].shape)
+
+    def evaluate(self, overlap: int = 1) -> np.ndarray:
+        conv = None if self.functor.converter is None else self.functor.converter.factors()
+        return self.functor.lattice.field_export(self.key, self.field, overlap, conv)
 
 
 class SuperLatticeDensity3D(_Functor):
@@ -158,11 +186,18 @@ class SuperVTMwriter3D:
     def write(self, it: int, origin=(0.0, 0.0, 0.0), delta: float = 1.0, cuboids=None):
         """One time step: the .pvd entry, the .vtm and a .vti per cuboid.
         `cuboids` lists (functors, origin) per cuboid; by default the added
-        functors form cuboid 0 at `origin`."""
+        functors give one piece per cuboid of their lattice (a SuperLattice's
+        cuboids at `origin` + `delta` x their origin in cells, a block as
+        cuboid 0 at `origin`)."""
         if not self.functors and not cuboids:
             raise RuntimeError("No functor to write")
         if cuboids is None:
-            cuboids = [(self.functors, origin)]
+            pieces = [f.pieces() for f in self.functors]
+            if len({len(p) for p in pieces}) != 1:
+                raise ValueError("the added functors belong to lattices of different cuboids")
+            cuboids = [([p[i][1] for p in pieces],
+                        tuple(origin[d] + delta * pieces[0][i][0][d] for d in range(3)))
+                       for i in range(len(pieces[0]))]
         pvd = self.out_dir + create_file_name(self.name) + ".pvd"
         with open(pvd) as fh:
             text = fh.read()

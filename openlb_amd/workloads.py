@@ -34,6 +34,7 @@ class Case:
     force: tuple[float, float, float] = (0.0, 0.0, 0.0)
     walls: dict = dataclasses.field(default_factory=dict)   # id -> (rho, (ux, uy, uz))
     local: dict = dataclasses.field(default_factory=dict)   # id -> "velocity" | "pressure"
+    force_field: np.ndarray | None = None  # float64 (3, nx, ny, nz): per-cell FORCE of the forced kinds
 
     @property
     def ncore(self) -> int:
@@ -268,6 +269,36 @@ def turbulent_channel(shape=(384, 192, 192), tau=0.505, smago=0.14, u_max=0.05,
     rho_u[1:, ids == 2] = 0.0
     return Case("turbulent_channel", q, "smagorinsky", tuple(shape), (1, 0, 1), ids, rho_u,
                 1.0 / tau, smago=smago)
+
+
+def kolmogorov_flow(shape=(16, 14, 12), amplitude=2e-5, tau=0.62, q=19, kind="forced_bgk", walls=False,
+                    seed=3):
+    """Forced flow with a FORCE field that differs from cell to cell: a
+    Kolmogorov shear force along x varying with y, plus smaller transverse
+    components varying with z and x, F(x, y, z) = A (sin(2 pi (y+1/2)/ny),
+    cos(2 pi (z+1/2)/nz) / 2, sin(2 pi (x+1/2)/nx) / 4).  Fully periodic, or
+    (walls=True) periodic in x and z between BounceBack walls at the y bounds
+    with a few seeded BounceBack obstacles.  Starts at rest."""
+    x, y, z = _grid(shape)
+    nx, ny, nz = shape
+    F = np.empty((3,) + tuple(shape))
+    F[0] = amplitude * np.sin(2 * math.pi * (y + 0.5) / ny)
+    F[1] = 0.5 * amplitude * np.cos(2 * math.pi * (z + 0.5) / nz)
+    F[2] = 0.25 * amplitude * np.sin(2 * math.pi * (x + 0.5) / nx)
+    ids = np.ones(shape, dtype=np.uint8)
+    periodic = (1, 1, 1)
+    if walls:
+        periodic = (1, 0, 1)
+        ids[:, 0, :] = 2
+        ids[:, -1, :] = 2
+        rng = np.random.Generator(np.random.PCG64(seed))
+        for _ in range(3):
+            cx, cy, cz = rng.integers(0, nx), rng.integers(2, ny - 2), rng.integers(0, nz)
+            ids[cx, cy, cz] = 2
+    rho_u = np.zeros((4,) + tuple(shape))
+    rho_u[0] = 1.0
+    name = "kolmogorov_channel" if walls else "kolmogorov_flow"
+    return Case(name, q, kind, tuple(shape), periodic, ids, rho_u, 1.0 / tau, force_field=F)
 
 
 def random_periodic(shape, seed, q=19, kind="bgk", tau=0.7, u_amp=0.04, rho_amp=0.01,

@@ -119,11 +119,17 @@ def cell_roles(case):
     return role, face
 
 
-def fields(case):
+def fields(case, within=None):
     """[(typeid name, components, which, (dir, orient))] in OpenLB's order,
-    POPULATION excluded (it sorts first)."""
+    POPULATION excluded (it sorts first).  `within` = (whole case, origin of
+    this block in it) for one cuboid of a split lattice: OpenLB defines
+    dynamics on the overlap too, so walls of the neighbouring cuboids seen in
+    the overlap allocate their fields in this block as well."""
     role, face = cell_roles(case)
     present = set(np.unique(role))
+    if within is not None:
+        img = _image_roles(case, role, within)
+        present |= set(np.unique(img)) & {"bounce_back", "moving_wall"}
     out = []
     if case.kind in ("forced_bgk", "forced_trt"):
         out.append((_PRE + "11descriptors5FORCEEEE", 3, "force", None))
@@ -142,13 +148,20 @@ def fields(case):
     return out
 
 
-def _image_roles(case, role):
-    """Role of the core cell every padded cell is, or is a periodic image of."""
+def _image_roles(case, role, within=None):
+    """Role of the core cell every padded cell is, or is a periodic image of.
+    With `within` = (whole case, origin of this block in it), of the cell of
+    the whole lattice every padded cell copies (a neighbouring cuboid's cell,
+    or a periodic image of the whole lattice)."""
     P = padded_shape(case)
+    lo = (0, 0, 0)
+    if within is not None:
+        case, lo = within
+        role, _ = cell_roles(case)
     out = np.full(P, "none", dtype=object)
     idx = []
     for a in range(3):
-        g = np.arange(P[a]) - 1
+        g = np.arange(P[a]) - 1 + lo[a]
         ok = (g >= 0) & (g < case.shape[a])
         if case.periodic[a]:
             g = g % case.shape[a]
@@ -165,7 +178,7 @@ def _image_roles(case, role):
     return out
 
 
-def field_values(case, field):
+def field_values(case, field, within=None):
     """float64 [components][padded cells] of one field, as OpenLB's set-up
     leaves it: defineRhoU's values on the core, initialize()'s (RHO = 1) on
     overlap cells that are periodic images of BounceBack / BounceBackVelocity
@@ -185,7 +198,7 @@ def field_values(case, field):
         if which == "force":
             sel = sel & (role == "bulk")
             for d in range(3):
-                vals[d][sel] = case.force[d]
+                vals[d][sel] = case.force[d] if case.force_field is None else case.force_field[d][sel]
         elif which == "wall_velocity":
             sel = sel & (role == "moving_wall")
             for d in range(3):
@@ -204,7 +217,7 @@ def field_values(case, field):
     for d in range(comps):
         out[d][core] = vals[d]
     if which == "rho":
-        img = _image_roles(case, role)
+        img = _image_roles(case, role, within)
         pad = np.ones(P, dtype=bool)
         pad[core] = False
         # defineDynamics covers the overlap; boundary::set skips cells of
@@ -214,21 +227,21 @@ def field_values(case, field):
     return out.reshape(comps, -1)
 
 
-def serializable_size(case, real: str) -> int:
+def serializable_size(case, real: str, within=None) -> int:
     n = int(np.prod(padded_shape(case)))
     t = np.dtype(real_dtype(real)).itemsize
-    cols = sum(f[1] for f in fields(case))
+    cols = sum(f[1] for f in fields(case, within))
     return case.q * (16 + t * n) + cols * (8 + t * n)
 
 
-def unpack(case, raw: bytes, real: str):
+def unpack(case, raw: bytes, real: str, within=None):
     """(signed shifts [q], populations [q][padded cells] in logical cell order,
     {field name: [components][padded cells]}) of a checkpoint."""
     dt = real_dtype(real)
     t = np.dtype(dt).itemsize
     n = int(np.prod(padded_shape(case)))
-    if len(raw) != serializable_size(case, real):
-        raise ValueError(f"checkpoint holds {len(raw)} bytes, expected {serializable_size(case, real)}")
+    if len(raw) != serializable_size(case, real, within):
+        raise ValueError(f"checkpoint holds {len(raw)} bytes, expected {serializable_size(case, real, within)}")
     off = 0
     shifts = np.zeros(case.q, dtype=np.int64)
     pops = np.empty((case.q, n), dtype=dt)
@@ -242,7 +255,7 @@ def unpack(case, raw: bytes, real: str):
         shifts[i] = s
         pops[i] = data[(np.arange(n) + s) % n]
     out = {}
-    for f in fields(case):
+    for f in fields(case, within):
         cols = np.empty((f[1], n), dtype=dt)
         for d in range(f[1]):
             (cnt,) = struct.unpack_from("<Q", raw, off)
@@ -255,7 +268,7 @@ def unpack(case, raw: bytes, real: str):
     return shifts, pops, out
 
 
-def pack(case, pops_padded: np.ndarray, shifts: np.ndarray, real: str) -> bytes:
+def pack(case, pops_padded: np.ndarray, shifts: np.ndarray, real: str, within=None) -> bytes:
     """Checkpoint bytes from populations [q][padded cells] in logical order and
     signed shifts; the other fields from the set-up."""
     dt = real_dtype(real)
@@ -267,8 +280,8 @@ def pack(case, pops_padded: np.ndarray, shifts: np.ndarray, real: str) -> bytes:
         data[(np.arange(n) + s) % n] = np.asarray(pops_padded[i], dtype=dt)
         parts.append(struct.pack("<qQ", s, n))
         parts.append(data.tobytes())
-    for f in fields(case):
-        vals = field_values(case, f)
+    for f in fields(case, within):
+        vals = field_values(case, f, within)
         for d in range(f[1]):
             parts.append(struct.pack("<Q", n))
             parts.append(vals[d].astype(dt).tobytes())

@@ -18,6 +18,7 @@ struct olo_lattice {
   int64_t *halo_dst, *halo_src;
   int kind;
   double omega, magic, smago, force[3];
+  double* force_field;  /* per padded cell, 3 values (NULL: the uniform force) */
   double wall_rho[256], wall_u[256][3];
   /* local boundaries, per padded cell: kind (OLK_*; 0 = none), discrete
    * normal or (plane, n1, n2), and the defined rho / u */
@@ -126,6 +127,14 @@ static uint16_t float_to_half(float f)
   const uint32_t rest = m & 0x1fffu;
   if (rest > 0x1000u || (rest == 0x1000u && (h & 1u))) h += 1; /* may carry into exponent */
   return sign | (uint16_t)h;
+}
+
+/* FORCE field of the cell at padded index n: its own value where a per-cell
+ * field is set (ForcedBGKdynamics reads the cell's FORCE, momenta/elements.h:
+ * 1480-1488, forcing.h:37-74), else the lattice's uniform force */
+static const double* cell_force(const olo_lattice* L, int64_t n)
+{
+  return L->force_field ? L->force_field + 3 * n : L->force;
 }
 
 /* ---- instantiations */
@@ -441,6 +450,7 @@ void olo_destroy(olo_lattice* L)
 {
   if (!L) return;
   free(L->pop);
+  free(L->force_field);
   free(L->ids);
   free(L->halo_dst);
   free(L->halo_src);
@@ -473,6 +483,25 @@ int olo_set_bulk(olo_lattice* L, int kind, double omega, double magic, double sm
   L->magic = magic;
   L->smago = smago;
   L->force[0] = fx; L->force[1] = fy; L->force[2] = fz;
+  return 0;
+}
+
+int olo_set_force_field(olo_lattice* L, const double* force)
+{
+  free(L->force_field);
+  L->force_field = NULL;
+  if (!force) return 0;
+  L->force_field = (double*)calloc((size_t)L->ncells * 3, sizeof(double));
+  if (!L->force_field) { set_err("olo_set_force_field: out of memory"); return -1; }
+  const int p = L->pad;
+  const int64_t nc = L->ncore;
+  for (int x = 0; x < L->n[0]; ++x)
+    for (int y = 0; y < L->n[1]; ++y)
+      for (int z = 0; z < L->n[2]; ++z) {
+        const int64_t c = ((int64_t)x * L->n[1] + y) * L->n[2] + z;
+        const int64_t n = (int64_t)(x + p) * L->proj[0] + (int64_t)(y + p) * L->proj[1] + z + p;
+        for (int d = 0; d < 3; ++d) L->force_field[3 * n + d] = force[(int64_t)d * nc + c];
+      }
   return 0;
 }
 
@@ -567,6 +596,12 @@ int olo_moments(const olo_lattice* L, double* rho_u)
 int olo_equilibrium(int q, int kind, int storage, int64_t n, const double* rho_u,
                     double fx, double fy, double fz, double* f)
 {
+  return olo_equilibrium_field(q, kind, storage, n, rho_u, fx, fy, fz, NULL, f);
+}
+
+int olo_equilibrium_field(int q, int kind, int storage, int64_t n, const double* rho_u,
+                          double fx, double fy, double fz, const double* force, double* f)
+{
   olo_lattice L;
   memset(&L, 0, sizeof L);
   if (q != 19 && q != 27) { set_err("olo_equilibrium: q"); return -1; }
@@ -575,9 +610,9 @@ int olo_equilibrium(int q, int kind, int storage, int64_t n, const double* rho_u
   L.force[0] = fx; L.force[1] = fy; L.force[2] = fz;
   fill_descriptor(&L);
   switch (storage) {
-  case OLO_F64: equilibrium_f64(&L, n, rho_u, f); break;
-  case OLO_F32: equilibrium_f32(&L, n, rho_u, f); break;
-  default: equilibrium_f16(&L, n, rho_u, f); break;
+  case OLO_F64: equilibrium_f64(&L, n, rho_u, force, f); break;
+  case OLO_F32: equilibrium_f32(&L, n, rho_u, force, f); break;
+  default: equilibrium_f16(&L, n, rho_u, force, f); break;
   }
   return 0;
 }

@@ -65,6 +65,10 @@ void olo_destroy(olo_lattice* L);
 int olo_set_ids(olo_lattice* L, const uint8_t* ids);
 int olo_set_bulk(olo_lattice* L, int kind, double omega, double magic, double smago,
                  double fx, double fy, double fz);
+/* per-cell FORCE field of the forced kinds, [3][nx][ny][nz] (NULL: back to
+ * the uniform force of olo_set_bulk); ForcedBGKdynamics / ForcedTRTdynamics
+ * read each cell's own FORCE (momenta/elements.h:1480-1488, forcing.h:37-74) */
+int olo_set_force_field(olo_lattice* L, const double* force);
 int olo_set_wall(olo_lattice* L, int id, double rho, double ux, double uy, double uz);
 /* populations of the core cells, shifted values, layout [q][nx][ny][nz] */
 int olo_upload(olo_lattice* L, const double* f);
@@ -81,6 +85,9 @@ int olo_field_export(const olo_lattice* L, int field, int overlap, const double*
  * semantics (forced kinds inverse-shift u by F/2), layout [q][n] */
 int olo_equilibrium(int q, int kind, int storage, int64_t n, const double* rho_u,
                     double fx, double fy, double fz, double* f);
+/* the same with a per-cell FORCE field force [3][n] (NULL: fx, fy, fz) */
+int olo_equilibrium_field(int q, int kind, int storage, int64_t n, const double* rho_u,
+                          double fx, double fy, double fz, const double* force, double* f);
 /* Local boundaries with BGK bulk dynamics (boundary::set<LocalVelocity> /
  * <LocalPressure>): every core cell of `material` in the material map
  * ([nx][ny][nz]; 1 fluid, 0 outside, anything else wall) is classified as

@@ -18,8 +18,9 @@ static inline V FN(tw)(const olo_lattice* L, int i)
 }
 
 /* Bulk momenta: rho = sum f + 1, j = sum c f, u = j / rho        lbm.h:266-307
- * Forced momenta add F/2 to u                           momenta/elements.h:1480-1488 */
-static void FN(rho_u)(const olo_lattice* L, const V* f, int forced, V* rho, V* u)
+ * Forced momenta add F/2 to u, F the cell's FORCE field   momenta/elements.h:1480-1488
+ * (F == NULL: plain momenta) */
+static void FN(rho_u)(const olo_lattice* L, const V* f, const double* F, V* rho, V* u)
 {
   V r = 0;
   for (int i = 0; i < L->q; ++i) r += f[i];
@@ -28,8 +29,8 @@ static void FN(rho_u)(const olo_lattice* L, const V* f, int forced, V* rho, V* u
   for (int i = 0; i < L->q; ++i)
     for (int d = 0; d < 3; ++d) j[d] += f[i] * (V)L->c[i][d];
   for (int d = 0; d < 3; ++d) u[d] = j[d] / r;
-  if (forced)
-    for (int d = 0; d < 3; ++d) u[d] += (V)L->force[d] * (V)0.5;
+  if (F)
+    for (int d = 0; d < 3; ++d) u[d] += (V)F[d] * (V)0.5;
   *rho = r;
 }
 
@@ -82,7 +83,7 @@ static void FN(trt)(const olo_lattice* L, V* f, V rho, const V* u, V omega, V ma
 }
 
 /* lbm<D>::addExternalForce (Guo source)                           lbm.h:545-566 */
-static void FN(guo)(const olo_lattice* L, V* f, V rho, const V* u, V omega)
+static void FN(guo)(const olo_lattice* L, V* f, V rho, const V* u, V omega, const double* F)
 {
   const V inv = (V)3;
   for (int i = 0; i < L->q; ++i) {
@@ -91,7 +92,7 @@ static void FN(guo)(const olo_lattice* L, V* f, V rho, const V* u, V omega)
     cu *= inv * inv;
     V term = 0;
     for (int d = 0; d < 3; ++d)
-      term += (((V)L->c[i][d] - u[d]) * inv + cu * (V)L->c[i][d]) * (V)L->force[d];
+      term += (((V)L->c[i][d] - u[d]) * inv + cu * (V)L->c[i][d]) * (V)F[d];
     term *= FN(tw)(L, i);
     term *= (V)1 - omega * (V)0.5;
     term *= rho;
@@ -338,33 +339,34 @@ static V FN(collide_local)(const olo_lattice* L, V* f, int64_t n, V* usq_out)
   return rho;
 }
 
-static void FN(collide_cell)(const olo_lattice* L, V* f, int id)
+static void FN(collide_cell)(const olo_lattice* L, V* f, int id, int64_t n)
 {
+  const double* F = cell_force(L, n);
   if (id == 2) { FN(revert)(L, f); return; }
   if (id >= 3) { FN(moving_wall)(L, f, id); return; }
   const V omega = (V)L->omega;
   V rho, u[3];
   switch (L->kind) {
   case OLO_BGK:
-    FN(rho_u)(L, f, 0, &rho, u);
+    FN(rho_u)(L, f, NULL, &rho, u);
     FN(bgk)(L, f, rho, u, omega);
     break;
   case OLO_FORCED_BGK:
-    FN(rho_u)(L, f, 1, &rho, u);
+    FN(rho_u)(L, f, F, &rho, u);
     FN(bgk)(L, f, rho, u, omega);
-    FN(guo)(L, f, rho, u, omega);
+    FN(guo)(L, f, rho, u, omega, F);
     break;
   case OLO_TRT:
-    FN(rho_u)(L, f, 0, &rho, u);
+    FN(rho_u)(L, f, NULL, &rho, u);
     FN(trt)(L, f, rho, u, omega, (V)L->magic);
     break;
   case OLO_FORCED_TRT:
-    FN(rho_u)(L, f, 1, &rho, u);
+    FN(rho_u)(L, f, F, &rho, u);
     FN(trt)(L, f, rho, u, omega, (V)L->magic);
-    FN(guo)(L, f, rho, u, omega);
+    FN(guo)(L, f, rho, u, omega, F);
     break;
   case OLO_SMAGORINSKY: {
-    FN(rho_u)(L, f, 0, &rho, u);
+    FN(rho_u)(L, f, NULL, &rho, u);
     const V w = FN(smago_omega)(L, f, rho, u, omega, (V)L->smago);
     FN(bgk)(L, f, rho, u, w);
     break;
@@ -422,7 +424,7 @@ static void FN(cell_rho_u)(const olo_lattice* L, int64_t n, V* rho, V* u)
   if (id == 1) {
     V f[27];
     FN(load_cell)(L, n, f);
-    FN(rho_u)(L, f, L->kind == OLO_FORCED_BGK || L->kind == OLO_FORCED_TRT, rho, u);
+    FN(rho_u)(L, f, (L->kind == OLO_FORCED_BGK || L->kind == OLO_FORCED_TRT) ? cell_force(L, n) : NULL, rho, u);
     return;
   }
   u[0] = u[1] = u[2] = 0;
@@ -647,7 +649,7 @@ static void FN(statistics)(const olo_lattice* L, double* out)
           rho = FN(collide_local)(L, f, n, &usq);
           if (rho < 0) continue;
         } else if (id == 1) {
-          FN(rho_u)(L, f, forced, &rho, u);
+          FN(rho_u)(L, f, forced ? cell_force(L, n) : NULL, &rho, u);
           usq = FN(usqr)(u);
         } else {
           rho = (V)L->wall_rho[id];
@@ -689,7 +691,7 @@ static void FN(step)(olo_lattice* L, int threads)
           V usq;
           (void)FN(collide_local)(L, f, n, &usq);
         } else {
-          FN(collide_cell)(L, f, id);
+          FN(collide_cell)(L, f, id, n);
         }
         for (int i = 0; i < q; ++i) pop[idx[i]] = STOREV(f[i]);
       }
@@ -766,13 +768,14 @@ static void FN(moments)(const olo_lattice* L, double* out)
           if (k >= N) k -= N;
           f[i] = LOAD(pop[(int64_t)i * N + k]);
         }
-        FN(rho_u)(L, f, forced && L->ids[n] == 1, &rho, u);
+        FN(rho_u)(L, f, (forced && L->ids[n] == 1) ? cell_force(L, n) : NULL, &rho, u);
         out[c] = (double)rho;
         for (int d = 0; d < 3; ++d) out[(int64_t)(d + 1) * nc + c] = (double)u[d];
       }
 }
 
-static void FN(equilibrium)(olo_lattice* L, int64_t n, const double* rho_u, double* f)
+/* force: NULL (the lattice's uniform FORCE) or [3][n] per cell */
+static void FN(equilibrium)(olo_lattice* L, int64_t n, const double* rho_u, const double* force, double* f)
 {
   const int forced = (L->kind == OLO_FORCED_BGK || L->kind == OLO_FORCED_TRT);
   for (int64_t c = 0; c < n; ++c) {
@@ -781,7 +784,8 @@ static void FN(equilibrium)(olo_lattice* L, int64_t n, const double* rho_u, doub
     for (int d = 0; d < 3; ++d) {
       u[d] = (V)rho_u[(int64_t)(d + 1) * n + c];
       /* ForcedMomentum::inverseShift, dynamics/interface.h:111-121 */
-      if (forced) u[d] -= (V)L->force[d] * (V)0.5;
+      const double Fd = force ? force[(int64_t)d * n + c] : L->force[d];
+      if (forced) u[d] -= (V)Fd * (V)0.5;
     }
     const V usq = FN(usqr)(u);
     for (int i = 0; i < L->q; ++i) {

@@ -50,6 +50,8 @@ def lib():
         L.olo_moments.argtypes = [vp, vp]
         L.olo_field_export.argtypes = [vp, i32, i32, vp, vp]
         L.olo_equilibrium.argtypes = [i32, i32, i32, i64, vp, dbl, dbl, dbl, vp]
+        L.olo_equilibrium_field.argtypes = [i32, i32, i32, i64, vp, dbl, dbl, dbl, vp, vp]
+        L.olo_set_force_field.argtypes = [vp, vp]
         L.olo_round_to_storage.argtypes = [i32, i64, vp]
         L.olo_define_local.argtypes = [vp, vp, i32, i32, dbl, dbl, dbl, dbl]
         L.olo_classify.argtypes = [i32] * 6 + [vp, vp]
@@ -82,6 +84,13 @@ class OracleLattice:
         if L.olo_set_bulk(h, KINDS[case.kind], case.omega, case.magic, case.smago,
                           *case.force) != 0:
             raise RuntimeError(L.olo_last_error().decode())
+        field = getattr(case, "force_field", None)
+        if field is not None:
+            self._force = np.ascontiguousarray(field, dtype=np.float64)
+            if self._force.shape != (3,) + self.shape:
+                raise ValueError("force_field must be [3][nx][ny][nz]")
+            if L.olo_set_force_field(h, _ptr(self._force)) != 0:
+                raise RuntimeError(L.olo_last_error().decode())
         local = getattr(case, "local", {}) or {}
         for wid, (rho, u) in case.walls.items():
             if wid in local:
@@ -165,8 +174,14 @@ def equilibrium(case, storage: str = "f64", rho_u: np.ndarray | None = None) -> 
     rho_u = np.ascontiguousarray(rho_u, dtype=np.float64)
     n = int(np.prod(rho_u.shape[1:]))
     out = np.empty((case.q,) + rho_u.shape[1:])
-    lib().olo_equilibrium(case.q, KINDS[case.kind], STORAGE[storage], n, _ptr(rho_u),
-                          *case.force, _ptr(out))
+    field = getattr(case, "force_field", None)
+    if field is not None:
+        field = np.ascontiguousarray(field, dtype=np.float64)
+        lib().olo_equilibrium_field(case.q, KINDS[case.kind], STORAGE[storage], n, _ptr(rho_u),
+                                    *case.force, _ptr(field), _ptr(out))
+    else:
+        lib().olo_equilibrium(case.q, KINDS[case.kind], STORAGE[storage], n, _ptr(rho_u),
+                              *case.force, _ptr(out))
     return out
 
 

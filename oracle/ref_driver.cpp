@@ -15,6 +15,8 @@
 //   ids.bin   uint8  [nx][ny][nz]  per-cell dynamics id
 //             (0 none, 1 bulk, 2 BounceBack, 3.. BounceBackVelocity walls)
 //   rhou.bin  double [4][nx][ny][nz] initial rho, ux, uy, uz
+//   force.bin (optional) double [3][nx][ny][nz] per-cell FORCE of the bulk
+//             cells, in place of the uniform fx fy fz
 // Output (same directory):
 //   pops_<step>.bin  double [q][nx][ny][nz] shifted populations of the core
 //                    cells after <step> collideAndStream() calls (0 = after
@@ -86,6 +88,7 @@ struct CaseSpec {
   int split = 0;   // splitRegular width; 0 keeps one cuboid
   std::vector<unsigned char> ids;
   std::vector<double> rhou;
+  std::vector<double> forceField;  // [3][nCore], empty: the uniform force
 };
 
 static std::size_t coreIndex(const CaseSpec& c, int x, int y, int z)
@@ -140,6 +143,12 @@ static CaseSpec readCase(const std::string& dir)
   f = std::fopen((dir + "/rhou.bin").c_str(), "rb");
   if (!f || std::fread(c.rhou.data(), sizeof(double), 4 * nCore, f) != 4 * nCore) throw std::runtime_error("rhou.bin");
   std::fclose(f);
+  f = std::fopen((dir + "/force.bin").c_str(), "rb");
+  if (f) {
+    c.forceField.resize(3 * nCore);
+    if (std::fread(c.forceField.data(), sizeof(double), 3 * nCore, f) != 3 * nCore) throw std::runtime_error("force.bin");
+    std::fclose(f);
+  }
   return c;
 }
 
@@ -288,8 +297,13 @@ static void runCase(const CaseSpec& c, const std::string& dir)
     auto cell = lattice.getBlock(iC).get(local);
     if constexpr (FORCED) {
       if (id == 1) {
-        cell.template setField<descriptors::FORCE>(
-          Vector<T,3>{T(c.force[0]), T(c.force[1]), T(c.force[2])});
+        if (c.forceField.empty()) {
+          cell.template setField<descriptors::FORCE>(
+            Vector<T,3>{T(c.force[0]), T(c.force[1]), T(c.force[2])});
+        } else {
+          cell.template setField<descriptors::FORCE>(
+            Vector<T,3>{T(c.forceField[k]), T(c.forceField[nCore + k]), T(c.forceField[2 * nCore + k])});
+        }
       }
     }
     cell.defineRhoU(rho, u);
@@ -390,6 +404,7 @@ int main(int argc, char** argv)
     if (c.dyn == "bgk") runCase<D27, BGKdynamics<T,D27>, false>(c, dir);
     else if (c.dyn == "forced_bgk") runCase<D27F, ForcedBGKdynamics<T,D27F>, true>(c, dir);
     else if (c.dyn == "trt") runCase<D27, TRTdynamics<T,D27>, false>(c, dir);
+    else if (c.dyn == "forced_trt") runCase<D27F, ForcedTRTdynamics<T,D27F>, true>(c, dir);
     else if (c.dyn == "smagorinsky") runCase<D27, SmagorinskyBGKdynamics<T,D27>, false>(c, dir);
     else throw std::runtime_error("unknown dyn " + c.dyn);
   } else {

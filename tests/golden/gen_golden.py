@@ -88,6 +88,14 @@ def cases():
     c = W.velocity_channel((24, 12, 10), u_in=0.03, rho_out=1.0, tau=0.6, inlet="interp_velocity",
                            outlet="interp_pressure")
     out.append(("channel19_interp_velocity_pressure_f64", c, [0, 200], "f64"))
+    # a FORCE field that differs from cell to cell (Kolmogorov shear plus
+    # transverse components), periodic and between walls with obstacles
+    c = W.kolmogorov_flow((16, 14, 12), amplitude=2e-5, tau=0.62)
+    out.append(("kolmogorov19_forced_bgk_f64", c, [0, 200], "f64"))
+    out.append(("kolmogorov19_forced_bgk_f32", c, [0, 100], "f32"))
+    c = W.kolmogorov_flow((10, 12, 8), amplitude=3e-5, tau=0.7, q=27, kind="forced_trt", walls=True)
+    c.magic = 3.0 / 16.0
+    out.append(("kolmogorov27_forced_trt_f64", c, [0, 100], "f64"))
     return out
 
 
@@ -124,6 +132,8 @@ def write_normals():
 def write_case_dir(d, case, steps, split=0):
     case.ids.astype(np.uint8).tofile(os.path.join(d, "ids.bin"))
     np.ascontiguousarray(case.rho_u, dtype=np.float64).tofile(os.path.join(d, "rhou.bin"))
+    if case.force_field is not None:
+        np.ascontiguousarray(case.force_field, dtype=np.float64).tofile(os.path.join(d, "force.bin"))
     lines = [f"q {case.q}", f"dyn {case.kind}"]
     lines += [f"n{a} {n}" for a, n in zip("xyz", case.shape)]
     lines += [f"p{a} {p}" for a, p in zip("xyz", case.periodic)]
@@ -197,13 +207,23 @@ def write_sampled(manifest):
         print(f"{name}: shape {case.shape} steps {steps} (sampled)", flush=True)
 
 
-def main():
+def main(prefix=None):
+    """All population fixtures, or (prefix) those whose name starts with it,
+    merged into the existing manifest."""
     manifest = {}
-    write_sampled(manifest)
+    if prefix is None:
+        write_sampled(manifest)
+    else:
+        with open(os.path.join(HERE, "manifest.json")) as fh:
+            manifest = json.load(fh)
     for name, case, steps, precision in cases():
+        if prefix is not None and not name.startswith(prefix):
+            continue
         snaps = run_reference(case, steps, precision)
         store = np.float64 if precision == "f64" else np.float32
         arrays = {"ids": case.ids.astype(np.uint8), "rho_u": case.rho_u}
+        if case.force_field is not None:
+            arrays["force_field"] = np.ascontiguousarray(case.force_field, dtype=np.float64)
         for s, a in zip(steps, snaps):
             arrays[f"f_{s}"] = a.astype(store)
         meta = meta_of(case, steps, precision)
@@ -233,10 +253,12 @@ def checkpoint_cases():
         ("cavity27_local_velocity_f64", W.local_velocity_cavity((8, 7, 6), tau=0.7, lid_u=0.1, q=27), 5, 25, "f64"),
         ("channel19_velocity_pressure_zperiodic_f64",
          W.velocity_channel((10, 7, 6), u_in=0.03, rho_out=1.0, tau=0.6, periodic_z=True), 5, 25, "f64"),
+        ("kolmogorov19_forced_bgk_f64", W.kolmogorov_flow((8, 9, 6), amplitude=2e-5, tau=0.7, walls=True, seed=3),
+         5, 25, "f64"),
     ]
 
 
-def write_checkpoints():
+def write_checkpoints(only=None):
     """checkpoints/<name>.npz: OpenLB's own checkpoint file of the lattice after
     k steps (Serializable::save, the .dat text as bytes), its populations
     after k and m steps, and the files its SuperVTMwriter3D writes for the
@@ -246,7 +268,12 @@ def write_checkpoints():
     out_dir = os.path.join(HERE, "checkpoints")
     os.makedirs(out_dir, exist_ok=True)
     manifest = {}
+    if only is not None:  # rewrite one fixture, keep the others' files and manifest entries
+        with open(os.path.join(out_dir, "manifest.json")) as fh:
+            manifest = json.load(fh)
     for name, case, k, m, precision in checkpoint_cases():
+        if only is not None and name != only:
+            continue
         exe = os.path.join(ROOT, "oracle", "_ref", f"olb_ref_{precision}")
         with tempfile.TemporaryDirectory() as d:
             write_case_dir(d, case, [k, m])
@@ -270,6 +297,8 @@ def write_checkpoints():
         meta["checkpoint_step"] = k
         arrays = {"ids": case.ids.astype(np.uint8), "rho_u": case.rho_u, "dat": dat,
                   f"f_{k}": pops[k], f"f_{m}": pops[m], **vtk}
+        if case.force_field is not None:
+            arrays["force_field"] = np.ascontiguousarray(case.force_field, dtype=np.float64)
         np.savez_compressed(os.path.join(out_dir, name + ".npz"), meta=np.array(json.dumps(meta)), **arrays)
         meta["sha256"] = {a: hashlib.sha256(np.ascontiguousarray(v).tobytes()).hexdigest()
                           for a, v in arrays.items()}
@@ -401,11 +430,15 @@ def write_multiblock():
 
 
 if __name__ == "__main__":
+    if "--only" in sys.argv:
+        main(sys.argv[sys.argv.index("--only") + 1])
+        sys.exit(0)
     if "--multiblock-only" in sys.argv:
         write_multiblock()
         sys.exit(0)
     if "--checkpoints-only" in sys.argv:
-        write_checkpoints()
+        at = sys.argv.index("--checkpoints-only") + 1
+        write_checkpoints(sys.argv[at] if at < len(sys.argv) else None)
         sys.exit(0)
     if "--uq-only" in sys.argv:
         write_uq()

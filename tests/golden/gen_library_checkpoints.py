@@ -2,7 +2,7 @@
 
 Run on a machine with the GPU and the built library (no reference needed):
 
-    python tests/golden/gen_library_checkpoints.py <output dir>
+    python tests/golden/gen_library_checkpoints.py <output dir> [<split-lattice output dir>]
 
 For every fixture in tests/golden/checkpoints/, the library restores OpenLB's
 checkpoint after k steps, advances `EXTRA` steps on the device, and writes its
@@ -31,8 +31,10 @@ from openlb_amd.lattice import BlockLattice  # noqa: E402
 EXTRA = 8
 
 
-def main(out_dir: str):
+def main(out_dir: str, multiblock_dir: str | None = None):
     os.makedirs(out_dir, exist_ok=True)
+    if multiblock_dir is not None:
+        write_multiblock(multiblock_dir)
     for name in G.checkpoint_names():
         case, meta, _, text = G.load_checkpoint(name)
         k, m = meta["steps"]
@@ -49,5 +51,29 @@ def main(out_dir: str):
         print(f"library checkpoint {name}: step {k + EXTRA}", flush=True)
 
 
+def write_multiblock(out_dir: str):
+    """The same for the split-lattice fixtures of tests/golden/multiblock/ that
+    hold OpenLB's checkpoint: a SuperLattice on OpenLB's cuboids restores it,
+    advances EXTRA steps and writes its own (every block, in cuboid order)."""
+    from openlb_amd.super_lattice import SuperLattice
+    os.makedirs(out_dir, exist_ok=True)
+    for name in G.multiblock_names():
+        case, meta, arrays = G.load_multiblock(name)
+        k, real = meta["checkpoint_step"], meta["precision"]
+        if k is None:
+            continue
+        with SuperLattice(case, real, tuple(meta["blocks"])) as sup:
+            sup.restore(CK.decode_dat(arrays["dat"].tobytes().decode("ascii")), real)
+            sup.collide_and_stream(EXTRA)
+            dat = CK.encode_dat(sup.checkpoint(real))
+        info = {"fixture": name, "step": k + EXTRA, "precision": real, "blocks": meta["blocks"],
+                "source": "SuperLattice.checkpoint after restoring OpenLB's checkpoint and "
+                          f"{EXTRA} device steps"}
+        np.savez_compressed(os.path.join(out_dir, name + ".npz"), meta=np.array(json.dumps(info)),
+                            dat=np.frombuffer(dat.encode("ascii"), dtype=np.uint8))
+        print(f"library split checkpoint {name}: step {k + EXTRA}", flush=True)
+
+
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else os.path.join(HERE, "checkpoints", "library"))
+    main(sys.argv[1] if len(sys.argv) > 1 else os.path.join(HERE, "checkpoints", "library"),
+         sys.argv[2] if len(sys.argv) > 2 else os.path.join(HERE, "multiblock", "library"))

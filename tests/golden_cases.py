@@ -44,7 +44,7 @@ def load(name: str):
     local = {int(k): v for k, v in meta.get("local", {}).items()}
     case = W.Case(name, meta["q"], meta["kind"], tuple(meta["shape"]), tuple(meta["periodic"]),
                   arrays["ids"], arrays["rho_u"], meta["omega"], meta["magic"], meta["smago"],
-                  tuple(meta["force"]), walls, local)
+                  tuple(meta["force"]), walls, local, arrays.get("force_field"))
     return case, meta, arrays
 
 
@@ -121,7 +121,7 @@ def load_checkpoint(name: str):
     local = {int(k): v for k, v in meta.get("local", {}).items()}
     case = W.Case(name, meta["q"], meta["kind"], tuple(meta["shape"]), tuple(meta["periodic"]),
                   arrays["ids"], arrays["rho_u"], meta["omega"], meta["magic"], meta["smago"],
-                  tuple(meta["force"]), walls, local)
+                  tuple(meta["force"]), walls, local, arrays.get("force_field"))
     return case, meta, arrays, z["dat"].tobytes().decode("ascii")
 
 
@@ -150,5 +150,5 @@ def load_multiblock(name: str):
     local = {int(k): v for k, v in meta.get("local", {}).items()}
     case = W.Case(name, meta["q"], meta["kind"], tuple(meta["shape"]), tuple(meta["periodic"]),
                   arrays["ids"], arrays["rho_u"], meta["omega"], meta["magic"], meta["smago"],
-                  tuple(meta["force"]), walls, local)
+                  tuple(meta["force"]), walls, local, arrays.get("force_field"))
     return case, meta, arrays

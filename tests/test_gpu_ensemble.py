@@ -246,3 +246,29 @@ def test_autotune_leaves_the_members_as_they_were(gpu_available):
             finals.append([ens.download(m) for m in members])
     for a, b in zip(*finals):
         np.testing.assert_array_equal(a, b)
+
+
+def test_guarded_ensemble_members_match_plain_ensembles(gpu_available):
+    """The precision guard routes members with tau < FP16_TAU_GUARD to FP32
+    storage; every member is bit-identical to a plain ensemble of the storage
+    it runs in."""
+    from openlb_amd.ensemble import FP16_TAU_GUARD, GuardedEnsemble
+    cases = small_members(seed=5, count=6)
+    cases[1] = W.lid_driven_cavity((7, 9, 6), tau=0.515, lid_u=0.08)
+    cases[4] = W.lid_driven_cavity((5, 8, 11), tau=0.525, lid_u=0.05)
+    with GuardedEnsemble.for_cases(cases) as ens:
+        members = [ens.add(c, steps=19) for c in cases]
+        for m, c in zip(members, cases):
+            ens.ini_equilibrium(m, c.rho_u)
+        ens.step(25, graph=True)
+        got = {m: ens.download(m) for m in members}
+        where = {m: ens.storage_of(m) for m in members}
+        assert all(ens.status(m) == (19, 0) for m in members)
+    for m, c in zip(members, cases):
+        want_st = "f32" if 1.0 / c.omega < FP16_TAU_GUARD else "f16"
+        assert where[m] == want_st
+        with Ensemble(19, want_st, "bgk", max_members=1, population_bytes=16 << 20, cell_capacity=1 << 20) as one:
+            k = one.add(c, steps=19)
+            one.ini_equilibrium(k, c.rho_u)
+            one.step(25)
+            np.testing.assert_array_equal(got[m], one.download(k))

@@ -18,7 +18,9 @@ import pytest
 
 import golden_cases as G
 from oracle import oracle as O
+from openlb_amd import abi
 from openlb_amd import workloads as W
+from openlb_amd.ensemble import Ensemble
 from openlb_amd.lattice import BlockLattice
 
 pytestmark = pytest.mark.gpu
@@ -222,6 +224,59 @@ def test_ini_equilibrium_and_moments(gpu_available, storage):
     assert abs(red[0] - rho_sum) <= rtol * rho_sum
     np.testing.assert_allclose(red[1:], j.sum(axis=1), rtol=1e-6,
                                atol=1e-12 if storage == "f64" else 1e-5)
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_force_field_equilibrium_and_moments(gpu_available, storage):
+    """A FORCE field that differs from cell to cell: equilibrium initialisation
+    (u - F/2 per cell), the moments (+F/2 per cell) and their reduction against
+    the oracle's; clearing the field returns to the uniform force."""
+    case = W.kolmogorov_flow((12, 10, 8), amplitude=3e-4, tau=0.7, walls=True)
+    rng = np.random.Generator(np.random.PCG64(4))
+    case.rho_u[0] = 1.0 + 0.01 * rng.standard_normal(case.shape)
+    case.rho_u[1:] = 0.03 * rng.standard_normal((3,) + case.shape)
+    fluid = case.ids == 1
+    with BlockLattice.from_case(case, storage) as lat:
+        lat.ini_equilibrium(case.rho_u)
+        f = lat.download_populations("f64")
+        mom = lat.moments()
+        red = lat.reduce_moments()
+        lat.set_force_field(None)
+        mom0 = lat.moments()
+    tol = {"f64": 1e-15, "f32": 3e-7}[storage]
+    assert G.rel_error(f, O.initial_populations(case, storage), 19, case.ids != 0) <= tol
+    ora = O.OracleLattice(case, storage)
+    ora.upload(f)
+    om = ora.moments()
+    mtol = {"f64": 1e-14, "f32": 1e-6}[storage]
+    np.testing.assert_allclose(mom[:, fluid], om[:, fluid], rtol=0, atol=mtol)
+    # the requested velocity comes back on every bulk cell, whatever its force
+    np.testing.assert_allclose(mom[1:, fluid], case.rho_u[1:, fluid], rtol=0, atol=mtol * 10)
+    j = (om[1:, fluid] - 0.5 * case.force_field[:, fluid]) * om[0][fluid]
+    np.testing.assert_allclose(red[1:], j.sum(axis=1), rtol=1e-6, atol=1e-12 if storage == "f64" else 1e-5)
+    # without the field the force is the uniform one (zero here): u = j / rho
+    np.testing.assert_allclose(mom0[1:, fluid], mom[1:, fluid] - 0.5 * case.force_field[:, fluid],
+                               rtol=0, atol=mtol * 10)
+
+
+def test_force_field_is_refused_where_it_does_not_apply(gpu_available):
+    case = W.kolmogorov_flow((6, 5, 4))
+    with BlockLattice.from_case(case, "f64") as lat:
+        with pytest.raises(ValueError):
+            lat.set_force_field(np.zeros((3, 6, 5, 3)))
+        bad = case.force_field.copy()
+        bad[1, 2, 3, 1] = np.nan
+        with pytest.raises(abi.LbmhError):
+            lat.set_force_field(bad)
+    with BlockLattice.from_case(dataclasses.replace(case, kind="bgk", force_field=None), "f64") as lat:
+        with pytest.raises(abi.LbmhError):
+            lat.set_force_field(case.force_field)
+    ens = Ensemble(19, "f64", kind="forced_bgk", max_members=1, population_bytes=1 << 20, cell_capacity=1 << 12)
+    try:
+        with pytest.raises(ValueError):
+            ens.add(case)
+    finally:
+        ens.close()
 
 
 def test_full_size_porous_channel_conserves_mass(gpu_available):

@@ -85,3 +85,131 @@ def test_split_lattice_matches_openlb_multicuboid(gpu_available, name):
                   flush=True)
             assert err <= TOL[precision]
             assert err_flow <= TOL[precision]
+
+
+def _with_checkpoint():
+    return [n for n in G.multiblock_names() if G.load_multiblock(n)[1]["checkpoint_step"] is not None]
+
+
+def _decoded(arrays):
+    from openlb_amd import checkpoint as CK
+    return CK.decode_dat(arrays["dat"].tobytes().decode("ascii"))
+
+
+@pytest.mark.parametrize("name", _with_checkpoint())
+def test_split_lattice_resumes_from_openlb_checkpoint(gpu_available, name):
+    """OpenLB's checkpoint of the split lattice (every block, in cuboid order)
+    restores OpenLB's k-step populations exactly and continues to OpenLB's
+    populations at the next snapshot; loading and saving again gives the file's
+    bytes."""
+    case, meta, arrays = G.load_multiblock(name)
+    k, real = meta["checkpoint_step"], meta["precision"]
+    raw = _decoded(arrays)
+    flow = (case.ids != 0)[None] & ~G.streamed_from_outside(case)
+    with SuperLattice(case, real, tuple(meta["blocks"])) as sup:
+        sup.restore(raw, real)
+        assert sup.checkpoint(real) == raw
+        np.testing.assert_array_equal(sup.download_populations(real), arrays[f"f_{k}"])
+        m = min(s for s in meta["steps"] if s > k)
+        sup.collide_and_stream(m - k)
+        got = sup.download_populations("f64")
+    err = G.rel_error(got, arrays[f"f_{m}"].astype(np.float64), case.q, flow)
+    print(f"{name}: resumed at {k}, step {m}: rel {err:.3e}", flush=True)
+    assert err <= TOL[real]
+
+
+@pytest.mark.parametrize("name", _with_checkpoint())
+def test_split_checkpoint_after_steps_has_openlb_layout(gpu_available, name):
+    """Each block's part of a checkpoint written after further steps has
+    OpenLB's shifts for that step count, the block's core populations and
+    every field OpenLB's set-up writes, overlap included (the oracle's
+    restatement, pinned to OpenLB's file by test_multiblock_cpu.py)."""
+    from oracle import checkpoint as K
+    from openlb_amd.super_lattice import _sub_case, split_axis
+    case, meta, arrays = G.load_multiblock(name)
+    k, real = meta["checkpoint_step"], meta["precision"]
+    with SuperLattice(case, real, tuple(meta["blocks"])) as sup:
+        sup.restore(_decoded(arrays), real)
+        sup.collide_and_stream(7)
+        raw = sup.checkpoint(real)
+        cores = {key: lat.download_populations(real) for key, lat in sup.blocks.items()}
+        ranges = sup.ranges
+    self_periodic = tuple(int(case.periodic[d] and len(ranges[d]) == 1) for d in range(3))
+    off = 0
+    for key, core in cores.items():
+        lo = tuple(ranges[d][key[d]][0] for d in range(3))
+        hi = tuple(ranges[d][key[d]][1] for d in range(3))
+        sub = _sub_case(case, lo, hi, self_periodic)
+        n = K.serializable_size(sub, real, (case, lo))
+        shifts, pops, fields = K.unpack(sub, raw[off:off + n], real, (case, lo))
+        off += n
+        np.testing.assert_array_equal(shifts, K.rotate_shifts(sub, k + 7))
+        np.testing.assert_array_equal(K.core_of(sub, pops), core)
+        want = K.unpack(sub, K.pack(sub, K.padded_of(sub, core), shifts, real, (case, lo)), real, (case, lo))[2]
+        assert fields.keys() == want.keys()
+        for f in fields:
+            np.testing.assert_array_equal(fields[f], want[f], err_msg=f"{key} {f}")
+    assert off == len(raw)
+    assert split_axis(case.shape[0], meta["blocks"][0]) == ranges[0]
+
+
+@pytest.mark.parametrize("name", _with_checkpoint())
+def test_split_vtk_pieces_match_openlb(gpu_available, name, tmp_path):
+    """SuperVTMwriter3D on the split lattice writes OpenLB's pvd and vtm and one
+    piece per cuboid with OpenLB's header and arrays (within one float rounding,
+    as the single-block export)."""
+    from openlb_amd import vtk as V
+    case, meta, arrays = G.load_multiblock(name)
+    k, real = meta["checkpoint_step"], meta["precision"]
+    conv = V.UnitConverter(0.01, 1e-4, 0.1, 1.0, 1e-5, 1000.0, 100.0, real)
+    with SuperLattice(case, real, tuple(meta["blocks"])) as sup:
+        sup.restore(_decoded(arrays), real)
+        w = V.SuperVTMwriter3D("fields", out_dir=str(tmp_path / "vtkData"))
+        for f in (V.SuperLatticeDensity3D(sup), V.SuperLatticeVelocity3D(sup),
+                  V.SuperLatticePhysVelocity3D(sup, conv), V.SuperLatticePhysPressure3D(sup, conv)):
+            w.add_functor(f)
+        w.create_master_file()
+        w.write(k)
+    out = tmp_path / "vtkData"
+    assert (out / "fields.pvd").read_text() == arrays["pvd"].tobytes().decode()
+    assert (out / "data" / f"fields_iT{k:07d}.vtm").read_text() == arrays["vtm"].tobytes().decode()
+    for ic in range(int(np.prod(meta["blocks"]))):
+        path = out / "data" / f"fields_iT{k:07d}iC{ic:05d}.vti"
+        ref_text = arrays[f"vti_{ic}"].tobytes().decode()
+        head = [ln for ln in path.read_text().splitlines() if ln.startswith("<")]
+        assert head == [ln for ln in ref_text.splitlines() if ln.startswith("<")]
+        ref_path = tmp_path / f"ref_{ic}.vti"
+        ref_path.write_text(ref_text)
+        got, ref = V.read_vti_arrays(str(path)), V.read_vti_arrays(str(ref_path))
+        for key in ref:
+            g, r = got[key].astype(np.float64), ref[key].astype(np.float64)
+            tol = np.maximum(np.abs(r) * 2.0 ** -23, 1e-7 * np.abs(r).max())
+            assert not (np.abs(g - r) > tol).any(), (ic, key, float(np.abs(g - r).max()))
+
+
+def test_split_restore_is_all_or_nothing(gpu_available):
+    """A checkpoint of the wrong size is refused; one whose last block is
+    refused leaves every block as it was, the blocks restored before it
+    included."""
+    import struct
+
+    from openlb_amd import abi
+    case = W.lid_driven_cavity((10, 9, 8), tau=0.6, lid_u=0.05)
+    with SuperLattice(case, "f64", (2, 1, 1)) as sup:
+        sup.ini_equilibrium(case.rho_u)
+        sup.collide_and_stream(3)
+        good = sup.checkpoint("f64")
+        sup.collide_and_stream(2)
+        after = sup.download_populations("f64")
+        with pytest.raises(ValueError):
+            sup.restore(good[:-8], "f64")
+        sizes = [lat.checkpoint_size("f64") for lat in sup.blocks.values()]
+        bad = bytearray(good)
+        last = sum(sizes[:-1])
+        bad[last:last + 8] = struct.pack("<q", 1 << 40)     # the last block's first shift, out of range
+        with pytest.raises(abi.LbmhError):
+            sup.restore(bytes(bad), "f64")
+        np.testing.assert_array_equal(sup.download_populations("f64"), after)
+        sup.restore(good, "f64")
+        sup.collide_and_stream(2)
+        np.testing.assert_array_equal(sup.download_populations("f64"), after)

@@ -96,3 +96,31 @@ def test_fifo_when_everything_fits(host_server):
     srv = host_server(budget_bytes=1 << 30, storages=("f32",), max_members=16, tick_steps=4)
     res = srv.run(reqs, realtime=False)
     assert sorted(r.rid for r in res) == list(range(6)) and srv.max_active == 6
+
+
+def test_precision_guard_runs_low_tau_fp16_requests_in_fp32(host_server):
+    reqs = [serving.channel_request(0, 0.0, 8, 20, 0.05, "f16", tau=0.6),
+            serving.channel_request(1, 0.0, 8, 20, 0.05, "f16", tau=0.51),
+            serving.channel_request(2, 0.0, 8, 20, 0.05, "f32", tau=0.51)]
+    srv = host_server(budget_bytes=1 << 26, storages=("f32", "f16"), max_members=8, tick_steps=8)
+    res = {r.rid: r for r in srv.run(reqs, realtime=False)}
+    assert [(res[k].storage, res[k].promoted) for k in range(3)] == [("f16", False), ("f32", True),
+                                                                     ("f32", False)]
+    assert serving.summarize(list(res.values()))["promoted_to_f32"] == 1
+    off = host_server(budget_bytes=1 << 26, storages=("f32", "f16"), max_members=8, tick_steps=8,
+                      precision_guard=None)
+    assert {r.rid: r.storage for r in off.run(reqs, realtime=False)} == {0: "f16", 1: "f16", 2: "f32"}
+    # a server without an FP32 ensemble keeps FP16 requests where they are
+    only16 = host_server(budget_bytes=1 << 26, storages=("f16",), max_members=8, tick_steps=8)
+    assert {r.rid: r.storage for r in only16.run(reqs[:2], realtime=False)} == {0: "f16", 1: "f16"}
+
+
+def test_guarded_storage():
+    from openlb_amd import workloads as W
+    from openlb_amd.ensemble import FP16_TAU_GUARD, guarded_storage
+    low = W.lid_driven_cavity((6, 6, 6), tau=0.52, lid_u=0.05)
+    high = W.lid_driven_cavity((6, 6, 6), tau=FP16_TAU_GUARD, lid_u=0.05)
+    assert guarded_storage(low, "f16") == "f32"
+    assert guarded_storage(high, "f16") == "f16"
+    assert guarded_storage(low, "f16", None) == "f16"
+    assert guarded_storage(low, "f64") == "f64" and guarded_storage(low, "f32") == "f32"
