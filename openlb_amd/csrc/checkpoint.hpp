@@ -71,11 +71,9 @@ struct CheckpointBlock {
   int storage = LBMH_F32;
   int kind = LBMH_BGK;
   uint32_t n_padded = 0;
-  uint32_t ring = 0;              // elements of a device column (>= n_padded)
   uint64_t stride = 0;            // elements between columns
   void* d_pop = nullptr;          // column 0 on the device
-  int64_t shift[27] = {};         // OpenLB's PS shifts, in [0, n_padded)
-  int64_t rshift[27] = {};        // the device columns' shifts, in [0, ring)
+  int64_t shift[27] = {};         // PS shifts in [0, n_padded)
   int64_t dist[27] = {};          // c_i . projection
   const uint8_t* ids = nullptr;   // host ids [nx][ny][nz]
   const double* walls = nullptr;  // [id][4] rho, u
@@ -186,33 +184,6 @@ inline int64_t openlb_shift(int64_t shift, int64_t dist, int64_t count)
   return (dist > 0 && shift != 0) ? shift - count : shift;
 }
 
-// A column in OpenLB's storage order (n cells, element k = logical cell
-// (k - s) mod n) against a device column of ring >= n elements with its own
-// shift r (logical cell c at element (c + r) mod ring).  With ring == n and
-// r == s the two are the same array.  Both copy n values in at most three
-// contiguous runs.
-template <typename T>
-void ring_to_openlb(const T* ring, uint32_t m, int64_t r, T* out, uint32_t n, int64_t s)
-{
-  for (uint32_t c = 0; c < n;) {
-    const uint64_t k = (uint64_t(c) + uint64_t(s)) % n, e = (uint64_t(c) + uint64_t(r)) % m;
-    const uint32_t run = uint32_t(std::min<uint64_t>({uint64_t(n - c), n - k, m - e}));
-    std::memcpy(out + k, ring + e, size_t(run) * sizeof(T));
-    c += run;
-  }
-}
-
-template <typename T>
-void openlb_to_ring(const T* in, uint32_t n, int64_t s, T* ring, uint32_t m, int64_t r)
-{
-  for (uint32_t c = 0; c < n;) {
-    const uint64_t k = (uint64_t(c) + uint64_t(s)) % n, e = (uint64_t(c) + uint64_t(r)) % m;
-    const uint32_t run = uint32_t(std::min<uint64_t>({uint64_t(n - c), n - k, m - e}));
-    std::memcpy(ring + e, in + k, size_t(run) * sizeof(T));
-    c += run;
-  }
-}
-
 // Core-cell value of component `comp` of field `f` at core cell `cell` (flat
 // core index), and whether the block's dynamics read it.
 inline double field_value(const CheckpointBlock& b, const CheckpointField& f, int id, int comp, bool& read,
@@ -310,25 +281,23 @@ inline T take(const uint8_t*& p)
 template <typename S, typename T>
 int checkpoint_save_typed(const CheckpointBlock& b, uint8_t* out)
 {
-  const uint32_t n = b.n_padded, m = b.ring;
+  const uint32_t n = b.n_padded;
   uint8_t* p = out;
   DeviceBuffer staging;
-  if (!std::is_same_v<S, T>) LBMH_HIP(hipMalloc(&staging.p, size_t(m) * sizeof(T)));
-  std::vector<T> ring(m);
+  if (!std::is_same_v<S, T>) LBMH_HIP(hipMalloc(&staging.p, size_t(n) * sizeof(T)));
   LBMH_HIP(hipDeviceSynchronize());
   for (int i = 0; i < b.q; ++i) {
     put<int64_t>(p, openlb_shift(b.shift[i], b.dist[i], n));
     put<uint64_t>(p, n);
     const S* col = static_cast<const S*>(b.d_pop) + uint64_t(i) * b.stride;
     if constexpr (std::is_same_v<S, T>) {
-      LBMH_HIP(hipMemcpy(ring.data(), col, size_t(m) * sizeof(T), hipMemcpyDeviceToHost));
+      LBMH_HIP(hipMemcpy(p, col, size_t(n) * sizeof(T), hipMemcpyDeviceToHost));
     } else {
-      hipLaunchKernelGGL((column_export_kernel<S, T>), dim3(grid_for(m, 256)), dim3(256), 0, 0,
-                         static_cast<T*>(staging.p), col, m);
+      hipLaunchKernelGGL((column_export_kernel<S, T>), dim3(grid_for(n, 256)), dim3(256), 0, 0,
+                         static_cast<T*>(staging.p), col, n);
       LBMH_HIP(hipGetLastError());
-      LBMH_HIP(hipMemcpy(ring.data(), staging.p, size_t(m) * sizeof(T), hipMemcpyDeviceToHost));
+      LBMH_HIP(hipMemcpy(p, staging.p, size_t(n) * sizeof(T), hipMemcpyDeviceToHost));
     }
-    ring_to_openlb<T>(ring.data(), m, b.rshift[i], reinterpret_cast<T*>(p), n, b.shift[i]);
     p += size_t(n) * sizeof(T);
   }
   std::vector<T> column(n);
@@ -389,23 +358,17 @@ int checkpoint_load_typed(const CheckpointBlock& b, const uint8_t* in, int64_t* 
           }
     }
   }
-  // the device columns take the file's shifts: logical cell c goes to ring
-  // element (c + shift) mod ring
-  const uint32_t m = b.ring;
   DeviceBuffer staging;
-  if (!std::is_same_v<S, T>) LBMH_HIP(hipMalloc(&staging.p, size_t(m) * sizeof(T)));
-  std::vector<T> ring(m, T(0));
+  if (!std::is_same_v<S, T>) LBMH_HIP(hipMalloc(&staging.p, size_t(n) * sizeof(T)));
   LBMH_HIP(hipDeviceSynchronize());
   for (int i = 0; i < b.q; ++i) {
     S* col = static_cast<S*>(b.d_pop) + uint64_t(i) * b.stride;
-    std::vector<T> column(data[i], data[i] + n);
-    openlb_to_ring<T>(column.data(), n, new_shift[i], ring.data(), m, new_shift[i]);
     if constexpr (std::is_same_v<S, T>) {
-      LBMH_HIP(hipMemcpy(col, ring.data(), size_t(m) * sizeof(T), hipMemcpyHostToDevice));
+      LBMH_HIP(hipMemcpy(col, data[i], size_t(n) * sizeof(T), hipMemcpyHostToDevice));
     } else {
-      LBMH_HIP(hipMemcpy(staging.p, ring.data(), size_t(m) * sizeof(T), hipMemcpyHostToDevice));
-      hipLaunchKernelGGL((column_import_kernel<S, T>), dim3(grid_for(m, 256)), dim3(256), 0, 0, col,
-                         static_cast<const T*>(staging.p), m);
+      LBMH_HIP(hipMemcpy(staging.p, data[i], size_t(n) * sizeof(T), hipMemcpyHostToDevice));
+      hipLaunchKernelGGL((column_import_kernel<S, T>), dim3(grid_for(n, 256)), dim3(256), 0, 0, col,
+                         static_cast<const T*>(staging.p), n);
       LBMH_HIP(hipGetLastError());
     }
   }

@@ -2,16 +2,11 @@
 //
 // Memory layout in HBM: one column per discrete velocity i, each holding all
 // n_padded cells of the padded block (padding 1, x-major, z contiguous,
-// src/core/blockStructure.h:65-78) in a ring of `n_padded` (the StepArgs
-// field: the ring length, >= the cell count) elements, columns `stride`
-// elements apart.  A column is addressed through its periodic shift: logical
-// cell n of column i lives at element (n + shift_i) mod ring (src/core/
-// platform/cpu/sisd/column.h:160-204).  Streaming is shift_i -= c_i .
-// (sx, sy, 1) on the host, so a time step is a single pass that reads and
-// writes every population once.  A single lattice maps each ring twice back
-// to back (stride = 2 ring); its pass reads column i through a pointer
-// already advanced by shift_i, at element n, with no wrap (MIRRORED).
-// Ensemble members share one arena and wrap lane by lane.
+// src/core/blockStructure.h:65-78), columns `stride` elements apart.  A column
+// is addressed through its periodic shift: logical cell n of column i lives at
+// element (n + shift_i) mod n_padded (src/core/platform/cpu/sisd/column.h:
+// 160-204).  Streaming is shift_i -= c_i . (sx, sy, 1) on the host, so a time
+// step is a single pass that reads and writes every population once.
 //
 // One lane owns one core cell.  Lanes of a wave take consecutive z, so each of
 // the Q loads and stores of a wave is one contiguous 64-element run of a
@@ -40,9 +35,8 @@ template <int Q, typename V>
 struct StepArgs {
   void* pop;
   uint64_t stride;          // elements between population columns
-  uint32_t n_padded;        // ring length of a column (modulus of the shifts)
+  uint32_t n_padded;        // cells per column (modulus of the shifts)
   uint32_t shift[Q];        // PS shift of each column, in [0, n_padded)
-  void* col[Q];             // mirrored columns: column i advanced by shift[i]
   const uint8_t* ids;       // per core cell dynamics id (flat core index)
   uint32_t ncore;
   uint32_t nx, ny, nz;
@@ -130,7 +124,7 @@ enum LocalMode : int { kLocalNone = 0, kLocalSkip = 1, kLocalCollide = 2, kLocal
 
 // Time step of core cell c of one block: collide in place, refresh the
 // periodic overlap.  The caller advances the shifts afterwards.
-template <int Q, typename S, int KIND, bool IDS, int LOCAL = kLocalNone, bool MIRRORED = false>
+template <int Q, typename S, int KIND, bool IDS, int LOCAL = kLocalNone>
 __device__ __forceinline__ void collide_stream_cell(const StepArgs<Q, typename StorageTraits<S>::V>& a, uint32_t c)
 {
   using V = typename StorageTraits<S>::V;
@@ -167,10 +161,7 @@ __device__ __forceinline__ void collide_stream_cell(const StepArgs<Q, typename S
   V f[Q];
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
-    if constexpr (MIRRORED)
-      f[i] = load_pop<S, V>(column_at(static_cast<S*>(a.col[i]), n));
-    else
-      f[i] = load_pop<S, V>(column_at(pop + uint64_t(i) * a.stride, wrap_index(n, a.shift[i], a.n_padded)));
+    f[i] = load_pop<S, V>(column_at(pop + uint64_t(i) * a.stride, wrap_index(n, a.shift[i], a.n_padded)));
   }
   // The store addresses are recomputed after the collision from an opaque
   // copy of n instead of held through it (Q fewer live registers: 30-55 VGPRs
@@ -225,10 +216,7 @@ __device__ __forceinline__ void collide_stream_cell(const StepArgs<Q, typename S
     }
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      if constexpr (MIRRORED)
-        store_pop<S, V>(column_at(static_cast<S*>(a.col[i]), ns), f[i]);
-      else
-        store_pop<S, V>(column_at(pop + uint64_t(i) * a.stride, wrap_index(ns, a.shift[i], a.n_padded)), f[i]);
+      store_pop<S, V>(column_at(pop + uint64_t(i) * a.stride, wrap_index(ns, a.shift[i], a.n_padded)), f[i]);
     }
   }
   if (edge) {
@@ -246,7 +234,7 @@ __global__ __launch_bounds__(kBlock) void collide_stream_kernel(StepArgs<Q, type
 {
   uint32_t b = blockIdx.x;
   if (a.xcd_group) b = remap_xcd(b, gridDim.x, a.xcd_group);
-  collide_stream_cell<Q, S, KIND, IDS, LOCAL, true>(a, b * kBlock + threadIdx.x);
+  collide_stream_cell<Q, S, KIND, IDS, LOCAL>(a, b * kBlock + threadIdx.x);
 }
 
 // Measurement kernel, not part of a time step: the step kernel's access
@@ -268,15 +256,16 @@ __global__ __launch_bounds__(kBlock) void pattern_ceiling_kernel(StepArgs<Q, flo
   const uint32_t x = a.div_y.div(t);
   const uint32_t y = t - x * a.ny;
   const uint32_t n = (x + 1) * a.sx + (y + 1) * a.sy + (z + 1);
+  S* pop = static_cast<S*>(a.pop);
   S f[Q];
 #pragma unroll
-  for (int i = 0; i < Q; ++i) f[i] = *column_at(static_cast<S*>(a.col[i]), n);
+  for (int i = 0; i < Q; ++i) f[i] = *column_at(pop + uint64_t(i) * a.stride, wrap_index(n, a.shift[i], a.n_padded));
   uint32_t ns = n;
   asm volatile("" : "+v"(ns));
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     asm volatile("" : "+v"(f[i]));
-    *column_at(static_cast<S*>(a.col[i]), ns) = f[i];
+    *column_at(pop + uint64_t(i) * a.stride, wrap_index(ns, a.shift[i], a.n_padded)) = f[i];
   }
 }
 
@@ -288,37 +277,7 @@ __global__ __launch_bounds__(64) void local_boundary_kernel(StepArgs<Q, typename
 {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= count) return;
-  collide_stream_cell<Q, S, KIND, true, kLocalCollide, true>(a, cells[k]);
-}
-
-// The ring of a lattice column is longer than its n_padded cells (whole
-// allocation granules, plus at least one plane of slack), so after a shift the
-// |d_i| slots that become live at one end of the column hold what a column of
-// exactly n_padded elements would have wrapped into them: the other end's
-// values.  Each step copies them over (|d_i| <= sx + sy + 1 elements per
-// column), which keeps every padded cell -- overlap included -- equal to
-// OpenLB's CPU columns.  Column i copies count_i = begin[i+1] - begin[i]
-// elements from element src[i] to element dst[i] of its double mapping
-// (src, dst < ring and count <= ring - n_padded: both ranges stay inside the
-// mapping and never overlap).
-struct WrapFixArgs {
-  void* pop;
-  uint64_t stride;
-  uint32_t src[27], dst[27];
-  uint32_t begin[28];
-  int q;
-};
-
-template <typename S>
-__global__ __launch_bounds__(256) void wrap_fixup_kernel(WrapFixArgs a)
-{
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.begin[a.q]) return;
-  int i = 0;
-  while (t >= a.begin[i + 1]) ++i;
-  const uint32_t j = t - a.begin[i];
-  S* col = static_cast<S*>(a.pop) + uint64_t(i) * a.stride;
-  col[uint64_t(a.dst[i]) + j] = col[uint64_t(a.src[i]) + j];
+  collide_stream_cell<Q, S, KIND, true, kLocalCollide>(a, cells[k]);
 }
 
 // ---- data movement and evaluation kernels (not on the time-step path) ----

@@ -70,17 +70,12 @@ struct lbmh_lattice {
   int kind = LBMH_BGK;
   uint32_t size[3] = {0, 0, 0};
   uint32_t n_padded = 0;
-  uint32_t ring = 0;          // elements of one physical column (>= n_padded, whole allocation granules)
-  uint64_t stride = 0;        // 2 * ring: each column is mapped twice, back to back
+  uint64_t stride = 0;
   uint32_t ncore = 0;
   int64_t steps = 0;
-  int64_t shift[27] = {};     // OpenLB's CyclicColumn shift, in [0, n_padded)
-  int64_t rshift[27] = {};    // the device column's shift, in [0, ring)
+  int64_t shift[27] = {};
   int64_t dist[27] = {};
-  void* d_pop = nullptr;      // the columns' virtual range (2 * ring * q elements)
-  size_t va_bytes = 0;
-  size_t column_bytes = 0;
-  hipMemGenericAllocationHandle_t column_mem[27] = {};
+  void* d_pop = nullptr;
   uint8_t* d_ids = nullptr;
   double* d_walls = nullptr;
   double* d_partials = nullptr;
@@ -152,7 +147,7 @@ CoreGeom core_geom(const lbmh_lattice* L)
   g.nz = L->n[2];
   g.sx = L->size[1] * L->size[2];
   g.sy = L->size[2];
-  g.n_padded = L->ring;
+  g.n_padded = L->n_padded;
   g.stride = L->stride;
   g.div_z = FastDiv(uint32_t(L->n[2]));
   g.div_y = FastDiv(uint32_t(L->n[1]));
@@ -163,7 +158,7 @@ template <int Q>
 Shifts<Q> shifts_of(const lbmh_lattice* L)
 {
   Shifts<Q> s;
-  for (int i = 0; i < Q; ++i) s.s[i] = uint32_t(L->rshift[i]);
+  for (int i = 0; i < Q; ++i) s.s[i] = uint32_t(L->shift[i]);
   return s;
 }
 
@@ -188,8 +183,8 @@ PostArgs<Q> post_args(const lbmh_lattice* L)
   PostArgs<Q> a;
   a.pop = L->d_pop;
   a.stride = L->stride;
-  a.n_padded = L->ring;
-  for (int i = 0; i < Q; ++i) a.shift[i] = uint32_t(L->rshift[i]);
+  a.n_padded = L->n_padded;
+  for (int i = 0; i < Q; ++i) a.shift[i] = uint32_t(L->shift[i]);
   a.ids = L->d_ids;
   a.walls = L->d_walls;
   a.info = L->d_info;
@@ -304,7 +299,7 @@ StepArgs<Q, V> step_args(const lbmh_lattice* L)
   StepArgs<Q, V> a;
   a.pop = L->d_pop;
   a.stride = L->stride;
-  a.n_padded = L->ring;
+  a.n_padded = L->n_padded;
   a.ids = L->d_ids;
   a.ncore = L->ncore;
   a.nx = L->n[0];
@@ -320,61 +315,19 @@ StepArgs<Q, V> step_args(const lbmh_lattice* L)
   a.info = L->d_info;
   a.force = static_cast<const V*>(L->d_force);
   a.bp = bulk_params<V>(L);
-  for (int i = 0; i < Q; ++i) {
-    a.shift[i] = uint32_t(L->rshift[i]);
-    // column i pre-shifted: logical cell n is element n of this pointer in the
-    // double mapping, for every n < n_padded
-    a.col[i] = static_cast<char*>(L->d_pop) + (uint64_t(i) * L->stride + uint64_t(L->rshift[i])) * storage_size(L->storage);
-  }
+  for (int i = 0; i < Q; ++i) a.shift[i] = uint32_t(L->shift[i]);
   return a;
 }
 
-// stream: shift_i -= c_i . projection (blockLattice.hh:317-339), for OpenLB's
-// shift (modulo the padded cells) and for the device column's (modulo the ring)
+// stream: shift_i -= c_i . projection (blockLattice.hh:317-339)
 void advance_shifts(lbmh_lattice* L)
 {
-  const int64_t N = L->n_padded, M = L->ring;
+  const int64_t N = L->n_padded;
   for (int i = 0; i < L->q; ++i) {
     int64_t s = (L->shift[i] - L->dist[i]) % N;
     if (s < 0) s += N;
     L->shift[i] = s;
-    int64_t r = (L->rshift[i] - L->dist[i]) % M;
-    if (r < 0) r += M;
-    L->rshift[i] = r;
   }
-}
-
-// Before the shift of a step: refill the slots the shift makes live from the
-// column's other end (wrap_fixup_kernel).
-template <typename S>
-int launch_wrap_fixup(lbmh_lattice* L, hipStream_t stream)
-{
-  WrapFixArgs w;
-  w.pop = L->d_pop;
-  w.stride = L->stride;
-  w.q = L->q;
-  const int64_t N = L->n_padded, M = L->ring;
-  uint32_t total = 0;
-  for (int i = 0; i < L->q; ++i) {
-    const int64_t d = L->dist[i], r = L->rshift[i];
-    w.begin[i] = total;
-    if (d > 0) {
-      w.src[i] = uint32_t((r + N - d) % M);
-      w.dst[i] = uint32_t((r - d + M) % M);
-      total += uint32_t(d);
-    } else if (d < 0) {
-      w.src[i] = uint32_t(r);
-      w.dst[i] = uint32_t((r + N) % M);
-      total += uint32_t(-d);
-    } else {
-      w.src[i] = w.dst[i] = 0;
-    }
-  }
-  w.begin[L->q] = total;
-  if (total == 0) return LBMH_OK;
-  hipLaunchKernelGGL((wrap_fixup_kernel<S>), dim3(grid_for(total, 256)), dim3(256), 0, stream, w);
-  LBMH_HIP(hipGetLastError());
-  return LBMH_OK;
 }
 
 template <int Q, typename S, int KIND>
@@ -408,7 +361,6 @@ int launch_kind(lbmh_lattice* L, hipStream_t stream, int phase)
   LBMH_HIP(hipGetLastError());
   }
   if (!(phase & kPhaseStream)) return LBMH_OK;
-  if (const int rc = launch_wrap_fixup<S>(L, stream); rc != LBMH_OK) return rc;
   advance_shifts(L);
   L->steps += 1;
   if (L->has_local) return launch_post_processors<Q, S>(L, stream);
@@ -477,71 +429,6 @@ int lbmh_device_arch(int device, char* name, size_t len)
   return LBMH_OK;
 }
 
-// Population columns, each mapped twice back to back into one virtual range
-// (the layout of OpenLB's CUDA CyclicColumn, src/core/platform/gpu/cuda/
-// column.hh:217-267): a column of `ring` elements occupies [2 i ring,
-// 2 (i+1) ring), and element k of the second mapping is element k of the
-// first.  Logical cell n of a column shifted by r lies at (n + r) mod ring,
-// i.e. at element n + r of the pair for every n < n_padded <= ring: the step
-// kernel addresses the pre-shifted column with no wrap at all.  The ring is
-// n_padded rounded up to whole allocation granules.
-hipError_t map_columns(lbmh_lattice* L)
-{
-  hipMemAllocationProp prop = {};
-  prop.type = hipMemAllocationTypePinned;
-  prop.location.type = hipMemLocationTypeDevice;
-  prop.location.id = L->device;
-  size_t granule = 0;
-  hipError_t e = hipMemGetAllocationGranularity(&granule, &prop, hipMemAllocationGranularityMinimum);
-  if (e != hipSuccess) return e;
-  const size_t es = storage_size(L->storage);
-  if (granule == 0 || granule % es != 0) return hipErrorInvalidValue;
-  // slack for the largest shift of one step, so that wrap_fixup_kernel's
-  // source and destination never overlap
-  int64_t slack = 0;
-  for (int i = 0; i < L->q; ++i) slack = std::max<int64_t>(slack, L->dist[i] < 0 ? -L->dist[i] : L->dist[i]);
-  L->column_bytes = ((size_t(L->n_padded) + size_t(slack)) * es + granule - 1) / granule * granule;
-  L->ring = uint32_t(L->column_bytes / es);
-  L->stride = 2 * uint64_t(L->ring);
-  L->va_bytes = 2 * L->column_bytes * size_t(L->q);
-  void* va = nullptr;
-  e = hipMemAddressReserve(&va, L->va_bytes, granule, nullptr, 0);
-  if (e != hipSuccess) {
-    L->va_bytes = 0;
-    return e;
-  }
-  L->d_pop = va;
-  for (int i = 0; i < L->q; ++i) {
-    e = hipMemCreate(&L->column_mem[i], L->column_bytes, &prop, 0);
-    if (e != hipSuccess) {
-      L->column_mem[i] = {};
-      return e;
-    }
-    char* base = static_cast<char*>(va) + 2 * size_t(i) * L->column_bytes;
-    e = hipMemMap(base, L->column_bytes, 0, L->column_mem[i], 0);
-    if (e == hipSuccess) e = hipMemMap(base + L->column_bytes, L->column_bytes, 0, L->column_mem[i], 0);
-    if (e != hipSuccess) return e;
-  }
-  hipMemAccessDesc access = {};
-  access.location = prop.location;
-  access.flags = hipMemAccessFlagsProtReadWrite;
-  return hipMemSetAccess(va, L->va_bytes, &access, 1);
-}
-
-void unmap_columns(lbmh_lattice* L)
-{
-  if (!L->d_pop) return;
-  for (int i = 0; i < L->q; ++i) {
-    char* base = static_cast<char*>(L->d_pop) + 2 * size_t(i) * L->column_bytes;
-    (void)hipMemUnmap(base, L->column_bytes);
-    (void)hipMemUnmap(base + L->column_bytes, L->column_bytes);
-    if (L->column_mem[i]) (void)hipMemRelease(L->column_mem[i]);
-    L->column_mem[i] = {};
-  }
-  (void)hipMemAddressFree(L->d_pop, L->va_bytes);
-  L->d_pop = nullptr;
-}
-
 int lbmh_lattice_create(const lbmh_lattice_desc* desc, lbmh_lattice** out)
 {
   if (!desc || !out) return fail(LBMH_E_INVALID, "null argument");
@@ -579,9 +466,11 @@ int lbmh_lattice_create(const lbmh_lattice_desc* desc, lbmh_lattice** out)
   L->size[2] = uint32_t(sz);
   L->n_padded = uint32_t(npad);
   L->ncore = uint32_t(uint64_t(desc->nx) * desc->ny * desc->nz);
+  // columns start on 256-byte boundaries
+  const uint64_t align = 256 / storage_size(L->storage);
+  L->stride = (npad + align - 1) / align * align;
   for (int i = 0; i < L->q; ++i) {
     L->shift[i] = 0;
-    L->rshift[i] = 0;
     L->dist[i] = L->q == 19 ? neighbor_distance<19>(i, int64_t(sy * sz), int64_t(sz))
                             : neighbor_distance<27>(i, int64_t(sy * sz), int64_t(sz));
   }
@@ -592,16 +481,15 @@ int lbmh_lattice_create(const lbmh_lattice_desc* desc, lbmh_lattice** out)
   for (int w = 0; w < kMaxWalls; ++w) L->walls[w][0] = 1.0;
 
   DeviceGuard guard(L->device);
+  const size_t pop_bytes = size_t(L->stride) * L->q * storage_size(L->storage);
   L->n_partials = 1024;
-  hipError_t e = map_columns(L);
-  const size_t pop_bytes = L->column_bytes * L->q;
+  hipError_t e = hipMalloc(&L->d_pop, pop_bytes);
   if (e == hipSuccess) e = hipMalloc((void**)&L->d_ids, L->ncore);
   if (e == hipSuccess) e = hipMalloc((void**)&L->d_walls, sizeof(double) * 4 * kMaxWalls);
   if (e == hipSuccess) e = hipMalloc((void**)&L->d_partials, sizeof(double) * 4 * (L->n_partials + 1));
   if (e == hipSuccess) e = hipMalloc((void**)&L->d_info, sizeof(uint32_t) * kMaxWalls);
   if (e == hipSuccess) e = hipMemcpy(L->d_info, L->info, sizeof(L->info), hipMemcpyHostToDevice);
-  for (int i = 0; i < L->q && e == hipSuccess; ++i)
-    e = hipMemset(static_cast<char*>(L->d_pop) + 2 * size_t(i) * L->column_bytes, 0, L->column_bytes);
+  if (e == hipSuccess) e = hipMemset(L->d_pop, 0, pop_bytes);
   if (e == hipSuccess) e = hipMemset(L->d_ids, 0, L->ncore);
   if (e == hipSuccess) e = hipMemcpy(L->d_walls, L->walls, sizeof(L->walls), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
@@ -620,7 +508,7 @@ int lbmh_lattice_destroy(lbmh_lattice* L)
   DeviceGuard guard(L->device);
   for (lbmh_halo* H : L->halos_in) H->dst = nullptr;
   for (lbmh_halo* H : L->halos_out) H->src = nullptr;
-  unmap_columns(L);
+  if (L->d_pop) (void)hipFree(L->d_pop);
   if (L->d_force) (void)hipFree(L->d_force);
   if (L->d_ids) (void)hipFree(L->d_ids);
   if (L->d_walls) (void)hipFree(L->d_walls);
@@ -983,7 +871,7 @@ int lbmh_column_upload(lbmh_lattice* L, int i, const void* host, int64_t shift, 
   int rc = column_map(L, padding, m);
   if (rc != LBMH_OK) return rc;
   if (shift <= -m.count || shift >= m.count) return fail(LBMH_E_INVALID, "column shift out of (-count, count)");
-  const int64_t n = L->n_padded, ring = L->ring, s_dev = L->rshift[i];
+  const int64_t n = L->n_padded, s_dev = L->shift[i];
   DeviceGuard guard(L->device);
   LBMH_HIP(hipDeviceSynchronize());
   return with_storage(L->storage, [&](auto s) -> int {
@@ -992,16 +880,16 @@ int lbmh_column_upload(lbmh_lattice* L, int i, const void* host, int64_t shift, 
     auto put = [&](auto t) -> int {
       using T = decltype(t);
       const T* src = static_cast<const T*>(host);
-      std::vector<T> dev(static_cast<size_t>(ring), T(0));
-      for (int64_t d = 0; d < n; ++d) dev[size_t(wrap64(d + s_dev, ring))] = src[size_t(wrap64(m.host_of[size_t(d)] + shift, m.count))];
+      std::vector<T> dev(static_cast<size_t>(n), T(0));
+      for (int64_t d = 0; d < n; ++d) dev[size_t(wrap64(d + s_dev, n))] = src[size_t(wrap64(m.host_of[size_t(d)] + shift, m.count))];
       if constexpr (std::is_same_v<S, T>) {
-        LBMH_HIP(hipMemcpy(col, dev.data(), size_t(ring) * sizeof(T), hipMemcpyHostToDevice));
+        LBMH_HIP(hipMemcpy(col, dev.data(), size_t(n) * sizeof(T), hipMemcpyHostToDevice));
       } else {
         DeviceBuffer staging;
-        LBMH_HIP(hipMalloc(&staging.p, size_t(ring) * sizeof(T)));
-        LBMH_HIP(hipMemcpy(staging.p, dev.data(), size_t(ring) * sizeof(T), hipMemcpyHostToDevice));
-        hipLaunchKernelGGL((column_import_kernel<S, T>), dim3(grid_for(uint32_t(ring), 256)), dim3(256), 0, 0, col,
-                           static_cast<const T*>(staging.p), uint32_t(ring));
+        LBMH_HIP(hipMalloc(&staging.p, size_t(n) * sizeof(T)));
+        LBMH_HIP(hipMemcpy(staging.p, dev.data(), size_t(n) * sizeof(T), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL((column_import_kernel<S, T>), dim3(grid_for(uint32_t(n), 256)), dim3(256), 0, 0, col,
+                           static_cast<const T*>(staging.p), uint32_t(n));
         LBMH_HIP(hipGetLastError());
         LBMH_HIP(hipDeviceSynchronize());
       }
@@ -1020,7 +908,7 @@ int lbmh_column_download(lbmh_lattice* L, int i, void* host, int64_t shift, int 
   int rc = column_map(L, padding, m);
   if (rc != LBMH_OK) return rc;
   if (shift <= -m.count || shift >= m.count) return fail(LBMH_E_INVALID, "column shift out of (-count, count)");
-  const int64_t n = L->n_padded, ring = L->ring, s_dev = L->rshift[i];
+  const int64_t n = L->n_padded, s_dev = L->shift[i];
   DeviceGuard guard(L->device);
   LBMH_HIP(hipDeviceSynchronize());
   return with_storage(L->storage, [&](auto s) -> int {
@@ -1028,19 +916,19 @@ int lbmh_column_download(lbmh_lattice* L, int i, void* host, int64_t shift, int 
     const S* col = static_cast<const S*>(L->d_pop) + uint64_t(i) * L->stride;
     auto get = [&](auto t) -> int {
       using T = decltype(t);
-      std::vector<T> dev(static_cast<size_t>(ring));
+      std::vector<T> dev(static_cast<size_t>(n));
       if constexpr (std::is_same_v<S, T>) {
-        LBMH_HIP(hipMemcpy(dev.data(), col, size_t(ring) * sizeof(T), hipMemcpyDeviceToHost));
+        LBMH_HIP(hipMemcpy(dev.data(), col, size_t(n) * sizeof(T), hipMemcpyDeviceToHost));
       } else {
         DeviceBuffer staging;
-        LBMH_HIP(hipMalloc(&staging.p, size_t(ring) * sizeof(T)));
-        hipLaunchKernelGGL((column_export_kernel<S, T>), dim3(grid_for(uint32_t(ring), 256)), dim3(256), 0, 0,
-                           static_cast<T*>(staging.p), col, uint32_t(ring));
+        LBMH_HIP(hipMalloc(&staging.p, size_t(n) * sizeof(T)));
+        hipLaunchKernelGGL((column_export_kernel<S, T>), dim3(grid_for(uint32_t(n), 256)), dim3(256), 0, 0,
+                           static_cast<T*>(staging.p), col, uint32_t(n));
         LBMH_HIP(hipGetLastError());
-        LBMH_HIP(hipMemcpy(dev.data(), staging.p, size_t(ring) * sizeof(T), hipMemcpyDeviceToHost));
+        LBMH_HIP(hipMemcpy(dev.data(), staging.p, size_t(n) * sizeof(T), hipMemcpyDeviceToHost));
       }
       T* dst = static_cast<T*>(host);
-      for (int64_t d = 0; d < n; ++d) dst[size_t(wrap64(m.host_of[size_t(d)] + shift, m.count))] = dev[size_t(wrap64(d + s_dev, ring))];
+      for (int64_t d = 0; d < n; ++d) dst[size_t(wrap64(m.host_of[size_t(d)] + shift, m.count))] = dev[size_t(wrap64(d + s_dev, n))];
       return LBMH_OK;
     };
     return host_dtype == LBMH_F64 ? get(double{}) : get(float{});
@@ -1200,8 +1088,8 @@ int lbmh_halo_copy(lbmh_halo* H, void* stream)
     constexpr int Q = decltype(qc)::value;
     using S = decltype(st);
     hipLaunchKernelGGL((halo_copy_kernel<Q, S>), dim3(grid_for(H->count, 256), Q), dim3(256), 0, s,
-                       static_cast<S*>(D->d_pop), D->stride, D->ring, shifts_of<Q>(D),
-                       static_cast<const S*>(S_->d_pop), S_->stride, S_->ring, shifts_of<Q>(S_), H->d_pairs,
+                       static_cast<S*>(D->d_pop), D->stride, D->n_padded, shifts_of<Q>(D),
+                       static_cast<const S*>(S_->d_pop), S_->stride, S_->n_padded, shifts_of<Q>(S_), H->d_pairs,
                        H->count);
     LBMH_HIP(hipGetLastError());
     return LBMH_OK;
@@ -1245,12 +1133,10 @@ CheckpointBlock checkpoint_block(const lbmh_lattice* L, std::vector<uint8_t>& no
   b.storage = L->storage;
   b.kind = L->kind;
   b.n_padded = L->n_padded;
-  b.ring = L->ring;
   b.stride = L->stride;
   b.d_pop = L->d_pop;
   for (int i = 0; i < L->q; ++i) {
     b.shift[i] = L->shift[i];
-    b.rshift[i] = L->rshift[i];
     b.dist[i] = L->dist[i];
   }
   if (L->h_ids.size() == L->ncore) {
@@ -1311,7 +1197,7 @@ int lbmh_checkpoint_load(lbmh_lattice* L, int real, const void* buffer, size_t b
   int64_t shift[27];
   const int rc = checkpoint_load(checkpoint_block(L, no_ids, overlap), real, buffer, bytes, shift);
   if (rc != LBMH_OK) return rc;
-  for (int i = 0; i < L->q; ++i) L->shift[i] = L->rshift[i] = shift[i];
+  for (int i = 0; i < L->q; ++i) L->shift[i] = shift[i];
   return LBMH_OK;
 }
 

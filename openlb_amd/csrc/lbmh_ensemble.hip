@@ -822,11 +822,10 @@ int member_block(lbmh_ensemble* E, int m, CheckpointBlock& b, std::vector<double
     constexpr int Q = decltype(qc)::value;
     const EnsembleEntry<Q>& en = entry<Q>(E, m);
     b.n_padded = en.n_padded;
-    b.ring = en.n_padded;
     b.stride = en.stride;
     b.d_pop = static_cast<char*>(E->d_pop) + en.pop_offset * storage_size(E->storage);
     for (int i = 0; i < Q; ++i) {
-      b.shift[i] = b.rshift[i] = en.shift[i];
+      b.shift[i] = en.shift[i];
       b.dist[i] = neighbor_distance<Q>(i, int64_t(en.sx), int64_t(en.sy));
     }
     return LBMH_OK;
