@@ -123,6 +123,54 @@ def calibrate_copy(mib: int, repeats: int = 5):
                       "repeats": repeats, "copy_gbs": round(2 * nbytes / dt / 1e9, 1)}), flush=True)
 
 
+def kernel_profile(steps: int = 10):
+    """A short fixed run for hardware-counter passes (rocprofv3 --pmc): the
+    kernels whose counters are tabulated, each launched `steps` times, plus
+    two torch kernels of known traffic that calibrate the byte counters (a
+    read-only float32 sum of 1 GiB, and a float32 elementwise negation of
+    1 GiB into another 1 GiB).  It prints one JSON line naming each kernel's
+    algorithmic bytes per launch."""
+    import torch
+
+    from openlb_amd import workloads as W
+    from openlb_amd.ensemble import Ensemble
+    from openlb_amd.lattice import BlockLattice
+    out = {}
+    for label, case, storage in (("porous_channel 256^3", W.porous_channel((256, 256, 256)), "f32"),
+                                 ("turbulent_channel 384x192x192", W.turbulent_channel((384, 192, 192)), "f16")):
+        with BlockLattice.from_case(case, storage) as lat:
+            lat.ini_equilibrium(case.rho_u)
+            lat.collide_and_stream(steps)
+            lat.synchronize()
+        out[f"collide_stream_kernel<{case.q},{storage},{case.kind}> {label}"] = {
+            "launches": steps, "cells": case.n_active,
+            "algorithmic_bytes": 2 * case.q * STORE_BYTES[storage] * case.n_active}
+    cases = W.cavity_ensemble(128)
+    pop = sum(Ensemble.bytes_for(19, "f16", c.shape)[0] for c in cases)
+    ids = sum(Ensemble.bytes_for(19, "f16", c.shape)[1] for c in cases)
+    with Ensemble(19, "f16", "bgk", max_members=len(cases), population_bytes=pop + (1 << 20),
+                  cell_capacity=ids + (1 << 20)) as ens:
+        for c in cases:
+            m = ens.add(c)
+            ens.ini_equilibrium(m, c.rho_u)
+        ens.step(steps)
+        ens.status(m)
+    cells = sum(int((c.ids != 0).sum()) for c in cases)
+    out["ensemble_step_kernel<19,f16,bgk> cavity_ensemble x128"] = {
+        "launches": steps, "cells": cells, "algorithmic_bytes": 2 * 19 * 2 * cells}
+    n = 1 << 28
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    total = float(x.sum().item())
+    torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    if total != float(n):
+        raise SystemExit("calibration sum is wrong")
+    out["torch sum (reduce_kernel) 1 GiB float32"] = {"launches": 1, "bytes_read": 4 * n, "bytes_written": 0}
+    out["torch neg (elementwise) 1 GiB float32"] = {"launches": 1, "bytes_read": 4 * n, "bytes_written": 4 * n}
+    print(json.dumps({"kernel_profile": out}), flush=True)
+
+
 def copy_ceiling(bytes_moved: int, repeats: int = 10):
     """Plain copies moving the step kernel's algorithmic bytes per launch, half
     read and half written, timed with HIP events on the null stream: the
@@ -636,6 +684,8 @@ def main():
     ap.add_argument("--serving-rate", type=float, default=200.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (counter calibration)")
+    ap.add_argument("--kernel-profile", action="store_true",
+                    help="only run the short fixed kernel sequence the counter tables come from")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -649,6 +699,9 @@ def main():
 
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.kernel_profile:
+        kernel_profile()
         return
 
     case = make_case(args)
