@@ -229,6 +229,33 @@ int lbmh_moments(lbmh_lattice* lattice, double* rho_u);
 int lbmh_field_export(lbmh_lattice* lattice, int field, int overlap, const double conversion[3], float* host,
                       size_t floats);
 
+/* Reduction of a cell field over the core cells whose dynamics id is
+ * selected: the sum, minimum or maximum of each component, in FP64, and the
+ * number of cells visited.  Fields: the four of lbmh_field_export, each cell
+ * evaluated through its own momenta as there (in the lattice's arithmetic
+ * type, not rounded to float), and LBMH_FIELD_FORCE, the FORCE the bulk
+ * cells carry (the per-cell field of lbmh_set_force_field or the uniform
+ * force; 0 on other cells).  `materials` holds 256 flags, nonzero for the
+ * materials to visit -- the ids given to lbmh_set_dynamics_ids, whose cells
+ * lbmh_set_local_boundary relabels but which still count as their material,
+ * like OpenLB's SuperGeometry::getMaterialIndicator -- or is NULL for every
+ * material but 0 (NoDynamics).  Empty selections give
+ * the reduction's identity: 0, +DBL_MAX (min) or -DBL_MAX (max).  `out`
+ * holds 1 (density, pressure) or 3 values; `cells` may be NULL.  Replaces
+ * SuperLatticeFieldReductionO<T,DESCRIPTOR,FIELD,SumO|MinO|MaxO,CONDITION>::
+ * compute() (src/core/superLatticeFieldReductionO.h:41-160, the operators of
+ * src/core/blockLatticeFieldReductionO.h:35-100, the GPU kernels of
+ * src/core/platform/gpu/cuda/fieldReduction.hh) and the SuperSum3D /
+ * SuperMin3D / SuperMax3D / SuperAverage3D functors over a material
+ * indicator (src/functors/lattice/integral/superIntegralF3D.h:37-62,
+ * superMax3D.h, superMin3D.h, superAverage3D.h; average = sum / cells). */
+#define LBMH_FIELD_FORCE 4
+#define LBMH_REDUCE_SUM 0
+#define LBMH_REDUCE_MIN 1
+#define LBMH_REDUCE_MAX 2
+int lbmh_reduce_field(lbmh_lattice* lattice, int field, int op, const uint8_t* materials, const double conversion[3],
+                      double out[3], uint64_t* cells);
+
 /* FP64 sums over the cells with dynamics id 1: out = {sum rho, sum j_x,
  * sum j_y, sum j_z} (j = rho u without forcing shift).  Replaces
  * SuperLatticeFieldReductionO / reductionFunctionDevice

@@ -23,7 +23,7 @@ EXPORTS = (
     "lbmh_set_bulk_params", "lbmh_set_force_field", "lbmh_set_wall", "lbmh_populations_upload",
     "lbmh_populations_download", "lbmh_column_upload", "lbmh_column_download", "lbmh_ini_equilibrium", "lbmh_step",
     "lbmh_get_step_count", "lbmh_get_shifts", "lbmh_checkpoint_size", "lbmh_checkpoint_save",
-    "lbmh_checkpoint_load", "lbmh_field_export", "lbmh_moments", "lbmh_reduce_moments",
+    "lbmh_checkpoint_load", "lbmh_field_export", "lbmh_moments", "lbmh_reduce_field", "lbmh_reduce_moments",
     "lbmh_reduce_moments_noalloc", "lbmh_statistics",
     "lbmh_set_local_boundary", "lbmh_classify_boundary", "lbmh_initialize",
     "lbmh_collide", "lbmh_stream", "lbmh_halo_create", "lbmh_halo_destroy", "lbmh_halo_copy",
@@ -98,6 +98,7 @@ def load() -> ctypes.CDLL:
     L.lbmh_set_dynamics_ids.argtypes = [vp, vp]
     L.lbmh_set_bulk_params.argtypes = [vp, ctypes.POINTER(BulkParams)]
     L.lbmh_set_force_field.argtypes = [vp, vp]
+    L.lbmh_reduce_field.argtypes = [vp, i32, i32, vp, vp, vp, vp]
     L.lbmh_set_wall.argtypes = [vp, i32, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
     L.lbmh_populations_upload.argtypes = [vp, vp, i32]
     L.lbmh_populations_download.argtypes = [vp, vp, i32]

@@ -124,4 +124,126 @@ __global__ __launch_bounds__(256) void field_export_kernel(PostArgs<Q> a, int ov
   }
 }
 
+// ---- reductions of a cell field ---------------------------------------------
+
+enum ReduceOp : int { kReduceSum = 0, kReduceMin = 1, kReduceMax = 2 };
+constexpr int kFieldForce = 4;
+
+// The dynamics ids a reduction visits: bit id of bits[id / 32].
+struct IdMask {
+  uint32_t bits[8];
+};
+
+__device__ __forceinline__ double reduce_combine(int op, double a, double b)
+{
+  return op == kReduceSum ? a + b : (op == kReduceMin ? fmin(a, b) : fmax(a, b));
+}
+
+// SumO / MinO / MaxO's reset values (blockLatticeFieldReductionO.h:54-100)
+__host__ __device__ __forceinline__ double reduce_identity(int op)
+{
+  return op == kReduceSum ? 0.0 : (op == kReduceMin ? 1.7976931348623157e308 : -1.7976931348623157e308);
+}
+
+__device__ __forceinline__ double wave_reduce(int op, double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = reduce_combine(op, v, __shfl_down(v, off, 64));
+  return v;
+}
+
+// First level of a field reduction: core cells whose id is in `mask`, each
+// cell's value as lbmh_field_export evaluates it at that core point (in the
+// arithmetic type, not rounded to float) or its FORCE; one partial per
+// workgroup and component plus the count of visited cells, in a fixed order.
+template <int Q, typename S>
+__global__ __launch_bounds__(256) void field_reduce_kernel(PostArgs<Q> a, int field, int op, FieldConversion conv,
+                                                           int forced, double hfx, double hfy, double hfz,
+                                                           const typename StorageTraits<S>::V* force, IdMask mask,
+                                                           double* partial)
+{
+  using V = typename StorageTraits<S>::V;
+  const uint32_t ncore = uint32_t(a.nx) * uint32_t(a.ny) * uint32_t(a.nz);
+  const double init = reduce_identity(op);
+  double acc[3] = {init, init, init};
+  double count = 0.0;
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < ncore; c += gridDim.x * blockDim.x) {
+    const int id = a.ids[c];
+    if (!((mask.bits[id >> 5] >> (id & 31)) & 1u)) continue;
+    count += 1.0;
+    double v[3] = {0.0, 0.0, 0.0};
+    if (field == kFieldForce) {
+      if (id == kIdBulk) {
+        v[0] = double(half_force<V>(force, ncore, c, 0, hfx) * V(2));
+        v[1] = double(half_force<V>(force, ncore, c, 1, hfy) * V(2));
+        v[2] = double(half_force<V>(force, ncore, c, 2, hfz) * V(2));
+      }
+    } else {
+      int x, y, z;
+      core_coords(a.div_z, a.div_y, a.ny, a.nz, c, x, y, z);
+      V rho = V(1), u[3] = {V(0), V(0), V(0)};
+      if (id != 0) cell_rho_u<Q, S, V>(a, x, y, z, rho, u);
+      if (forced && id == kIdBulk) {
+        u[0] += half_force<V>(force, ncore, c, 0, hfx);
+        u[1] += half_force<V>(force, ncore, c, 1, hfy);
+        u[2] += half_force<V>(force, ncore, c, 2, hfz);
+      }
+      if (field == kFieldDensity) {
+        v[0] = double(rho);
+      } else if (field == kFieldVelocity) {
+        for (int d = 0; d < 3; ++d) v[d] = double(u[d]);
+      } else if (field == kFieldPhysVelocity) {
+        for (int d = 0; d < 3; ++d) v[d] = double(V(conv.velocity) * u[d]);
+      } else {
+        const V lattice_pressure = V((double(rho) - 1.0) / 3.0);
+        if constexpr (sizeof(V) == 4)
+          v[0] = double(__builtin_fmaf(float(conv.pressure), lattice_pressure, float(conv.pressure_offset)));
+        else
+          v[0] = __builtin_fma(conv.pressure, lattice_pressure, conv.pressure_offset);
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) acc[d] = reduce_combine(op, acc[d], v[d]);
+  }
+  __shared__ double lds[4][4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double r = wave_reduce(op, acc[d]);
+    if (lane == 0) lds[d][wave] = r;
+  }
+  const double n = wave_reduce(kReduceSum, count);
+  if (lane == 0) lds[3][wave] = n;
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int k = int(threadIdx.x), kop = k == 3 ? kReduceSum : op;
+    double r = lds[k][0];
+    for (int w = 1; w < 4; ++w) r = reduce_combine(kop, r, lds[k][w]);
+    partial[blockIdx.x * 4 + k] = r;
+  }
+}
+
+// Second level: the partials in a fixed order (deterministic).
+static __global__ __launch_bounds__(256) void field_reduce_partials_kernel(double* out, const double* partial, uint32_t nparts,
+                                                                    int op)
+{
+  __shared__ double lds[4][4];
+  double acc[4] = {reduce_identity(op), reduce_identity(op), reduce_identity(op), 0.0};
+  for (uint32_t p = threadIdx.x; p < nparts; p += blockDim.x)
+    for (int k = 0; k < 4; ++k) acc[k] = reduce_combine(k == 3 ? kReduceSum : op, acc[k], partial[p * 4 + k]);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double r = wave_reduce(k == 3 ? kReduceSum : op, acc[k]);
+    if (lane == 0) lds[k][wave] = r;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int k = int(threadIdx.x), kop = k == 3 ? kReduceSum : op;
+    double r = lds[k][0];
+    for (int w = 1; w < 4; ++w) r = reduce_combine(kop, r, lds[k][w]);
+    out[k] = r;
+  }
+}
+
 }  // namespace lbmh

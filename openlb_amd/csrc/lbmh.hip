@@ -91,6 +91,7 @@ struct lbmh_lattice {
   std::vector<lbmh_halo*> halos_in;  // halos that fill this block's overlap
   std::vector<lbmh_halo*> halos_out; // halos that read this block
   bool id_defined[kMaxWalls] = {};   // ids given a wall or boundary entry
+  uint8_t material_of[kMaxWalls] = {};  // the material (id of lbmh_set_dynamics_ids) each dynamics id stands for
   uint32_t info[kMaxWalls] = {};     // packed LocalInfo per id
   uint32_t* d_info = nullptr;
   bool has_local = false;
@@ -478,7 +479,10 @@ int lbmh_lattice_create(const lbmh_lattice_desc* desc, lbmh_lattice** out)
   // measured best shapes on MI355X for the 256^3 D3Q19 step: FP16 columns
   // want full occupancy, FP32/FP64 three to four workgroups per CU
   L->launch = default_launch(L->storage, false);
-  for (int w = 0; w < kMaxWalls; ++w) L->walls[w][0] = 1.0;
+  for (int w = 0; w < kMaxWalls; ++w) {
+    L->walls[w][0] = 1.0;
+    L->material_of[w] = uint8_t(w);
+  }
 
   DeviceGuard guard(L->device);
   const size_t pop_bytes = size_t(L->stride) * L->q * storage_size(L->storage);
@@ -551,6 +555,7 @@ int lbmh_set_dynamics_ids(lbmh_lattice* L, const uint8_t* ids)
   for (int w = 0; w < kMaxWalls; ++w) {
     L->info[w] = pack_local_info(kMovingWall, 0, 0, 0);
     L->id_defined[w] = false;
+    L->material_of[w] = uint8_t(w);
   }
   LBMH_HIP(hipMemcpy(L->d_info, L->info, sizeof(L->info), hipMemcpyHostToDevice));
   return LBMH_OK;
@@ -684,6 +689,7 @@ int lbmh_set_local_boundary(lbmh_lattice* L, int id, int kind, double rho, const
           if (free_id < 3) return fail(LBMH_E_INVALID, "no free dynamics ids left for the boundary's cell types");
           used[free_id] = true;
           L->info[free_id] = pack_local_info(k, a, b, cc);
+          L->material_of[free_id] = uint8_t(id);
           L->walls[free_id][0] = rho;
           for (int d = 0; d < 3; ++d) L->walls[free_id][d + 1] = u[d];
           L->id_defined[free_id] = true;
@@ -1234,6 +1240,46 @@ int lbmh_field_export(lbmh_lattice* L, int field, int overlap, const double conv
   });
   if (rc != LBMH_OK) return rc;
   LBMH_HIP(hipMemcpy(host, out.p, want * sizeof(float), hipMemcpyDeviceToHost));
+  return LBMH_OK;
+}
+
+int lbmh_reduce_field(lbmh_lattice* L, int field, int op, const uint8_t* materials, const double conversion[3],
+                      double out[3], uint64_t* cells)
+{
+  if (!L || !out) return fail(LBMH_E_INVALID, "null argument");
+  if (field < LBMH_FIELD_DENSITY || field > LBMH_FIELD_FORCE) return fail(LBMH_E_INVALID, "unknown field");
+  if (op < LBMH_REDUCE_SUM || op > LBMH_REDUCE_MAX) return fail(LBMH_E_INVALID, "unknown reduction");
+  if ((field == LBMH_FIELD_PHYS_VELOCITY || field == LBMH_FIELD_PHYS_PRESSURE) && !conversion)
+    return fail(LBMH_E_INVALID, "physical fields need the unit conversion factors");
+  IdMask mask{};
+  for (int id = 0; id < 256; ++id) {
+    const int m = L->material_of[id];
+    if (materials ? materials[m] != 0 : m != 0) mask.bits[id >> 5] |= 1u << (id & 31);
+  }
+  FieldConversion conv{0.0, 0.0, 0.0};
+  if (conversion) conv = FieldConversion{conversion[0], conversion[1], conversion[2]};
+  const int forced = (L->kind == LBMH_FORCED_BGK || L->kind == LBMH_FORCED_TRT) ? 1 : 0;
+  const double* F = L->params.force;
+  DeviceGuard guard(L->device);
+  const uint32_t blocks = std::min<uint32_t>(L->n_partials, grid_for(L->ncore, 256));
+  LBMH_HIP(hipDeviceSynchronize());
+  const int rc = with_types(L, [&](auto qc, auto s) -> int {
+    constexpr int Q = decltype(qc)::value;
+    using S = decltype(s);
+    hipLaunchKernelGGL((field_reduce_kernel<Q, S>), dim3(blocks), dim3(256), 0, 0, post_args<Q>(L), field, op, conv,
+                       forced, F[0] * 0.5, F[1] * 0.5, F[2] * 0.5, force_field<S>(L), mask, L->d_partials);
+    LBMH_HIP(hipGetLastError());
+    return LBMH_OK;
+  });
+  if (rc != LBMH_OK) return rc;
+  double* result = L->d_partials + 4 * L->n_partials;
+  hipLaunchKernelGGL(field_reduce_partials_kernel, dim3(1), dim3(256), 0, 0, result, L->d_partials, blocks, op);
+  LBMH_HIP(hipGetLastError());
+  double r[4];
+  LBMH_HIP(hipMemcpy(r, result, sizeof(r), hipMemcpyDeviceToHost));
+  const int comps = (field == LBMH_FIELD_VELOCITY || field == LBMH_FIELD_PHYS_VELOCITY || field == LBMH_FIELD_FORCE) ? 3 : 1;
+  for (int d = 0; d < comps; ++d) out[d] = r[d];
+  if (cells) *cells = uint64_t(r[3]);
   return LBMH_OK;
 }
 

@@ -218,6 +218,35 @@ class BlockLattice:
         """Read `<file_name>_rank0_size1.dat`, as OpenLB's load(fileName) does."""
         self.restore(checkpoint.read_file(checkpoint.parallel_file_name(file_name), enforce_uint), real)
 
+    REDUCE_FIELDS = {"density": 0, "velocity": 1, "physVelocity": 2, "physPressure": 3, "force": 4}
+    REDUCE_OPS = {"sum": 0, "min": 1, "max": 2}
+
+    def reduce_field(self, field: str = "density", op: str = "sum", materials=None, conversion=None):
+        """(values, cells): the sum, minimum or maximum of a cell field over the
+        core cells of the `materials` (the ids given to define_dynamics; cells a
+        local boundary relabels keep their material; default: every material
+        but 0), in
+        FP64, as OpenLB's SuperLatticeFieldReductionO with SumO / MinO / MaxO
+        (src/core/superLatticeFieldReductionO.h:41-160) or SuperSum3D /
+        SuperMin3D / SuperMax3D over a material indicator compute it; the
+        average of SuperAverage3D is values / cells.  `conversion` = the
+        {velocity, pressure, charPhysPressure} factors of the physical fields
+        (vtk.UnitConverter.factors())."""
+        mask = None
+        if materials is not None:
+            mask = np.zeros(256, dtype=np.uint8)
+            for m in materials:
+                mask[int(m)] = 1
+        conv = None if conversion is None else np.ascontiguousarray(conversion, dtype=np.float64)
+        out = np.zeros(3)
+        cells = ctypes.c_uint64(0)
+        abi.check(abi.load().lbmh_reduce_field(self._h, self.REDUCE_FIELDS[field], self.REDUCE_OPS[op],
+                                               None if mask is None else _ptr(mask),
+                                               None if conv is None else _ptr(conv), _ptr(out),
+                                               ctypes.byref(cells)))
+        comps = 3 if field in ("velocity", "physVelocity", "force") else 1
+        return out[:comps].copy(), int(cells.value)
+
     def moments(self) -> np.ndarray:
         out = np.empty((4,) + self.shape)
         abi.check(abi.load().lbmh_moments(self._h, _ptr(out)))
