@@ -425,8 +425,11 @@ int lbmh_ensemble_add(lbmh_ensemble* ensemble, const int extent[3], const int pe
 int lbmh_ensemble_remove(lbmh_ensemble* ensemble, int member);
 int lbmh_ensemble_set_wall(lbmh_ensemble* ensemble, int member, int id, double rho, const double u[3]);
 /* Local velocity / pressure boundary on the member's cells carrying `id`, as
- * lbmh_set_local_boundary does for a lattice; the per-type ids come from the
- * member's free ids 3..63.  Needs an LBMH_BGK ensemble. */
+ * lbmh_set_local_boundary does for a lattice (every kind: LocalVelocity,
+ * LocalPressure, InterpolatedVelocity, InterpolatedPressure); the per-type
+ * ids come from the member's free ids 3..63.  Needs an LBMH_BGK ensemble.
+ * The first InterpolatedPressure member allocates the ensemble's plane
+ * workspace (boundary capacity x Q values); later members reuse it. */
 int lbmh_ensemble_set_local_boundary(lbmh_ensemble* ensemble, int member, int id, int kind, double rho,
                                      const double u[3]);
 /* initialize() of one member: its edge and corner post-processors once. */

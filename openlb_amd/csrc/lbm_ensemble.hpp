@@ -133,14 +133,20 @@ __global__ __launch_bounds__(64) void ensemble_local_kernel(const EnsembleEntry<
 
 // Plane, edge or corner post-processors (KIND) of the members that stepped
 // last (or of member `only` when only >= 0: initialize()).  Eight lanes per
-// (member, cell) entry, as outer_post_kernel.
-template <int Q, typename S, int KIND>
+// (member, cell) entry, as outer_post_kernel.  The SCRATCH plane pass
+// computes entry k's populations into scratch[k * Q ...] and
+// ensemble_plane_store_kernel writes them afterwards: pressure faces read the
+// velocity of their face neighbours from populations the pass rewrites
+// (plane_scratch_kernel / plane_store_kernel of a lattice).  The two are
+// separate instantiations, as the lattice's kernels are, so that each
+// compiles the reconstruction exactly as the lattice does.
+template <int Q, typename S, int KIND, bool SCRATCH = false>
 __global__ __launch_bounds__(64) void ensemble_post_kernel(const EnsembleEntry<Q>* __restrict__ table,
                                                            const uint2* __restrict__ cells, uint32_t count,
                                                            S* __restrict__ arena, const uint8_t* __restrict__ ids,
                                                            const double* __restrict__ walls,
                                                            const uint32_t* __restrict__ info, EnsembleLimits limits,
-                                                           int only)
+                                                           int only, typename StorageTraits<S>::V* scratch = nullptr)
 {
   using V = typename StorageTraits<S>::V;
   __shared__ V nb[64 / kPostLanes][kPostSlots][4];
@@ -181,7 +187,37 @@ __global__ __launch_bounds__(64) void ensemble_post_kernel(const EnsembleEntry<Q
   }
   if (valid && slot < kPostSlots) post_gather<Q, S, KIND>(a, lc.y, slot, nb[j][slot]);
   __syncthreads();
-  if (valid && slot == 0) post_finish<Q, S, KIND>(a, lc.y, int(a.ids[lc.y]), nb[j]);
+  if (valid && slot == 0) {
+    if constexpr (SCRATCH)
+      post_finish<Q, S, KIND>(a, lc.y, int(a.ids[lc.y]), nb[j], scratch + uint64_t(k) * Q);
+    else
+      post_finish<Q, S, KIND>(a, lc.y, int(a.ids[lc.y]), nb[j]);
+  }
+}
+
+// Second half of a two-phase plane pass: entry k's populations from scratch.
+template <int Q, typename S>
+__global__ __launch_bounds__(64) void ensemble_plane_store_kernel(const EnsembleEntry<Q>* __restrict__ table,
+                                                                  const uint2* __restrict__ cells, uint32_t count,
+                                                                  S* __restrict__ arena,
+                                                                  const typename StorageTraits<S>::V* scratch,
+                                                                  EnsembleLimits limits, int only)
+{
+  using V = typename StorageTraits<S>::V;
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const uint2 lc = cells[k];
+  if (lc.x >= limits.members) return;
+  const EnsembleEntry<Q>& e = table[lc.x];
+  if (!((only >= 0 ? int(lc.x) == only : e.stepped != 0) && entry_fits(e, limits) && lc.y < e.ncore)) return;
+  int x, y, z;
+  core_coords(e.div_z, e.div_y, int(e.ny), int(e.nz), lc.y, x, y, z);
+  const uint32_t n = uint32_t(x + 1) * e.sx + uint32_t(y + 1) * e.sy + uint32_t(z + 1);
+  S* pop = arena + e.pop_offset;
+#pragma unroll
+  for (int i = 0; i < Q; ++i)
+    store_pop<S, V>(column_at(pop + uint64_t(i) * e.stride, wrap_index(n, e.shift[i], e.n_padded)),
+                    scratch[uint64_t(k) * Q + i]);
 }
 
 // Streaming of every lattice that stepped: rotate its columns, count the step.

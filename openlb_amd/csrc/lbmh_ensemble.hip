@@ -91,6 +91,7 @@ struct Member {
   uint32_t info[kEnsembleWalls] = {};
   bool id_defined[kEnsembleWalls] = {};
   std::vector<uint32_t> local, edges, corners, planes;
+  bool pressure_planes = false;  // InterpolatedPressure faces: its planes run the two-phase pass
 };
 
 }  // namespace
@@ -120,9 +121,13 @@ struct lbmh_ensemble {
   uint2* d_local = nullptr;
   uint2* d_edges = nullptr;
   uint2* d_corners = nullptr;
-  uint2* d_planes = nullptr;
+  uint2* d_planes = nullptr;    // plane cells of members without pressure planes: one pass
+  uint2* d_pplanes = nullptr;   // plane cells of members with pressure planes: two-phase pass
   size_t boundary_capacity = 0;
-  uint32_t n_local = 0, n_edges = 0, n_corners = 0, n_planes = 0;
+  // the two-phase plane pass's populations, boundary_capacity x Q values,
+  // allocated with the first member that has InterpolatedPressure faces
+  void* d_plane_scratch = nullptr;
+  uint32_t n_local = 0, n_edges = 0, n_corners = 0, n_planes = 0, n_pplanes = 0;
   bool has_local = false;            // some member ever had local boundaries
 };
 
@@ -199,23 +204,26 @@ int check_member(const lbmh_ensemble* E, int m)
 // Concatenate the local-boundary lists of the active members into the device lists.
 int rebuild_lists(lbmh_ensemble* E)
 {
-  std::vector<uint2> all[4];
+  std::vector<uint2> all[5];
   for (int m = 0; m < E->max_members; ++m) {
     const Member& mem = E->members[m];
     if (!mem.active) continue;
     const std::vector<uint32_t>* src[4] = {&mem.local, &mem.edges, &mem.corners, &mem.planes};
-    for (int k = 0; k < 4; ++k)
-      for (uint32_t c : *src[k]) all[k].push_back(make_uint2(uint32_t(m), c));
+    for (int k = 0; k < 4; ++k) {
+      const int to = (k == 3 && mem.pressure_planes) ? 4 : k;
+      for (uint32_t c : *src[k]) all[to].push_back(make_uint2(uint32_t(m), c));
+    }
   }
-  for (int k = 0; k < 4; ++k)
+  for (int k = 0; k < 5; ++k)
     if (all[k].size() > E->boundary_capacity) return fail(LBMH_E_MEMORY, "boundary list capacity exceeded");
-  uint2* dst[4] = {E->d_local, E->d_edges, E->d_corners, E->d_planes};
-  for (int k = 0; k < 4; ++k)
+  uint2* dst[5] = {E->d_local, E->d_edges, E->d_corners, E->d_planes, E->d_pplanes};
+  for (int k = 0; k < 5; ++k)
     if (!all[k].empty()) LBMH_HIP(hipMemcpy(dst[k], all[k].data(), all[k].size() * sizeof(uint2), hipMemcpyHostToDevice));
   E->n_local = uint32_t(all[0].size());
   E->n_edges = uint32_t(all[1].size());
   E->n_corners = uint32_t(all[2].size());
   E->n_planes = uint32_t(all[3].size());
+  E->n_pplanes = uint32_t(all[4].size());
   E->version += 1;
   return LBMH_OK;
 }
@@ -254,6 +262,18 @@ int launch_post(lbmh_ensemble* E, const EnsembleEntry<Q>* table, S* arena, const
   if (E->n_planes) {
     hipLaunchKernelGGL((ensemble_post_kernel<Q, S, kPostPlane>), dim3(grid_for(E->n_planes, 64 / kPostLanes)), dim3(64),
                        0, stream, table, E->d_planes, E->n_planes, arena, E->d_ids, E->d_walls, E->d_info, limits, only);
+    LBMH_HIP(hipGetLastError());
+  }
+  if (E->n_pplanes) {
+    // members with pressure faces: every plane cell into the workspace, then stored
+    using V = typename StorageTraits<S>::V;
+    V* scratch = static_cast<V*>(E->d_plane_scratch);
+    hipLaunchKernelGGL((ensemble_post_kernel<Q, S, kPostPlane, true>), dim3(grid_for(E->n_pplanes, 64 / kPostLanes)),
+                       dim3(64), 0, stream, table, E->d_pplanes, E->n_pplanes, arena, E->d_ids, E->d_walls, E->d_info,
+                       limits, only, scratch);
+    LBMH_HIP(hipGetLastError());
+    hipLaunchKernelGGL((ensemble_plane_store_kernel<Q, S>), dim3(grid_for(E->n_pplanes, 64)), dim3(64), 0, stream,
+                       table, E->d_pplanes, E->n_pplanes, arena, static_cast<const V*>(scratch), limits, only);
     LBMH_HIP(hipGetLastError());
   }
   if (E->n_edges) {
@@ -364,7 +384,7 @@ int lbmh_ensemble_create(const lbmh_ensemble_desc* d, lbmh_ensemble** out)
   E->boundary_capacity = d->boundary_capacity ? d->boundary_capacity : d->cell_capacity / 4 + 64;
   if (e == hipSuccess) e = hipMalloc((void**)&E->d_info, sizeof(uint32_t) * kEnsembleWalls * E->max_members);
   if (e == hipSuccess) e = hipMemset(E->d_info, 0, sizeof(uint32_t) * kEnsembleWalls * E->max_members);
-  for (uint2** list : {&E->d_local, &E->d_edges, &E->d_corners, &E->d_planes})
+  for (uint2** list : {&E->d_local, &E->d_edges, &E->d_corners, &E->d_planes, &E->d_pplanes})
     if (e == hipSuccess) e = hipMalloc((void**)list, sizeof(uint2) * E->boundary_capacity);
   if (e == hipSuccess) e = hipMemcpy(E->d_table, E->h_table.data(), E->h_table.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&E->capture_stream, hipStreamNonBlocking);
@@ -394,6 +414,8 @@ int lbmh_ensemble_destroy(lbmh_ensemble* E)
   if (E->d_edges) (void)hipFree(E->d_edges);
   if (E->d_corners) (void)hipFree(E->d_corners);
   if (E->d_planes) (void)hipFree(E->d_planes);
+  if (E->d_pplanes) (void)hipFree(E->d_pplanes);
+  if (E->d_plane_scratch) (void)hipFree(E->d_plane_scratch);
   delete E;
   return LBMH_OK;
 }
@@ -554,11 +576,11 @@ int lbmh_ensemble_set_local_boundary(lbmh_ensemble* E, int m, int id, int kind, 
   int rc = check_member(E, m);
   if (rc != LBMH_OK) return rc;
   if (!u) return fail(LBMH_E_INVALID, "null velocity");
-  if (kind < LBMH_LOCAL_VELOCITY || kind > LBMH_INTERP_VELOCITY)
-    return fail(LBMH_E_INVALID, "ensembles support LocalVelocity, LocalPressure and InterpolatedVelocity boundaries");
+  if (kind < LBMH_LOCAL_VELOCITY || kind > LBMH_INTERP_PRESSURE) return fail(LBMH_E_INVALID, "unknown local boundary kind");
   if (id < 2 || id >= kEnsembleWalls) return fail(LBMH_E_INVALID, "ensemble boundary id must be in [2, 63]");
   if (E->kind != LBMH_BGK) return fail(LBMH_E_INVALID, "local boundaries need BGK bulk dynamics");
-  if (kind == LBMH_LOCAL_PRESSURE && !(rho > 0)) return fail(LBMH_E_INVALID, "pressure boundary density must be positive");
+  if ((kind == LBMH_LOCAL_PRESSURE || kind == LBMH_INTERP_PRESSURE) && !(rho > 0))
+    return fail(LBMH_E_INVALID, "pressure boundary density must be positive");
   Member& mem = E->members[m];
   const int n[3] = {mem.n[0], mem.n[1], mem.n[2]};
   bool used[kEnsembleWalls] = {};
@@ -604,14 +626,19 @@ int lbmh_ensemble_set_local_boundary(lbmh_ensemble* E, int m, int id, int kind, 
           it = entry_ids.emplace(key, free_id).first;
         }
         next[c] = uint8_t(it->second);
-        if (k != kFlatVelocity && k != kFlatPressure && k != kFdVelocity) cells.push_back(c);
-        if (k == kFdVelocity) planes.push_back(c);
+        if (k != kFlatVelocity && k != kFlatPressure && k != kFdVelocity && k != kFdPressure) cells.push_back(c);
+        if (k == kFdVelocity || k == kFdPressure) planes.push_back(c);
         if (k == kOuterEdge) edges.push_back(c);
         if (k == kOuterCorner) corners.push_back(c);
       }
   if (entry_ids.empty()) return LBMH_OK;
   DeviceGuard guard(E->device);
   LBMH_HIP(hipDeviceSynchronize());
+  if (kind == LBMH_INTERP_PRESSURE && !planes.empty() && !E->d_plane_scratch) {
+    const size_t elem = E->storage == LBMH_F64 ? sizeof(double) : sizeof(float);
+    LBMH_HIP(hipMalloc(&E->d_plane_scratch, elem * size_t(E->q) * E->boundary_capacity));
+  }
+  if (kind == LBMH_INTERP_PRESSURE && !planes.empty()) mem.pressure_planes = true;
   for (const auto& kv : entry_ids) {
     const int w = kv.second;
     mem.id_defined[w] = true;
@@ -634,7 +661,7 @@ int lbmh_ensemble_initialize(lbmh_ensemble* E, int m)
 {
   int rc = check_member(E, m);
   if (rc != LBMH_OK) return rc;
-  if (!E->n_edges && !E->n_corners && !E->n_planes) return LBMH_OK;
+  if (!E->n_edges && !E->n_corners && !E->n_planes && !E->n_pplanes) return LBMH_OK;
   DeviceGuard guard(E->device);
   rc = with_types(E, [&](auto qc, auto s) -> int {
     constexpr int Q = decltype(qc)::value;

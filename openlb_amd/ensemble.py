@@ -97,7 +97,8 @@ class Ensemble:
             for mid, kind in local.items():
                 rho, u = case.walls.get(mid, (1.0, (0.0, 0.0, 0.0)))
                 abi.check(abi.load().lbmh_ensemble_set_local_boundary(
-                    self._h, m.value, int(mid), {"velocity": 0, "pressure": 1, "interp_velocity": 2}[kind],
+                    self._h, m.value, int(mid),
+                    {"velocity": 0, "pressure": 1, "interp_velocity": 2, "interp_pressure": 3}[kind],
                     float(rho), (ctypes.c_double * 3)(*map(float, u))))
         except BaseException:
             # a half-configured member must not keep its slot and arena space

@@ -188,15 +188,18 @@ def test_statistics_flag_a_diverged_lattice(gpu_available):
 
 @pytest.mark.parametrize("storage", ["f64", "f32"])
 def test_local_boundary_members_match_standalone(gpu_available, storage):
-    """Members with LocalVelocity / LocalPressure boundaries (edge and corner
-    post-processors included) beside bounce-back members: each is bit-identical
+    """Members with LocalVelocity / LocalPressure / InterpolatedVelocity /
+    InterpolatedPressure boundaries (edge, corner and plane post-processors
+    included) beside bounce-back members: each is bit-identical
     to the same lattice stepped alone, in eager and graph steps, and member 0's
     initialize() equals the lattice's."""
     cases = [W.local_velocity_cavity((9, 8, 11), tau=0.6, lid_u=0.07),
              W.lid_driven_cavity((7, 9, 6), tau=0.7, lid_u=0.05),
              W.velocity_channel((16, 8, 7), u_in=0.03, tau=0.65),
              W.local_velocity_cavity((6, 7, 5), tau=0.8, lid_u=0.1),
-             W.local_velocity_cavity((8, 6, 7), tau=0.7, lid_u=0.06, boundary="interp_velocity")]
+             W.local_velocity_cavity((8, 6, 7), tau=0.7, lid_u=0.06, boundary="interp_velocity"),
+             W.velocity_channel((14, 7, 6), u_in=0.03, rho_out=1.0, tau=0.6, inlet="interp_velocity",
+                                outlet="interp_pressure")]
     with Ensemble(19, storage, "bgk", max_members=8, population_bytes=64 << 20, cell_capacity=1 << 20) as ens:
         members, f0s = [], []
         for c in cases:
@@ -213,7 +216,7 @@ def test_local_boundary_members_match_standalone(gpu_available, storage):
         ens.step(32, graph=True)
         for m, c, f0 in zip(members, cases, f0s):
             assert ens.status(m) == (41, -1)
-            np.testing.assert_array_equal(ens.download(m), standalone(c, storage, f0, 41))
+            np.testing.assert_array_equal(ens.download(m), standalone(c, storage, f0, 41), err_msg=f"{m} {c.name}")
 
 
 def test_autotune_leaves_the_members_as_they_were(gpu_available):

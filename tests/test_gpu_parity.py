@@ -107,6 +107,40 @@ def test_gpu_tgv64_fp64_1000_steps(gpu_available):
     assert err_o <= 1e-12
 
 
+def test_gpu_turbulent_channel_fp64_matches_oracle_at_size(gpu_available):
+    """Config 3's flow (D3Q27 Smagorinsky-BGK, tau 0.505, C 0.14, periodic in x
+    and z between BounceBack walls, laminar profile with noise) at 96x48x48
+    for 400 steps: the FP64 device path against the oracle on the bulk cells,
+    at the 1e-12 bound of the fixture-size pin.  FP32 storage stays within its
+    2e-5 population bound of the oracle's FP64 run, and the FP16 velocity
+    error against that run is recorded."""
+    case = W.turbulent_channel((96, 48, 48), noise=0.2, seed=11)
+    f0 = O.initial_populations(case, "f64")
+    steps = 400
+    fluid = case.ids == 1
+    got = run_gpu(case, "f64", f0, [steps])[steps]
+    ora = O.OracleLattice(case, "f64")
+    ora.upload(f0)
+    ora.step(steps, threads=16)
+    want = ora.download()
+    err = G.rel_error(got, want, 27, fluid)
+    print(f"turbulent channel 96x48x48 FP64, {steps} steps vs oracle: rel {err:.3e}", flush=True)
+    assert err <= 1e-12
+    ref = O.OracleLattice(case, "f64")
+    ref.upload(want)
+    u_ref = ref.moments()[1:, fluid]
+    umax = float(np.abs(u_ref).max())
+    for storage in ("f32", "f16"):
+        f = run_gpu(case, storage, O.initial_populations(case, storage), [steps])[steps]
+        m = O.OracleLattice(case, "f64")
+        m.upload(f)
+        du = float(np.abs(m.moments()[1:, fluid] - u_ref).max()) / umax
+        perr = G.rel_error(f, want, 27, fluid)
+        print(f"  {storage}: max |du| / u_max {du:.2e}, populations rel {perr:.2e}", flush=True)
+        if storage == "f32":
+            assert perr <= 2e-5
+
+
 RAGGED = [
     ("cell1_periodic", lambda: W.random_periodic((1, 1, 1), seed=1, tau=0.7)),
     ("line_periodic", lambda: W.random_periodic((1, 3, 67), seed=2, tau=0.8)),
