@@ -160,10 +160,13 @@ private:
       }
     }
 
-    // walls and local velocity boundaries, grouped by their values
+    // walls and local boundaries, grouped by their values
     std::vector<std::uint8_t> ids(ncore, 0);
     std::map<std::array<T,4>, int> walls;     // (rho, u) -> id
-    std::map<std::array<T,3>, int> velocity;  // u -> material id of a local velocity group
+    std::map<std::array<T,3>, int> velocity;  // u -> material id of a velocity group (faces, edges, corners)
+    std::map<int, int> velocityKind;          // velocity group -> LBMH_LOCAL_VELOCITY / LBMH_INTERP_VELOCITY
+    // (rho, u with its normal component 0) -> material id; [0] LocalPressure, [1] InterpolatedPressure faces
+    std::map<std::array<T,4>, int> pressure[2];
     int next = LBMH_ID_FIRST_WALL;
     auto take = [&]() {
       if (next > 200) {
@@ -200,7 +203,30 @@ private:
             ids[c] = std::uint8_t(it->second);
             break;
           }
-          default: {  // the LocalVelocity family
+          case Role::PressureFlat:
+          case Role::InterpPressureFlat: {
+            // fixed density and the tangential velocity; the normal component
+            // comes from the populations every step
+            auto cell = _block.get(x, y, z);
+            T u[3] = {};
+            cell.computeU(u);
+            const auto& nrm = role[c]->info.normal;
+            for (int d = 0; d < 3; ++d) {
+              if (nrm[d] != 0) {
+                u[d] = T(0);
+              }
+            }
+            const std::array<T,4> key{cell.computeRho(), u[0], u[1], u[2]};
+            auto& groups = pressure[r == Role::InterpPressureFlat ? 1 : 0];
+            auto it = groups.find(key);
+            if (it == groups.end()) {
+              it = groups.emplace(key, take()).first;
+            }
+            ids[c] = std::uint8_t(it->second);
+            local = true;
+            break;
+          }
+          default: {  // the velocity family: faces, outer / inner edges and corners
             auto cell = _block.get(x, y, z);
             T u[3] = {};
             cell.computeU(u);
@@ -210,6 +236,15 @@ private:
               it = velocity.emplace(key, take()).first;
             }
             ids[c] = std::uint8_t(it->second);
+            if (r == Role::VelocityFlat || r == Role::InterpVelocityFlat) {
+              // the faces decide whether the group is local or interpolated
+              const int kind = r == Role::InterpVelocityFlat ? LBMH_INTERP_VELOCITY : LBMH_LOCAL_VELOCITY;
+              auto k = velocityKind.emplace(it->second, kind).first;
+              if (k->second != kind) {
+                throw std::runtime_error("GPU_HIP: LocalVelocity and InterpolatedVelocity faces with the same "
+                                         "velocity share edges; give them different velocities or blocks");
+              }
+            }
             local = true;
             break;
           }
@@ -230,7 +265,16 @@ private:
     }
     for (const auto& [key, id] : velocity) {
       const double u[3] = {double(key[0]), double(key[1]), double(key[2])};
-      check(lbmh_set_local_boundary(_lattice, id, LBMH_LOCAL_VELOCITY, 1.0, u), "lbmh_set_local_boundary");
+      auto k = velocityKind.find(id);
+      const int kind = k == velocityKind.end() ? LBMH_LOCAL_VELOCITY : k->second;
+      check(lbmh_set_local_boundary(_lattice, id, kind, 1.0, u), "lbmh_set_local_boundary");
+    }
+    for (int interp = 0; interp < 2; ++interp) {
+      for (const auto& [key, id] : pressure[interp]) {
+        const double u[3] = {double(key[1]), double(key[2]), double(key[3])};
+        check(lbmh_set_local_boundary(_lattice, id, interp ? LBMH_INTERP_PRESSURE : LBMH_LOCAL_PRESSURE,
+                                      double(key[0]), u), "lbmh_set_local_boundary");
+      }
     }
     readParameters(bulk, role);
     _structureChanged = _valuesChanged = false;
@@ -267,7 +311,8 @@ private:
           const std::array<int,3> n{cls[ncore + c], cls[2 * ncore + c], cls[3 * ncore + c]};
           int want = -1;
           std::array<int,3> normal = info.normal;
-          if (info.role == Role::VelocityFlat) {
+          if (info.role == Role::VelocityFlat || info.role == Role::PressureFlat ||
+              info.role == Role::InterpVelocityFlat || info.role == Role::InterpPressureFlat) {
             want = kFlat;
           } else if (info.role == Role::VelocityInnerCorner) {
             want = kInternalCorner;

@@ -2,8 +2,10 @@
  *
  * The device lattice runs a fixed set of cell behaviours (include/lbmh.h):
  * one bulk collision per block (BGK, Guo-forced BGK, TRT, forced TRT,
- * Smagorinsky-BGK), NoDynamics, BounceBack, BounceBackVelocity walls and the
- * LocalVelocity family of boundaries.  Each OpenLB type is mapped here, from
+ * Smagorinsky-BGK), NoDynamics, BounceBack, BounceBackVelocity walls, the
+ * LocalVelocity family of boundaries, LocalPressure faces and the
+ * InterpolatedVelocity / InterpolatedPressure faces with their plane
+ * finite-difference post-processors.  Each OpenLB type is mapped here, from
  * its template arguments, to one of those roles; any other type is
  * Role::Unsupported and a block holding it refuses to simulate.
  */
@@ -26,7 +28,10 @@ enum class Role : std::uint8_t {
   VelocityFlat,     // LocalVelocity face               src/boundary/localVelocity3D.h:49-80
   VelocityFixed,    // LocalVelocity outer edge/corner  localVelocity3D.h:82-85, 92-95
   VelocityInnerCorner,  //                              localVelocity3D.h:87-90
-  VelocityInnerEdge     //                              localVelocity3D.h:97-100
+  VelocityInnerEdge,    //                              localVelocity3D.h:97-100
+  PressureFlat,         // LocalPressure face               src/boundary/localPressure3D.h:44-55
+  InterpVelocityFlat,   // InterpolatedVelocity face        src/boundary/interpolatedVelocity3D.h:44-48
+  InterpPressureFlat    // InterpolatedPressure face        src/boundary/interpolatedPressure3D.h:44-66
 };
 
 /// Boundary types in the numbering of lbmh_classify_boundary
@@ -115,6 +120,40 @@ struct RoleOf<T,DESCRIPTOR,CombinedRLBdynamics<T,DESCRIPTOR,BGKdynamics<T,DESCRI
   static constexpr RoleInfo info() { return {Role::VelocityInnerEdge, -1, edgeNormal(plane, normal1, normal2)}; }
 };
 
+/// LocalPressure faces: CombinedRLBdynamics with the regularized pressure momenta
+template <typename T, typename DESCRIPTOR, int direction, int orientation>
+struct RoleOf<T,DESCRIPTOR,CombinedRLBdynamics<T,DESCRIPTOR,BGKdynamics<T,DESCRIPTOR>,
+                                               momenta::RegularizedPressureBoundaryTuple<direction,orientation>>> {
+  static constexpr RoleInfo info()
+  {
+    RoleInfo r{Role::PressureFlat};
+    r.normal[direction] = orientation;
+    return r;
+  }
+};
+
+/// Interpolated faces: BGK with the Dirichlet velocity / pressure momenta
+/// (the mixin's exchange_momenta), completed by a PlaneFdBoundaryProcessor3D
+template <typename T, typename DESCRIPTOR, int direction, int orientation>
+struct RoleOf<T,DESCRIPTOR,BGKdynamics<T,DESCRIPTOR,momenta::BasicDirichletVelocityBoundaryTuple<direction,orientation>>> {
+  static constexpr RoleInfo info()
+  {
+    RoleInfo r{Role::InterpVelocityFlat};
+    r.normal[direction] = orientation;
+    return r;
+  }
+};
+
+template <typename T, typename DESCRIPTOR, int direction, int orientation>
+struct RoleOf<T,DESCRIPTOR,BGKdynamics<T,DESCRIPTOR,momenta::BasicDirichletPressureBoundaryTuple<direction,orientation>>> {
+  static constexpr RoleInfo info()
+  {
+    RoleInfo r{Role::InterpPressureFlat};
+    r.normal[direction] = orientation;
+    return r;
+  }
+};
+
 /// Post-processors the device step runs itself
 struct PostInfo {
   bool handled = false;
@@ -140,6 +179,18 @@ struct PostOf<OuterVelocityEdgeProcessor3D<T,DESCRIPTOR,plane,normal1,normal2>> 
 template <typename T, typename DESCRIPTOR, int x, int y, int z>
 struct PostOf<OuterVelocityCornerProcessor3D<T,DESCRIPTOR,x,y,z>> {
   static constexpr PostInfo info() { return {true, kExternalCorner, {x, y, z}}; }
+};
+
+/// The plane finite-difference processor of the interpolated faces
+/// (src/boundary/boundaryPostProcessors3D.h:41-70)
+template <typename T, typename DESCRIPTOR, int direction, int orientation>
+struct PostOf<PlaneFdBoundaryProcessor3D<T,DESCRIPTOR,direction,orientation>> {
+  static constexpr PostInfo info()
+  {
+    PostInfo p{true, kFlat};
+    p.normal[direction] = orientation;
+    return p;
+  }
 };
 
 }

@@ -70,3 +70,38 @@ def test_float_run_as_close_to_double_as_openlbs_own(gpu_available, runs):
         cpu = max_err(runs["cavity3d_cpu"][f], ref)
         print(f"f32 {f}: |gpu_hip - cpu f64| = {gpu:.3e}, |cpu f32 - cpu f64| = {cpu:.3e}", flush=True)
         assert gpu <= 2.0 * cpu + 1e-7
+
+
+DUCT = {k: os.path.join(BUILD, k) for k in ("duct3d_gpu_hip", "duct3d_cpu")}
+
+
+@pytest.mark.skipif(not all(os.path.exists(b) for b in DUCT.values()),
+                    reason="duct3d binaries not built (make -C integration/openlb)")
+@pytest.mark.parametrize("family", ["local", "interpolated"])
+def test_duct_boundaries_match_openlb(gpu_available, family, tmp_path):
+    """integration/openlb/examples/duct3d.cpp, an OpenLB program of this
+    repository (T = double): a duct with BounceBack walls, a velocity inlet and
+    a pressure outlet, LocalVelocity + LocalPressure or InterpolatedVelocity +
+    InterpolatedPressure (the plane finite-difference post-processors), built
+    on CPU_SISD and on GPU_HIP.  The GPU_HIP run's physical velocity and
+    pressure equal the CPU run's to the float precision of the VTK output."""
+    out = {}
+    for name, binary in DUCT.items():
+        cwd = tmp_path / name
+        cwd.mkdir()
+        cmd = [binary, "--steps", "300"] + (["--interpolated"] if family == "interpolated" else [])
+        print("running", " ".join(cmd), flush=True)
+        p = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True, timeout=240)
+        assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+        out[name] = {f: vtk.read_vti_arrays(str(cwd / "tmp" / "vtkData" / "data" / f"{f}_iT0000000iC00000.vti"))[f]
+                     for f in FIELDS}
+        line = [ln for ln in p.stdout.splitlines() if ln.count(",") == 5][-1]
+        out[name]["platform"] = int(line.split(",")[5])
+    # the GPU_HIP build runs its block on another platform than CPU_SISD
+    assert out["duct3d_gpu_hip"]["platform"] != out["duct3d_cpu"]["platform"]
+    for f in FIELDS:
+        a, b = out["duct3d_gpu_hip"][f], out["duct3d_cpu"][f]
+        scale = float(np.abs(b).max())
+        err = max_err(a, b)
+        print(f"duct {family} {f}: max |gpu_hip - cpu| = {err:.3e} of max {scale:.3e}", flush=True)
+        assert scale > 0 and err <= 1e-6 * scale
