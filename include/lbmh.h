@@ -151,7 +151,9 @@ int lbmh_step(lbmh_lattice* lattice, int64_t n_steps, void* stream);
 /* The two halves of lbmh_step for blocks that exchange overlap cells with
  * other blocks: lbmh_collide launches the collision (and the periodic
  * self-overlap refresh) without streaming; lbmh_stream shifts every column
- * and runs the post-processors.  SuperLattice::collideAndStream collides all
+ * and runs the post-processors (for blocks whose post-processors are joined,
+ * once the last of them has streamed: lbmh_set_overlap_ids).
+ * SuperLattice::collideAndStream collides all
  * blocks, communicates, then streams all blocks (src/core/superLattice.hh:
  * 601-663); one lbmh_step equals lbmh_collide followed by lbmh_stream. */
 int lbmh_collide(lbmh_lattice* lattice, void* stream);
@@ -318,9 +320,43 @@ int lbmh_set_local_boundary(lbmh_lattice* lattice, int id, int kind, double rho,
  * (src/geometry/discreteNormals.hh:174-1190). */
 int lbmh_classify_boundary(const int extent[3], const int periodic[3], const uint8_t* ids, int8_t* out);
 
+/* The same classification for one block of a split lattice: beyond each
+ * non-periodic face lies the block's first overlap layer, whose materials
+ * `ring` gives as a padded map uint8 [nx+2][ny+2][nz+2] (core entries
+ * unused): the ids of the neighbouring blocks' cells there, 0 where the
+ * domain ends.  computeBoundaryTypeAndNormal sees the same on a block
+ * geometry whose overlap holds the neighbours' materials
+ * (src/boundary/setBoundary.h:83-101).  Host only. */
+int lbmh_classify_boundary_overlap(const int extent[3], const int periodic[3], const uint8_t* ids,
+                                   const uint8_t* ring, int8_t* out);
+
+/* Materials of the first overlap layer of a block of a split lattice, as
+ * lbmh_classify_boundary_overlap takes them.  lbmh_set_local_boundary then
+ * classifies the block's boundary cells with them, and reads every
+ * post-processor neighbour beyond a non-periodic face from the block that
+ * owns it: the halos into this block name the owner of the first-layer cell
+ * on the way to the neighbour, and the rest of the offset is taken inside
+ * that owner.  A neighbour whose first-layer cell has no halo source is
+ * outside the domain (no dynamics: rho 1, u 0).
+ * Blocks whose post-processors are joined through halos run them together,
+ * stage by stage (planes, edges, corners): before each stage every block
+ * fetches the neighbours that stage reads in other blocks.  A neighbour in
+ * another block is so read after the earlier stages and before its own, as
+ * a neighbour inside the block is; the reference gets the same by running
+ * the post-processors on each block's overlap copies too, after the
+ * PostStream exchange (src/boundary/setBoundary.h:83-101,
+ * src/core/superLattice.hh:639-640).  The group runs when the last of its
+ * blocks reaches lbmh_stream for the step (the others' lbmh_stream only
+ * shift), or lbmh_initialize; every block must have collided, and the halo
+ * copies run, before any of them streams.  Call before
+ * lbmh_set_local_boundary (LBMH_E_STATE after). */
+int lbmh_set_overlap_ids(lbmh_lattice* lattice, const uint8_t* ring);
+
 /* SuperLattice::initialize() for the local boundaries: run the edge and
  * corner post-processors once on the current populations
- * (src/core/blockLattice.hh:56-67).  No-op without local boundaries. */
+ * (src/core/blockLattice.hh:56-67).  No-op without local boundaries.  A block
+ * of a split lattice whose post-processors are joined to other blocks' (see
+ * lbmh_set_overlap_ids) runs them when the last of those blocks calls. */
 int lbmh_initialize(lbmh_lattice* lattice);
 
 /* Lattice statistics in FP64 over the cells whose collision reports one:

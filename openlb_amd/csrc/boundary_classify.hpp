@@ -12,7 +12,9 @@
 //   F  fluid
 //   N  not fluid
 // and the normal points out of the fluid.  Axes flagged periodic wrap; beyond
-// a non-periodic face lies the outside.
+// a non-periodic face lies the outside, or, for a block of a split lattice,
+// the first overlap layer with the materials of the neighbouring blocks (the
+// reference classifies on the block geometry, whose overlap holds them).
 #pragma once
 
 #include <cstdint>
@@ -84,24 +86,37 @@ inline const NormalRule* normal_rules(int& count)
   return rules;
 }
 
-// Material (dynamics id) of cell (x, y, z) of an nx*ny*nz map; 0 beyond a
-// non-periodic face.
-inline int material_at(const int n[3], const int periodic[3], const uint8_t* m, int x, int y, int z)
+// Material (dynamics id) of cell (x, y, z) of an nx*ny*nz map.  Periodic axes
+// wrap.  Beyond a non-periodic face lies the first overlap layer: its
+// materials come from `ring`, a padded (nx+2)*(ny+2)*(nz+2) map whose core
+// entries are unused (the materials of the neighbouring blocks' cells, 0 where
+// the domain ends), or are 0 when there is no ring.  Deeper than one layer is
+// outside.
+inline int material_at(const int n[3], const int periodic[3], const uint8_t* m, int x, int y, int z,
+                       const uint8_t* ring = nullptr)
 {
   int r[3] = {x, y, z};
+  bool overlap = false;
   for (int d = 0; d < 3; ++d) {
     if (r[d] < 0 || r[d] >= n[d]) {
-      if (!periodic[d]) return 0;
-      r[d] = (r[d] % n[d] + n[d]) % n[d];
+      if (!periodic[d]) {
+        if (!ring || r[d] < -1 || r[d] > n[d]) return 0;
+        overlap = true;
+      } else {
+        r[d] = (r[d] % n[d] + n[d]) % n[d];
+      }
     }
+  }
+  if (overlap) {
+    return ring[(int64_t(r[0] + 1) * (n[1] + 2) + (r[1] + 1)) * (n[2] + 2) + (r[2] + 1)];
   }
   return m[(int64_t(r[0]) * n[1] + r[1]) * n[2] + r[2]];
 }
 
 // Type and normal of wall cell (x, y, z); type kTypeFlat with a zero normal
-// when no pattern matches.
+// when no pattern matches.  `ring` as for material_at.
 inline void classify_boundary_cell(const int n[3], const int periodic[3], const uint8_t* m, int x, int y, int z,
-                                   int& type, int normal[3])
+                                   int& type, int normal[3], const uint8_t* ring = nullptr)
 {
   int count = 0;
   const NormalRule* rules = normal_rules(count);
@@ -121,7 +136,7 @@ inline void classify_boundary_cell(const int n[3], const int periodic[3], const 
       }
       p += 7;
       while (*p == ' ') ++p;
-      const int mat = material_at(n, periodic, m, x + off[0], y + off[1], z + off[2]);
+      const int mat = material_at(n, periodic, m, x + off[0], y + off[1], z + off[2], ring);
       const bool fluid = mat == 1, outside = mat == 0;
       ok = pred == 'W' ? (!fluid && !outside) : (pred == 'F' ? fluid : !fluid);
     }

@@ -173,6 +173,7 @@ __global__ __launch_bounds__(64) void ensemble_post_kernel(const EnsembleEntry<Q
       a.info = info + uint64_t(lc.x) * kEnsembleWalls;
       a.cells = nullptr;
       a.recs = nullptr;
+      a.remote = nullptr;
       a.count = 0;
       a.nx = int(e.nx);
       a.ny = int(e.ny);

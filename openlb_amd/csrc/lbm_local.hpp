@@ -444,6 +444,7 @@ struct PostArgs {
   const uint32_t* info;     // [256] packed LocalInfo
   const uint32_t* cells;    // core indices of the cells to process
   const struct PostCell* recs;  // the same cells with their neighbours resolved (lattices), or null
+  const void* remote;       // rho, u of the neighbours in other blocks (4 values each, arithmetic type), or null
   uint32_t count;
   int nx, ny, nz;
   uint32_t sx, sy;
@@ -466,6 +467,9 @@ __device__ __forceinline__ void load_core_cell(const PostArgs<Q>& a, int x, int 
 // through its dynamics' momenta; n == kPostOutside is the overlap beyond a
 // non-periodic face (no dynamics: OneDensity, ZeroMomentum).
 constexpr uint32_t kPostOutside = 0xffffffffu;
+// A record slot kPostRemote | k names entry k of the block's remote buffer:
+// a neighbour that lies in another block of a split lattice.
+constexpr uint32_t kPostRemote = 0x80000000u;
 
 template <int Q, typename S, typename V>
 __device__ void rho_u_at(const PostArgs<Q>& a, uint32_t n, int id, V& rho, V (&u)[3])
@@ -609,6 +613,13 @@ struct PostCell {
 //   plane (dir, orient)     PlaneFdBoundaryProcessor3D::apply, boundaryPostProcessors3D.hh:38-92
 //     slots, axis d = 0..2: -orient e_d and -2 orient e_d along the normal, +e_d and -e_d along the face
 enum PostKind : int { kPostEdge = 0, kPostCorner = 1, kPostPlane = 2 };
+// The order in which a lattice runs the kinds after the shift: the plane
+// processors, then edges, then corners (PostStream priorities).
+enum PostStage : int { kStagePlanes = 0, kStageEdges = 1, kStageCorners = 2, kPostStages = 3 };
+__host__ __device__ constexpr int post_stage(int kind)
+{
+  return kind == kPostPlane ? kStagePlanes : (kind == kPostEdge ? kStageEdges : kStageCorners);
+}
 
 __host__ __device__ inline void post_offset(const LocalInfo& li, int kind, int slot, int (&o)[3])
 {
@@ -663,8 +674,15 @@ __device__ void post_gather_rec(const PostArgs<Q>& a, const PostCell* r, int slo
                                 typename StorageTraits<S>::V (&out)[4])
 {
   using V = typename StorageTraits<S>::V;
+  const uint32_t n = r->nb[slot];
+  if (n != kPostOutside && (n & kPostRemote)) {
+    const V* m = static_cast<const V*>(a.remote) + uint64_t(n & ~kPostRemote) * 4;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out[k] = m[k];
+    return;
+  }
   V rho, u[3];
-  rho_u_at<Q, S, V>(a, r->nb[slot], int(r->nb_id[slot]), rho, u);
+  rho_u_at<Q, S, V>(a, n, int(r->nb_id[slot]), rho, u);
   out[0] = rho;
   out[1] = u[0];
   out[2] = u[1];
@@ -761,6 +779,36 @@ __global__ __launch_bounds__(64) void outer_post_kernel(PostArgs<Q> a)
   if (valid && slot < kPostSlots) post_gather_rec<Q, S>(a, a.recs + k, slot, nb[j][slot]);
   __syncthreads();
   if (valid && slot == 0) post_finish<Q, S, KIND>(a, a.recs[k].c, int(a.recs[k].id), nb[j]);
+}
+
+// Neighbours of post-processed cells that lie in another block: entry `slot`
+// of the consuming block's remote buffer takes rho and u of padded cell
+// `src_padded` (core index `src_core`) of the block that owns it.
+struct RemotePair {
+  uint32_t src_padded;
+  uint32_t src_core;
+  uint32_t slot;
+};
+
+// The PostStream exchange of the post-processors' neighbourhood
+// (superLattice.hh:639-640, the overlap cells boundary::set requests through
+// addPoints2CommBC): rho and u of the listed cells of the source block `a`,
+// as its post-processors would read them, into the consumer's remote buffer.
+template <int Q, typename S>
+__global__ __launch_bounds__(64) void remote_moments_kernel(PostArgs<Q> a, const RemotePair* pairs, uint32_t count,
+                                                            typename StorageTraits<S>::V* out)
+{
+  using V = typename StorageTraits<S>::V;
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const RemotePair p = pairs[k];
+  V rho, u[3];
+  rho_u_at<Q, S, V>(a, p.src_padded, int(a.ids[p.src_core]), rho, u);
+  V* o = out + uint64_t(p.slot) * 4;
+  o[0] = rho;
+  o[1] = u[0];
+  o[2] = u[1];
+  o[3] = u[2];
 }
 
 }  // namespace lbmh

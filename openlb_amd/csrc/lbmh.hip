@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -15,6 +16,7 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <unordered_map>
 #include <vector>
 
 #include "boundary_classify.hpp"
@@ -111,6 +113,18 @@ struct lbmh_lattice {
   PostCell* d_edge_recs = nullptr;
   PostCell* d_corner_recs = nullptr;
   PostCell* d_plane_recs = nullptr;
+  // block of a split lattice: materials of the first overlap layer (padded
+  // map, core entries unused), empty for a lone block
+  std::vector<uint8_t> h_ring;
+  // post-processor neighbours beyond the core on non-periodic axes of such a
+  // block, in block coordinates, and the post-processor stage that reads each:
+  // they lie in other blocks and are read from d_remote (rho, u: 4 values
+  // each, in the arithmetic type), which the halos into this block fill
+  std::vector<std::array<int, 3>> remote_pos;
+  std::vector<uint8_t> remote_stage;
+  bool remote_resolved = true;
+  void* d_remote = nullptr;
+  int64_t init_step = -1;  // step count at which lbmh_initialize asked for the post-processors
 };
 
 // Library default launch shape, from sweeps of (LDS reservation, XCD group)
@@ -135,6 +149,9 @@ struct lbmh_halo {
   uint2* d_pairs = nullptr;   // (destination overlap cell, source core cell), padded ids
   std::vector<uint2> pairs;   // the same on the host: which cell each overlap cell copies
   uint32_t count = 0;
+  // post-processor neighbours of dst that src owns, by post-processor stage
+  std::vector<RemotePair> remote[kPostStages];
+  RemotePair* d_remote[kPostStages] = {};
 };
 
 namespace {
@@ -191,6 +208,7 @@ PostArgs<Q> post_args(const lbmh_lattice* L)
   a.info = L->d_info;
   a.cells = nullptr;
   a.recs = nullptr;
+  a.remote = L->d_remote;
   a.count = 0;
   a.nx = L->n[0];
   a.ny = L->n[1];
@@ -206,8 +224,11 @@ PostArgs<Q> post_args(const lbmh_lattice* L)
 
 // The cells of a post-processor list with their neighbours resolved, as the
 // device's cell_rho_u would find them: offsets of the processor's slots,
-// periodic axes wrapped, kPostOutside beyond a non-periodic face.
-std::vector<PostCell> post_records(const lbmh_lattice* L, const std::vector<uint32_t>& cells, int kind)
+// periodic axes wrapped, kPostOutside beyond a non-periodic face.  A block of
+// a split lattice (one with overlap materials) has neighbours there instead:
+// each such slot becomes an entry of L->remote_pos, read from the remote
+// buffer.
+std::vector<PostCell> post_records(lbmh_lattice* L, const std::vector<uint32_t>& cells, int kind)
 {
   const int n[3] = {L->n[0], L->n[1], L->n[2]};
   const uint32_t sx = L->size[1] * L->size[2], sy = L->size[2];
@@ -232,7 +253,12 @@ std::vector<PostCell> post_records(const lbmh_lattice* L, const std::vector<uint
           else q[d] = (q[d] % n[d] + n[d]) % n[d];
         }
       }
-      if (outside) {
+      if (outside && !L->h_ring.empty()) {
+        p.nb[slot] = kPostRemote | uint32_t(L->remote_pos.size());
+        p.nb_id[slot] = 0;
+        L->remote_pos.push_back({q[0], q[1], q[2]});
+        L->remote_stage.push_back(uint8_t(post_stage(kind)));
+      } else if (outside) {
         p.nb[slot] = kPostOutside;
         p.nb_id[slot] = 0;
       } else {
@@ -245,14 +271,145 @@ std::vector<PostCell> post_records(const lbmh_lattice* L, const std::vector<uint
   return out;
 }
 
-// PostStream stage of the local boundaries: outer edges (priority 0), then
-// outer corners (priority 1), on the populations after the shift.
-template <int Q, typename S>
-int launch_post_processors(lbmh_lattice* L, hipStream_t stream)
+size_t arith_size(int storage) { return storage == LBMH_F64 ? sizeof(double) : sizeof(float); }
+
+void drop_remote(lbmh_lattice* L)
 {
-  if (L->h_edges.empty() && L->h_corners.empty() && L->h_planes.empty()) return LBMH_OK;
+  for (lbmh_halo* H : L->halos_in) {
+    for (int st = 0; st < kPostStages; ++st) {
+      H->remote[st].clear();
+      if (H->d_remote[st]) (void)hipFree(H->d_remote[st]);
+      H->d_remote[st] = nullptr;
+    }
+  }
+  if (L->d_remote) (void)hipFree(L->d_remote);
+  L->d_remote = nullptr;
+  L->remote_resolved = L->remote_pos.empty();
+}
+
+// Finds the block that owns each remote neighbour of L: the halo source of
+// the first overlap layer cell on the way to it, stepped on by the rest of
+// the offset inside that block.  Neighbours with no halo source lie outside
+// the domain and keep the moments of a cell without dynamics (rho 1, u 0).
+int resolve_remote(lbmh_lattice* L)
+{
+  if (L->remote_resolved) return LBMH_OK;
+  DeviceGuard guard(L->device);
+  drop_remote(L);
+  const size_t count = L->remote_pos.size();
+  std::unordered_map<uint32_t, std::pair<lbmh_halo*, uint32_t>> src_of;
+  for (lbmh_halo* H : L->halos_in) {
+    if (!H->src) continue;
+    for (const uint2& pr : H->pairs) src_of[pr.x] = {H, pr.y};
+  }
+  const uint32_t sx = L->size[1] * L->size[2], sy = L->size[2];
+  for (size_t k = 0; k < count; ++k) {
+    const auto& q = L->remote_pos[k];
+    int r[3], delta[3];
+    for (int d = 0; d < 3; ++d) {
+      r[d] = std::min(std::max(q[d], -1), L->n[d]);
+      delta[d] = q[d] - r[d];
+    }
+    const uint32_t pr = uint32_t(r[0] + 1) * sx + uint32_t(r[1] + 1) * sy + uint32_t(r[2] + 1);
+    auto it = src_of.find(pr);
+    if (it == src_of.end()) continue;
+    lbmh_halo* H = it->second.first;
+    const lbmh_lattice* S = H->src;
+    const uint32_t s = it->second.second;
+    const uint32_t ssx = S->size[1] * S->size[2], ssy = S->size[2];
+    int t[3] = {int(s / ssx) - 1 + delta[0], int((s % ssx) / ssy) - 1 + delta[1], int(s % ssy) - 1 + delta[2]};
+    for (int d = 0; d < 3; ++d) {
+      if (t[d] < 0 || t[d] >= S->n[d])
+        return fail(LBMH_E_INVALID, "a post-processor neighbour lies beyond the block next to this one");
+    }
+    const uint32_t tp = uint32_t(t[0] + 1) * ssx + uint32_t(t[1] + 1) * ssy + uint32_t(t[2] + 1);
+    const uint32_t tc = (uint32_t(t[0]) * uint32_t(S->n[1]) + uint32_t(t[1])) * uint32_t(S->n[2]) + uint32_t(t[2]);
+    H->remote[L->remote_stage[k]].push_back(RemotePair{tp, tc, uint32_t(k)});
+  }
+  if (count) {
+    // every entry starts at the moments of a cell without dynamics
+    const size_t es = arith_size(L->storage);
+    std::vector<uint8_t> init(count * 4 * es);
+    for (size_t k = 0; k < count; ++k) {
+      if (es == sizeof(double)) {
+        const double v[4] = {1.0, 0.0, 0.0, 0.0};
+        std::memcpy(init.data() + k * 4 * es, v, sizeof(v));
+      } else {
+        const float v[4] = {1.0f, 0.0f, 0.0f, 0.0f};
+        std::memcpy(init.data() + k * 4 * es, v, sizeof(v));
+      }
+    }
+    LBMH_HIP(hipMalloc(&L->d_remote, init.size()));
+    LBMH_HIP(hipMemcpy(L->d_remote, init.data(), init.size(), hipMemcpyHostToDevice));
+  }
+  for (lbmh_halo* H : L->halos_in) {
+    for (int st = 0; st < kPostStages; ++st) {
+      const auto& v = H->remote[st];
+      if (v.empty()) continue;
+      LBMH_HIP(hipMalloc((void**)&H->d_remote[st], sizeof(RemotePair) * v.size()));
+      LBMH_HIP(hipMemcpy(H->d_remote[st], v.data(), sizeof(RemotePair) * v.size(), hipMemcpyHostToDevice));
+    }
+  }
+  L->remote_resolved = true;
+  return LBMH_OK;
+}
+
+// Moments of H's remote neighbours read by post-processor stage `stage`, as
+// H->src holds them now, into H->dst's remote buffer.  A source without
+// post-processors may not have streamed this step yet (step `at`): it is read
+// through the shifts it will have, its post-collision populations are final.
+template <int Q, typename S>
+int launch_remote(const lbmh_halo* H, int stage, int64_t at, hipStream_t stream)
+{
+  const lbmh_lattice* src = H->src;
+  PostArgs<Q> a = post_args<Q>(src);
+  if (src->steps + 1 == at) {
+    const int64_t N = src->n_padded;
+    for (int i = 0; i < Q; ++i) {
+      int64_t v = (src->shift[i] - src->dist[i]) % N;
+      if (v < 0) v += N;
+      a.shift[i] = uint32_t(v);
+    }
+  } else if (src->steps != at) {
+    return fail(LBMH_E_STATE, "blocks of a split lattice are out of step");
+  }
+  const uint32_t count = uint32_t(H->remote[stage].size());
+  hipLaunchKernelGGL((remote_moments_kernel<Q, S>), dim3(grid_for(count, 64)), dim3(64), 0, stream, a,
+                     H->d_remote[stage], count, static_cast<typename StorageTraits<S>::V*>(H->dst->d_remote));
+  LBMH_HIP(hipGetLastError());
+  return LBMH_OK;
+}
+
+bool has_post(const lbmh_lattice* L)
+{
+  return L->has_local && (!L->h_edges.empty() || !L->h_corners.empty() || !L->h_planes.empty());
+}
+
+// The blocks whose post-processors must run together with L's: those joined
+// to L through halos, directly or through one another, that have
+// post-processors of their own.
+std::vector<lbmh_lattice*> post_group(lbmh_lattice* L)
+{
+  std::vector<lbmh_lattice*> group{L};
+  for (size_t k = 0; k < group.size(); ++k) {
+    lbmh_lattice* X = group[k];
+    auto visit = [&](lbmh_lattice* Y) {
+      if (Y && has_post(Y) && std::find(group.begin(), group.end(), Y) == group.end()) group.push_back(Y);
+    };
+    for (lbmh_halo* H : X->halos_in) visit(const_cast<lbmh_lattice*>(H->src));
+    for (lbmh_halo* H : X->halos_out) visit(H->dst);
+  }
+  return group;
+}
+
+// One PostStream stage of the local boundaries of one block: the plane
+// finite-difference processors (stage 0), outer edges (1), outer corners (2),
+// on the populations after the shift.
+template <int Q, typename S>
+int launch_post_stage(lbmh_lattice* L, int stage, hipStream_t stream)
+{
   PostArgs<Q> a = post_args<Q>(L);
-  if (!L->h_planes.empty()) {
+  if (stage == kStagePlanes && !L->h_planes.empty()) {
     using V = typename StorageTraits<S>::V;
     a.cells = L->d_planes;
     a.recs = L->d_plane_recs;
@@ -269,7 +426,7 @@ int launch_post_processors(lbmh_lattice* L, hipStream_t stream)
     }
     LBMH_HIP(hipGetLastError());
   }
-  if (!L->h_edges.empty()) {
+  if (stage == kStageEdges && !L->h_edges.empty()) {
     a.cells = L->d_edges;
     a.recs = L->d_edge_recs;
     a.count = uint32_t(L->h_edges.size());
@@ -277,13 +434,60 @@ int launch_post_processors(lbmh_lattice* L, hipStream_t stream)
                        stream, a);
     LBMH_HIP(hipGetLastError());
   }
-  if (!L->h_corners.empty()) {
+  if (stage == kStageCorners && !L->h_corners.empty()) {
     a.cells = L->d_corners;
     a.recs = L->d_corner_recs;
     a.count = uint32_t(L->h_corners.size());
     hipLaunchKernelGGL((outer_post_kernel<Q, S, kPostCorner>), dim3(grid_for(a.count, 64 / kPostLanes)), dim3(64), 0,
                        stream, a);
     LBMH_HIP(hipGetLastError());
+  }
+  return LBMH_OK;
+}
+
+// The PostStream post-processors of L and of every block they are coupled to
+// (post_group), stage by stage: before each stage every block fetches the
+// moments its processors of that stage read in other blocks, then every
+// block runs the stage.  A neighbour in another block is thus read after the
+// earlier stages and before its own stage, exactly as a neighbour inside the
+// block -- the reference's blocks get the same by post-processing their
+// overlap copies too (boundary::set covers the overlap cells within the
+// padding, src/boundary/setBoundary.h:83-101, after the PostStream exchange,
+// src/core/superLattice.hh:639-640).  The group runs once every member has
+// reached the call (lbmh_stream of its step, or lbmh_initialize); a member
+// that calls earlier only records it.
+template <int Q, typename S>
+int run_post(lbmh_lattice* L, hipStream_t stream, bool init)
+{
+  if (!has_post(L)) return LBMH_OK;
+  if (init) L->init_step = L->steps;
+  const std::vector<lbmh_lattice*> group = post_group(L);
+  for (lbmh_lattice* X : group) {
+    if (X->steps > L->steps) return fail(LBMH_E_STATE, "blocks of a split lattice are out of step");
+    if (X->steps < L->steps) return LBMH_OK;
+    if (init && X->init_step != L->steps) return LBMH_OK;
+  }
+  for (lbmh_lattice* X : group) {
+    if (X->remote_pos.empty()) continue;
+    const int rc = resolve_remote(X);
+    if (rc != LBMH_OK) return rc;
+  }
+  for (int stage = 0; stage < kPostStages; ++stage) {
+    for (lbmh_lattice* X : group) {
+      for (lbmh_halo* H : X->halos_in) {
+        if (H->remote[stage].empty()) continue;
+        if (!H->src) return fail(LBMH_E_INVALID, "a block of this halo was destroyed");
+        const int rc = launch_remote<Q, S>(H, stage, X->steps, stream);
+        if (rc != LBMH_OK) return rc;
+      }
+    }
+    for (lbmh_lattice* X : group) {
+      const int rc = launch_post_stage<Q, S>(X, stage, stream);
+      if (rc != LBMH_OK) return rc;
+    }
+  }
+  if (init) {
+    for (lbmh_lattice* X : group) X->init_step = -1;
   }
   return LBMH_OK;
 }
@@ -364,7 +568,7 @@ int launch_kind(lbmh_lattice* L, hipStream_t stream, int phase)
   if (!(phase & kPhaseStream)) return LBMH_OK;
   advance_shifts(L);
   L->steps += 1;
-  if (L->has_local) return launch_post_processors<Q, S>(L, stream);
+  if (L->has_local) return run_post<Q, S>(L, stream, false);
   return LBMH_OK;
 }
 
@@ -511,7 +715,11 @@ int lbmh_lattice_destroy(lbmh_lattice* L)
   if (!L) return LBMH_OK;
   DeviceGuard guard(L->device);
   for (lbmh_halo* H : L->halos_in) H->dst = nullptr;
-  for (lbmh_halo* H : L->halos_out) H->src = nullptr;
+  for (lbmh_halo* H : L->halos_out) {
+    H->src = nullptr;
+    if (H->dst) H->dst->remote_resolved = H->dst->remote_pos.empty();
+  }
+  if (L->d_remote) (void)hipFree(L->d_remote);
   if (L->d_pop) (void)hipFree(L->d_pop);
   if (L->d_force) (void)hipFree(L->d_force);
   if (L->d_ids) (void)hipFree(L->d_ids);
@@ -552,6 +760,9 @@ int lbmh_set_dynamics_ids(lbmh_lattice* L, const uint8_t* ids)
   L->h_local.clear();
   L->h_planes.clear();
   L->pressure_planes = false;
+  L->remote_pos.clear();
+  L->remote_stage.clear();
+  drop_remote(L);
   for (int w = 0; w < kMaxWalls; ++w) {
     L->info[w] = pack_local_info(kMovingWall, 0, 0, 0);
     L->id_defined[w] = false;
@@ -639,6 +850,37 @@ int lbmh_classify_boundary(const int extent[3], const int periodic[3], const uin
   return LBMH_OK;
 }
 
+int lbmh_classify_boundary_overlap(const int extent[3], const int periodic[3], const uint8_t* ids,
+                                   const uint8_t* ring, int8_t* out)
+{
+  if (!extent || !periodic || !ids || !ring || !out) return fail(LBMH_E_INVALID, "null argument");
+  if (extent[0] < 1 || extent[1] < 1 || extent[2] < 1) return fail(LBMH_E_INVALID, "extent must be >= 1");
+  const int n[3] = {extent[0], extent[1], extent[2]};
+  const int per[3] = {periodic[0] ? 1 : 0, periodic[1] ? 1 : 0, periodic[2] ? 1 : 0};
+  const int64_t nc = int64_t(n[0]) * n[1] * n[2];
+  for (int x = 0; x < n[0]; ++x)
+    for (int y = 0; y < n[1]; ++y)
+      for (int z = 0; z < n[2]; ++z) {
+        const int64_t c = (int64_t(x) * n[1] + y) * n[2] + z;
+        out[c] = -1;
+        out[nc + c] = out[2 * nc + c] = out[3 * nc + c] = 0;
+        if (ids[c] == 0 || ids[c] == 1) continue;
+        int type, nrm[3];
+        classify_boundary_cell(n, per, ids, x, y, z, type, nrm, ring);
+        out[c] = int8_t(type);
+        for (int d = 0; d < 3; ++d) out[(d + 1) * nc + c] = int8_t(nrm[d]);
+      }
+  return LBMH_OK;
+}
+
+int lbmh_set_overlap_ids(lbmh_lattice* L, const uint8_t* ring)
+{
+  if (!L || !ring) return fail(LBMH_E_INVALID, "null argument");
+  if (L->has_local) return fail(LBMH_E_STATE, "set the overlap ids before the local boundaries");
+  L->h_ring.assign(ring, ring + L->n_padded);
+  return LBMH_OK;
+}
+
 int lbmh_set_local_boundary(lbmh_lattice* L, int id, int kind, double rho, const double u[3])
 {
   if (!L || !u) return fail(LBMH_E_INVALID, "null argument");
@@ -666,7 +908,8 @@ int lbmh_set_local_boundary(lbmh_lattice* L, int id, int kind, double rho, const
         const uint32_t c = uint32_t((int64_t(x) * n[1] + y) * n[2] + z);
         if (L->h_ids[c] != id) continue;
         int type, nrm[3];
-        classify_boundary_cell(n, per, L->h_ids.data(), x, y, z, type, nrm);
+        classify_boundary_cell(n, per, L->h_ids.data(), x, y, z, type, nrm,
+                               L->h_ring.empty() ? nullptr : L->h_ring.data());
         const bool flat = type == kTypeFlat && (nrm[0] || nrm[1] || nrm[2]);
         const uint32_t k = local_kind(kind, type, flat);
         if (k == kInvalidKind) {
@@ -739,9 +982,13 @@ int lbmh_set_local_boundary(lbmh_lattice* L, int id, int kind, double rho, const
     LBMH_HIP(hipMemcpy(d, recs.data(), sizeof(PostCell) * recs.size(), hipMemcpyHostToDevice));
     return LBMH_OK;
   };
+  L->remote_pos.clear();
+  L->remote_stage.clear();
+  drop_remote(L);
   if (rc == LBMH_OK) rc = upload_records(L->h_edges, kPostEdge, L->d_edge_recs);
   if (rc == LBMH_OK) rc = upload_records(L->h_corners, kPostCorner, L->d_corner_recs);
   if (rc == LBMH_OK) rc = upload_records(L->h_planes, kPostPlane, L->d_plane_recs);
+  L->remote_resolved = L->remote_pos.empty();
   if (rc == LBMH_OK && L->pressure_planes) {
     if (L->d_scratch) {
       LBMH_HIP(hipFree(L->d_scratch));
@@ -765,7 +1012,7 @@ int lbmh_initialize(lbmh_lattice* L)
   const int rc = with_types(L, [&](auto qc, auto s) -> int {
     constexpr int Q = decltype(qc)::value;
     using S = decltype(s);
-    return launch_post_processors<Q, S>(L, nullptr);
+    return run_post<Q, S>(L, nullptr, true);
   });
   if (rc != LBMH_OK) return rc;
   LBMH_HIP(hipDeviceSynchronize());
@@ -1051,6 +1298,7 @@ int lbmh_halo_create(lbmh_lattice* dst, const lbmh_lattice* src, const uint32_t*
   H->count = uint32_t(n);
   H->pairs = pairs;
   dst->halos_in.push_back(H);
+  dst->remote_resolved = dst->remote_pos.empty();
   const_cast<lbmh_lattice*>(src)->halos_out.push_back(H);
   DeviceGuard guard(dst->device);
   if (n) {
@@ -1070,8 +1318,14 @@ int lbmh_halo_destroy(lbmh_halo* H)
 {
   if (!H) return LBMH_OK;
   DeviceGuard guard(H->device);
-  if (H->dst) H->dst->halos_in.erase(std::remove(H->dst->halos_in.begin(), H->dst->halos_in.end(), H),
-                                     H->dst->halos_in.end());
+  if (H->dst) {
+    H->dst->halos_in.erase(std::remove(H->dst->halos_in.begin(), H->dst->halos_in.end(), H),
+                           H->dst->halos_in.end());
+    H->dst->remote_resolved = H->dst->remote_pos.empty();
+  }
+  for (int st = 0; st < kPostStages; ++st) {
+    if (H->d_remote[st]) (void)hipFree(H->d_remote[st]);
+  }
   if (H->src) {
     auto& out = const_cast<lbmh_lattice*>(H->src)->halos_out;
     out.erase(std::remove(out.begin(), out.end(), H), out.end());

@@ -957,6 +957,7 @@ int lbmh_ensemble_field_export(lbmh_ensemble* E, int m, int field, int overlap, 
     a.walls = E->d_walls + size_t(m) * kEnsembleWalls * 4;
     a.info = E->d_info + size_t(m) * kEnsembleWalls;
     a.cells = nullptr;
+    a.remote = nullptr;
     a.count = 0;
     a.nx = int(en.nx);
     a.ny = int(en.ny);

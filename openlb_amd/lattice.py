@@ -45,9 +45,15 @@ class BlockLattice:
         self._h = h
 
     @classmethod
-    def from_case(cls, case, storage: str = "f32", device: int = 0):
+    def from_case(cls, case, storage: str = "f32", device: int = 0, overlap_ids=None):
+        """The lattice of a case.  ``overlap_ids`` (uint8, shape + 2 per axis)
+        are the materials of the first overlap layer of a block of a split
+        lattice: its local boundaries are classified with them
+        (``lbmh_set_overlap_ids``)."""
         lat = cls(case.q, case.shape, case.periodic, storage, case.kind, device)
         lat.define_dynamics(case.ids)
+        if overlap_ids is not None:
+            lat.set_overlap_ids(overlap_ids)
         lat.set_parameters(omega=case.omega, magic=case.magic, smago=case.smago, force=case.force)
         if getattr(case, "force_field", None) is not None:
             lat.set_force_field(case.force_field)
@@ -91,6 +97,15 @@ class BlockLattice:
         if ids.shape != self.shape:
             raise ValueError(f"ids shape {ids.shape} != lattice shape {self.shape}")
         abi.check(abi.load().lbmh_set_dynamics_ids(self._h, _ptr(ids)))
+
+    def set_overlap_ids(self, ring: np.ndarray):
+        """Materials of the first overlap layer (padded map, core entries
+        unused; 0 where the domain ends), before any local boundary."""
+        ring = np.ascontiguousarray(ring, dtype=np.uint8)
+        want = tuple(n + 2 for n in self.shape)
+        if ring.shape != want:
+            raise ValueError(f"overlap ids shape {ring.shape} != padded shape {want}")
+        abi.check(abi.load().lbmh_set_overlap_ids(self._h, _ptr(ring)))
 
     def set_local_boundary(self, material: int, kind: str = "velocity", rho: float = 1.0,
                            u=(0.0, 0.0, 0.0)):
@@ -287,13 +302,22 @@ class BlockLattice:
         return b.value
 
 
-def classify_boundary(ids: np.ndarray, periodic=(0, 0, 0)) -> np.ndarray:
+def classify_boundary(ids: np.ndarray, periodic=(0, 0, 0), overlap_ids=None) -> np.ndarray:
     """Boundary type and discrete normal of every wall cell of an id map, as
     OpenLB's computeBoundaryTypeAndNormal gives them (host-side, no GPU):
-    int8 [4][nx][ny][nz] = type (-1 on fluid and outside cells), n_x, n_y, n_z."""
+    int8 [4][nx][ny][nz] = type (-1 on fluid and outside cells), n_x, n_y, n_z.
+    With ``overlap_ids`` (padded map) the map is one block of a split lattice
+    and its first overlap layer holds the neighbouring blocks' materials."""
     m = np.ascontiguousarray(ids, dtype=np.uint8)
     ext = np.array(m.shape, dtype=np.int32)
     per = np.array([int(bool(p)) for p in periodic], dtype=np.int32)
     out = np.empty((4,) + m.shape, dtype=np.int8)
-    abi.check(abi.load().lbmh_classify_boundary(_ptr(ext), _ptr(per), _ptr(m), _ptr(out)))
+    lib = abi.load()
+    if overlap_ids is None:
+        abi.check(lib.lbmh_classify_boundary(_ptr(ext), _ptr(per), _ptr(m), _ptr(out)))
+    else:
+        ring = np.ascontiguousarray(overlap_ids, dtype=np.uint8)
+        if ring.shape != tuple(n + 2 for n in m.shape):
+            raise ValueError(f"overlap ids shape {ring.shape} != padded shape of {m.shape}")
+        abi.check(lib.lbmh_classify_boundary_overlap(_ptr(ext), _ptr(per), _ptr(m), _ptr(ring), _ptr(out)))
     return out

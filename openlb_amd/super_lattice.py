@@ -54,9 +54,6 @@ class SuperLattice:
     ``CuboidDecomposition::splitRegular`` uses."""
 
     def __init__(self, case, storage: str = "f32", blocks=(2, 1, 1), device: int = 0):
-        if getattr(case, "local", None):
-            # their classification and post-processors read neighbours across block interfaces
-            raise ValueError("local boundaries are supported on single blocks only")
         self.case = case
         self.storage = storage
         self.shape = tuple(case.shape)
@@ -72,7 +69,8 @@ class SuperLattice:
             lo = tuple(self.ranges[d][key[d]][0] for d in range(3))
             hi = tuple(self.ranges[d][key[d]][1] for d in range(3))
             sub = _sub_case(case, lo, hi, self_periodic)
-            self.blocks[key] = BlockLattice.from_case(sub, storage, device)
+            ring = overlap_ids(case, lo, hi)
+            self.blocks[key] = BlockLattice.from_case(sub, storage, device, overlap_ids=ring)
             self.origin[key] = lo
         self._build_halos(self_periodic)
 
@@ -143,6 +141,14 @@ class SuperLattice:
                 abi.check(lib.lbmh_halo_copy(h, None))
             for lat in self.blocks.values():
                 abi.check(lib.lbmh_stream(lat.handle, None))
+
+    def initialize(self):
+        """SuperLattice::initialize(): every block's post-processors once on the
+        current populations (src/core/superLattice.hh:191-204); blocks whose
+        post-processors read other blocks take those neighbours as they stand
+        before any block's post-processors ran."""
+        for lat in self.blocks.values():
+            lat.initialize()
 
     # -- data ------------------------------------------------------------------
 
@@ -306,6 +312,29 @@ def piece_sources(case, ranges, key):
                 lists[0].append(((z + 1) * ey + (y + 1)) * ex + (x + 1))
                 lists[1].append((s[2] * sn[1] + s[1]) * sn[0] + s[0])
     return {k: (np.asarray(a, dtype=np.int64), np.asarray(b, dtype=np.int64)) for k, (a, b) in out.items()}
+
+
+def overlap_ids(case, lo, hi) -> np.ndarray:
+    """Materials around the block [lo, hi) of a case: the padded map of the
+    block's first overlap layer, read from the whole domain (periodic axes
+    wrapped, 0 beyond a non-periodic face), as the block geometry's overlap
+    holds them for OpenLB's boundary classification
+    (src/boundary/setBoundary.h:83-101).  The core entries are the block's own."""
+    shape = tuple(case.shape)
+    idx = []
+    inside = np.ones([hi[d] - lo[d] + 2 for d in range(3)], dtype=bool)
+    for d in range(3):
+        g = np.arange(lo[d] - 1, hi[d] + 1)
+        ok = (g >= 0) & (g < shape[d])
+        if case.periodic[d]:
+            g = g % shape[d]
+            ok[:] = True
+        dims = [1, 1, 1]
+        dims[d] = len(g)
+        inside &= ok.reshape(dims)
+        idx.append(np.clip(g, 0, shape[d] - 1))
+    ids = np.asarray(case.ids, dtype=np.uint8)[np.ix_(*idx)]
+    return np.where(inside, ids, 0).astype(np.uint8)
 
 
 def _sub_case(case, lo, hi, periodic):

@@ -12,6 +12,13 @@
 //             split <width>: CuboidDecomposition::splitRegular(0, width),
 //             i.e. max(1, n_d / width) cuboids per axis cut by Cuboid::divide,
 //             cuboidDecomposition.hh:755-763, cuboid.hh:184-211)
+//             overlap <n>: the geometry's (and lattice's) overlap, default 1;
+//             with this key the overlap cells inside the domain (or its
+//             periodic images) are initialised like the core cells, as an
+//             example program's defineRhoU / iniEquilibrium over a material
+//             indicator does (blockLattice.hh:69-120 iterate every spatial
+//             location, overlap included).  OpenLB's own default is 3
+//             (superGeometry.h:94-96).
 //   ids.bin   uint8  [nx][ny][nz]  per-cell dynamics id
 //             (0 none, 1 bulk, 2 BounceBack, 3.. BounceBackVelocity walls)
 //   rhou.bin  double [4][nx][ny][nz] initial rho, ux, uy, uz
@@ -86,6 +93,7 @@ struct CaseSpec {
   std::vector<WallSpec> walls;
   std::map<int, std::string> local;
   int split = 0;   // splitRegular width; 0 keeps one cuboid
+  int overlap = 0; // 0: overlap 1 with core-only initialisation
   std::vector<unsigned char> ids;
   std::vector<double> rhou;
   std::vector<double> forceField;  // [3][nCore], empty: the uniform force
@@ -122,6 +130,7 @@ static CaseSpec readCase(const std::string& dir)
     else if (key == "fz") ls >> c.force[2];
     else if (key == "steps") { long s; while (ls >> s) c.steps.push_back(s); }
     else if (key == "split") ls >> c.split;
+    else if (key == "overlap") ls >> c.overlap;
     else if (key == "local") {
       int id; std::string kind; ls >> id >> kind;
       if (kind != "velocity" && kind != "pressure" && kind != "interp_velocity" && kind != "interp_pressure")
@@ -170,6 +179,37 @@ static void forEachCoreCell(SuperGeometry<T,3>& geometry, F f)
       for (int y = 0; y < extent[1]; ++y)
         for (int z = 0; z < extent[2]; ++z)
           f(iC, LatticeR<3>{x, y, z}, LatticeR<3>{o[0] + x, o[1] + y, o[2] + z});
+  }
+}
+
+// Calls f(local block index, local cell, global cell) for every spatial
+// location of every block -- core and `overlap` layers -- that lies in the
+// domain or in one of its periodic images (global cell wrapped into the
+// domain).
+template <typename F>
+static void forEachSpatialCell(const CaseSpec& c, SuperGeometry<T,3>& geometry, int overlap, F f)
+{
+  auto& load = geometry.getLoadBalancer();
+  auto& cuboids = geometry.getCuboidDecomposition();
+  for (int iC = 0; iC < load.size(); ++iC) {
+    const auto& cuboid = cuboids.get(load.glob(iC));
+    const auto origin = cuboid.getOrigin();
+    const auto extent = cuboid.getExtent();
+    const int o[3] = {int(std::lround(double(origin[0]))), int(std::lround(double(origin[1]))),
+                      int(std::lround(double(origin[2])))};
+    for (int x = -overlap; x < extent[0] + overlap; ++x)
+      for (int y = -overlap; y < extent[1] + overlap; ++y)
+        for (int z = -overlap; z < extent[2] + overlap; ++z) {
+          int g[3] = {o[0] + x, o[1] + y, o[2] + z};
+          bool inside = true;
+          for (int d = 0; d < 3; ++d) {
+            if (g[d] < 0 || g[d] >= c.n[d]) {
+              if (!c.periodic[d]) inside = false;
+              else g[d] = (g[d] % c.n[d] + c.n[d]) % c.n[d];
+            }
+          }
+          if (inside) f(iC, LatticeR<3>{x, y, z}, LatticeR<3>{g[0], g[1], g[2]});
+        }
   }
 }
 
@@ -232,8 +272,9 @@ static void runCase(const CaseSpec& c, const std::string& dir)
   }
   cuboids.setPeriodicity({bool(c.periodic[0]), bool(c.periodic[1]), bool(c.periodic[2])});
   BlockLoadBalancer<T> load(0, 1, cuboids.size(), 0);
-  // overlap 1: the halo depth the time step actually refreshes (superLattice.hh:138-143)
-  SuperGeometry<T,3> geometry(cuboids, load, 1);
+  // overlap 1 by default: the halo depth the time step actually refreshes
+  // (superLattice.hh:138-143)
+  SuperGeometry<T,3> geometry(cuboids, load, c.overlap > 0 ? c.overlap : 1);
   forEachCoreCell(geometry, [&](int iC, LatticeR<3> local, LatticeR<3> global) {
     geometry.getBlockGeometry(iC).set(local, int(c.ids[coreIndex(c, global[0], global[1], global[2])]));
   });
@@ -288,7 +329,11 @@ static void runCase(const CaseSpec& c, const std::string& dir)
   // Per-cell rho/u in the same order as the example programs: define the
   // moments, then initialise populations at equilibrium.
   const std::size_t nCore = c.ids.size();
-  forEachCoreCell(geometry, [&](int iC, LatticeR<3> local, LatticeR<3> global) {
+  auto forEachInitCell = [&](auto f) {
+    if (c.overlap > 0) forEachSpatialCell(c, geometry, c.overlap, f);
+    else forEachCoreCell(geometry, f);
+  };
+  forEachInitCell([&](int iC, LatticeR<3> local, LatticeR<3> global) {
     const std::size_t k = coreIndex(c, global[0], global[1], global[2]);
     const int id = c.ids[k];
     if (id == 0) return;
@@ -311,7 +356,7 @@ static void runCase(const CaseSpec& c, const std::string& dir)
   });
   // Moving walls: fixed density and wall velocity (cavity3d.cpp:124-136).
   for (const auto& w : c.walls) {
-    forEachCoreCell(geometry, [&](int iC, LatticeR<3> local, LatticeR<3> global) {
+    forEachInitCell([&](int iC, LatticeR<3> local, LatticeR<3> global) {
       if (c.ids[coreIndex(c, global[0], global[1], global[2])] != w.id) return;
       auto cell = lattice.getBlock(iC).get(local);
       T u[3] = {T(w.u[0]), T(w.u[1]), T(w.u[2])};

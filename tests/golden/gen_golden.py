@@ -129,7 +129,7 @@ def write_normals():
     np.savez_compressed(os.path.join(HERE, "normals.npz"), **arrays)
 
 
-def write_case_dir(d, case, steps, split=0):
+def write_case_dir(d, case, steps, split=0, overlap=0):
     case.ids.astype(np.uint8).tofile(os.path.join(d, "ids.bin"))
     np.ascontiguousarray(case.rho_u, dtype=np.float64).tofile(os.path.join(d, "rhou.bin"))
     if case.force_field is not None:
@@ -146,6 +146,8 @@ def write_case_dir(d, case, steps, split=0):
         lines.append(f"local {mid} {kind}")
     if split:
         lines.append(f"split {split}")
+    if overlap:
+        lines.append(f"overlap {overlap}")
     with open(os.path.join(d, "case.txt"), "w") as fh:
         fh.write("\n".join(lines) + "\n")
 
@@ -370,6 +372,36 @@ def multiblock_cases():
         ("channel27_smagorinsky_f64_split2x2x1",
          W.turbulent_channel((12, 12, 7), tau=0.51, smago=0.14, u_max=0.05, noise=0.2, seed=8, q=27), 6, None,
          [0, 60], "f64", False),
+    ] + multiblock_local_cases()
+
+
+def multiblock_local_cases():
+    """Local boundaries on several cuboids, with OpenLB's own default overlap of
+    3 (superGeometry.h:94-96) and the overlap cells initialised as the core
+    cells, as an example program's defineRhoU / iniEquilibrium over a material
+    indicator does: the overlap then carries the boundary dynamics that the
+    post-processors of the neighbouring block read.  (With overlap 1 the
+    overlap cells get no boundary dynamics -- boundary::set skips cells within
+    the neighbourhood radius of the padded block's edge -- and OpenLB's split
+    run departs from its undivided one.)  The eighth entry is the overlap."""
+    return [
+        ("cavity19_local_velocity_f64_split2x2x2", W.local_velocity_cavity((16, 17, 18), tau=0.6, lid_u=0.08), 8,
+         None, [0, 1, 100], "f64", True, 3),
+        ("cavity19_local_velocity_f32_split2x2x2", W.local_velocity_cavity((16, 17, 18), tau=0.6, lid_u=0.08), 8,
+         None, [0, 100], "f32", True, 3),
+        # cuboids of 2-3 cells: corner and edge post-processors read cells two blocks over
+        ("cavity19_local_velocity_f64_split4x4x5", W.local_velocity_cavity((8, 9, 10), tau=0.6, lid_u=0.08), 2,
+         None, [0, 1, 100], "f64", True, 3),
+        ("channel19_velocity_pressure_f64_split4x2x1",
+         W.velocity_channel((24, 12, 10), u_in=0.03, rho_out=1.0, tau=0.6), 6, None, [0, 1, 100], "f64", True, 3),
+        ("channel19_interp_velocity_pressure_f64_split4x2x1",
+         W.velocity_channel((24, 12, 10), u_in=0.03, rho_out=1.0, tau=0.6, inlet="interp_velocity",
+                            outlet="interp_pressure"), 6, None, [0, 1, 100], "f64", True, 3),
+        ("step19_local_velocity_f64_split2x1x2", W.step_cavity((14, 12, 16), block=(6, 5, 7), tau=0.6, lid_u=0.05),
+         7, None, [0, 100], "f64", True, 3),
+        ("cavity27_interp_velocity_f64_split2x2x2",
+         W.local_velocity_cavity((10, 8, 9), tau=0.7, lid_u=0.1, q=27, boundary="interp_velocity"), 4, None,
+         [0, 1, 100], "f64", True, 3),
     ]
 
 
@@ -388,12 +420,13 @@ def write_multiblock():
     out_dir = os.path.join(HERE, "multiblock")
     os.makedirs(out_dir, exist_ok=True)
     manifest = {}
-    for name, case, width, k, steps, precision, keep_f0 in multiblock_cases():
+    for name, case, width, k, steps, precision, keep_f0, *rest in multiblock_cases():
+        overlap = rest[0] if rest else 0
         exe = os.path.join(ROOT, "oracle", "_ref", f"olb_ref_{precision}")
         blocks = division_of(case, width)
         arrays = {"ids": case.ids.astype(np.uint8), "rho_u": case.rho_u}
         with tempfile.TemporaryDirectory() as d:
-            write_case_dir(d, case, steps, split=width)
+            write_case_dir(d, case, steps, split=width, overlap=overlap)
             env = dict(os.environ, OLB_REF_SAVE="1") if k is not None else None
             subprocess.run([exe, d], check=True, cwd=d, stdout=subprocess.DEVNULL, env=env)
             if k is not None:
@@ -417,7 +450,7 @@ def write_multiblock():
         for s in kept:
             arrays[f"f_{s}"] = pops[s].astype(store)
         meta = meta_of(case, kept, precision)
-        meta.update({"split_width": width, "blocks": blocks, "checkpoint_step": k,
+        meta.update({"split_width": width, "blocks": blocks, "checkpoint_step": k, "overlap": overlap or 1,
                      "start": "f_0" if keep_f0 else "oracle",
                      "source": "OpenLB 1.8 CPU_SISD, oracle/ref_driver.cpp, CuboidDecomposition::splitRegular"})
         np.savez_compressed(os.path.join(out_dir, name + ".npz"), meta=np.array(json.dumps(meta)), **arrays)

@@ -89,6 +89,42 @@ def streamed_from_outside(case) -> np.ndarray:
     return out
 
 
+def without_dynamics(case) -> np.ndarray:
+    """bool [nx][ny][nz]: cells OpenLB gives no dynamics -- material 0, and
+    cells of a local-boundary material whose classification gives them no
+    boundary type (boundary::set leaves a velocity boundary cell without a
+    discrete normal, and a non-flat InterpolatedPressure cell, without
+    dynamics: src/boundary/localVelocity3D.h, interpolatedPressure3D.h).
+    Their populations only stream; what they hold depends on the overlap."""
+    from openlb_amd.lattice import classify_boundary
+    out = case.ids == 0
+    local = getattr(case, "local", None) or {}
+    if local:
+        cls = classify_boundary(case.ids, case.periodic)
+        flat_normal = (cls[0] == 0) & np.any(cls[1:] != 0, axis=0)
+        for mid, kind in local.items():
+            cells = case.ids == mid
+            if kind == "interp_pressure" or kind == "pressure":
+                out |= cells & ~flat_normal
+            else:
+                out |= cells & (cls[0] == 0) & ~flat_normal
+    return out
+
+
+def flow_mask(case) -> np.ndarray:
+    """bool [q][nx][ny][nz]: the populations of cells with dynamics that come
+    from a cell with dynamics -- every slot a comparison of two runs can hold
+    to account (streamed_from_outside, without_dynamics)."""
+    from oracle.checkpoint import VELOCITIES
+    dead = without_dynamics(case)
+    mask = ~dead[None] & ~streamed_from_outside(case)
+    for i, c in enumerate(VELOCITIES[case.q]):
+        # population i of r came from r - c_i: roll the source map by +c_i
+        src_dead = np.roll(dead, shift=tuple(int(v) for v in c), axis=(0, 1, 2))
+        mask[i] &= ~src_dead
+    return mask
+
+
 def statistics(name: str) -> dict:
     """OpenLB's lattice statistics of the fixture's last snapshot (statistics.json)."""
     with open(os.path.join(GOLDEN, "statistics.json")) as fh:

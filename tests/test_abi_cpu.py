@@ -92,6 +92,45 @@ def test_library_classification_matches_reference(geometry):
     np.testing.assert_array_equal(got, z[f"{geometry}_normals"])
 
 
+SPLITS_FOR_NORMALS = [(2, 1, 1), (1, 3, 2), (2, 2, 2), (3, 2, 3)]
+
+
+@pytest.mark.parametrize("blocks", SPLITS_FOR_NORMALS, ids=["x".join(map(str, b)) for b in SPLITS_FOR_NORMALS])
+@pytest.mark.parametrize("geometry", ["cavity", "channel", "step", "spheres", "spheres_closed"])
+def test_split_block_classification_matches_reference(geometry, blocks):
+    """Each block of a split lattice classifies its wall cells with the
+    materials of its first overlap layer (lbmh_classify_boundary_overlap, as
+    lbmh_set_local_boundary does after lbmh_set_overlap_ids): OpenLB classifies
+    on the block geometry, whose overlap holds the neighbours' materials, so
+    every block must reproduce OpenLB's normals of the whole domain on its own
+    cells.  Without the overlap materials the block faces would read as the
+    end of the domain."""
+    import golden_cases as G
+    from openlb_amd.lattice import classify_boundary
+    from openlb_amd.super_lattice import overlap_ids, split_axis
+    from openlb_amd import workloads as W
+
+    z = np.load(f"{G.GOLDEN}/normals.npz", allow_pickle=False)
+    ids = z[f"{geometry}_ids"]
+    periodic = tuple(int(p) for p in z[f"{geometry}_periodic"])
+    want = z[f"{geometry}_normals"]
+    case = W.Case(geometry, 19, "bgk", ids.shape, periodic, ids, None, 1.0, 0.25, 0.1, (0.0, 0.0, 0.0), {}, {})
+    ranges = [split_axis(n, min(b, n)) for n, b in zip(ids.shape, blocks)]
+    self_periodic = [int(periodic[d] and len(ranges[d]) == 1) for d in range(3)]
+    interfaces = 0
+    for rx in ranges[0]:
+        for ry in ranges[1]:
+            for rz in ranges[2]:
+                lo, hi = (rx[0], ry[0], rz[0]), (rx[1], ry[1], rz[1])
+                sl = tuple(slice(lo[d], hi[d]) for d in range(3))
+                got = classify_boundary(ids[sl], self_periodic, overlap_ids(case, lo, hi))
+                np.testing.assert_array_equal(got, want[(slice(None),) + sl], err_msg=f"block {lo}-{hi}")
+                alone = classify_boundary(ids[sl], self_periodic)
+                interfaces += int(np.any(alone != want[(slice(None),) + sl]))
+    if int(np.prod([len(r) for r in ranges])) > 1:
+        assert interfaces > 0, "no block of this split has a wall cell on a block face"
+
+
 def test_split_axis_cuts_as_openlb_divides_a_cuboid():
     """Cuboid::divide (src/geometry/cuboid.hh:184-211) gives piece k of n cells
     in p parts (n + p - k - 1) / p cells: 10 in 3 -> 4, 3, 3; 7 in 2 -> 4, 3."""

@@ -62,6 +62,106 @@ def test_split_lattice_equals_single_block(gpu_available, label, make, storage, 
 ORACLE_TOL = {"f64": 1e-13, "f32": 1e-5}
 TOL = {"f64": 1e-12, "f32": 2e-5}
 
+# Local boundaries on split lattices: each block classifies its wall cells
+# with its neighbours' materials, and post-processors read the neighbours that
+# lie in other blocks as they stand after the stream and before any
+# post-processor.  With block interfaces more than two cells from every edge
+# and corner, that is what a single block reads too, so the split lattice is
+# bit-identical to the undivided one on every population with dynamics behind
+# it (flow_mask: not the slots streamed in from beyond the domain or from cells
+# without dynamics, whose contents depend on the padded extent).
+LOCAL_CASES = [
+    ("cavity_local_velocity", lambda: W.local_velocity_cavity((17, 15, 19), tau=0.6, lid_u=0.08), "f64"),
+    ("cavity_interp_velocity",
+     lambda: W.local_velocity_cavity((16, 14, 18), tau=0.6, lid_u=0.08, boundary="interp_velocity"), "f64"),
+    ("channel_interp_periodic_z",
+     lambda: W.velocity_channel((26, 12, 14), tau=0.6, inlet="interp_velocity", outlet="interp_pressure",
+                                periodic_z=True), "f64"),
+    ("channel_local_pressure", lambda: W.velocity_channel((24, 13, 11), tau=0.6), "f32"),
+    ("step_local_velocity", lambda: W.step_cavity((14, 12, 16), block=(6, 5, 7), tau=0.6, lid_u=0.05), "f64"),
+    ("cavity27_local_velocity", lambda: W.local_velocity_cavity((10, 12, 11), tau=0.7, lid_u=0.1, q=27), "f32"),
+]
+LOCAL_SPLITS = [(2, 1, 1), (1, 3, 2), (2, 2, 2)]
+
+
+@pytest.mark.parametrize("blocks", LOCAL_SPLITS, ids=["x".join(map(str, b)) for b in LOCAL_SPLITS])
+@pytest.mark.parametrize("label,make,storage", LOCAL_CASES, ids=[c[0] for c in LOCAL_CASES])
+def test_split_lattice_with_local_boundaries_equals_single_block(gpu_available, label, make, storage, blocks):
+    case = make()
+    f0 = O.initial_populations(case, storage)
+    steps = 23
+    print(f"{label} {storage} split {blocks}", flush=True)
+    with BlockLattice.from_case(case, storage) as lat:
+        lat.upload_populations(f0, "f64")
+        lat.initialize()
+        lat.collide_and_stream(steps)
+        ref = lat.download_populations("f64")
+    with SuperLattice(case, storage, blocks) as sup:
+        sup.upload_populations(f0, "f64")
+        sup.initialize()
+        sup.collide_and_stream(steps)
+        got = sup.download_populations("f64")
+    mask = G.flow_mask(case)
+    np.testing.assert_array_equal(got[mask], ref[mask])
+
+
+@pytest.mark.parametrize("boundary", ["velocity", "interp_velocity"])
+def test_split_local_boundaries_do_not_depend_on_stream_order(gpu_available, boundary):
+    """Blocks whose post-processors read each other give the same populations
+    whichever of them lbmh_stream (and lbmh_initialize) reaches first: the
+    joined blocks run their post-processors together, stage by stage, once
+    the last of them has streamed.  Blocks of 2-3 cells put every corner and
+    edge processor next to an interface, reading edges and planes two blocks
+    over; the result must equal the undivided lattice's as well."""
+    from openlb_amd import abi
+    case = W.local_velocity_cavity((8, 9, 10), tau=0.6, lid_u=0.08, boundary=boundary)
+    f0 = O.initial_populations(case, "f64")
+    lib = abi.load()
+    out = []
+    for order in (1, -1):
+        with SuperLattice(case, "f64", (4, 4, 5)) as sup:
+            sup.upload_populations(f0, "f64")
+            blocks = list(sup.blocks.values())[::order]
+            for lat in blocks:
+                lat.initialize()
+            for _ in range(17):
+                for lat in blocks:
+                    abi.check(lib.lbmh_collide(lat.handle, None))
+                for h in sup.halos:
+                    abi.check(lib.lbmh_halo_copy(h, None))
+                for lat in blocks:
+                    abi.check(lib.lbmh_stream(lat.handle, None))
+            out.append(sup.download_populations("f64"))
+    with BlockLattice.from_case(case, "f64") as lat:
+        lat.upload_populations(f0, "f64")
+        lat.initialize()
+        lat.collide_and_stream(17)
+        ref = lat.download_populations("f64")
+    mask = G.flow_mask(case)
+    np.testing.assert_array_equal(out[0][mask], out[1][mask])
+    np.testing.assert_array_equal(out[0][mask], ref[mask])
+
+
+def _local_fixtures():
+    return [n for n in G.multiblock_names() if G.load_multiblock(n)[0].local]
+
+
+@pytest.mark.parametrize("name", _local_fixtures())
+def test_split_initialize_matches_openlb(gpu_available, name):
+    """SuperLattice::initialize() on OpenLB's cuboids: every block's
+    post-processors on the equilibrium state, the ones at block interfaces
+    reading their neighbours in the other blocks, reach OpenLB's state after
+    initialize() (its f_0)."""
+    case, meta, arrays = G.load_multiblock(name)
+    precision = meta["precision"]
+    with SuperLattice(case, precision, tuple(meta["blocks"])) as sup:
+        sup.upload_populations(O.initial_populations(case, precision), "f64")
+        sup.initialize()
+        got = sup.download_populations("f64")
+    err = G.rel_error(got, arrays["f_0"].astype(np.float64), case.q, G.flow_mask(case))
+    print(f"{name} initialize: rel {err:.3e}", flush=True)
+    assert err <= TOL[precision]
+
 
 @pytest.mark.parametrize("name", G.multiblock_names())
 def test_split_lattice_matches_openlb_multicuboid(gpu_available, name):
@@ -69,7 +169,7 @@ def test_split_lattice_matches_openlb_multicuboid(gpu_available, name):
     case, meta, arrays = G.load_multiblock(name)
     precision = meta["precision"]
     f0 = arrays["f_0"].astype(np.float64) if meta["start"] == "f_0" else O.initial_populations(case, precision)
-    flow = (case.ids != 0)[None] & ~G.streamed_from_outside(case)
+    flow = G.flow_mask(case)
     with SuperLattice(case, precision, tuple(meta["blocks"])) as sup:
         assert len(sup.blocks) == int(np.prod(meta["blocks"]))
         sup.upload_populations(f0, "f64")
@@ -105,7 +205,7 @@ def test_split_lattice_resumes_from_openlb_checkpoint(gpu_available, name):
     case, meta, arrays = G.load_multiblock(name)
     k, real = meta["checkpoint_step"], meta["precision"]
     raw = _decoded(arrays)
-    flow = (case.ids != 0)[None] & ~G.streamed_from_outside(case)
+    flow = G.flow_mask(case)
     with SuperLattice(case, real, tuple(meta["blocks"])) as sup:
         sup.restore(raw, real)
         assert sup.checkpoint(real) == raw

@@ -46,7 +46,7 @@ def test_oracle_matches_multicuboid_reference(name):
         assert err <= TOL[precision], f"{name} step {s}: rel error {err:.3e}"
         # cells with dynamics, except populations that stream in from beyond a
         # non-periodic face (what the overlap there holds is each side's own)
-        flow = (case.ids != 0)[None] & ~G.streamed_from_outside(case)
+        flow = G.flow_mask(case)
         err_all = G.rel_error(got, ref, case.q, flow)
         assert err_all <= TOL[precision], f"{name} step {s} (cells with dynamics): {err_all:.3e}"
 
@@ -200,7 +200,7 @@ def test_openlb_resumes_from_library_split_checkpoint(name):
         subprocess.run([exe, d], check=True, cwd=d, stdout=subprocess.DEVNULL,
                        env=dict(os.environ, OLB_REF_LOAD=os.path.join(d, "library")))
         got = np.fromfile(os.path.join(d, f"pops_{m - start}.bin")).reshape((case.q,) + tuple(case.shape))
-    flow = (case.ids != 0)[None] & ~G.streamed_from_outside(case)
+    flow = G.flow_mask(case)
     err = G.rel_error(got, arrays[f"f_{m}"].astype(np.float64), case.q, flow)
     print(f"OpenLB resumed from the library's step-{start} split checkpoint of {name}: rel {err:.2e}", flush=True)
     assert err <= TOL[real], err
