@@ -5,6 +5,9 @@
  * x = nx - 1 of one boundary family, chosen on the command line:
  *   (default)         boundary::LocalVelocity + boundary::LocalPressure
  *   --interpolated    boundary::InterpolatedVelocity + boundary::InterpolatedPressure
+ * --cuboids N divides the domain into N cuboids (CuboidDecomposition with
+ * OpenLB's default overlap of 3), whose blocks exchange their overlap and
+ * whose boundary post-processors read cells of the neighbouring blocks.
  * The inflow velocity is uniform over the inlet face; the fluid starts at
  * rest.  After the steps the program writes the physical velocity and
  * pressure (SuperVTMwriter3D) and prints "nx, ny, nz, steps, MLUPs,
@@ -57,6 +60,7 @@ int main(int argc, char** argv)
   const int nz = args.getValueOrFallback<int>("--nz", 12);
   const std::size_t steps = args.getValueOrFallback<std::size_t>("--steps", 200);
   const bool interpolated = args.contains("--interpolated");
+  const int nCuboids = args.getValueOrFallback<int>("--cuboids", 1);
   singleton::directories().setOutputDir("./tmp/");
 
   // ny - 1 voxels per unit length, lattice inflow velocity 0.04, Re 20
@@ -67,7 +71,7 @@ int main(int argc, char** argv)
   const T lx = T(nx - 1) * dx;
   IndicatorCuboid3D<T> box(Vector<T,3>(lx + 0.25 * dx, T(ny - 1) * dx + 0.25 * dx, T(nz - 1) * dx + 0.25 * dx),
                            Vector<T,3>(0, 0, 0));
-  CuboidDecomposition3D<T> cuboids(box, dx, 1);
+  CuboidDecomposition3D<T> cuboids(box, dx, nCuboids);
   BlockLoadBalancer<T> load(0, 1, cuboids.size(), 0);
   SuperGeometry<T,3> geometry(cuboids, load);
   prepareGeometry(geometry, box, dx, lx);
@@ -109,5 +113,6 @@ int main(int argc, char** argv)
   writer.write(pressure);
   std::cout << nx << ", " << ny << ", " << nz << ", " << steps << ", " << timer.getTotalMLUPs() << ", "
             << static_cast<int>(lattice.getBlock(0).getPlatform()) << std::endl;
+  std::cout << "cuboids " << cuboids.size() << std::endl;
   return 0;
 }

@@ -31,6 +31,26 @@ namespace olb {
 
 namespace gpu::hip {
 
+/// The device halos of the process's GPU_HIP block communicators, by
+/// communicator.  A block's first post-processing (BlockLattice::initialize)
+/// comes before any communication; its post-processors that read cells of
+/// other blocks find them through the halos, so every halo is built first.
+template <typename T, typename DESCRIPTOR>
+struct HaloBuilders {
+  static std::map<const void*, std::function<void()>>& all()
+  {
+    static std::map<const void*, std::function<void()>> builders;
+    return builders;
+  }
+
+  static void buildAll()
+  {
+    for (auto& entry : all()) {
+      entry.second();
+    }
+  }
+};
+
 template <typename T, typename DESCRIPTOR>
 class BlockState final : public PopulationMirror {
 public:
@@ -253,12 +273,16 @@ private:
       }
     }
 
+    // the first overlap layer as the classification sees it: the materials of
+    // the neighbouring blocks' cells (or periodic images), 0 beyond the domain
+    const std::vector<std::uint8_t> ring = overlapMaterials(byDynamics);
     if (local) {
-      crossCheckBoundaries(ids, role);
+      crossCheckBoundaries(ids, ring, role);
     }
     // everything above is host work that validates the block; the device comes last
     ensureLattice(bulk ? bulk->info.bulkKind : LBMH_BGK);
     check(lbmh_set_dynamics_ids(_lattice, ids.data()), "lbmh_set_dynamics_ids");
+    check(lbmh_set_overlap_ids(_lattice, ring.data()), "lbmh_set_overlap_ids");
     for (const auto& [key, id] : walls) {
       const double u[3] = {double(key[1]), double(key[2]), double(key[3])};
       check(lbmh_set_wall(_lattice, id, double(key[0]), u), "lbmh_set_wall");
@@ -281,17 +305,47 @@ private:
     std::fill(_uploaded.begin(), _uploaded.end(), false);
   }
 
-  /// The library classifies local-boundary cells from the id map as OpenLB's
-  /// computeBoundaryTypeAndNormal does; every cell must come out with the type
-  /// and normal its OpenLB dynamics (and post-processor) were chosen for.
-  void crossCheckBoundaries(const std::vector<std::uint8_t>& ids, const std::vector<const DynamicsEntry*>& role)
+  /// Materials of the first overlap layer, padded [nx+2][ny+2][nz+2] (core
+  /// entries 0): OpenLB defines dynamics on the overlap cells as on the core
+  /// ones (defineDynamics and boundary::set visit every spatial location,
+  /// src/core/blockLattice.hh:277-301, src/boundary/setBoundary.h:83-101), so
+  /// a cell's collision operator tells fluid (1), outside (0) and wall (2)
+  /// apart there as in the core
+  std::vector<std::uint8_t> overlapMaterials(
+    const std::map<const Dynamics<T,DESCRIPTOR>*, const DynamicsEntry*>& byDynamics) const
+  {
+    const int nx = _block.getNx(), ny = _block.getNy(), nz = _block.getNz();
+    std::vector<std::uint8_t> ring(std::size_t(nx + 2) * (ny + 2) * (nz + 2), 0);
+    for (int x = -1; x <= nx; ++x) {
+      for (int y = -1; y <= ny; ++y) {
+        for (int z = -1; z <= nz; ++z) {
+          if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz) {
+            continue;
+          }
+          auto it = byDynamics.find(_block.getDynamics(_block.getCellId(x, y, z)));
+          const Role r = it == byDynamics.end() ? Role::Unsupported : it->second->info.role;
+          const std::uint8_t m = r == Role::None ? LBMH_ID_NONE : (r == Role::Bulk ? LBMH_ID_BULK : LBMH_ID_BOUNCE_BACK);
+          ring[(std::size_t(x + 1) * (ny + 2) + std::size_t(y + 1)) * (nz + 2) + std::size_t(z + 1)] = m;
+        }
+      }
+    }
+    return ring;
+  }
+
+  /// The library classifies local-boundary cells from the id map (and the
+  /// overlap materials) as OpenLB's computeBoundaryTypeAndNormal does; every
+  /// cell must come out with the type and normal its OpenLB dynamics (and
+  /// post-processor) were chosen for.
+  void crossCheckBoundaries(const std::vector<std::uint8_t>& ids, const std::vector<std::uint8_t>& ring,
+                            const std::vector<const DynamicsEntry*>& role)
   {
     const int nx = _block.getNx(), ny = _block.getNy(), nz = _block.getNz();
     const std::size_t ncore = ids.size();
     const int extent[3] = {nx, ny, nz};
     const int periodic[3] = {0, 0, 0};
     std::vector<std::int8_t> cls(4 * ncore);
-    check(lbmh_classify_boundary(extent, periodic, ids.data(), cls.data()), "lbmh_classify_boundary");
+    check(lbmh_classify_boundary_overlap(extent, periodic, ids.data(), ring.data(), cls.data()),
+          "lbmh_classify_boundary_overlap");
     std::map<CellID, const PostEntry*> post;
     for (const auto& p : _posts) {
       for (CellID iCell : *p.cells) {
@@ -528,13 +582,16 @@ public:
   }
 
   /// A post-processor the device runs itself: inside a time step lbmh_stream
-  /// ran it already, outside (BlockLattice::initialize) it runs once now
+  /// ran it already, outside (BlockLattice::initialize) it runs once now --
+  /// for blocks whose post-processors read one another, once the last of
+  /// them got here (lbmh_initialize), through halos built now
   void postStream()
   {
     if (_postDone) {
       return;
     }
     prepare();
+    HaloBuilders<T,DESCRIPTOR>::buildAll();
     check(lbmh_initialize(_lattice), "lbmh_initialize");
     _postDone = true;
   }

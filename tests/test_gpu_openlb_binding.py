@@ -105,3 +105,71 @@ def test_duct_boundaries_match_openlb(gpu_available, family, tmp_path):
         err = max_err(a, b)
         print(f"duct {family} {f}: max |gpu_hip - cpu| = {err:.3e} of max {scale:.3e}", flush=True)
         assert scale > 0 and err <= 1e-6 * scale
+
+
+def _pieces(cwd, field: str) -> list:
+    """The per-cuboid pieces SuperVTMwriter3D wrote for a field, in cuboid order."""
+    import glob
+    files = sorted(glob.glob(os.path.join(str(cwd), "tmp", "vtkData", "data", f"{field}_iT0000000iC*.vti")))
+    return [vtk.read_vti_arrays(f)[field] for f in files]
+
+
+@pytest.mark.skipif(not all(os.path.exists(b) for b in DUCT.values()),
+                    reason="duct3d binaries not built (make -C integration/openlb)")
+@pytest.mark.parametrize("family", ["local", "interpolated"])
+def test_duct_on_several_cuboids_matches_openlb(gpu_available, family, tmp_path):
+    """The duct on 6 cuboids (CuboidDecomposition, OpenLB's default overlap of
+    3): the inlet and outlet faces and their post-processors cross cuboid
+    interfaces.  Each GPU_HIP block classifies its boundary cells with the
+    materials of its overlap and reads the neighbours of its post-processors
+    from the neighbouring blocks (lbmh_set_overlap_ids); every piece of the
+    GPU_HIP run's output equals the CPU run's piece to the VTK float precision,
+    and the CPU run on 6 cuboids equals OpenLB's own run on one."""
+    out = {}
+    runs = [("duct3d_gpu_hip", 6), ("duct3d_cpu", 6), ("duct3d_cpu", 1)]
+    for name, n in runs:
+        cwd = tmp_path / f"{name}_{n}"
+        cwd.mkdir()
+        cmd = [DUCT[name], "--steps", "300", "--cuboids", str(n)]
+        cmd += ["--interpolated"] if family == "interpolated" else []
+        print("running", " ".join(cmd), flush=True)
+        p = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True, timeout=240)
+        assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+        assert f"cuboids {n}" in p.stdout
+        out[(name, n)] = {f: _pieces(cwd, f) for f in FIELDS}
+    for f in FIELDS:
+        gpu, cpu = out[("duct3d_gpu_hip", 6)][f], out[("duct3d_cpu", 6)][f]
+        assert len(gpu) == len(cpu) == 6
+        scale = max(float(np.abs(b).max()) for b in cpu)
+        err = max(max_err(a, b) for a, b in zip(gpu, cpu))
+        print(f"duct {family} 6 cuboids {f}: max |gpu_hip - cpu| = {err:.3e} of max {scale:.3e}", flush=True)
+        assert scale > 0 and err <= 1e-6 * scale
+        # OpenLB itself: the 6 pieces hold the one-cuboid run's values (a
+        # piece also holds its overlap-1 ring; compare the value sets' extremes)
+        one = out[("duct3d_cpu", 1)][f][0]
+        assert abs(max(float(b.max()) for b in cpu) - float(one.max())) <= 1e-6 * scale
+        assert abs(min(float(b.min()) for b in cpu) - float(one.min())) <= 1e-6 * scale
+
+
+@pytest.mark.parametrize("cuboids", [2, 4])
+def test_cavity_on_several_cuboids_matches_openlb(gpu_available, cuboids, tmp_path):
+    """cavity3dBenchmark (T = double, LocalVelocity on every wall) with
+    --cuboids-per-process: the edges and corners of the cavity and their
+    post-processors sit on cuboid interfaces.  Every piece of the GPU_HIP run
+    equals the CPU run's."""
+    out = {}
+    for name in ("cavity3d_gpu_hip_f64", "cavity3d_cpu_f64"):
+        cwd = tmp_path / name
+        cwd.mkdir()
+        cmd = [BINARIES[name], "--size", str(SIZE), "--steps", str(STEPS), "--cuboids-per-process", str(cuboids)]
+        print("running", " ".join(cmd), flush=True)
+        p = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True, timeout=240)
+        assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+        out[name] = {f: _pieces(cwd, f) for f in FIELDS}
+    for f in FIELDS:
+        gpu, cpu = out["cavity3d_gpu_hip_f64"][f], out["cavity3d_cpu_f64"][f]
+        assert len(gpu) == len(cpu) >= 2
+        scale = max(float(np.abs(b).max()) for b in cpu)
+        err = max(max_err(a, b) for a, b in zip(gpu, cpu))
+        print(f"cavity {cuboids} cuboids {f}: max |gpu_hip - cpu| = {err:.3e} of max {scale:.3e}", flush=True)
+        assert scale > 0 and err <= 1e-6 * scale
