@@ -279,7 +279,7 @@ int lbmh_reduce_moments(lbmh_lattice* lattice, double out[4]);
 int lbmh_reduce_moments_noalloc(lbmh_lattice* lattice, double* out_device, void* stream);
 
 /* Local (regularized) boundaries, OpenLB's boundary::LocalVelocity and
- * boundary::LocalPressure with BGK bulk dynamics mixed in
+ * boundary::LocalPressure with BGKdynamics mixed in, their default mixin
  * (src/boundary/localVelocity3D.h:36-116, localPressure3D.h:36-66, set by
  * boundary::set, src/boundary/setBoundary.h:83-101).  Every core cell that
  * currently carries dynamics id `id` (the boundary's material) is classified
@@ -297,7 +297,11 @@ int lbmh_reduce_moments_noalloc(lbmh_lattice* lattice, double* out_device, void*
  * per (type, normal); rho/u are what defineRhoU gives the material: the
  * velocity of a velocity boundary, the density (and tangential velocity) of a
  * pressure boundary.  The post-processors run after every step's shift, edges
- * before corners (PostStream priorities).  Requires LBMH_BGK bulk dynamics;
+ * before corners (PostStream priorities).  The bulk may run any kind: the
+ * boundary cells collide with the BGK mixin at OMEGA (as next to the
+ * Smagorinsky bulk of examples/turbulence/aorta3d/aorta3d.cpp:108-127), and
+ * the post-processors read a forced bulk neighbour's velocity with F/2 added
+ * (momenta/elements.h:1473-1488).  Other mixins are not built.
  * lbmh_set_dynamics_ids drops all local boundaries. */
 #define LBMH_LOCAL_VELOCITY 0
 #define LBMH_LOCAL_PRESSURE 1

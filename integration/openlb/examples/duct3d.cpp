@@ -5,6 +5,8 @@
  * x = nx - 1 of one boundary family, chosen on the command line:
  *   (default)         boundary::LocalVelocity + boundary::LocalPressure
  *   --interpolated    boundary::InterpolatedVelocity + boundary::InterpolatedPressure
+ * --smagorinsky runs a SmagorinskyBGKdynamics bulk (C = 0.14) next to the
+ * boundaries' own BGK mixin, as aorta3d.cpp does.
  * --cuboids N divides the domain into N cuboids (CuboidDecomposition with
  * OpenLB's default overlap of 3), whose blocks exchange their overlap and
  * whose boundary post-processors read cells of the neighbouring blocks.
@@ -40,9 +42,14 @@ void prepareGeometry(SuperGeometry<T,3>& geometry, IndicatorF3D<T>& box, T dx, T
 }
 
 template <typename INLET, typename OUTLET>
-void prepareLattice(SuperLattice<T,DESCRIPTOR>& lattice, SuperGeometry<T,3>& geometry, T omega)
+void prepareLattice(SuperLattice<T,DESCRIPTOR>& lattice, SuperGeometry<T,3>& geometry, T omega, bool smagorinsky)
 {
-  lattice.defineDynamics<Bulk>(geometry, 1);
+  if (smagorinsky) {
+    lattice.defineDynamics<SmagorinskyBGKdynamics<T,DESCRIPTOR>>(geometry, 1);
+    lattice.setParameter<collision::LES::SMAGORINSKY>(T(0.14));
+  } else {
+    lattice.defineDynamics<Bulk>(geometry, 1);
+  }
   boundary::set<boundary::BounceBack>(lattice, geometry, 2);
   boundary::set<INLET>(lattice, geometry, 3);
   boundary::set<OUTLET>(lattice, geometry, 4);
@@ -61,6 +68,7 @@ int main(int argc, char** argv)
   const std::size_t steps = args.getValueOrFallback<std::size_t>("--steps", 200);
   const bool interpolated = args.contains("--interpolated");
   const int nCuboids = args.getValueOrFallback<int>("--cuboids", 1);
+  const bool smagorinsky = args.contains("--smagorinsky");
   singleton::directories().setOutputDir("./tmp/");
 
   // ny - 1 voxels per unit length, lattice inflow velocity 0.04, Re 20
@@ -81,10 +89,10 @@ int main(int argc, char** argv)
   const T omega = converter.getLatticeRelaxationFrequency();
   if (interpolated) {
     prepareLattice<boundary::InterpolatedVelocity<T,DESCRIPTOR,Bulk>,
-                   boundary::InterpolatedPressure<T,DESCRIPTOR,Bulk>>(lattice, geometry, omega);
+                   boundary::InterpolatedPressure<T,DESCRIPTOR,Bulk>>(lattice, geometry, omega, smagorinsky);
   } else {
     prepareLattice<boundary::LocalVelocity<T,DESCRIPTOR,Bulk>,
-                   boundary::LocalPressure<T,DESCRIPTOR,Bulk>>(lattice, geometry, omega);
+                   boundary::LocalPressure<T,DESCRIPTOR,Bulk>>(lattice, geometry, omega, smagorinsky);
   }
 
   AnalyticalConst3D<T,T> rho(1);

@@ -174,6 +174,8 @@ __global__ __launch_bounds__(64) void ensemble_post_kernel(const EnsembleEntry<Q
       a.cells = nullptr;
       a.recs = nullptr;
       a.remote = nullptr;
+      a.forced = 0;
+      a.force = nullptr;
       a.count = 0;
       a.nx = int(e.nx);
       a.ny = int(e.ny);

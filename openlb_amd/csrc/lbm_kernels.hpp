@@ -346,14 +346,6 @@ __global__ __launch_bounds__(256) void halo_copy_kernel(S* dst, uint64_t dst_str
       src[uint64_t(i) * src_stride + wrap_index(p.y, ssh.s[i], src_n)];
 }
 
-// Half the FORCE of core cell c, component d: the per-cell field [3][ncore]
-// where the lattice has one, else the uniform value (F/2 given in FP64).
-template <typename V>
-__device__ __forceinline__ V half_force(const V* force, uint32_t ncore, uint32_t c, int d, double uniform_half)
-{
-  return force ? force[uint64_t(d) * ncore + c] * V(0.5) : V(uniform_half);
-}
-
 // Equilibrium from per-cell (rho, u), iniEquilibrium semantics.
 template <int Q, typename S>
 __global__ void ini_equilibrium_kernel(const double* rho_u, const uint8_t* ids, S* pop, CoreGeom g,

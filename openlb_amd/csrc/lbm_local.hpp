@@ -433,6 +433,14 @@ __device__ __forceinline__ void collide_flat(V (&f)[Q], const LocalInfo& li, con
 
 // ---- post-processors (PostStream stage) ----------------------------------
 
+// Half the FORCE of core cell c, component d: the per-cell field [3][ncore]
+// where the lattice has one, else the uniform value (F/2 given in FP64).
+template <typename V>
+__device__ __forceinline__ V half_force(const V* force, uint32_t ncore, uint32_t c, int d, double uniform_half)
+{
+  return force ? force[uint64_t(d) * ncore + c] * V(0.5) : V(uniform_half);
+}
+
 template <int Q>
 struct PostArgs {
   void* pop;
@@ -445,6 +453,12 @@ struct PostArgs {
   const uint32_t* cells;    // core indices of the cells to process
   const struct PostCell* recs;  // the same cells with their neighbours resolved (lattices), or null
   const void* remote;       // rho, u of the neighbours in other blocks (4 values each, arithmetic type), or null
+  // bulk cells of a forced kind report u + F/2 (forcing::Guo's momenta,
+  // momenta/elements.h:1473-1488): the per-cell FORCE [3][ncore] in the
+  // arithmetic type, or null and the uniform F/2
+  uint32_t forced;
+  const void* force;
+  double half_force[3];
   uint32_t count;
   int nx, ny, nz;
   uint32_t sx, sy;
@@ -496,6 +510,15 @@ __device__ void rho_u_at(const PostArgs<Q>& a, uint32_t n, int id, V& rho, V (&u
     u[0] = j[0] / rho;
     u[1] = j[1] / rho;
     u[2] = j[2] / rho;
+    if (a.forced) {
+      // the core index of padded cell n, for a per-cell FORCE
+      const uint32_t x = n / a.sx - 1, y = (n % a.sx) / a.sy - 1, z = n % a.sy - 1;
+      const uint32_t ncore = uint32_t(a.nx) * uint32_t(a.ny) * uint32_t(a.nz);
+      const uint32_t c = (x * uint32_t(a.ny) + y) * uint32_t(a.nz) + z;
+      const V* force = static_cast<const V*>(a.force);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) u[d] += half_force<V>(force, ncore, c, d, a.half_force[d]);
+    }
     return;
   }
   const double* w = a.walls + 4 * id;

@@ -209,6 +209,9 @@ PostArgs<Q> post_args(const lbmh_lattice* L)
   a.cells = nullptr;
   a.recs = nullptr;
   a.remote = L->d_remote;
+  a.forced = 0;
+  a.force = nullptr;
+  a.half_force[0] = a.half_force[1] = a.half_force[2] = 0.0;
   a.count = 0;
   a.nx = L->n[0];
   a.ny = L->n[1];
@@ -220,6 +223,17 @@ PostArgs<Q> post_args(const lbmh_lattice* L)
   a.div_y = FastDiv(uint32_t(L->n[1]));
   a.omega = L->params.omega;
   return a;
+}
+
+// The post-processors read a forced bulk neighbour's velocity with F/2 added,
+// as its momenta report it
+template <int Q>
+void post_forcing(const lbmh_lattice* L, PostArgs<Q>& a)
+{
+  if (L->kind != LBMH_FORCED_BGK && L->kind != LBMH_FORCED_TRT) return;
+  a.forced = 1;
+  a.force = L->d_force;
+  for (int d = 0; d < 3; ++d) a.half_force[d] = L->params.force[d] * 0.5;
 }
 
 // The cells of a post-processor list with their neighbours resolved, as the
@@ -363,6 +377,7 @@ int launch_remote(const lbmh_halo* H, int stage, int64_t at, hipStream_t stream)
 {
   const lbmh_lattice* src = H->src;
   PostArgs<Q> a = post_args<Q>(src);
+  post_forcing<Q>(src, a);
   if (src->steps + 1 == at) {
     const int64_t N = src->n_padded;
     for (int i = 0; i < Q; ++i) {
@@ -409,6 +424,7 @@ template <int Q, typename S>
 int launch_post_stage(lbmh_lattice* L, int stage, hipStream_t stream)
 {
   PostArgs<Q> a = post_args<Q>(L);
+  post_forcing<Q>(L, a);
   if (stage == kStagePlanes && !L->h_planes.empty()) {
     using V = typename StorageTraits<S>::V;
     a.cells = L->d_planes;
@@ -544,20 +560,15 @@ int launch_kind(lbmh_lattice* L, hipStream_t stream, int phase)
   // the dynamic LDS is never touched: it only limits resident workgroups per CU
   const unsigned lds = unsigned(L->launch.lds_bytes);
   if (phase & kPhaseCollide) {
-  bool launched = false;
-  if constexpr (KIND == kBGK) {
-    if (L->has_local) {
-      hipLaunchKernelGGL((collide_stream_kernel<Q, S, KIND, true, kLocalFlat>), dim3(grid), dim3(kBlock), lds, stream, a);
-      LBMH_HIP(hipGetLastError());
-      const uint32_t count = uint32_t(L->h_local.size());
-      if (count) {
-        hipLaunchKernelGGL((local_boundary_kernel<Q, S, KIND>), dim3(grid_for(count, 64)), dim3(64), 0, stream, a,
-                           static_cast<const uint32_t*>(L->d_local), count);
-      }
-      launched = true;
+  if (L->has_local) {
+    // the boundary cells collide with their BGK mixin (OMEGA) whatever the bulk kind
+    hipLaunchKernelGGL((collide_stream_kernel<Q, S, KIND, true, kLocalFlat>), dim3(grid), dim3(kBlock), lds, stream, a);
+    LBMH_HIP(hipGetLastError());
+    const uint32_t count = uint32_t(L->h_local.size());
+    if (count) {
+      hipLaunchKernelGGL((local_boundary_kernel<Q, S, KIND>), dim3(grid_for(count, 64)), dim3(64), 0, stream, a,
+                         static_cast<const uint32_t*>(L->d_local), count);
     }
-  }
-  if (launched) {
   } else if (L->uniform_bulk) {
     hipLaunchKernelGGL((collide_stream_kernel<Q, S, KIND, false>), dim3(grid), dim3(kBlock), lds, stream, a);
   } else {
@@ -886,7 +897,6 @@ int lbmh_set_local_boundary(lbmh_lattice* L, int id, int kind, double rho, const
   if (!L || !u) return fail(LBMH_E_INVALID, "null argument");
   if (kind < LBMH_LOCAL_VELOCITY || kind > LBMH_INTERP_PRESSURE) return fail(LBMH_E_INVALID, "unknown local boundary kind");
   if (id < 2 || id >= kMaxWalls) return fail(LBMH_E_INVALID, "boundary id must be in [2, 255]");
-  if (L->kind != LBMH_BGK) return fail(LBMH_E_INVALID, "local boundaries need BGK bulk dynamics");
   if (L->h_ids.size() != L->ncore) return fail(LBMH_E_INVALID, "set the dynamics ids first");
   if ((kind == LBMH_LOCAL_PRESSURE || kind == LBMH_INTERP_PRESSURE) && !(rho > 0))
     return fail(LBMH_E_INVALID, "pressure boundary density must be positive");

@@ -958,6 +958,8 @@ int lbmh_ensemble_field_export(lbmh_ensemble* E, int m, int field, int overlap, 
     a.info = E->d_info + size_t(m) * kEnsembleWalls;
     a.cells = nullptr;
     a.remote = nullptr;
+    a.forced = 0;
+    a.force = nullptr;
     a.count = 0;
     a.nx = int(en.nx);
     a.ny = int(en.ny);

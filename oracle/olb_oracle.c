@@ -311,7 +311,6 @@ int olo_define_local(olo_lattice* L, const uint8_t* materials, int material, int
                      double rho, double ux, double uy, double uz)
 {
   if (kind < OLO_LOCAL_VELOCITY || kind > OLO_INTERP_PRESSURE) { set_err("olo_define_local: kind"); return -1; }
-  if (L->kind != OLO_BGK) { set_err("olo_define_local: local boundaries need BGK bulk dynamics"); return -1; }
   const int p = L->pad;
   if (!L->lkind) {
     L->lkind = (uint8_t*)calloc((size_t)L->ncells, 1);

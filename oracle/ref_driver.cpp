@@ -50,10 +50,11 @@
 //                    type -1 elsewhere.
 //
 // A material listed as `local <id> velocity` (`pressure`) gets OpenLB's
-// LocalVelocity (LocalPressure) boundary with the bulk dynamics mixed in, as
-// cavity3d.cpp:72-73 sets it; `interp_velocity` (`interp_pressure`) gets
-// InterpolatedVelocity (InterpolatedPressure).  Its `wall` line gives the
-// velocity (density) that defineRhoU assigns.
+// LocalVelocity (LocalPressure) boundary with BGKdynamics mixed in -- the
+// bulk dynamics of cavity3d.cpp:72-73, and the boundaries' default mixin next
+// to any other bulk (aorta3d.cpp:118-127); `interp_velocity`
+// (`interp_pressure`) gets InterpolatedVelocity (InterpolatedPressure).  Its
+// `wall` line gives the velocity (density) that defineRhoU assigns.
 //
 // The setup follows the reference's example programs:
 //   geometry/materials  examples/laminar/cavity3dBenchmark/cavity3d.cpp:39-57
@@ -302,20 +303,18 @@ static void runCase(const CaseSpec& c, const std::string& dir)
     lattice.template defineDynamics<BounceBackVelocity<T,DESCRIPTOR>>(geometry, w.id);
   }
   for (const auto& [id, kind] : c.local) {
-    // local boundaries are generated for plain BGK bulk dynamics only (as in
-    // cavity3d.cpp), which keeps the driver's compile time down
-    if constexpr (std::is_same_v<BULK, BGKdynamics<T,DESCRIPTOR>>) {
-      if (kind == "velocity") {
-        boundary::set<boundary::LocalVelocity<T,DESCRIPTOR,BULK>>(lattice, geometry, id);
-      } else if (kind == "pressure") {
-        boundary::set<boundary::LocalPressure<T,DESCRIPTOR,BULK>>(lattice, geometry, id);
-      } else if (kind == "interp_velocity") {
-        boundary::set<boundary::InterpolatedVelocity<T,DESCRIPTOR,BULK>>(lattice, geometry, id);
-      } else {
-        boundary::set<boundary::InterpolatedPressure<T,DESCRIPTOR,BULK>>(lattice, geometry, id);
-      }
+    // the boundaries' own default mixin, BGKdynamics, whatever the bulk
+    // dynamics: boundary::set<boundary::InterpolatedVelocity>(lattice,
+    // geometry, m) as aorta3d.cpp:118-127 writes it next to a Smagorinsky bulk
+    using MIXIN = BGKdynamics<T,DESCRIPTOR>;
+    if (kind == "velocity") {
+      boundary::set<boundary::LocalVelocity<T,DESCRIPTOR,MIXIN>>(lattice, geometry, id);
+    } else if (kind == "pressure") {
+      boundary::set<boundary::LocalPressure<T,DESCRIPTOR,MIXIN>>(lattice, geometry, id);
+    } else if (kind == "interp_velocity") {
+      boundary::set<boundary::InterpolatedVelocity<T,DESCRIPTOR,MIXIN>>(lattice, geometry, id);
     } else {
-      throw std::runtime_error("local boundaries need dyn bgk");
+      boundary::set<boundary::InterpolatedPressure<T,DESCRIPTOR,MIXIN>>(lattice, geometry, id);
     }
   }
   lattice.template setParameter<descriptors::OMEGA>(T(c.omega));

@@ -20,6 +20,7 @@ UQ module (``--statistics-only`` / ``--normals-only`` / ``--checkpoints-only``
 """
 from __future__ import annotations
 
+import dataclasses
 import hashlib
 import json
 import os
@@ -88,6 +89,28 @@ def cases():
     c = W.velocity_channel((24, 12, 10), u_in=0.03, rho_out=1.0, tau=0.6, inlet="interp_velocity",
                            outlet="interp_pressure")
     out.append(("channel19_interp_velocity_pressure_f64", c, [0, 200], "f64"))
+    # local boundaries next to other bulk dynamics keep their default BGK
+    # mixin: aorta3d.cpp runs a SmagorinskyBGK bulk with InterpolatedVelocity /
+    # InterpolatedPressure faces, testFlow3dSolver.h a ForcedBGK bulk with
+    # LocalVelocity walls
+    c = W.velocity_channel((24, 12, 10), u_in=0.03, rho_out=1.0, tau=0.52, inlet="interp_velocity",
+                           outlet="interp_pressure")
+    c = dataclasses.replace(c, kind="smagorinsky", smago=0.14)
+    out.append(("channel19_interp_smagorinsky_f64", c, [0, 200], "f64"))
+    out.append(("channel19_interp_smagorinsky_f32", c, [0, 100], "f32"))
+    c = W.velocity_channel((24, 12, 10), u_in=0.02, rho_out=1.0, tau=0.6)
+    ff = np.zeros((3,) + c.shape)
+    y = np.arange(c.shape[1])[None, :, None]
+    ff[0] = 2e-5 * np.sin(2 * np.pi * y / c.shape[1]) * (c.ids == 1)
+    ff[1] = 5e-6 * (c.ids == 1)
+    c = dataclasses.replace(c, kind="forced_bgk", force_field=ff)
+    out.append(("channel19_local_forced_bgk_f64", c, [0, 200], "f64"))
+    c = W.local_velocity_cavity((12, 10, 14), tau=0.6, lid_u=0.08)
+    c = dataclasses.replace(c, kind="trt", magic=0.25)
+    out.append(("cavity19_local_velocity_trt_f64", c, [0, 200], "f64"))
+    c = W.local_velocity_cavity((10, 8, 9), tau=0.7, lid_u=0.1, q=27, boundary="interp_velocity")
+    c = dataclasses.replace(c, kind="forced_trt", magic=3.0 / 16.0, force=(1e-5, 0.0, -5e-6))
+    out.append(("cavity27_interp_forced_trt_f64", c, [0, 100], "f64"))
     # a FORCE field that differs from cell to cell (Kolmogorov shear plus
     # transverse components), periodic and between walls with obstacles
     c = W.kolmogorov_flow((16, 14, 12), amplitude=2e-5, tau=0.62)
@@ -402,6 +425,11 @@ def multiblock_local_cases():
         ("cavity27_interp_velocity_f64_split2x2x2",
          W.local_velocity_cavity((10, 8, 9), tau=0.7, lid_u=0.1, q=27, boundary="interp_velocity"), 4, None,
          [0, 1, 100], "f64", True, 3),
+        # a Smagorinsky bulk with interpolated faces (their BGK mixin), as aorta3d.cpp
+        ("channel19_interp_smagorinsky_f64_split4x2x1",
+         dataclasses.replace(W.velocity_channel((24, 12, 10), u_in=0.03, rho_out=1.0, tau=0.52,
+                                                inlet="interp_velocity", outlet="interp_pressure"),
+                             kind="smagorinsky", smago=0.14), 6, None, [0, 1, 100], "f64", True, 3),
     ]
 
 

@@ -116,6 +116,32 @@ def _pieces(cwd, field: str) -> list:
 
 @pytest.mark.skipif(not all(os.path.exists(b) for b in DUCT.values()),
                     reason="duct3d binaries not built (make -C integration/openlb)")
+@pytest.mark.parametrize("cuboids", [1, 4])
+def test_duct_smagorinsky_bulk_matches_openlb(gpu_available, cuboids, tmp_path):
+    """A SmagorinskyBGK bulk next to InterpolatedVelocity / InterpolatedPressure
+    faces with their default BGK mixin, the set-up of aorta3d.cpp: the GPU_HIP
+    run equals the CPU run piece by piece, on one cuboid and on four."""
+    out = {}
+    for name in ("duct3d_gpu_hip", "duct3d_cpu"):
+        cwd = tmp_path / name
+        cwd.mkdir()
+        cmd = [DUCT[name], "--steps", "300", "--interpolated", "--smagorinsky", "--cuboids", str(cuboids)]
+        print("running", " ".join(cmd), flush=True)
+        p = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True, timeout=240)
+        assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+        out[name] = {f: _pieces(cwd, f) for f in FIELDS}
+    for f in FIELDS:
+        gpu, cpu = out["duct3d_gpu_hip"][f], out["duct3d_cpu"][f]
+        assert len(gpu) == len(cpu) == cuboids
+        scale = max(float(np.abs(b).max()) for b in cpu)
+        err = max(max_err(a, b) for a, b in zip(gpu, cpu))
+        print(f"duct smagorinsky {cuboids} cuboids {f}: max |gpu_hip - cpu| = {err:.3e} of max {scale:.3e}",
+              flush=True)
+        assert scale > 0 and err <= 1e-6 * scale
+
+
+@pytest.mark.skipif(not all(os.path.exists(b) for b in DUCT.values()),
+                    reason="duct3d binaries not built (make -C integration/openlb)")
 @pytest.mark.parametrize("family", ["local", "interpolated"])
 def test_duct_on_several_cuboids_matches_openlb(gpu_available, family, tmp_path):
     """The duct on 6 cuboids (CuboidDecomposition, OpenLB's default overlap of

@@ -10,6 +10,8 @@ itself computes on the same cuboids (tests/golden/multiblock/, OpenLB run with
 CuboidDecomposition::splitRegular: the overlap exchange of
 src/communication/superCommunicator.hh:189-218 with periodic images resolved
 through src/geometry/cuboidDecomposition.hh:155-177)."""
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -80,6 +82,14 @@ LOCAL_CASES = [
     ("channel_local_pressure", lambda: W.velocity_channel((24, 13, 11), tau=0.6), "f32"),
     ("step_local_velocity", lambda: W.step_cavity((14, 12, 16), block=(6, 5, 7), tau=0.6, lid_u=0.05), "f64"),
     ("cavity27_local_velocity", lambda: W.local_velocity_cavity((10, 12, 11), tau=0.7, lid_u=0.1, q=27), "f32"),
+    # other bulk dynamics; the boundary cells keep their BGK mixin
+    ("channel_interp_smagorinsky",
+     lambda: dataclasses.replace(W.velocity_channel((26, 12, 14), tau=0.52, inlet="interp_velocity",
+                                                    outlet="interp_pressure"), kind="smagorinsky", smago=0.14),
+     "f32"),
+    ("cavity_local_forced_trt",
+     lambda: dataclasses.replace(W.local_velocity_cavity((16, 14, 15), tau=0.6, lid_u=0.08), kind="forced_trt",
+                                 magic=0.25, force=(1e-5, 0.0, 0.0)), "f64"),
 ]
 LOCAL_SPLITS = [(2, 1, 1), (1, 3, 2), (2, 2, 2)]
 
