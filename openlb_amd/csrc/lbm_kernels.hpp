@@ -147,11 +147,11 @@ __device__ __forceinline__ void collide_stream_cell(const StepArgs<Q, typename S
       if (id >= 3 && (a.info[id] & 0xffu) != kMovingWall) return;
     }
     if constexpr (LOCAL == kLocalFlat) {
-      // flat faces stay in this pass; edges, corners and the rest go to the list pass
+      // flat faces, and the outer edges and corners (BGK at their fixed
+      // velocity), stay in this pass; inner edges and corners go to the list pass
       if (id >= 3) {
         const uint32_t k = a.info[id] & 0xffu;
-        if (k != kMovingWall && k != kFlatVelocity && k != kFlatPressure && k != kFdVelocity && k != kFdPressure)
-          return;
+        if (k == kInnerEdge || k == kInnerCorner || k == kDeadWall) return;
       }
     }
   }
@@ -201,7 +201,8 @@ __device__ __forceinline__ void collide_stream_cell(const StepArgs<Q, typename S
       }
       if constexpr (LOCAL == kLocalFlat) {
         const LocalInfo li = unpack_local_info(a.info[id]);
-        if (li.kind == kFdVelocity || li.kind == kFdPressure) {
+        if (li.kind == kFdVelocity || li.kind == kFdPressure || li.kind == kOuterEdge || li.kind == kOuterCorner) {
+          // BGK at the momenta of the cell's type: CombinedRLB's plain case
           wall = false;
           collide_dirichlet<Q, V>(f, li, w, a.bp.omega);
         } else if (li.kind != kMovingWall) {

@@ -949,8 +949,9 @@ int lbmh_set_local_boundary(lbmh_lattice* L, int id, int kind, double rho, const
           it = entry.emplace(key, free_id).first;
         }
         next[c] = uint8_t(it->second);
-        // flat faces are collided by the whole-lattice pass, the rest by the list pass
-        if (k != kFlatVelocity && k != kFlatPressure && k != kFdVelocity && k != kFdPressure) cells.push_back(c);
+        // faces, outer edges and corners are collided by the whole-lattice pass,
+        // inner edges and corners (and cells without dynamics) by the list pass
+        if (k == kInnerEdge || k == kInnerCorner || k == kDeadWall) cells.push_back(c);
         if (k == kFdVelocity || k == kFdPressure) planes.push_back(c);
         if (k == kFdPressure) L->pressure_planes = true;
         if (k == kOuterEdge) edges.push_back(c);

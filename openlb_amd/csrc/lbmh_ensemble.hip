@@ -626,7 +626,7 @@ int lbmh_ensemble_set_local_boundary(lbmh_ensemble* E, int m, int id, int kind, 
           it = entry_ids.emplace(key, free_id).first;
         }
         next[c] = uint8_t(it->second);
-        if (k != kFlatVelocity && k != kFlatPressure && k != kFdVelocity && k != kFdPressure) cells.push_back(c);
+        if (k == kInnerEdge || k == kInnerCorner || k == kDeadWall) cells.push_back(c);
         if (k == kFdVelocity || k == kFdPressure) planes.push_back(c);
         if (k == kOuterEdge) edges.push_back(c);
         if (k == kOuterCorner) corners.push_back(c);
