@@ -7,6 +7,10 @@
  *   --interpolated    boundary::InterpolatedVelocity + boundary::InterpolatedPressure
  * --smagorinsky runs a SmagorinskyBGKdynamics bulk (C = 0.14) next to the
  * boundaries' own BGK mixin, as aorta3d.cpp does.
+ * --load FILE replaces the state after initialize() by OpenLB checkpoint FILE
+ * (SuperLattice::load, then the switch to ProcessingContext::Simulation that
+ * hands the loaded populations to the platform); --save FILE writes one after
+ * the steps (switch to Evaluation, SuperLattice::save).
  * --cuboids N divides the domain into N cuboids (CuboidDecomposition with
  * OpenLB's default overlap of 3), whose blocks exchange their overlap and
  * whose boundary post-processors read cells of the neighbouring blocks.
@@ -69,6 +73,8 @@ int main(int argc, char** argv)
   const bool interpolated = args.contains("--interpolated");
   const int nCuboids = args.getValueOrFallback<int>("--cuboids", 1);
   const bool smagorinsky = args.contains("--smagorinsky");
+  const std::string loadFrom = args.getValueOrFallback<std::string>("--load", "");
+  const std::string saveTo = args.getValueOrFallback<std::string>("--save", "");
   singleton::directories().setOutputDir("./tmp/");
 
   // ny - 1 voxels per unit length, lattice inflow velocity 0.04, Re 20
@@ -104,6 +110,12 @@ int main(int argc, char** argv)
   AnalyticalConst3D<T,T> inflow(Vector<T,3>{converter.getCharLatticeVelocity(), 0, 0});
   lattice.defineU(geometry, 3, inflow);
   lattice.initialize();
+  if (!loadFrom.empty()) {
+    if (!lattice.template load<false>(loadFrom)) {
+      throw std::runtime_error("cannot load " + loadFrom);
+    }
+    lattice.setProcessingContext(ProcessingContext::Simulation);
+  }
 
   util::Timer<T> timer(steps, geometry.getStatistics().getNvoxel());
   timer.start();
@@ -114,6 +126,9 @@ int main(int argc, char** argv)
   timer.update(steps);
 
   lattice.setProcessingContext(ProcessingContext::Evaluation);
+  if (!saveTo.empty() && !lattice.template save<false>(saveTo)) {
+    throw std::runtime_error("cannot save " + saveTo);
+  }
   SuperVTMwriter3D<T> writer("duct3d");
   SuperLatticePhysVelocity3D<T,DESCRIPTOR> velocity(lattice, converter);
   SuperLatticePhysPressure3D<T,DESCRIPTOR> pressure(lattice, converter);

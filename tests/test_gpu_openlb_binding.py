@@ -199,3 +199,35 @@ def test_cavity_on_several_cuboids_matches_openlb(gpu_available, cuboids, tmp_pa
         err = max(max_err(a, b) for a, b in zip(gpu, cpu))
         print(f"cavity {cuboids} cuboids {f}: max |gpu_hip - cpu| = {err:.3e} of max {scale:.3e}", flush=True)
         assert scale > 0 and err <= 1e-6 * scale
+
+
+@pytest.mark.skipif(not all(os.path.exists(b) for b in DUCT.values()),
+                    reason="duct3d binaries not built (make -C integration/openlb)")
+@pytest.mark.parametrize("first,second", [("duct3d_gpu_hip", "duct3d_cpu"), ("duct3d_cpu", "duct3d_gpu_hip")])
+def test_checkpoint_moves_between_platforms(gpu_available, first, second, tmp_path):
+    """OpenLB's own serializer on the duct (interpolated faces, 2 cuboids): one
+    platform runs 150 steps and saves (SuperLattice::save after the switch to
+    Evaluation), the other loads that file and runs 150 more.  The result
+    equals a CPU_SISD run of 300 steps to the VTK float precision: a GPU_HIP
+    block's checkpoint is OpenLB's CPU bytes, and a loaded one reaches the
+    device."""
+    common = ["--interpolated", "--cuboids", "2"]
+    ck = str(tmp_path / "duct")
+    runs = [(first, ["--steps", "150", "--save", ck]), (second, ["--steps", "150", "--load", ck]),
+            ("duct3d_cpu", ["--steps", "300"])]
+    out = []
+    for k, (name, extra) in enumerate(runs):
+        cwd = tmp_path / f"run{k}"
+        cwd.mkdir()
+        cmd = [DUCT[name]] + common + extra
+        print("running", " ".join(cmd), flush=True)
+        p = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True, timeout=240)
+        assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+        out.append({f: _pieces(cwd, f) for f in FIELDS})
+    assert os.path.exists(ck + "_rank0_size1.dat")
+    for f in FIELDS:
+        resumed, ref = out[1][f], out[2][f]
+        scale = max(float(np.abs(b).max()) for b in ref)
+        err = max(max_err(a, b) for a, b in zip(resumed, ref))
+        print(f"{first} -> {second} {f}: max |resumed - cpu 300| = {err:.3e} of max {scale:.3e}", flush=True)
+        assert scale > 0 and err <= 1e-6 * scale
