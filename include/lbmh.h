@@ -467,7 +467,8 @@ int lbmh_ensemble_set_wall(lbmh_ensemble* ensemble, int member, int id, double r
 /* Local velocity / pressure boundary on the member's cells carrying `id`, as
  * lbmh_set_local_boundary does for a lattice (every kind: LocalVelocity,
  * LocalPressure, InterpolatedVelocity, InterpolatedPressure); the per-type
- * ids come from the member's free ids 3..63.  Needs an LBMH_BGK ensemble.
+ * ids come from the member's free ids 3..63.  Any bulk kind: the boundary
+ * cells keep their BGK mixin, as on a lattice.
  * The first InterpolatedPressure member allocates the ensemble's plane
  * workspace (boundary capacity x Q values); later members reuse it. */
 int lbmh_ensemble_set_local_boundary(lbmh_ensemble* ensemble, int member, int id, int kind, double rho,

@@ -146,7 +146,8 @@ __global__ __launch_bounds__(64) void ensemble_post_kernel(const EnsembleEntry<Q
                                                            S* __restrict__ arena, const uint8_t* __restrict__ ids,
                                                            const double* __restrict__ walls,
                                                            const uint32_t* __restrict__ info, EnsembleLimits limits,
-                                                           int only, typename StorageTraits<S>::V* scratch = nullptr)
+                                                           int only, int forced,
+                                                           typename StorageTraits<S>::V* scratch = nullptr)
 {
   using V = typename StorageTraits<S>::V;
   __shared__ V nb[64 / kPostLanes][kPostSlots][4];
@@ -174,8 +175,12 @@ __global__ __launch_bounds__(64) void ensemble_post_kernel(const EnsembleEntry<Q
       a.cells = nullptr;
       a.recs = nullptr;
       a.remote = nullptr;
-      a.forced = 0;
+      // a forced bulk neighbour reports u + F/2 (the member's uniform FORCE)
+      a.forced = uint32_t(forced);
       a.force = nullptr;
+      a.half_force[0] = e.bp.half_force[0];
+      a.half_force[1] = e.bp.half_force[1];
+      a.half_force[2] = e.bp.half_force[2];
       a.count = 0;
       a.nx = int(e.nx);
       a.ny = int(e.ny);

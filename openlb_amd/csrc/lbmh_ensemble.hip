@@ -259,9 +259,11 @@ template <int Q, typename S>
 int launch_post(lbmh_ensemble* E, const EnsembleEntry<Q>* table, S* arena, const EnsembleLimits& limits,
                 hipStream_t stream, int only)
 {
+  const int forced = (E->kind == LBMH_FORCED_BGK || E->kind == LBMH_FORCED_TRT) ? 1 : 0;
   if (E->n_planes) {
     hipLaunchKernelGGL((ensemble_post_kernel<Q, S, kPostPlane>), dim3(grid_for(E->n_planes, 64 / kPostLanes)), dim3(64),
-                       0, stream, table, E->d_planes, E->n_planes, arena, E->d_ids, E->d_walls, E->d_info, limits, only);
+                       0, stream, table, E->d_planes, E->n_planes, arena, E->d_ids, E->d_walls, E->d_info, limits, only,
+                       forced);
     LBMH_HIP(hipGetLastError());
   }
   if (E->n_pplanes) {
@@ -270,7 +272,7 @@ int launch_post(lbmh_ensemble* E, const EnsembleEntry<Q>* table, S* arena, const
     V* scratch = static_cast<V*>(E->d_plane_scratch);
     hipLaunchKernelGGL((ensemble_post_kernel<Q, S, kPostPlane, true>), dim3(grid_for(E->n_pplanes, 64 / kPostLanes)),
                        dim3(64), 0, stream, table, E->d_pplanes, E->n_pplanes, arena, E->d_ids, E->d_walls, E->d_info,
-                       limits, only, scratch);
+                       limits, only, forced, scratch);
     LBMH_HIP(hipGetLastError());
     hipLaunchKernelGGL((ensemble_plane_store_kernel<Q, S>), dim3(grid_for(E->n_pplanes, 64)), dim3(64), 0, stream,
                        table, E->d_pplanes, E->n_pplanes, arena, static_cast<const V*>(scratch), limits, only);
@@ -279,12 +281,13 @@ int launch_post(lbmh_ensemble* E, const EnsembleEntry<Q>* table, S* arena, const
   if (E->n_edges) {
     hipLaunchKernelGGL((ensemble_post_kernel<Q, S, kPostEdge>), dim3(grid_for(E->n_edges, 64 / kPostLanes)), dim3(64), 0,
                        stream, table,
-                       E->d_edges, E->n_edges, arena, E->d_ids, E->d_walls, E->d_info, limits, only);
+                       E->d_edges, E->n_edges, arena, E->d_ids, E->d_walls, E->d_info, limits, only, forced);
     LBMH_HIP(hipGetLastError());
   }
   if (E->n_corners) {
     hipLaunchKernelGGL((ensemble_post_kernel<Q, S, kPostCorner>), dim3(grid_for(E->n_corners, 64 / kPostLanes)), dim3(64), 0,
-                       stream, table, E->d_corners, E->n_corners, arena, E->d_ids, E->d_walls, E->d_info, limits, only);
+                       stream, table, E->d_corners, E->n_corners, arena, E->d_ids, E->d_walls, E->d_info, limits, only,
+                       forced);
     LBMH_HIP(hipGetLastError());
   }
   return LBMH_OK;
@@ -313,10 +316,18 @@ int launch_steps(lbmh_ensemble* E, int64_t n, hipStream_t stream)
 #define LBMH_ENSEMBLE_STEP(KIND_)                                                                               \
   hipLaunchKernelGGL((ensemble_step_kernel<Q, S, KIND_>), dim3(E->total_chunks), dim3(kBlock), lds, stream, table, \
                      E->d_chunk, arena, E->d_ids, E->d_walls, E->d_info, group, limits)
+#define LBMH_ENSEMBLE_STEP_LOCAL(KIND_)                                                                         \
+  hipLaunchKernelGGL((ensemble_step_kernel<Q, S, KIND_, kLocalFlat>), dim3(E->total_chunks), dim3(kBlock), lds,    \
+                     stream, table, E->d_chunk, arena, E->d_ids, E->d_walls, E->d_info, group, limits)
         if (local) {
-          // local boundaries need BGK bulk dynamics (checked when they are defined)
-          hipLaunchKernelGGL((ensemble_step_kernel<Q, S, kBGK, kLocalFlat>), dim3(E->total_chunks), dim3(kBlock), lds,
-                             stream, table, E->d_chunk, arena, E->d_ids, E->d_walls, E->d_info, group, limits);
+          // the boundary cells collide with their BGK mixin whatever the bulk kind
+          switch (E->kind) {
+          case LBMH_BGK: LBMH_ENSEMBLE_STEP_LOCAL(kBGK); break;
+          case LBMH_FORCED_BGK: LBMH_ENSEMBLE_STEP_LOCAL(kForcedBGK); break;
+          case LBMH_TRT: LBMH_ENSEMBLE_STEP_LOCAL(kTRT); break;
+          case LBMH_FORCED_TRT: LBMH_ENSEMBLE_STEP_LOCAL(kForcedTRT); break;
+          default: LBMH_ENSEMBLE_STEP_LOCAL(kSmagorinsky); break;
+          }
         } else {
           switch (E->kind) {
           case LBMH_BGK: LBMH_ENSEMBLE_STEP(kBGK); break;
@@ -326,12 +337,22 @@ int launch_steps(lbmh_ensemble* E, int64_t n, hipStream_t stream)
           default: LBMH_ENSEMBLE_STEP(kSmagorinsky); break;
           }
         }
+#undef LBMH_ENSEMBLE_STEP_LOCAL
 #undef LBMH_ENSEMBLE_STEP
         LBMH_HIP(hipGetLastError());
       }
       if (local && E->n_local) {
-        hipLaunchKernelGGL((ensemble_local_kernel<Q, S, kBGK>), dim3(grid_for(E->n_local, 64)), dim3(64), 0, stream,
-                           table, E->d_local, E->n_local, arena, E->d_ids, E->d_walls, E->d_info, limits);
+#define LBMH_ENSEMBLE_LOCAL(KIND_)                                                                              \
+  hipLaunchKernelGGL((ensemble_local_kernel<Q, S, KIND_>), dim3(grid_for(E->n_local, 64)), dim3(64), 0, stream,      \
+                     table, E->d_local, E->n_local, arena, E->d_ids, E->d_walls, E->d_info, limits)
+        switch (E->kind) {
+        case LBMH_BGK: LBMH_ENSEMBLE_LOCAL(kBGK); break;
+        case LBMH_FORCED_BGK: LBMH_ENSEMBLE_LOCAL(kForcedBGK); break;
+        case LBMH_TRT: LBMH_ENSEMBLE_LOCAL(kTRT); break;
+        case LBMH_FORCED_TRT: LBMH_ENSEMBLE_LOCAL(kForcedTRT); break;
+        default: LBMH_ENSEMBLE_LOCAL(kSmagorinsky); break;
+        }
+#undef LBMH_ENSEMBLE_LOCAL
         LBMH_HIP(hipGetLastError());
       }
       hipLaunchKernelGGL((ensemble_advance_kernel<Q>), dim3(adv_grid), dim3(256), 0, stream, table,
@@ -578,7 +599,6 @@ int lbmh_ensemble_set_local_boundary(lbmh_ensemble* E, int m, int id, int kind, 
   if (!u) return fail(LBMH_E_INVALID, "null velocity");
   if (kind < LBMH_LOCAL_VELOCITY || kind > LBMH_INTERP_PRESSURE) return fail(LBMH_E_INVALID, "unknown local boundary kind");
   if (id < 2 || id >= kEnsembleWalls) return fail(LBMH_E_INVALID, "ensemble boundary id must be in [2, 63]");
-  if (E->kind != LBMH_BGK) return fail(LBMH_E_INVALID, "local boundaries need BGK bulk dynamics");
   if ((kind == LBMH_LOCAL_PRESSURE || kind == LBMH_INTERP_PRESSURE) && !(rho > 0))
     return fail(LBMH_E_INVALID, "pressure boundary density must be positive");
   Member& mem = E->members[m];

@@ -5,6 +5,8 @@ parameters, so a member must be bit-identical to the same lattice advanced
 on its own by lbmh_step. The single-lattice path is pinned to the oracle and
 to OpenLB by test_gpu_parity.py, so bit-identity extends that pin to every
 member."""
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -217,6 +219,40 @@ def test_local_boundary_members_match_standalone(gpu_available, storage):
         for m, c, f0 in zip(members, cases, f0s):
             assert ens.status(m) == (41, -1)
             np.testing.assert_array_equal(ens.download(m), standalone(c, storage, f0, 41), err_msg=f"{m} {c.name}")
+
+
+@pytest.mark.parametrize("kind", ["smagorinsky", "forced_trt"])
+def test_local_boundary_members_with_other_bulk_kinds_match_standalone(gpu_available, kind):
+    """Local-boundary members of a Smagorinsky or forced-TRT ensemble (the
+    boundary cells keep their BGK mixin, the post-processors read a forced
+    neighbour's velocity with the member's F/2): each member equals the same
+    lattice stepped alone.  Smagorinsky members are bit-identical; forced-TRT
+    members agree to 1e-14 (about one ulp of the populations after 25 steps):
+    the lattice and the ensemble run the forced collision in different
+    kernels, which may contract the same arithmetic into different FMAs."""
+    def make(c):
+        extra = {"smago": 0.14} if kind == "smagorinsky" else {"magic": 0.25, "force": (1e-5, 0.0, -4e-6)}
+        return dataclasses.replace(c, kind=kind, **extra)
+    cases = [make(W.velocity_channel((16, 8, 7), u_in=0.03, tau=0.56, inlet="interp_velocity",
+                                     outlet="interp_pressure")),
+             make(W.local_velocity_cavity((9, 8, 11), tau=0.6, lid_u=0.07)),
+             make(W.velocity_channel((14, 7, 6), u_in=0.03, rho_out=1.0, tau=0.6))]
+    with Ensemble(19, "f64", kind, max_members=4, population_bytes=64 << 20, cell_capacity=1 << 20) as ens:
+        members, f0s = [], []
+        for c in cases:
+            m = ens.add(c)
+            ens.ini_equilibrium(m, c.rho_u)
+            ens.initialize(m)
+            members.append(m)
+            f0s.append(ens.download(m))
+        ens.step(5)
+        ens.step(20, graph=True)
+        for m, c, f0 in zip(members, cases, f0s):
+            got, ref = ens.download(m), standalone(c, "f64", f0, 25)
+            if kind == "smagorinsky":
+                np.testing.assert_array_equal(got, ref, err_msg=f"{m} {c.name}")
+            else:
+                np.testing.assert_allclose(got, ref, rtol=0, atol=1e-14, err_msg=f"{m} {c.name}")
 
 
 def test_autotune_leaves_the_members_as_they_were(gpu_available):
