@@ -25,8 +25,16 @@ using namespace olb;
 using namespace olb::descriptors;
 
 using T = double;
+#ifdef DUCT_FORCED
+// built with -DDUCT_FORCED: a ForcedBGK bulk with a body force along the duct,
+// next to boundaries with their BGK mixin (testFlow3dSolver.h:203-213)
+using DESCRIPTOR = D3Q19<FORCE>;
+using Bulk = ForcedBGKdynamics<T,DESCRIPTOR>;
+#else
 using DESCRIPTOR = D3Q19<>;
 using Bulk = BGKdynamics<T,DESCRIPTOR>;
+#endif
+using Mixin = BGKdynamics<T,DESCRIPTOR>;
 
 namespace {
 
@@ -94,13 +102,17 @@ int main(int argc, char** argv)
   lattice.statisticsOff();
   const T omega = converter.getLatticeRelaxationFrequency();
   if (interpolated) {
-    prepareLattice<boundary::InterpolatedVelocity<T,DESCRIPTOR,Bulk>,
-                   boundary::InterpolatedPressure<T,DESCRIPTOR,Bulk>>(lattice, geometry, omega, smagorinsky);
+    prepareLattice<boundary::InterpolatedVelocity<T,DESCRIPTOR,Mixin>,
+                   boundary::InterpolatedPressure<T,DESCRIPTOR,Mixin>>(lattice, geometry, omega, smagorinsky);
   } else {
-    prepareLattice<boundary::LocalVelocity<T,DESCRIPTOR,Bulk>,
-                   boundary::LocalPressure<T,DESCRIPTOR,Bulk>>(lattice, geometry, omega, smagorinsky);
+    prepareLattice<boundary::LocalVelocity<T,DESCRIPTOR,Mixin>,
+                   boundary::LocalPressure<T,DESCRIPTOR,Mixin>>(lattice, geometry, omega, smagorinsky);
   }
 
+#ifdef DUCT_FORCED
+  AnalyticalConst3D<T,T> bodyForce(Vector<T,3>{T(2e-5), T(0), T(-5e-6)});
+  lattice.defineField<FORCE>(geometry, 1, bodyForce);
+#endif
   AnalyticalConst3D<T,T> rho(1);
   AnalyticalConst3D<T,T> rest(Vector<T,3>{});
   for (int m = 1; m <= 4; ++m) {

@@ -13,8 +13,9 @@
  * as well.  The halos also tell each block which block owns the neighbours
  * its boundary post-processors read (lbmh_set_overlap_ids), so every halo is
  * built before the blocks' first post-processing.  Only populations cross
- * this way; a neighbourhood that requests other fields of a GPU_HIP block is
- * refused.  Builds without MPI only.
+ * this way: other fields a neighbourhood requests (FORCE of the full
+ * exchange) keep the overlap values the set-up wrote, which is all the device
+ * needs, since it reads them at core cells.  Builds without MPI only.
  */
 #ifndef OLB_GPU_HIP_COMMUNICATOR_H
 #define OLB_GPU_HIP_COMMUNICATOR_H
@@ -109,10 +110,15 @@ public:
       if (neighborhood.getCellsInboundFrom(localC).empty()) {
         return;
       }
+      // only the populations cross: the device reads the other fields (FORCE)
+      // at core cells only, and OpenLB's set-up (defineField over a material
+      // indicator) writes their overlap copies too
+      bool populations = false;
       for (const std::type_index& field : neighborhood.getFieldsCommonWith(localC)) {
-        if (field != typeid(descriptors::POPULATION)) {
-          throw std::logic_error("GPU_HIP: blocks exchange populations only");
-        }
+        populations = populations || field == typeid(descriptors::POPULATION);
+      }
+      if (!populations) {
+        return;
       }
       if (loadBalancer.platform(loadBalancer.loc(localC)) != Platform::GPU_HIP) {
         throw std::logic_error("GPU_HIP: overlap communication with a block of another platform");

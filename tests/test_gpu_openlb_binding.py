@@ -73,6 +73,7 @@ def test_float_run_as_close_to_double_as_openlbs_own(gpu_available, runs):
 
 
 DUCT = {k: os.path.join(BUILD, k) for k in ("duct3d_gpu_hip", "duct3d_cpu")}
+FORCED = {k: os.path.join(BUILD, k) for k in ("duct3d_forced_gpu_hip", "duct3d_forced_cpu")}
 
 
 @pytest.mark.skipif(not all(os.path.exists(b) for b in DUCT.values()),
@@ -230,4 +231,30 @@ def test_checkpoint_moves_between_platforms(gpu_available, first, second, tmp_pa
         scale = max(float(np.abs(b).max()) for b in ref)
         err = max(max_err(a, b) for a, b in zip(resumed, ref))
         print(f"{first} -> {second} {f}: max |resumed - cpu 300| = {err:.3e} of max {scale:.3e}", flush=True)
+        assert scale > 0 and err <= 1e-6 * scale
+
+
+@pytest.mark.skipif(not all(os.path.exists(b) for b in FORCED.values()),
+                    reason="duct3d_forced binaries not built (make -C integration/openlb)")
+@pytest.mark.parametrize("family", ["local", "interpolated"])
+def test_duct_forced_bulk_matches_openlb(gpu_available, family, tmp_path):
+    """The duct built with a ForcedBGK bulk and a body force (D3Q19<FORCE>),
+    its faces keeping their BGK mixin, on 2 cuboids: the edge and plane
+    processors read the forced bulk neighbours' velocity with F/2 added.  The
+    GPU_HIP run equals the CPU run piece by piece."""
+    out = {}
+    for name in ("duct3d_forced_gpu_hip", "duct3d_forced_cpu"):
+        cwd = tmp_path / name
+        cwd.mkdir()
+        cmd = [FORCED[name], "--steps", "300", "--cuboids", "2"] + (["--interpolated"] if family == "interpolated" else [])
+        print("running", " ".join(cmd), flush=True)
+        p = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True, timeout=240)
+        assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+        out[name] = {f: _pieces(cwd, f) for f in FIELDS}
+    for f in FIELDS:
+        gpu, cpu = out["duct3d_forced_gpu_hip"][f], out["duct3d_forced_cpu"][f]
+        assert len(gpu) == len(cpu) == 2
+        scale = max(float(np.abs(b).max()) for b in cpu)
+        err = max(max_err(a, b) for a, b in zip(gpu, cpu))
+        print(f"duct forced {family} {f}: max |gpu_hip - cpu| = {err:.3e} of max {scale:.3e}", flush=True)
         assert scale > 0 and err <= 1e-6 * scale
