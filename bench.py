@@ -503,6 +503,23 @@ def cavity_bench(size=200, steps=200, cpu_size=100, cpu_seconds=10.0, no_cpu=Fal
                       "hbm_frac": round(mlups * 1e6 * 2 * 19 * 4 / (HBM_PEAK_GBS * 1e9), 4),
                       "max_u": st["max_u"], "average_energy": st["average_energy"]}
         log(f"cavity {label} {size}: {mlups:.0f} MLUPS")
+    # the same LocalVelocity cavity on 2x2x2 blocks of one device (OpenLB's
+    # cuboids): halo copies, and post-processors joined across the blocks
+    from openlb_amd.super_lattice import SuperLattice
+    case = W.cavity_benchmark_local(size)
+    with SuperLattice(case, "f32", (2, 2, 2)) as sup:
+        sup.ini_equilibrium(case.rho_u)
+        sup.initialize()
+        sup.collide_and_stream(10)
+        sup.synchronize()
+        t0 = time.perf_counter()
+        sup.collide_and_stream(steps)
+        sup.synchronize()
+        dt = time.perf_counter() - t0
+    mlups = case.n_active * steps / dt / 1e6
+    out["local_velocity_split_2x2x2"] = {"seconds": round(dt, 3), "mlups": round(mlups, 1),
+                                         "hbm_frac": round(mlups * 1e6 * 2 * 19 * 4 / (HBM_PEAK_GBS * 1e9), 4)}
+    log(f"cavity local_velocity {size} on 2x2x2 blocks: {mlups:.0f} MLUPS")
     if not no_cpu:
         from oracle import compare_cpu
         if os.path.exists(compare_cpu.REF_SIMD):

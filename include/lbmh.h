@@ -171,6 +171,13 @@ int lbmh_halo_create(lbmh_lattice* dst, const lbmh_lattice* src, const uint32_t*
                      const uint32_t* src_cells, size_t n, lbmh_halo** out);
 int lbmh_halo_destroy(lbmh_halo* halo);
 int lbmh_halo_copy(lbmh_halo* halo, void* stream);
+/* lbmh_halo_copy of n halos in one launch (the copies of OpenLB's block
+ * communicators for every block of a SuperLattice at once,
+ * src/communication/superCommunicator.hh:189-218).  The halos must share
+ * device, q and storage; the batch's descriptor table is kept by the first
+ * halo's destination block, so batches led by one block must run on one
+ * stream. */
+int lbmh_halo_copy_many(lbmh_halo* const* halos, size_t n, void* stream);
 
 /* Number of completed time steps and the PS shift of every column, in the
  * convention of CyclicColumn::_shift, normalised to [0, n_padded_cells)

@@ -26,7 +26,7 @@ EXPORTS = (
     "lbmh_checkpoint_load", "lbmh_field_export", "lbmh_moments", "lbmh_reduce_field", "lbmh_reduce_moments",
     "lbmh_reduce_moments_noalloc", "lbmh_statistics",
     "lbmh_set_local_boundary", "lbmh_classify_boundary", "lbmh_initialize",
-    "lbmh_classify_boundary_overlap", "lbmh_set_overlap_ids",
+    "lbmh_classify_boundary_overlap", "lbmh_set_overlap_ids", "lbmh_halo_copy_many",
     "lbmh_collide", "lbmh_stream", "lbmh_halo_create", "lbmh_halo_destroy", "lbmh_halo_copy",
     "lbmh_synchronize", "lbmh_device_synchronize", "lbmh_lattice_footprint", "lbmh_abi_version", "lbmh_device_count",
     "lbmh_device_arch", "lbmh_last_error", "lbmh_set_launch_config", "lbmh_get_launch_config",
@@ -125,6 +125,7 @@ def load() -> ctypes.CDLL:
     L.lbmh_classify_boundary.argtypes = [vp, vp, vp, vp]
     L.lbmh_classify_boundary_overlap.argtypes = [vp, vp, vp, vp, vp]
     L.lbmh_set_overlap_ids.argtypes = [vp, vp]
+    L.lbmh_halo_copy_many.argtypes = [vp, ctypes.c_size_t, vp]
     L.lbmh_initialize.argtypes = [vp]
     L.lbmh_collide.argtypes = [vp, vp]
     L.lbmh_stream.argtypes = [vp, vp]

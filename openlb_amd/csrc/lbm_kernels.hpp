@@ -347,6 +347,43 @@ __global__ __launch_bounds__(256) void halo_copy_kernel(S* dst, uint64_t dst_str
       src[uint64_t(i) * src_stride + wrap_index(p.y, ssh.s[i], src_n)];
 }
 
+// One halo of a batched copy: both blocks' columns and shifts, its pair
+// list, and where its pairs start in the batch.
+template <int Q, typename S>
+struct HaloCopyDesc {
+  S* dst;
+  const S* src;
+  const uint2* pairs;
+  uint64_t dst_stride, src_stride;
+  uint32_t dst_n, src_n;
+  uint32_t first, count;
+  uint32_t dst_shift[Q];
+  uint32_t src_shift[Q];
+};
+
+// The copies of halo_copy_kernel for many halos in one launch: lane k of
+// the batch finds its halo by bisection over the halos' first pairs.
+template <int Q, typename S>
+__global__ __launch_bounds__(256) void halo_copy_many_kernel(const HaloCopyDesc<Q, S>* __restrict__ descs,
+                                                             uint32_t halos, uint32_t total)
+{
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = blockIdx.y;
+  if (k >= total || i >= uint32_t(Q)) return;
+  uint32_t lo = 0, hi = halos;  // the last halo whose first pair is <= k
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (descs[mid].first <= k) lo = mid;
+    else hi = mid;
+  }
+  const HaloCopyDesc<Q, S>& d = descs[lo];
+  const uint32_t j = k - d.first;
+  if (j >= d.count) return;
+  const uint2 p = d.pairs[j];
+  d.dst[uint64_t(i) * d.dst_stride + wrap_index(p.x, d.dst_shift[i], d.dst_n)] =
+      d.src[uint64_t(i) * d.src_stride + wrap_index(p.y, d.src_shift[i], d.src_n)];
+}
+
 // Equilibrium from per-cell (rho, u), iniEquilibrium semantics.
 template <int Q, typename S>
 __global__ void ini_equilibrium_kernel(const double* rho_u, const uint8_t* ids, S* pop, CoreGeom g,

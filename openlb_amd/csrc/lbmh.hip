@@ -125,6 +125,9 @@ struct lbmh_lattice {
   bool remote_resolved = true;
   void* d_remote = nullptr;
   int64_t init_step = -1;  // step count at which lbmh_initialize asked for the post-processors
+  // descriptor table of the batched halo copies this block leads
+  void* d_halo_batch = nullptr;
+  size_t halo_batch_bytes = 0;
 };
 
 // Library default launch shape, from sweeps of (LDS reservation, XCD group)
@@ -731,6 +734,7 @@ int lbmh_lattice_destroy(lbmh_lattice* L)
     if (H->dst) H->dst->remote_resolved = H->dst->remote_pos.empty();
   }
   if (L->d_remote) (void)hipFree(L->d_remote);
+  if (L->d_halo_batch) (void)hipFree(L->d_halo_batch);
   if (L->d_pop) (void)hipFree(L->d_pop);
   if (L->d_force) (void)hipFree(L->d_force);
   if (L->d_ids) (void)hipFree(L->d_ids);
@@ -1362,6 +1366,70 @@ int lbmh_halo_copy(lbmh_halo* H, void* stream)
                        static_cast<S*>(D->d_pop), D->stride, D->n_padded, shifts_of<Q>(D),
                        static_cast<const S*>(S_->d_pop), S_->stride, S_->n_padded, shifts_of<Q>(S_), H->d_pairs,
                        H->count);
+    LBMH_HIP(hipGetLastError());
+    return LBMH_OK;
+  });
+}
+
+int lbmh_halo_copy_many(lbmh_halo* const* halos, size_t n, void* stream)
+{
+  if (n && !halos) return fail(LBMH_E_INVALID, "null argument");
+  if (n > 0xffffu) return fail(LBMH_E_INVALID, "too many halos in one batch");
+  uint64_t total = 0;
+  for (size_t h = 0; h < n; ++h) {
+    if (!halos[h]) return fail(LBMH_E_INVALID, "null halo");
+    if (!halos[h]->dst || !halos[h]->src) return fail(LBMH_E_INVALID, "a block of this halo was destroyed");
+    const lbmh_lattice* D = halos[h]->dst;
+    const lbmh_lattice* D0 = halos[0]->dst;
+    if (D->device != D0->device || D->q != D0->q || D->storage != D0->storage)
+      return fail(LBMH_E_INVALID, "halos of one batch must share the device, q and storage");
+    total += halos[h]->count;
+  }
+  if (total == 0) return LBMH_OK;
+  if (total > 0xffffffffu) return fail(LBMH_E_INVALID, "too many overlap cells in one batch");
+  lbmh_lattice* owner = halos[0]->dst;
+  DeviceGuard guard(owner->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return with_types(owner, [&](auto qc, auto st) -> int {
+    constexpr int Q = decltype(qc)::value;
+    using S = decltype(st);
+    using Desc = HaloCopyDesc<Q, S>;
+    std::vector<Desc> descs;
+    descs.reserve(n);
+    uint32_t first = 0;
+    for (size_t h = 0; h < n; ++h) {
+      const lbmh_halo* H = halos[h];
+      if (!H->count) continue;
+      Desc d{};
+      d.dst = static_cast<S*>(H->dst->d_pop);
+      d.src = static_cast<const S*>(H->src->d_pop);
+      d.pairs = H->d_pairs;
+      d.dst_stride = H->dst->stride;
+      d.src_stride = H->src->stride;
+      d.dst_n = H->dst->n_padded;
+      d.src_n = H->src->n_padded;
+      d.first = first;
+      d.count = H->count;
+      for (int i = 0; i < Q; ++i) {
+        d.dst_shift[i] = uint32_t(H->dst->shift[i]);
+        d.src_shift[i] = uint32_t(H->src->shift[i]);
+      }
+      descs.push_back(d);
+      first += H->count;
+    }
+    // the table lives with the first block; the previous batch's kernel on
+    // this stream has read it before the copy below runs
+    const size_t bytes = sizeof(Desc) * descs.size();
+    if (owner->halo_batch_bytes < bytes) {
+      if (owner->d_halo_batch) LBMH_HIP(hipFree(owner->d_halo_batch));
+      owner->d_halo_batch = nullptr;
+      owner->halo_batch_bytes = 0;
+      LBMH_HIP(hipMalloc(&owner->d_halo_batch, bytes));
+      owner->halo_batch_bytes = bytes;
+    }
+    LBMH_HIP(hipMemcpyAsync(owner->d_halo_batch, descs.data(), bytes, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL((halo_copy_many_kernel<Q, S>), dim3(grid_for(first, 256), Q), dim3(256), 0, s,
+                       static_cast<const Desc*>(owner->d_halo_batch), uint32_t(descs.size()), first);
     LBMH_HIP(hipGetLastError());
     return LBMH_OK;
   });

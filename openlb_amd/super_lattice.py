@@ -81,6 +81,7 @@ class SuperLattice:
             self.blocks[key] = BlockLattice.from_case(sub, storage, device, overlap_ids=ring)
             self.origin[key] = lo
         self._build_halos(self_periodic)
+        self._halo_table = (ctypes.c_void_p * max(1, len(self.halos)))(*[h.value for h in self.halos])
 
     # -- decomposition ---------------------------------------------------------
 
@@ -145,8 +146,9 @@ class SuperLattice:
         for _ in range(int(steps)):
             for lat in self.blocks.values():
                 abi.check(lib.lbmh_collide(lat.handle, None))
-            for h in self.halos:
-                abi.check(lib.lbmh_halo_copy(h, None))
+            if self.halos:
+                # every block's overlap copies in one launch
+                abi.check(lib.lbmh_halo_copy_many(self._halo_table, len(self.halos), None))
             for lat in self.blocks.values():
                 abi.check(lib.lbmh_stream(lat.handle, None))
 
