@@ -161,9 +161,15 @@ public:
   /// which the switch to ProcessingContext::Evaluation filled from the device.
   void copy() override
   {
+    // this block's neighbours in one launch
+    std::vector<lbmh_halo*> halos;
+    halos.reserve(_tasks.size());
     for (auto& task : _tasks) {
       build(task);
-      gpu::hip::check(lbmh_halo_copy(task.halo, nullptr), "lbmh_halo_copy");
+      halos.push_back(task.halo);
+    }
+    gpu::hip::check(lbmh_halo_copy_many(halos.data(), halos.size(), nullptr), "lbmh_halo_copy_many");
+    for (auto& task : _tasks) {
       if (task.deep) {
         auto& to = task.target->template getField<descriptors::POPULATION>();
         auto& from = task.source->template getField<descriptors::POPULATION>();
