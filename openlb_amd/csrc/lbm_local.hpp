@@ -815,19 +815,38 @@ struct RemotePair {
 
 // The PostStream exchange of the post-processors' neighbourhood
 // (superLattice.hh:639-640, the overlap cells boundary::set requests through
-// addPoints2CommBC): rho and u of the listed cells of the source block `a`,
-// as its post-processors would read them, into the consumer's remote buffer.
+// addPoints2CommBC): rho and u of the listed cells of each source block, as
+// its post-processors would read them, into the consumers' remote buffers --
+// many halos in one launch, entry k of the batch finding its halo by
+// bisection over the halos' first entries.
+template <int Q>
+struct RemoteBatchDesc {
+  PostArgs<Q> a;             // the source block, at the shifts to read it with
+  const RemotePair* pairs;
+  void* out;                 // the consumer's remote buffer
+  uint32_t first, count;
+};
+
 template <int Q, typename S>
-__global__ __launch_bounds__(64) void remote_moments_kernel(PostArgs<Q> a, const RemotePair* pairs, uint32_t count,
-                                                            typename StorageTraits<S>::V* out)
+__global__ __launch_bounds__(64) void remote_moments_many_kernel(const RemoteBatchDesc<Q>* __restrict__ descs,
+                                                                 uint32_t halos, uint32_t total)
 {
   using V = typename StorageTraits<S>::V;
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= count) return;
-  const RemotePair p = pairs[k];
+  if (k >= total) return;
+  uint32_t lo = 0, hi = halos;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (descs[mid].first <= k) lo = mid;
+    else hi = mid;
+  }
+  const RemoteBatchDesc<Q>& d = descs[lo];
+  const uint32_t j = k - d.first;
+  if (j >= d.count) return;
+  const RemotePair p = d.pairs[j];
   V rho, u[3];
-  rho_u_at<Q, S, V>(a, p.src_padded, int(a.ids[p.src_core]), rho, u);
-  V* o = out + uint64_t(p.slot) * 4;
+  rho_u_at<Q, S, V>(d.a, p.src_padded, int(d.a.ids[p.src_core]), rho, u);
+  V* o = static_cast<V*>(d.out) + uint64_t(p.slot) * 4;
   o[0] = rho;
   o[1] = u[0];
   o[2] = u[1];

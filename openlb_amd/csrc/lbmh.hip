@@ -125,9 +125,11 @@ struct lbmh_lattice {
   bool remote_resolved = true;
   void* d_remote = nullptr;
   int64_t init_step = -1;  // step count at which lbmh_initialize asked for the post-processors
-  // descriptor table of the batched halo copies this block leads
+  // descriptor tables of the batched halo copies and remote fetches this block leads
   void* d_halo_batch = nullptr;
   size_t halo_batch_bytes = 0;
+  void* d_remote_batch = nullptr;
+  size_t remote_batch_bytes = 0;
 };
 
 // Library default launch shape, from sweeps of (LDS reservation, XCD group)
@@ -375,11 +377,11 @@ int resolve_remote(lbmh_lattice* L)
 // H->src holds them now, into H->dst's remote buffer.  A source without
 // post-processors may not have streamed this step yet (step `at`): it is read
 // through the shifts it will have, its post-collision populations are final.
-template <int Q, typename S>
-int launch_remote(const lbmh_halo* H, int stage, int64_t at, hipStream_t stream)
+template <int Q>
+int remote_source_args(const lbmh_halo* H, int64_t at, PostArgs<Q>& a)
 {
   const lbmh_lattice* src = H->src;
-  PostArgs<Q> a = post_args<Q>(src);
+  a = post_args<Q>(src);
   post_forcing<Q>(src, a);
   if (src->steps + 1 == at) {
     const int64_t N = src->n_padded;
@@ -391,9 +393,44 @@ int launch_remote(const lbmh_halo* H, int stage, int64_t at, hipStream_t stream)
   } else if (src->steps != at) {
     return fail(LBMH_E_STATE, "blocks of a split lattice are out of step");
   }
-  const uint32_t count = uint32_t(H->remote[stage].size());
-  hipLaunchKernelGGL((remote_moments_kernel<Q, S>), dim3(grid_for(count, 64)), dim3(64), 0, stream, a,
-                     H->d_remote[stage], count, static_cast<typename StorageTraits<S>::V*>(H->dst->d_remote));
+  return LBMH_OK;
+}
+
+// The fetches of one post-processor stage for every block of a group in one
+// launch; the descriptor table is kept by the block that runs the group.
+template <int Q, typename S>
+int launch_remote_stage(lbmh_lattice* runner, const std::vector<lbmh_lattice*>& group, int stage,
+                        hipStream_t stream)
+{
+  std::vector<RemoteBatchDesc<Q>> descs;
+  uint32_t total = 0;
+  for (lbmh_lattice* X : group) {
+    for (lbmh_halo* H : X->halos_in) {
+      if (H->remote[stage].empty()) continue;
+      if (!H->src) return fail(LBMH_E_INVALID, "a block of this halo was destroyed");
+      RemoteBatchDesc<Q> d{};
+      const int rc = remote_source_args<Q>(H, X->steps, d.a);
+      if (rc != LBMH_OK) return rc;
+      d.pairs = H->d_remote[stage];
+      d.out = X->d_remote;
+      d.first = total;
+      d.count = uint32_t(H->remote[stage].size());
+      total += d.count;
+      descs.push_back(d);
+    }
+  }
+  if (descs.empty()) return LBMH_OK;
+  const size_t bytes = sizeof(RemoteBatchDesc<Q>) * descs.size();
+  if (runner->remote_batch_bytes < bytes) {
+    if (runner->d_remote_batch) LBMH_HIP(hipFree(runner->d_remote_batch));
+    runner->d_remote_batch = nullptr;
+    runner->remote_batch_bytes = 0;
+    LBMH_HIP(hipMalloc(&runner->d_remote_batch, bytes));
+    runner->remote_batch_bytes = bytes;
+  }
+  LBMH_HIP(hipMemcpyAsync(runner->d_remote_batch, descs.data(), bytes, hipMemcpyHostToDevice, stream));
+  hipLaunchKernelGGL((remote_moments_many_kernel<Q, S>), dim3(grid_for(total, 64)), dim3(64), 0, stream,
+                     static_cast<const RemoteBatchDesc<Q>*>(runner->d_remote_batch), uint32_t(descs.size()), total);
   LBMH_HIP(hipGetLastError());
   return LBMH_OK;
 }
@@ -492,14 +529,8 @@ int run_post(lbmh_lattice* L, hipStream_t stream, bool init)
     if (rc != LBMH_OK) return rc;
   }
   for (int stage = 0; stage < kPostStages; ++stage) {
-    for (lbmh_lattice* X : group) {
-      for (lbmh_halo* H : X->halos_in) {
-        if (H->remote[stage].empty()) continue;
-        if (!H->src) return fail(LBMH_E_INVALID, "a block of this halo was destroyed");
-        const int rc = launch_remote<Q, S>(H, stage, X->steps, stream);
-        if (rc != LBMH_OK) return rc;
-      }
-    }
+    const int rc = launch_remote_stage<Q, S>(L, group, stage, stream);
+    if (rc != LBMH_OK) return rc;
     for (lbmh_lattice* X : group) {
       const int rc = launch_post_stage<Q, S>(X, stage, stream);
       if (rc != LBMH_OK) return rc;
@@ -735,6 +766,7 @@ int lbmh_lattice_destroy(lbmh_lattice* L)
   }
   if (L->d_remote) (void)hipFree(L->d_remote);
   if (L->d_halo_batch) (void)hipFree(L->d_halo_batch);
+  if (L->d_remote_batch) (void)hipFree(L->d_remote_batch);
   if (L->d_pop) (void)hipFree(L->d_pop);
   if (L->d_force) (void)hipFree(L->d_force);
   if (L->d_ids) (void)hipFree(L->d_ids);
